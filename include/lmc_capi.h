@@ -1,0 +1,185 @@
+/* C ABI of the MI355X LatticeMC event-evaluation engine.
+ *
+ * Plain C types only (no torch, no STL). Every function returns LMC_OK (0) or a negative
+ * status; lmc_last_error() then returns a thread-local message. Pointers named d_* are device
+ * pointers (hipMalloc'ed or torch CUDA tensors); `stream` is a hipStream_t (NULL = default).
+ *
+ * Occupancy format: one uint8 per lattice site, indexed by the reference's lattice id
+ * (x-major rank of the fractional position, lmc/cfg/src/Config.cpp:466-470); the value is the
+ * species code = index of the element in sorted(element_set + {"X"}) by element string
+ * (lmc/cfg/include/Element.hpp:44-49), e.g. {Al, Mg, Zn} -> Al=0 Mg=1 X=2 Zn=3. Lattices are
+ * cubic FCC boxes of n x n x n conventional cells (N = 4 n^3 sites, n >= 4), a = 4.046 A.
+ * A batch of R replicas stores its occupancies back to back; d_offset[r] is the first byte of
+ * replica r and d_ncell[r] its n.
+ */
+#ifndef LMC_CAPI_H_
+#define LMC_CAPI_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMC_OK 0
+#define LMC_ERR_ARG (-1)      /* invalid argument */
+#define LMC_ERR_IO (-2)       /* file missing / unreadable / malformed */
+#define LMC_ERR_HIP (-3)      /* HIP runtime error (no GPU, launch failure, ...) */
+#define LMC_ERR_MODEL (-4)    /* unknown cluster type or missing element parameters */
+
+enum {
+  LMC_MODEL_ENERGY = 0,   /* pred::EnergyPredictor */
+  LMC_MODEL_E0 = 1,       /* pred::VacancyMigrationPredictorE0 */
+  LMC_MODEL_QUARTIC = 2,  /* pred::VacancyMigrationPredictorQuartic */
+  LMC_MODEL_PAIRSITE = 3  /* pred::EnergyChangePredictorPairSite */
+};
+
+typedef struct lmc_model lmc_model;
+typedef struct lmc_kmc lmc_kmc;
+typedef struct lmc_cmc lmc_cmc;
+
+const char *lmc_last_error(void);
+int lmc_version(void);
+
+/* ---- Models --------------------------------------------------------------------------------
+ * Replaces the predictor constructors: EnergyPredictor(json, element_set)
+ * (lmc/pred/include/EnergyPredictor.h:16-17), VacancyMigrationPredictorE0(json, reference_config,
+ * element_set) (VacancyMigrationPredictorE0.h:15-17), VacancyMigrationPredictorQuartic(...)
+ * (VacancyMigrationPredictorQuartic.h:18-20), EnergyChangePredictorPairSite(...)
+ * (EnergyChangePredictorPairSite.h:16-18). The reference config only supplied geometry; the
+ * engine's tables are independent of lattice size, so it is not needed here.
+ * elements: the element set without "X". Unknown element strings become "X" (Element.hpp:25-26).
+ */
+int lmc_model_load(const char *json_path, const char *const *elements, int32_t n_elements,
+                   int32_t kind, lmc_model **out);
+void lmc_model_free(lmc_model *m);
+int32_t lmc_model_num_species(const lmc_model *m);
+/* Species code of an element string, -1 if not in the model. */
+int32_t lmc_model_species_code(const lmc_model *m, const char *element);
+/* Sizes: number of ECIs (Base.theta), MMM and MM2 feature lengths. */
+int lmc_model_layout(const lmc_model *m, int32_t *n_eci, int32_t *n_mmm, int32_t *n_mm2);
+/* ECI key `index`: label (0..7) and its sorted species codes (-1 padded to 3). */
+int lmc_model_eci_key(const lmc_model *m, int32_t index, int32_t *label, int32_t *codes3);
+/* Orbit layout of the MMM (which=0) or MM2 (which=1) features; arrays of capacity `cap`:
+ * orbit size (clusters), kind (0 singlet, 1 pair, 2 symmetric pair), shell, feature base and
+ * length. Returns the number of orbits in *n_orbits. */
+int lmc_model_orbits(const lmc_model *m, int32_t which, int32_t cap, int32_t *n_orbits,
+                     int32_t *size, int32_t *kind, int32_t *shell, int32_t *base, int32_t *len);
+
+/* ---- Geometry (host) --------------------------------------------------------------------- */
+int64_t lmc_fcc_num_sites(int32_t n);
+/* Neighbour ids of `site` in shell 1/2/3, ascending (Config::UpdateNeighbors,
+ * Config.cpp:955-1045). Returns the count (12/6/24) or a negative status. */
+int32_t lmc_fcc_neighbors(int32_t n, int64_t site, int32_t shell, int64_t *out);
+/* Cluster tables of the jump v -> nb as lattice ids (for validation):
+ * state clusters as (label, s0, s1, s2) and MMM/MM2 orbit clusters as (orbit, sym, s0, s1). */
+int32_t lmc_jump_state_clusters(int32_t n, int64_t v, int64_t nb, int64_t *out, int32_t cap);
+int32_t lmc_jump_orbit_clusters(int32_t n, int64_t v, int64_t nb, int32_t which, int64_t *out,
+                                int32_t cap);
+
+/* ---- Stateless evaluation on the GPU ------------------------------------------------------
+ * The 12 vacancy jumps of each replica, in the reference's event order (neighbour ids
+ * ascending): replaces the 12 calls of GetBarrierAndDiffFromLatticeIdPair made by
+ * KineticMcFirstOmp::BuildEventList (lmc/mc/src/KineticMcFirstOmp.cpp:52-68,
+ * VacancyMigrationPredictorE0.cpp:153-159, VacancyMigrationPredictorQuartic.cpp:247-276).
+ * d_vac[r]: vacancy site; outputs d_nb/d_ea/d_de of length 12 R (eV).
+ * Asynchronous on `stream`; errors of the model (unknown cluster type, missing element
+ * parameters) are reported by lmc_check_errors(). */
+int lmc_eval_jumps(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                   const int32_t *d_ncell, const int64_t *d_vac, int32_t n_replicas,
+                   int64_t *d_nb, double *d_ea, double *d_de, void *stream);
+/* One jump v -> nb of one replica, synchronous, host outputs: replaces
+ * GetBarrierAndDiffFromLatticeIdPair(config, {v, nb}). */
+int lmc_eval_pair(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64_t v, int64_t nb,
+                  double *ea, double *de);
+/* Swap energy change of d_pairs[2r], d_pairs[2r+1] per replica:
+ * EnergyChangePredictorPairSite::GetDeFromLatticeIdPair (EnergyChangePredictorPairSite.cpp:69-81). */
+int lmc_swap_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                const int32_t *d_ncell, const int64_t *d_pairs, int32_t n_replicas, double *d_de,
+                void *stream);
+/* Energy change of setting d_sites[r] to species d_codes[r]:
+ * EnergyChangePredictorPairSite::GetDeFromLatticeIdSite (EnergyChangePredictorPairSite.cpp:154-192). */
+int lmc_site_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                const int32_t *d_ncell, const int64_t *d_sites, const uint8_t *d_codes,
+                int32_t n_replicas, double *d_de, void *stream);
+/* Total cluster-expansion energy per replica: EnergyPredictor::GetEnergy
+ * (lmc/pred/src/EnergyPredictor.cpp:173-177). */
+int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                     const int32_t *d_ncell, int32_t n_replicas, double *d_energy, void *stream);
+/* Returns LMC_ERR_MODEL (and clears the flag) if a kernel of model m met an unknown cluster
+ * type or an element without barrier parameters since the last check. Synchronous. */
+int lmc_check_errors(const lmc_model *m);
+
+/* ---- Batched KineticMcFirst driver ------------------------------------------------------
+ * Replaces KineticMcFirstOmp(config, ..., maximum_steps, ..., restart_steps, restart_energy,
+ * restart_time, temperature, element_set, json, time_temperature_filename, rate_corrector,
+ * early_stop, solute_disp) (lmc/mc/include/KineticMcFirstOmp.h:11-25) for R independent
+ * replicas, and Simulate() (KineticMcAbstract.cpp:184-190) by lmc_kmc_run. The initial
+ * absolute energy is EnergyPredictor::GetEnergy of the start config (McAbstract.cpp:26).
+ * Random numbers: Philox4x32-10 keyed by seed, counter (step, replica_id, 0). */
+typedef struct {
+  double temperature;      /* K */
+  uint64_t seed;
+  uint32_t replica_id;     /* Philox stream of this replica */
+  uint64_t restart_steps;
+  double restart_energy;
+  double restart_time;
+  uint64_t maximum_steps;  /* the run ends after step maximum_steps (steps_ <= maximum_steps) */
+  int32_t rate_corrector;  /* RateCorrector (lmc/pred/include/RateCorrector.hpp:17-24) */
+  int32_t early_stop;      /* IsEscaped (KineticMcAbstract.cpp:192-223) */
+  int32_t solute_disp;     /* solute centre-of-mass trajectory (KineticMcAbstract.cpp:163-177) */
+} lmc_kmc_replica;
+
+/* occ: host array of all replicas back to back (ncell[r] gives the sizes). */
+int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t n_replicas,
+                   const int32_t *ncell, const uint8_t *occ, const lmc_kmc_replica *params,
+                   lmc_kmc **out);
+void lmc_kmc_free(lmc_kmc *k);
+/* Piecewise-linear T(time) shared by all replicas (TimeTemperatureInterpolator). */
+int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const double *temp);
+/* Per-replica trace of the next `capacity` steps (selected neighbour, Ea, dE, time). 0 = off. */
+int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity);
+/* Advance every replica by `steps` steps (fewer if it reaches maximum_steps or escapes).
+ * Asynchronous on `stream`. */
+int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream);
+/* Host copies of the state arrays (each of length R, or 3R for disp/com); any may be NULL.
+ * disp: vacancy displacement in units of a/2; com: solute COM trajectory (A). Synchronous. */
+int lmc_kmc_state(lmc_kmc *k, uint64_t *steps, double *time, double *energy, double *abs_energy,
+                  double *temperature, int64_t *vacancy, int64_t *disp, double *com,
+                  int32_t *escaped);
+int lmc_kmc_occupancy(lmc_kmc *k, uint8_t *occ);
+/* Trace arrays [R][capacity]. */
+int lmc_kmc_trace(lmc_kmc *k, int64_t *sel, double *ea, double *de, double *time);
+/* Device views of the batch occupancy (for callers that evaluate it with lmc_eval_jumps). */
+int lmc_kmc_device_views(lmc_kmc *k, uint8_t **d_occ, const int64_t **d_offset,
+                         const int32_t **d_ncell, const int64_t **d_vac);
+
+/* ---- Batched CanonicalMc driver -----------------------------------------------------------
+ * Replaces CanonicalMcSerial(config, ..., maximum_steps, ..., restart_steps, restart_energy,
+ * temperature, element_set, json) (lmc/mc/include/CanonicalMcSerial.h:11-20) and Simulate()
+ * (CanonicalMcSerial.cpp:40-51). Temperature: t_start + (t_end - t_start) * step/maximum_steps
+ * (t_end = t_start gives the reference's constant temperature). Random numbers: Philox
+ * counter (step, replica_id, 2*attempt+1) for the site pair, (step, replica_id, 0xFFFFFFFE) for
+ * the Metropolis test. */
+typedef struct {
+  double t_start, t_end;
+  uint64_t seed;
+  uint32_t replica_id;
+  uint64_t restart_steps;
+  double restart_energy;
+  uint64_t maximum_steps;
+} lmc_cmc_replica;
+int lmc_cmc_create(const lmc_model *pairsite, const lmc_model *energy, int32_t n_replicas,
+                   const int32_t *ncell, const uint8_t *occ, const lmc_cmc_replica *params,
+                   lmc_cmc **out);
+void lmc_cmc_free(lmc_cmc *c);
+int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity);
+int lmc_cmc_run(lmc_cmc *c, uint64_t trials, void *stream);
+int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energy,
+                  uint64_t *accepted);
+int lmc_cmc_occupancy(lmc_cmc *c, uint8_t *occ);
+int lmc_cmc_trace(lmc_cmc *c, int64_t *p, int64_t *q, double *de, int8_t *accepted);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LMC_CAPI_H_ */

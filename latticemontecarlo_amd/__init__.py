@@ -1,0 +1,9 @@
+"""MI355X-native engine for the LatticeMC event-evaluation hot path.
+
+The compute path is liblmc.so (hand-written gfx950 HIP kernels behind the C ABI of
+include/lmc_capi.h); this package is the Python mirror of the reference's predictor and driver
+interface over that ABI.
+"""
+from .api import (E0, ENERGY, PAIRSITE, QUARTIC, CanonicalMc, KineticMcFirst, Model,  # noqa: F401
+                  ReplicaBatch, eval_jumps, fcc_neighbors, fcc_num_sites, jump_orbit_clusters,
+                  jump_state_clusters, site_de, swap_de, total_energy)

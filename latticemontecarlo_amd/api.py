@@ -1,0 +1,305 @@
+"""Python mirror of the reference's predictor and driver interface over the C ABI.
+
+Names follow zhucongx/LatticeMonteCarlo (lmc/pred/include/*.h, lmc/mc/include/*.h). Device
+buffers are torch CUDA tensors; every compute call goes to a HIP kernel through liblmc.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check, dptr, lib, ptr
+
+ENERGY, E0, QUARTIC, PAIRSITE = capi.ENERGY, capi.E0, capi.QUARTIC, capi.PAIRSITE
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Model:
+    """Coefficient model of one predictor kind (see lmc_model_load)."""
+
+    def __init__(self, json_path, element_set, kind):
+        els = sorted(set(element_set) - {"X"})
+        arr = (C.c_char_p * len(els))(*[e.encode() for e in els])
+        h = C.c_void_p()
+        check(lib().lmc_model_load(str(json_path).encode(), arr, len(els), kind, C.byref(h)))
+        self.h = h
+        self.kind = kind
+        self.element_set = els
+        n = lib().lmc_model_num_species(self.h)
+        self.species = sorted(els + ["X"])
+        assert len(self.species) == n
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lmc_model_free(self.h)
+            self.h = None
+
+    def code(self, element):
+        return lib().lmc_model_species_code(self.h, element.encode())
+
+    def layout(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().lmc_model_layout(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def eci_keys(self):
+        n_eci = self.layout()[0]
+        out = []
+        lab = C.c_int32()
+        codes = np.zeros(3, np.int32)
+        for i in range(n_eci):
+            check(lib().lmc_model_eci_key(self.h, i, C.byref(lab), ptr(codes, capi._i32p)))
+            out.append((lab.value, tuple(int(c) for c in codes if c >= 0)))
+        return out
+
+    def orbits(self, which):
+        cap = 512
+        arrs = [np.zeros(cap, np.int32) for _ in range(5)]
+        n = C.c_int32()
+        check(lib().lmc_model_orbits(self.h, which, cap, C.byref(n),
+                                     *[ptr(a, capi._i32p) for a in arrs]))
+        k = n.value
+        return dict(size=arrs[0][:k], kind=arrs[1][:k], shell=arrs[2][:k], base=arrs[3][:k],
+                    len=arrs[4][:k])
+
+    def check_errors(self):
+        check(lib().lmc_check_errors(self.h))
+
+
+def fcc_num_sites(n):
+    return int(lib().lmc_fcc_num_sites(n))
+
+
+def fcc_neighbors(n, site, shell):
+    out = np.zeros(24, np.int64)
+    k = lib().lmc_fcc_neighbors(n, site, shell, ptr(out, capi._i64p))
+    if k < 0:
+        check(k)
+    return out[:k].copy()
+
+
+def jump_state_clusters(n, v, nb):
+    out = np.zeros((1024, 4), np.int64)
+    k = lib().lmc_jump_state_clusters(n, v, nb, ptr(out, capi._i64p), 1024)
+    if k < 0:
+        check(k)
+    return out[:k].copy()
+
+
+def jump_orbit_clusters(n, v, nb, which):
+    out = np.zeros((1024, 4), np.int64)
+    k = lib().lmc_jump_orbit_clusters(n, v, nb, which, ptr(out, capi._i64p), 1024)
+    if k < 0:
+        check(k)
+    return out[:k].copy()
+
+
+class ReplicaBatch:
+    """Occupancies of R replicas on the GPU (concatenated uint8 with offsets)."""
+
+    def __init__(self, ncell, occ_list, device="cuda"):
+        torch = _torch()
+        self.ncell_host = np.asarray(ncell, np.int32)
+        sizes = [4 * int(n) ** 3 for n in self.ncell_host]
+        self.offset_host = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        host = np.concatenate([np.asarray(o, np.uint8) for o in occ_list])
+        self.occ = torch.from_numpy(host).to(device)
+        self.offset = torch.from_numpy(self.offset_host).to(device)
+        self.ncell = torch.from_numpy(self.ncell_host).to(device)
+        self.R = len(sizes)
+
+
+def eval_jumps(model, batch, vac, stream=None):
+    """12 jumps per replica in neighbour-id order: (nb[R,12], Ea[R,12], dE[R,12])."""
+    torch = _torch()
+    R = batch.R
+    vac_t = torch.as_tensor(np.asarray(vac, np.int64)).to(batch.occ.device)
+    nb = torch.empty((R, 12), dtype=torch.int64, device=batch.occ.device)
+    ea = torch.empty((R, 12), dtype=torch.float64, device=batch.occ.device)
+    de = torch.empty((R, 12), dtype=torch.float64, device=batch.occ.device)
+    check(lib().lmc_eval_jumps(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
+                               dptr(vac_t), R, dptr(nb), dptr(ea), dptr(de),
+                               capi.stream_ptr(stream)))
+    return nb, ea, de
+
+
+def swap_de(model, batch, pairs, stream=None):
+    torch = _torch()
+    p = torch.as_tensor(np.asarray(pairs, np.int64).reshape(-1)).to(batch.occ.device)
+    de = torch.empty(batch.R, dtype=torch.float64, device=batch.occ.device)
+    check(lib().lmc_swap_de(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
+                            dptr(p), batch.R, dptr(de), capi.stream_ptr(stream)))
+    return de
+
+
+def site_de(model, batch, sites, codes, stream=None):
+    torch = _torch()
+    s = torch.as_tensor(np.asarray(sites, np.int64)).to(batch.occ.device)
+    c = torch.as_tensor(np.asarray(codes, np.uint8)).to(batch.occ.device)
+    de = torch.empty(batch.R, dtype=torch.float64, device=batch.occ.device)
+    check(lib().lmc_site_de(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
+                            dptr(s), dptr(c), batch.R, dptr(de), capi.stream_ptr(stream)))
+    return de
+
+
+def total_energy(model, batch, stream=None):
+    torch = _torch()
+    e = torch.empty(batch.R, dtype=torch.float64, device=batch.occ.device)
+    check(lib().lmc_total_energy(model.h, dptr(batch.occ), dptr(batch.offset),
+                                 dptr(batch.ncell), batch.R, dptr(e), capi.stream_ptr(stream)))
+    return e
+
+
+class KineticMcFirst:
+    """Batched KineticMcFirst driver: R independent replicas advanced on the GPU."""
+
+    def __init__(self, barrier_model, energy_model, ncell, occ_list, temperature, seed,
+                 maximum_steps, replica_ids=None, restart_steps=0, restart_energy=0.0,
+                 restart_time=0.0, rate_corrector=False, early_stop=False, solute_disp=False):
+        R = len(ncell)
+        self.R = R
+        self.ncell = np.asarray(ncell, np.int32)
+        temps = np.broadcast_to(np.asarray(temperature, np.float64), (R,))
+        rid = np.arange(R) if replica_ids is None else np.asarray(replica_ids)
+        params = (capi.KmcReplica * R)()
+        for r in range(R):
+            p = params[r]
+            p.temperature = float(temps[r])
+            p.seed = int(seed)
+            p.replica_id = int(rid[r])
+            p.restart_steps = int(np.broadcast_to(restart_steps, (R,))[r])
+            p.restart_energy = float(np.broadcast_to(restart_energy, (R,))[r])
+            p.restart_time = float(np.broadcast_to(restart_time, (R,))[r])
+            p.maximum_steps = int(np.broadcast_to(maximum_steps, (R,))[r])
+            p.rate_corrector = int(rate_corrector)
+            p.early_stop = int(early_stop)
+            p.solute_disp = int(solute_disp)
+        occ = np.ascontiguousarray(np.concatenate([np.asarray(o, np.uint8) for o in occ_list]))
+        self.total_sites = occ.size
+        h = C.c_void_p()
+        check(lib().lmc_kmc_create(barrier_model.h, energy_model.h if energy_model else None, R,
+                                   ptr(self.ncell, capi._i32p), ptr(occ, capi._u8p), params,
+                                   C.byref(h)))
+        self.h = h
+        self.trace_cap = 0
+        self._models = (barrier_model, energy_model)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lmc_kmc_free(self.h)
+            self.h = None
+
+    def set_schedule(self, times, temps):
+        t = np.ascontiguousarray(times, np.float64)
+        T = np.ascontiguousarray(temps, np.float64)
+        check(lib().lmc_kmc_set_schedule(self.h, len(t), ptr(t, capi._f64p), ptr(T, capi._f64p)))
+
+    def set_trace(self, capacity):
+        check(lib().lmc_kmc_set_trace(self.h, capacity))
+        self.trace_cap = capacity
+
+    def run(self, steps, stream=None):
+        s = capi.stream_ptr(stream) if stream is not None else None
+        check(lib().lmc_kmc_run(self.h, steps, s))
+
+    def state(self):
+        R = self.R
+        out = dict(steps=np.zeros(R, np.uint64), time=np.zeros(R), energy=np.zeros(R),
+                   abs_energy=np.zeros(R), temperature=np.zeros(R), vacancy=np.zeros(R, np.int64),
+                   disp=np.zeros((R, 3), np.int64), com=np.zeros((R, 3)),
+                   escaped=np.zeros(R, np.int32))
+        check(lib().lmc_kmc_state(self.h, ptr(out["steps"], capi._u64p),
+                                  ptr(out["time"], capi._f64p), ptr(out["energy"], capi._f64p),
+                                  ptr(out["abs_energy"], capi._f64p),
+                                  ptr(out["temperature"], capi._f64p),
+                                  ptr(out["vacancy"], capi._i64p), ptr(out["disp"], capi._i64p),
+                                  ptr(out["com"], capi._f64p), ptr(out["escaped"], capi._i32p)))
+        return out
+
+    def occupancy(self):
+        out = np.zeros(self.total_sites, np.uint8)
+        check(lib().lmc_kmc_occupancy(self.h, ptr(out, capi._u8p)))
+        sizes = 4 * self.ncell.astype(np.int64) ** 3
+        return np.split(out, np.cumsum(sizes)[:-1])
+
+    def trace(self):
+        n = self.R * self.trace_cap
+        sel, ea, de, t = np.zeros(n, np.int64), np.zeros(n), np.zeros(n), np.zeros(n)
+        check(lib().lmc_kmc_trace(self.h, ptr(sel, capi._i64p), ptr(ea, capi._f64p),
+                                  ptr(de, capi._f64p), ptr(t, capi._f64p)))
+        shape = (self.R, self.trace_cap)
+        return dict(sel=sel.reshape(shape), ea=ea.reshape(shape), de=de.reshape(shape),
+                    time=t.reshape(shape))
+
+
+class CanonicalMc:
+    """Batched CanonicalMcSerial driver (optional linear temperature ramp)."""
+
+    def __init__(self, pairsite_model, energy_model, ncell, occ_list, t_start, t_end, seed,
+                 maximum_steps, replica_ids=None, restart_steps=0, restart_energy=0.0):
+        R = len(ncell)
+        self.R = R
+        self.ncell = np.asarray(ncell, np.int32)
+        rid = np.arange(R) if replica_ids is None else np.asarray(replica_ids)
+        params = (capi.CmcReplica * R)()
+        for r in range(R):
+            p = params[r]
+            p.t_start = float(np.broadcast_to(t_start, (R,))[r])
+            p.t_end = float(np.broadcast_to(t_end, (R,))[r])
+            p.seed = int(seed)
+            p.replica_id = int(rid[r])
+            p.restart_steps = int(np.broadcast_to(restart_steps, (R,))[r])
+            p.restart_energy = float(np.broadcast_to(restart_energy, (R,))[r])
+            p.maximum_steps = int(np.broadcast_to(maximum_steps, (R,))[r])
+        occ = np.ascontiguousarray(np.concatenate([np.asarray(o, np.uint8) for o in occ_list]))
+        self.total_sites = occ.size
+        h = C.c_void_p()
+        check(lib().lmc_cmc_create(pairsite_model.h, energy_model.h if energy_model else None, R,
+                                   ptr(self.ncell, capi._i32p), ptr(occ, capi._u8p), params,
+                                   C.byref(h)))
+        self.h = h
+        self.trace_cap = 0
+        self._models = (pairsite_model, energy_model)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lmc_cmc_free(self.h)
+            self.h = None
+
+    def set_trace(self, capacity):
+        check(lib().lmc_cmc_set_trace(self.h, capacity))
+        self.trace_cap = capacity
+
+    def run(self, trials, stream=None):
+        s = capi.stream_ptr(stream) if stream is not None else None
+        check(lib().lmc_cmc_run(self.h, trials, s))
+
+    def state(self):
+        R = self.R
+        out = dict(steps=np.zeros(R, np.uint64), energy=np.zeros(R), abs_energy=np.zeros(R),
+                   accepted=np.zeros(R, np.uint64))
+        check(lib().lmc_cmc_state(self.h, ptr(out["steps"], capi._u64p),
+                                  ptr(out["energy"], capi._f64p),
+                                  ptr(out["abs_energy"], capi._f64p),
+                                  ptr(out["accepted"], capi._u64p)))
+        return out
+
+    def occupancy(self):
+        out = np.zeros(self.total_sites, np.uint8)
+        check(lib().lmc_cmc_occupancy(self.h, ptr(out, capi._u8p)))
+        sizes = 4 * self.ncell.astype(np.int64) ** 3
+        return np.split(out, np.cumsum(sizes)[:-1])
+
+    def trace(self):
+        n = self.R * self.trace_cap
+        p, q = np.zeros(n, np.int64), np.zeros(n, np.int64)
+        de, acc = np.zeros(n), np.zeros(n, np.int8)
+        check(lib().lmc_cmc_trace(self.h, ptr(p, capi._i64p), ptr(q, capi._i64p),
+                                  ptr(de, capi._f64p), ptr(acc, capi._i8p)))
+        shape = (self.R, self.trace_cap)
+        return dict(p=p.reshape(shape), q=q.reshape(shape), de=de.reshape(shape),
+                    acc=acc.reshape(shape))

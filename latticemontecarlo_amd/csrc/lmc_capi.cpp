@@ -1,0 +1,920 @@
+// C ABI of the engine: model loading, table upload, stateless evaluation and the batched
+// KineticMcFirst / CanonicalMc drivers. Every compute entry point launches a HIP kernel; there
+// is no host fallback.
+#include "lmc_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lmc_device.h"
+#include "lmc_model.h"
+#include "lmc_tables.h"
+
+struct lmc_model {
+  lmc::Model *m;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int Fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ModelError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void Check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int Guard(F &&f) {
+  try {
+    return f();
+  } catch (const HipError &e) {
+    return Fail(LMC_ERR_HIP, e.what());
+  } catch (const ModelError &e) {
+    return Fail(LMC_ERR_MODEL, e.what());
+  } catch (const std::invalid_argument &e) {
+    return Fail(LMC_ERR_ARG, e.what());
+  } catch (const std::exception &e) {
+    return Fail(LMC_ERR_IO, e.what());
+  }
+}
+
+int32_t PackOff(const lmc::I3 &o) {
+  return (o.x & 0xff) | ((o.y & 0xff) << 8) | ((o.z & 0xff) << 16);
+}
+
+uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side) {
+  int other[2] = {63, 63}, k = 0;
+  for (int i = 0; i < c.size; ++i)
+    if (c.slots[i] != endpoint_slot) other[k++] = c.slots[i];
+  return static_cast<uint32_t>(c.label) | (n_side ? 8u : 0u) |
+         (static_cast<uint32_t>(other[0]) << 4) | (static_cast<uint32_t>(other[1]) << 10);
+}
+
+std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<int> &base) {
+  std::vector<uint32_t> v;
+  for (const auto &c : ot.clusters) {
+    const int kind = ot.orbit_kind[c.orbit];
+    const uint32_t b = static_cast<uint32_t>(base[c.orbit]);
+    if (b > 4095) throw std::invalid_argument("feature vector too long for the packed tables");
+    v.push_back(b | (static_cast<uint32_t>(kind) << 12) | (static_cast<uint32_t>(c.s0) << 14) |
+                (static_cast<uint32_t>(c.s1 < 0 ? 63 : c.s1) << 20));
+  }
+  return v;
+}
+
+struct BlobBuilder {
+  std::vector<char> bytes;
+  std::vector<std::pair<size_t, const void **>> fix;
+  template <typename T>
+  void Add(const std::vector<T> &v, const T **dst) {
+    const size_t at = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(at + std::max<size_t>(v.size() * sizeof(T), 8), 0);
+    if (!v.empty()) std::memcpy(bytes.data() + at, v.data(), v.size() * sizeof(T));
+    fix.emplace_back(at, reinterpret_cast<const void **>(dst));
+  }
+};
+
+int CurrentDevice() {
+  int d = -1;
+  Check(hipGetDevice(&d), "hipGetDevice");
+  return d;
+}
+
+}  // namespace
+
+namespace lmc {
+
+Model::~Model() {
+  if (dev) {
+    if (dev->blob) (void)hipFree(dev->blob);
+    delete dev;
+  }
+}
+
+const KTables &Upload(Model &m) {
+  const int device = CurrentDevice();
+  if (m.dev) {
+    if (m.dev->device != device)
+      throw std::invalid_argument("model was uploaded to another device");
+    return m.dev->k;
+  }
+  const JumpTables &jt = GetJumpTables();
+  const SiteTables &st = GetSiteTables();
+  KTables k{};
+  std::vector<int32_t> off;
+  for (int d = 0; d < 12; ++d)
+    for (int s = 0; s < kEnvSlots; ++s) off.push_back(PackOff(jt.off[d][s]));
+  std::vector<uint8_t> mirror(jt.mirror.begin(), jt.mirror.end());
+  std::vector<uint32_t> state;
+  for (const auto &c : jt.state_v_only) state.push_back(PackState(c, jt.v_slot, false));
+  for (const auto &c : jt.state_n_only) state.push_back(PackState(c, jt.n_slot, true));
+  const auto mmm = PackOrbits(jt.mmm, m.base_mmm);
+  const auto mm2 = PackOrbits(jt.mm2, m.base_mm2);
+  std::vector<uint32_t> site_cl;
+  for (const auto &c : st.clusters) site_cl.push_back(PackState(c, 0, false));
+  std::vector<int32_t> site_off;
+  for (int s = 0; s < kSiteSlots; ++s) site_off.push_back(PackOff(st.off[s]));
+  std::vector<double> par = m.par;
+  for (int c = 0; c < m.S; ++c)
+    if (!m.bar[c].present)
+      for (int i = 0; i < 6; ++i) par[6 * c + i] = std::numeric_limits<double>::quiet_NaN();
+  std::vector<int32_t> e1(m.eci.t1.begin(), m.eci.t1.end()), e2(m.eci.t2.begin(), m.eci.t2.end()),
+      e3(m.eci.t3.begin(), m.eci.t3.end());
+  for (auto *v : {&e1, &e2, &e3})
+    for (auto &x : *v)
+      if (x < 0) x = 0;  // invalid keys only occur with two vacancies in one cluster
+  if (state.size() > kMaxStateClusters || mmm.size() > kMaxOrbitClusters ||
+      site_cl.size() > kMaxSiteClusters || m.w.size() > kMaxW)
+    throw std::invalid_argument("tables exceed the kernel's LDS capacity (too many species)");
+
+  BlobBuilder b;
+  b.Add(off, &k.off);
+  b.Add(mirror, &k.mirror);
+  b.Add(state, &k.state);
+  b.Add(mmm, &k.mmm);
+  b.Add(mm2, &k.mm2);
+  b.Add(site_cl, &k.site_cl);
+  b.Add(site_off, &k.site_off);
+  b.Add(m.w, &k.w);
+  b.Add(m.b_mmm, &k.b1);
+  b.Add(m.b_mm2, &k.b2);
+  b.Add(par, &k.par);
+  b.Add(m.type_tab, &k.tt);
+  b.Add(m.theta, &k.theta);
+  b.Add(m.norm_total, &k.norm_tot);
+  b.Add(e1, &k.eci1);
+  b.Add(e2, &k.eci2);
+  b.Add(e3, &k.eci3);
+  std::vector<int32_t> errflag(1, 0);
+  const int32_t *err_ptr = nullptr;
+  b.Add(errflag, &err_ptr);
+
+  auto dev = std::make_unique<DeviceTables>();
+  dev->device = device;
+  Check(hipMalloc(&dev->blob, b.bytes.size()), "hipMalloc(model tables)");
+  Check(hipMemcpy(dev->blob, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice),
+        "hipMemcpy(model tables)");
+  for (auto &f : b.fix) *f.second = static_cast<char *>(dev->blob) + f.first;
+  k.vac_shell_mask = 0;
+  for (int s : jt.vac_shell_slots) k.vac_shell_mask |= 1ull << s;
+  k.n_state = static_cast<int>(state.size());
+  k.n_mmm = static_cast<int>(mmm.size());
+  k.n_mm2 = static_cast<int>(mm2.size());
+  k.n_site = static_cast<int>(site_cl.size());
+  k.n_eci = static_cast<int>(m.theta.size());
+  k.v_slot = jt.v_slot;
+  k.n_slot = jt.n_slot;
+  k.kind = m.kind;
+  k.S = m.S;
+  k.x_code = m.x_code;
+  k.F1 = m.F_mmm;
+  k.F2 = m.F_mm2;
+  k.nw = static_cast<int>(m.w.size());
+  dev->k = k;
+  m.dev = dev.release();
+  m.dev_err = const_cast<int32_t *>(err_ptr);
+  return m.dev->k;
+}
+
+}  // namespace lmc
+
+using lmc::Model;
+
+namespace {
+
+hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+void RequireModel(const lmc_model *m) {
+  if (!m || !m->m) throw std::invalid_argument("null model");
+}
+
+template <typename T>
+T *DevAlloc(size_t n) {
+  void *p = nullptr;
+  Check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+  return static_cast<T *>(p);
+}
+template <typename T>
+T *DevCopy(const std::vector<T> &v) {
+  T *p = DevAlloc<T>(v.size());
+  if (!v.empty())
+    Check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+  return p;
+}
+template <typename T>
+void DevFree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <typename T>
+void ToHost(T *dst, const T *src, size_t n) {
+  if (dst && n) Check(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+double ElementMass(const std::string &e) {  // Element::GetMass (Element.hpp:71-88)
+  if (e == "Al") return 26.98;
+  if (e == "Mg") return 24.31;
+  if (e == "Zn") return 65.38;
+  if (e == "Cu") return 63.55;
+  if (e == "Sn") return 118.71;
+  return 0.0;
+}
+
+struct Batch {
+  std::vector<int64_t> off;
+  int64_t total = 0;
+};
+Batch Layout(int32_t R, const int32_t *ncell) {
+  if (R <= 0) throw std::invalid_argument("n_replicas must be positive");
+  Batch b;
+  for (int r = 0; r < R; ++r) {
+    if (ncell[r] < 4 || ncell[r] > 1000)
+      throw std::invalid_argument("lattice size n must be in [4, 1000]");
+    b.off.push_back(b.total);
+    b.total += 4ll * ncell[r] * ncell[r] * ncell[r];
+  }
+  return b;
+}
+
+void ComputeTotalEnergy(const lmc_model *em, const uint8_t *d_occ, const int64_t *d_off,
+                        const int32_t *d_ncell, int32_t R, double *d_energy, hipStream_t s) {
+  const auto &t = lmc::Upload(*em->m);
+  unsigned long long *counts = DevAlloc<unsigned long long>(static_cast<size_t>(R) * t.n_eci);
+  const hipError_t e = lmc::LaunchTotalEnergy(t, d_occ, d_off, d_ncell, R, counts, d_energy, s);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(counts);
+  Check(e, "total energy kernel");
+}
+
+}  // namespace
+
+struct lmc_kmc {
+  const lmc_model *bm = nullptr;
+  lmc::KmcDev d;
+  uint64_t trace_pos = 0;
+  ~lmc_kmc() {
+    DevFree(d.occ);
+    DevFree(d.off);
+    DevFree(d.ncell);
+    DevFree(d.vac);
+    DevFree(d.steps);
+    DevFree(d.max_steps);
+    DevFree(d.seed);
+    DevFree(d.rid);
+    DevFree(d.time);
+    DevFree(d.energy);
+    DevFree(d.abs_energy);
+    DevFree(d.temp);
+    DevFree(d.cv);
+    DevFree(d.cs);
+    DevFree(d.solute_mass);
+    DevFree(d.flags);
+    DevFree(d.solvent);
+    DevFree(d.escaped);
+    DevFree(d.disp);
+    DevFree(d.com);
+    DevFree(d.error);
+    DevFree(d.tt_time);
+    DevFree(d.tt_temp);
+    DevFree(d.tr_sel);
+    DevFree(d.tr_ea);
+    DevFree(d.tr_de);
+    DevFree(d.tr_time);
+  }
+};
+
+struct lmc_cmc {
+  const lmc_model *pm = nullptr;
+  lmc::CmcDev d;
+  uint64_t trace_pos = 0;
+  ~lmc_cmc() {
+    DevFree(d.occ);
+    DevFree(d.off);
+    DevFree(d.ncell);
+    DevFree(d.steps);
+    DevFree(d.max_steps);
+    DevFree(d.seed);
+    DevFree(d.accepted);
+    DevFree(d.rid);
+    DevFree(d.t0);
+    DevFree(d.t1);
+    DevFree(d.energy);
+    DevFree(d.abs_energy);
+    DevFree(d.error);
+    DevFree(d.tr_p);
+    DevFree(d.tr_q);
+    DevFree(d.tr_de);
+    DevFree(d.tr_acc);
+  }
+};
+
+extern "C" {
+
+const char *lmc_last_error(void) { return g_error.c_str(); }
+int lmc_version(void) { return 1; }
+
+int lmc_model_load(const char *json_path, const char *const *elements, int32_t n_elements,
+                   int32_t kind, lmc_model **out) {
+  return Guard([&] {
+    if (!json_path || !elements || n_elements <= 0 || !out)
+      throw std::invalid_argument("lmc_model_load: null argument");
+    if (kind < LMC_MODEL_ENERGY || kind > LMC_MODEL_PAIRSITE)
+      throw std::invalid_argument("lmc_model_load: unknown kind");
+    std::vector<std::string> el;
+    for (int i = 0; i < n_elements; ++i) el.emplace_back(elements[i] ? elements[i] : "");
+    std::unique_ptr<Model> m(lmc::LoadModel(json_path, el, kind));
+    *out = new lmc_model{m.release()};
+    return LMC_OK;
+  });
+}
+void lmc_model_free(lmc_model *m) {
+  if (m) {
+    delete m->m;
+    delete m;
+  }
+}
+int32_t lmc_model_num_species(const lmc_model *m) { return m && m->m ? m->m->S : -1; }
+int32_t lmc_model_species_code(const lmc_model *m, const char *element) {
+  if (!m || !m->m || !element) return -1;
+  for (int i = 0; i < m->m->S; ++i)
+    if (m->m->eci.species[i] == element) return i;
+  return -1;
+}
+int lmc_model_layout(const lmc_model *m, int32_t *n_eci, int32_t *n_mmm, int32_t *n_mm2) {
+  return Guard([&] {
+    RequireModel(m);
+    if (n_eci) *n_eci = static_cast<int32_t>(m->m->eci.keys.size());
+    if (n_mmm) *n_mmm = m->m->F_mmm;
+    if (n_mm2) *n_mm2 = m->m->F_mm2;
+    return LMC_OK;
+  });
+}
+int lmc_model_eci_key(const lmc_model *m, int32_t index, int32_t *label, int32_t *codes3) {
+  return Guard([&] {
+    RequireModel(m);
+    const auto &keys = m->m->eci.keys;
+    if (index < 0 || index >= static_cast<int32_t>(keys.size()))
+      throw std::invalid_argument("ECI index out of range");
+    *label = keys[index].label;
+    for (int i = 0; i < 3; ++i)
+      codes3[i] = i < static_cast<int>(keys[index].codes.size()) ? keys[index].codes[i] : -1;
+    return LMC_OK;
+  });
+}
+int lmc_model_orbits(const lmc_model *m, int32_t which, int32_t cap, int32_t *n_orbits,
+                     int32_t *size, int32_t *kind, int32_t *shell, int32_t *base, int32_t *len) {
+  return Guard([&] {
+    RequireModel(m);
+    const auto &jt = lmc::GetJumpTables();
+    const auto &ot = which == 0 ? jt.mmm : jt.mm2;
+    const auto &bs = which == 0 ? m->m->base_mmm : m->m->base_mm2;
+    const auto &ln = which == 0 ? m->m->len_mmm : m->m->len_mm2;
+    const int no = static_cast<int>(ot.orbit_size.size());
+    if (no > cap) throw std::invalid_argument("orbit array capacity too small");
+    *n_orbits = no;
+    for (int o = 0; o < no; ++o) {
+      size[o] = ot.orbit_size[o];
+      kind[o] = ot.orbit_kind[o];
+      shell[o] = ot.orbit_shell[o];
+      base[o] = bs[o];
+      len[o] = ln[o];
+    }
+    return LMC_OK;
+  });
+}
+
+int64_t lmc_fcc_num_sites(int32_t n) { return n >= 4 ? 4ll * n * n * n : -1; }
+
+int32_t lmc_fcc_neighbors(int32_t n, int64_t site, int32_t shell, int64_t *out) {
+  if (n < 4 || site < 0 || site >= 4ll * n * n * n || shell < 1 || shell > 3 || !out)
+    return Fail(LMC_ERR_ARG, "lmc_fcc_neighbors: bad argument");
+  const lmc::I3 *tab = shell == 1 ? lmc::kNN1 : shell == 2 ? lmc::kNN2 : lmc::kNN3;
+  const int cnt = shell == 1 ? 12 : shell == 2 ? 6 : 24;
+  for (int i = 0; i < cnt; ++i) out[i] = lmc::SiteAdd(n, site, tab[i]);
+  std::sort(out, out + cnt);
+  return cnt;
+}
+
+static int FindDirection(int32_t n, int64_t v, int64_t nb) {
+  for (int k = 0; k < 12; ++k)
+    if (lmc::SiteAdd(n, v, lmc::kNN1[k]) == nb) return k;
+  return -1;
+}
+
+int32_t lmc_jump_state_clusters(int32_t n, int64_t v, int64_t nb, int64_t *out, int32_t cap) {
+  if (n < 4) return Fail(LMC_ERR_ARG, "n must be >= 4");
+  const int k = FindDirection(n, v, nb);
+  if (k < 0) return Fail(LMC_ERR_ARG, "sites are not first neighbours");
+  const auto &jt = lmc::GetJumpTables();
+  int32_t cnt = 0;
+  for (const auto &c : jt.state) {
+    if (cnt >= cap) return Fail(LMC_ERR_ARG, "capacity too small");
+    int64_t *r = out + 4 * cnt;
+    r[0] = c.label;
+    for (int i = 0; i < 3; ++i)
+      r[1 + i] = i < c.size ? lmc::SiteAdd(n, v, jt.off[k][c.slots[i]]) : -1;
+    ++cnt;
+  }
+  return cnt;
+}
+
+int32_t lmc_jump_orbit_clusters(int32_t n, int64_t v, int64_t nb, int32_t which, int64_t *out,
+                                int32_t cap) {
+  if (n < 4) return Fail(LMC_ERR_ARG, "n must be >= 4");
+  const int k = FindDirection(n, v, nb);
+  if (k < 0) return Fail(LMC_ERR_ARG, "sites are not first neighbours");
+  const auto &jt = lmc::GetJumpTables();
+  const auto &ot = which == 0 ? jt.mmm : jt.mm2;
+  int32_t cnt = 0;
+  for (const auto &c : ot.clusters) {
+    if (cnt >= cap) return Fail(LMC_ERR_ARG, "capacity too small");
+    int64_t *r = out + 4 * cnt;
+    r[0] = c.orbit;
+    r[1] = c.sym ? 1 : 0;
+    r[2] = lmc::SiteAdd(n, v, jt.off[k][c.s0]);
+    r[3] = c.s1 < 0 ? -1 : lmc::SiteAdd(n, v, jt.off[k][c.s1]);
+    ++cnt;
+  }
+  return cnt;
+}
+
+int lmc_eval_jumps(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                   const int32_t *d_ncell, const int64_t *d_vac, int32_t n_replicas,
+                   int64_t *d_nb, double *d_ea, double *d_de, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    if (m->m->kind != LMC_MODEL_E0 && m->m->kind != LMC_MODEL_QUARTIC)
+      throw std::invalid_argument("lmc_eval_jumps needs an E0 or Quartic model");
+    if (n_replicas < 0) throw std::invalid_argument("n_replicas < 0");
+    const auto &t = lmc::Upload(*m->m);
+    Check(lmc::LaunchEvalJumps(t, d_occ, d_offset, d_ncell, d_vac, n_replicas, d_nb, d_ea, d_de,
+                               m->m->dev_err, S(stream)),
+          "eval jumps kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_check_errors(const lmc_model *m) {
+  return Guard([&] {
+    RequireModel(m);
+    if (!m->m->dev) return LMC_OK;
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    int32_t e = 0;
+    Check(hipMemcpy(&e, m->m->dev_err, sizeof(e), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (e) {
+      const int32_t z = 0;
+      Check(hipMemcpy(m->m->dev_err, &z, sizeof(z), hipMemcpyHostToDevice), "hipMemcpy");
+      throw ModelError("Cluster not found in ClusterIndexer or element without barrier parameters");
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_eval_pair(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64_t v, int64_t nb,
+                  double *ea, double *de) {
+  return Guard([&] {
+    RequireModel(m);
+    const int k = FindDirection(ncell, v, nb);
+    if (k < 0) throw std::out_of_range("Neighbor not found for lattice pair in GetPairFlatIndex");
+    int64_t *buf = DevAlloc<int64_t>(2 + 12);
+    double *fb = DevAlloc<double>(24);
+    int32_t *nc = DevAlloc<int32_t>(1);
+    const int64_t h[2] = {0, v};
+    int rc = LMC_OK;
+    try {
+      Check(hipMemcpy(buf, h, sizeof(h), hipMemcpyHostToDevice), "hipMemcpy");
+      Check(hipMemcpy(nc, &ncell, sizeof(ncell), hipMemcpyHostToDevice), "hipMemcpy");
+      const auto &t = lmc::Upload(*m->m);
+      Check(lmc::LaunchEvalJumps(t, d_occ, buf, nc, buf + 1, 1, buf + 2, fb, fb + 12,
+                                 m->m->dev_err, nullptr),
+            "eval jumps kernel");
+      int64_t nbs[12];
+      double r[24];
+      Check(hipMemcpy(nbs, buf + 2, sizeof(nbs), hipMemcpyDeviceToHost), "hipMemcpy");
+      Check(hipMemcpy(r, fb, sizeof(r), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (int i = 0; i < 12; ++i)
+        if (nbs[i] == nb) {
+          *ea = r[i];
+          *de = r[12 + i];
+        }
+    } catch (...) {
+      (void)hipFree(buf);
+      (void)hipFree(fb);
+      (void)hipFree(nc);
+      throw;
+    }
+    (void)hipFree(buf);
+    (void)hipFree(fb);
+    (void)hipFree(nc);
+    rc = lmc_check_errors(m);
+    return rc;
+  });
+}
+
+int lmc_swap_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                const int32_t *d_ncell, const int64_t *d_pairs, int32_t n_replicas, double *d_de,
+                void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    const auto &t = lmc::Upload(*m->m);
+    Check(lmc::LaunchSwapDe(t, d_occ, d_offset, d_ncell, d_pairs, n_replicas, d_de, S(stream)),
+          "swap dE kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_site_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                const int32_t *d_ncell, const int64_t *d_sites, const uint8_t *d_codes,
+                int32_t n_replicas, double *d_de, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    const auto &t = lmc::Upload(*m->m);
+    Check(lmc::LaunchSiteDe(t, d_occ, d_offset, d_ncell, d_sites, d_codes, n_replicas, d_de,
+                            S(stream)),
+          "site dE kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                     const int32_t *d_ncell, int32_t n_replicas, double *d_energy, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    ComputeTotalEnergy(m, d_occ, d_offset, d_ncell, n_replicas, d_energy, S(stream));
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
+                   const int32_t *ncell, const uint8_t *occ, const lmc_kmc_replica *params,
+                   lmc_kmc **out) {
+  return Guard([&] {
+    RequireModel(barrier);
+    if (barrier->m->kind != LMC_MODEL_E0 && barrier->m->kind != LMC_MODEL_QUARTIC)
+      throw std::invalid_argument("KMC needs an E0 or Quartic barrier model");
+    if (!ncell || !occ || !params || !out) throw std::invalid_argument("null argument");
+    const Batch b = Layout(R, ncell);
+    const Model &m = *barrier->m;
+    auto k = std::make_unique<lmc_kmc>();
+    k->bm = barrier;
+    k->d.R = R;
+    std::vector<int64_t> vac(R);
+    std::vector<double> cv(R), cs(R), smass(R), temp(R), time(R), en(R);
+    std::vector<int32_t> flags(R), solvent(R);
+    std::vector<uint64_t> steps(R), maxs(R), seed(R);
+    std::vector<uint32_t> rid(R);
+    int al = -1;
+    for (int c = 0; c < m.S; ++c)
+      if (m.eci.species[c] == "Al") al = c;
+    for (int r = 0; r < R; ++r) {
+      const uint8_t *o = occ + b.off[r];
+      const int64_t N = 4ll * ncell[r] * ncell[r] * ncell[r];
+      std::vector<int64_t> cnt(m.S, 0);
+      int64_t nv = 0;
+      for (int64_t i = 0; i < N; ++i) {
+        if (o[i] >= m.S) throw std::invalid_argument("occupancy holds a species code >= S");
+        cnt[o[i]]++;
+        if (o[i] == m.x_code) {
+          vac[r] = i;
+          ++nv;
+        }
+      }
+      if (nv != 1) throw std::invalid_argument("each replica needs exactly one vacancy");
+      const lmc_kmc_replica &p = params[r];
+      if (!(p.temperature > 0)) throw std::invalid_argument("temperature must be positive");
+      cv[r] = static_cast<double>(nv) / static_cast<double>(N);
+      int64_t ns = 0;
+      for (int c = 0; c < m.S; ++c)
+        if (c != al && c != m.x_code) ns += cnt[c];
+      cs[r] = static_cast<double>(ns) / static_cast<double>(N);
+      int64_t mx = 0;  // Config::GetSolventElement: first species with the largest count
+      for (int c = 0; c < m.S; ++c)
+        if (cnt[c] > mx) {
+          mx = cnt[c];
+          solvent[r] = c;
+        }
+      double sm = 0;
+      for (int c = 0; c < m.S; ++c)
+        if (c != solvent[r]) sm += ElementMass(m.eci.species[c]) * static_cast<double>(cnt[c]);
+      smass[r] = sm;
+      temp[r] = p.temperature;
+      time[r] = p.restart_time;
+      en[r] = p.restart_energy;
+      steps[r] = p.restart_steps;
+      maxs[r] = p.maximum_steps;
+      seed[r] = p.seed;
+      rid[r] = p.replica_id;
+      flags[r] = (p.rate_corrector ? 1 : 0) | (p.early_stop ? 2 : 0) | (p.solute_disp ? 4 : 0);
+    }
+    for (int c = 0; c < m.S && c < lmc::kMaxSpecies; ++c)
+      k->d.mass[c] = ElementMass(m.eci.species[c]);
+    auto &d = k->d;
+    d.occ = DevCopy(std::vector<uint8_t>(occ, occ + b.total));
+    d.off = DevCopy(b.off);
+    d.ncell = DevCopy(std::vector<int32_t>(ncell, ncell + R));
+    d.vac = DevCopy(vac);
+    d.steps = DevCopy(steps);
+    d.max_steps = DevCopy(maxs);
+    d.seed = DevCopy(seed);
+    d.rid = DevCopy(rid);
+    d.time = DevCopy(time);
+    d.energy = DevCopy(en);
+    d.abs_energy = DevCopy(std::vector<double>(R, 0.0));
+    d.temp = DevCopy(temp);
+    d.cv = DevCopy(cv);
+    d.cs = DevCopy(cs);
+    d.solute_mass = DevCopy(smass);
+    d.flags = DevCopy(flags);
+    d.solvent = DevCopy(solvent);
+    d.escaped = DevCopy(std::vector<int32_t>(R, 0));
+    d.disp = DevCopy(std::vector<int64_t>(3 * R, 0));
+    d.com = DevCopy(std::vector<double>(3 * R, 0.0));
+    d.error = DevCopy(std::vector<int32_t>(1, 0));
+    lmc::Upload(*barrier->m);
+    if (energy) {
+      RequireModel(energy);
+      ComputeTotalEnergy(energy, d.occ, d.off, d.ncell, R, d.abs_energy, nullptr);
+    }
+    *out = k.release();
+    return LMC_OK;
+  });
+}
+
+void lmc_kmc_free(lmc_kmc *k) { delete k; }
+
+int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const double *temp) {
+  return Guard([&] {
+    if (!k || n_points < 0) throw std::invalid_argument("bad argument");
+    std::vector<std::pair<double, double>> pts;
+    for (int i = 0; i < n_points; ++i) pts.emplace_back(time[i], temp[i]);
+    std::sort(pts.begin(), pts.end());  // TimeTemperatureInterpolator::SortPoints (:32-43)
+    for (size_t i = 1; i < pts.size(); ++i)
+      if (std::abs(pts[i].first - pts[i - 1].first) < 1e-12)
+        throw std::invalid_argument("2 adjacent x values are equal.");
+    std::vector<double> tv, Tv;
+    for (auto &p : pts) {
+      tv.push_back(p.first);
+      Tv.push_back(p.second);
+    }
+    DevFree(k->d.tt_time);
+    DevFree(k->d.tt_temp);
+    k->d.n_tt = n_points;
+    if (n_points) {
+      k->d.tt_time = DevCopy(tv);
+      k->d.tt_temp = DevCopy(Tv);
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    DevFree(k->d.tr_sel);
+    DevFree(k->d.tr_ea);
+    DevFree(k->d.tr_de);
+    DevFree(k->d.tr_time);
+    k->d.trace_cap = capacity;
+    k->trace_pos = 0;
+    if (capacity) {
+      const size_t n = static_cast<size_t>(capacity) * k->d.R;
+      k->d.tr_sel = DevAlloc<int64_t>(n);
+      k->d.tr_ea = DevAlloc<double>(n);
+      k->d.tr_de = DevAlloc<double>(n);
+      k->d.tr_time = DevAlloc<double>(n);
+      Check(hipMemset(k->d.tr_sel, 0xff, n * sizeof(int64_t)), "hipMemset");
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    const auto &t = lmc::Upload(*k->bm->m);
+    Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream)), "KMC kernel");
+    k->trace_pos += steps;
+    return LMC_OK;
+  });
+}
+
+static int KmcErrors(lmc_kmc *k) {
+  int32_t e = 0;
+  Check(hipMemcpy(&e, k->d.error, sizeof(e), hipMemcpyDeviceToHost), "hipMemcpy");
+  if (e) throw ModelError("Cluster not found in ClusterIndexer or element without barrier parameters");
+  return LMC_OK;
+}
+
+int lmc_kmc_state(lmc_kmc *k, uint64_t *steps, double *time, double *energy, double *abs_energy,
+                  double *temperature, int64_t *vacancy, int64_t *disp, double *com,
+                  int32_t *escaped) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t R = static_cast<size_t>(k->d.R);
+    ToHost(steps, k->d.steps, R);
+    ToHost(time, k->d.time, R);
+    ToHost(energy, k->d.energy, R);
+    ToHost(abs_energy, k->d.abs_energy, R);
+    ToHost(temperature, k->d.temp, R);
+    ToHost(vacancy, k->d.vac, R);
+    ToHost(disp, k->d.disp, 3 * R);
+    ToHost(com, k->d.com, 3 * R);
+    ToHost(escaped, k->d.escaped, R);
+    return KmcErrors(k);
+  });
+}
+
+int lmc_kmc_occupancy(lmc_kmc *k, uint8_t *occ) {
+  return Guard([&] {
+    if (!k || !occ) throw std::invalid_argument("null argument");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    std::vector<int64_t> off(k->d.R);
+    std::vector<int32_t> nc(k->d.R);
+    ToHost(off.data(), k->d.off, off.size());
+    ToHost(nc.data(), k->d.ncell, nc.size());
+    const int64_t total = off.back() + 4ll * nc.back() * nc.back() * nc.back();
+    ToHost(occ, k->d.occ, static_cast<size_t>(total));
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_trace(lmc_kmc *k, int64_t *sel, double *ea, double *de, double *time) {
+  return Guard([&] {
+    if (!k || !k->d.trace_cap) throw std::invalid_argument("trace is off");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t n = static_cast<size_t>(k->d.trace_cap) * k->d.R;
+    ToHost(sel, k->d.tr_sel, n);
+    ToHost(ea, k->d.tr_ea, n);
+    ToHost(de, k->d.tr_de, n);
+    ToHost(time, k->d.tr_time, n);
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_device_views(lmc_kmc *k, uint8_t **d_occ, const int64_t **d_offset,
+                         const int32_t **d_ncell, const int64_t **d_vac) {
+  if (!k) return Fail(LMC_ERR_ARG, "null driver");
+  if (d_occ) *d_occ = k->d.occ;
+  if (d_offset) *d_offset = k->d.off;
+  if (d_ncell) *d_ncell = k->d.ncell;
+  if (d_vac) *d_vac = k->d.vac;
+  return LMC_OK;
+}
+
+int lmc_cmc_create(const lmc_model *pairsite, const lmc_model *energy, int32_t R,
+                   const int32_t *ncell, const uint8_t *occ, const lmc_cmc_replica *params,
+                   lmc_cmc **out) {
+  return Guard([&] {
+    RequireModel(pairsite);
+    if (!ncell || !occ || !params || !out) throw std::invalid_argument("null argument");
+    const Batch b = Layout(R, ncell);
+    const Model &m = *pairsite->m;
+    auto c = std::make_unique<lmc_cmc>();
+    c->pm = pairsite;
+    c->d.R = R;
+    std::vector<uint64_t> steps(R), maxs(R), seed(R);
+    std::vector<uint32_t> rid(R);
+    std::vector<double> t0(R), t1(R), en(R);
+    for (int r = 0; r < R; ++r) {
+      const int64_t N = 4ll * ncell[r] * ncell[r] * ncell[r];
+      const uint8_t *o = occ + b.off[r];
+      bool two = false;
+      for (int64_t i = 0; i < N; ++i) {
+        if (o[i] >= m.S) throw std::invalid_argument("occupancy holds a species code >= S");
+        two |= o[i] != o[0];
+      }
+      if (!two) throw std::invalid_argument("canonical MC needs at least two species");
+      const lmc_cmc_replica &p = params[r];
+      if (!(p.t_start > 0) || !(p.t_end > 0) || p.maximum_steps == 0)
+        throw std::invalid_argument("temperatures and maximum_steps must be positive");
+      steps[r] = p.restart_steps;
+      maxs[r] = p.maximum_steps;
+      seed[r] = p.seed;
+      rid[r] = p.replica_id;
+      t0[r] = p.t_start;
+      t1[r] = p.t_end;
+      en[r] = p.restart_energy;
+    }
+    auto &d = c->d;
+    d.occ = DevCopy(std::vector<uint8_t>(occ, occ + b.total));
+    d.off = DevCopy(b.off);
+    d.ncell = DevCopy(std::vector<int32_t>(ncell, ncell + R));
+    d.steps = DevCopy(steps);
+    d.max_steps = DevCopy(maxs);
+    d.seed = DevCopy(seed);
+    d.accepted = DevCopy(std::vector<uint64_t>(R, 0));
+    d.rid = DevCopy(rid);
+    d.t0 = DevCopy(t0);
+    d.t1 = DevCopy(t1);
+    d.energy = DevCopy(en);
+    d.abs_energy = DevCopy(std::vector<double>(R, 0.0));
+    d.error = DevCopy(std::vector<int32_t>(1, 0));
+    lmc::Upload(*pairsite->m);
+    if (energy) {
+      RequireModel(energy);
+      ComputeTotalEnergy(energy, d.occ, d.off, d.ncell, R, d.abs_energy, nullptr);
+    }
+    *out = c.release();
+    return LMC_OK;
+  });
+}
+
+void lmc_cmc_free(lmc_cmc *c) { delete c; }
+
+int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    DevFree(c->d.tr_p);
+    DevFree(c->d.tr_q);
+    DevFree(c->d.tr_de);
+    DevFree(c->d.tr_acc);
+    c->d.trace_cap = capacity;
+    c->trace_pos = 0;
+    if (capacity) {
+      const size_t n = static_cast<size_t>(capacity) * c->d.R;
+      c->d.tr_p = DevAlloc<int64_t>(n);
+      c->d.tr_q = DevAlloc<int64_t>(n);
+      c->d.tr_de = DevAlloc<double>(n);
+      c->d.tr_acc = DevAlloc<int8_t>(n);
+      Check(hipMemset(c->d.tr_p, 0xff, n * sizeof(int64_t)), "hipMemset");
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_run(lmc_cmc *c, uint64_t trials, void *stream) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    const auto &t = lmc::Upload(*c->pm->m);
+    Check(lmc::LaunchCmc(t, c->d, trials, c->trace_pos, S(stream)), "CMC kernel");
+    c->trace_pos += trials;
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energy,
+                  uint64_t *accepted) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t R = static_cast<size_t>(c->d.R);
+    ToHost(steps, c->d.steps, R);
+    ToHost(energy, c->d.energy, R);
+    ToHost(abs_energy, c->d.abs_energy, R);
+    ToHost(accepted, c->d.accepted, R);
+    int32_t e = 0;
+    Check(hipMemcpy(&e, c->d.error, sizeof(e), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (e) throw ModelError("no pair of distinct species found");
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_occupancy(lmc_cmc *c, uint8_t *occ) {
+  return Guard([&] {
+    if (!c || !occ) throw std::invalid_argument("null argument");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    std::vector<int64_t> off(c->d.R);
+    std::vector<int32_t> nc(c->d.R);
+    ToHost(off.data(), c->d.off, off.size());
+    ToHost(nc.data(), c->d.ncell, nc.size());
+    const int64_t total = off.back() + 4ll * nc.back() * nc.back() * nc.back();
+    ToHost(occ, c->d.occ, static_cast<size_t>(total));
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_trace(lmc_cmc *c, int64_t *p, int64_t *q, double *de, int8_t *accepted) {
+  return Guard([&] {
+    if (!c || !c->d.trace_cap) throw std::invalid_argument("trace is off");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t n = static_cast<size_t>(c->d.trace_cap) * c->d.R;
+    ToHost(p, c->d.tr_p, n);
+    ToHost(q, c->d.tr_q, n);
+    ToHost(de, c->d.tr_de, n);
+    ToHost(accepted, c->d.tr_acc, n);
+    return LMC_OK;
+  });
+}
+
+}  // extern "C"

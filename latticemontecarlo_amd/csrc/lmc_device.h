@@ -1,0 +1,116 @@
+// Device-side tables and kernel launchers (implemented in lmc_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lmc_model.h"
+
+namespace lmc {
+
+// Packed table entries (all slots < 64, 63 = none):
+//   offset   : int8 x | int8 y << 8 | int8 z << 16
+//   state    : label[0:3] | n_endpoint[3] | other1[4:10] | other2[10:16]
+//   orbit    : feature base[0:12] | kind[12:14] | s0[14:20] | s1[20:26]
+//   site     : label[0:3] | other1[4:10] | other2[10:16]
+constexpr int kMaxStateClusters = 512;
+constexpr int kMaxOrbitClusters = 640;
+constexpr int kMaxSiteClusters = 256;
+constexpr int kMaxW = 1024;  // w1 | w2 | w3 for S <= 6
+
+// Kernel view of a model (passed by value).
+struct KTables {
+  const int32_t *off;       // [12][60]
+  const uint8_t *mirror;    // [60]
+  const uint32_t *state;    // one-endpoint state clusters
+  const uint32_t *mmm;
+  const uint32_t *mm2;
+  const uint32_t *site_cl;
+  const int32_t *site_off;  // [43]
+  const double *w;          // theta/N by (label, codes)
+  const double *b1;         // [S][F1]
+  const double *b2;         // [S][F2]
+  const double *par;        // [S][6]
+  const int8_t *tt;         // [3][S][S]
+  const double *theta;      // [n_eci]
+  const double *norm_tot;   // [n_eci]
+  const int32_t *eci1;      // [S]
+  const int32_t *eci2;      // [3][S][S]
+  const int32_t *eci3;      // [4][S][S][S]
+  uint64_t vac_shell_mask;  // env slots within 3NN of the vacancy
+  int n_state, n_mmm, n_mm2, n_site, n_eci;
+  int v_slot, n_slot;
+  int kind, S, x_code, F1, F2, nw;
+};
+
+struct DeviceTables {
+  int device = -1;
+  void *blob = nullptr;  // one allocation holding every table
+  KTables k{};
+};
+
+// Uploads (once) and returns the kernel view of a model on the current device.
+const KTables &Upload(Model &m);
+
+// Per-replica KMC state, structure of arrays in device memory.
+struct KmcDev {
+  int32_t R = 0;
+  uint8_t *occ = nullptr;       // concatenated occupancy
+  int64_t *off = nullptr;       // [R] occupancy offsets
+  int32_t *ncell = nullptr;     // [R]
+  int64_t *vac = nullptr;       // [R]
+  uint64_t *steps = nullptr;    // [R]
+  uint64_t *max_steps = nullptr;
+  uint64_t *seed = nullptr;
+  uint32_t *rid = nullptr;      // Philox replica id
+  double *time = nullptr, *energy = nullptr, *abs_energy = nullptr, *temp = nullptr;
+  double *cv = nullptr, *cs = nullptr, *solute_mass = nullptr;
+  int32_t *flags = nullptr;     // bit0 rate corrector, bit1 early stop, bit2 solute disp
+  int32_t *solvent = nullptr;   // species code of the solvent
+  int32_t *escaped = nullptr;
+  int64_t *disp = nullptr;      // [R][3] vacancy displacement, a/2 units
+  double *com = nullptr;        // [R][3] solute centre-of-mass trajectory
+  int32_t *error = nullptr;     // [1] first error code
+  // optional temperature schedule (shared)
+  int32_t n_tt = 0;
+  double *tt_time = nullptr, *tt_temp = nullptr;
+  // optional trace [R][cap]
+  uint64_t trace_cap = 0;
+  int64_t *tr_sel = nullptr;
+  double *tr_ea = nullptr, *tr_de = nullptr, *tr_time = nullptr;
+  double mass[kMaxSpecies] = {};  // element masses by species code
+};
+
+struct CmcDev {
+  int32_t R = 0;
+  uint8_t *occ = nullptr;
+  int64_t *off = nullptr;
+  int32_t *ncell = nullptr;
+  uint64_t *steps = nullptr, *max_steps = nullptr, *seed = nullptr, *accepted = nullptr;
+  uint32_t *rid = nullptr;
+  double *t0 = nullptr, *t1 = nullptr, *energy = nullptr, *abs_energy = nullptr;
+  int32_t *error = nullptr;
+  uint64_t trace_cap = 0;
+  int64_t *tr_p = nullptr, *tr_q = nullptr;
+  double *tr_de = nullptr;
+  int8_t *tr_acc = nullptr;
+};
+
+hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *off,
+                           const int32_t *ncell, const int64_t *vac, int32_t R, int64_t *nb,
+                           double *ea, double *de, int32_t *err, hipStream_t s);
+hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
+                     hipStream_t s);
+hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off,
+                        const int32_t *ncell, const int64_t *pairs, int32_t R, double *de,
+                        hipStream_t s);
+hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off,
+                        const int32_t *ncell, const int64_t *sites, const uint8_t *codes,
+                        int32_t R, double *de, hipStream_t s);
+hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_t trace_base,
+                     hipStream_t s);
+hipError_t LaunchTotalEnergy(const KTables &t, const uint8_t *occ, const int64_t *off,
+                             const int32_t *ncell, int32_t R, unsigned long long *counts,
+                             double *energy, hipStream_t s);
+
+}  // namespace lmc

@@ -1,0 +1,199 @@
+"""GPU parity: the HIP kernels against the CPU oracle on the same seeded inputs.
+
+Bars: neighbour ids, event choice, occupancy and every integer state bit-exact; E_kra, dE, swap
+and total energies within 1e-9 eV (absolute); residence time within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import QUATERNARY, TERNARY, random_alloc
+
+pytestmark = pytest.mark.gpu
+
+TOL_E = 1e-9
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _oracle_models(path, els, n, kinds):
+    import oracle as O
+    lat = O.Lattice(n)
+    return lat, [O.Model(lat, path, els, k) for k in kinds]
+
+
+@pytest.mark.parametrize("name,kind", [("e0_3", 1), ("e0_3n", 1), ("e0_4n", 1), ("q_3n", 2),
+                                       ("q_4", 2)])
+@pytest.mark.parametrize("n", [5, 8])
+def test_eval_jumps_matches_oracle(torch, coeffs, name, kind, n):
+    import latticemontecarlo_amd as L
+    path, els = coeffs[name]
+    lat, (om,) = _oracle_models(path, els, n, [kind])
+    gm = L.Model(path, els, kind)
+    species = gm.species
+    N = 4 * n ** 3
+    occs, vacs = [], []
+    for r in range(6):
+        occ = random_alloc(n, species, 0.08 + 0.04 * r, seed=100 * n + r)
+        x = species.index("X")
+        v = [0, N - 1, 4 * n * n - 1][r] if r < 3 else int(np.flatnonzero(occ == x)[0])
+        if r < 3:  # move the vacancy to a boundary site
+            occ[occ == x] = occ[v]
+            occ[v] = x
+        occs.append(occ)
+        vacs.append(v)
+    batch = L.ReplicaBatch([n] * len(occs), occs)
+    nb, ea, de = L.eval_jumps(gm, batch, vacs)
+    torch.cuda.synchronize()
+    gm.check_errors()
+    nb, ea, de = nb.cpu().numpy(), ea.cpu().numpy(), de.cpu().numpy()
+    for r, (occ, v) in enumerate(zip(occs, vacs)):
+        ref_nb = lat.neighbors(v, 1)
+        assert np.array_equal(nb[r], ref_nb)
+        for i, b in enumerate(ref_nb):
+            e_ref, d_ref = om.barrier_de(occ, v, int(b))
+            assert abs(de[r, i] - d_ref) <= TOL_E, (r, i, de[r, i], d_ref)
+            assert abs(ea[r, i] - e_ref) <= TOL_E, (r, i, ea[r, i], e_ref)
+
+
+@pytest.mark.parametrize("name", ["e0_3n", "q_4"])
+def test_swap_and_site_de_match_oracle(torch, coeffs, name):
+    import latticemontecarlo_amd as L
+    path, els = coeffs[name]
+    n = 6
+    lat, (om,) = _oracle_models(path, els, n, [3])
+    gm = L.Model(path, els, L.PAIRSITE)
+    species = gm.species
+    N = 4 * n ** 3
+    rng = np.random.default_rng(7)
+    occ = random_alloc(n, species, 0.12, seed=77)
+    pairs = []
+    for r in range(48):
+        p = int(rng.integers(N))
+        if r % 3 == 0:
+            q = int(rng.integers(N))  # mostly far pairs
+        else:  # near pairs: 1st/2nd/3rd shells
+            q = int(rng.choice(lat.neighbors(p, 1 + r % 3)))
+        pairs.append((p, q))
+    batch = L.ReplicaBatch([n] * len(pairs), [occ] * len(pairs))
+    de = L.swap_de(gm, batch, pairs).cpu().numpy()
+    for r, (p, q) in enumerate(pairs):
+        assert abs(de[r] - om.swap_de(occ, p, q)) <= TOL_E
+    sites = [int(rng.integers(N)) for _ in range(24)]
+    codes = [int(rng.integers(len(species))) for _ in range(24)]
+    batch2 = L.ReplicaBatch([n] * 24, [occ] * 24)
+    sde = L.site_de(gm, batch2, sites, codes).cpu().numpy()
+    for r in range(24):
+        assert abs(sde[r] - om.site_de(occ, sites[r], codes[r])) <= TOL_E
+
+
+@pytest.mark.parametrize("name", ["e0_3", "e0_4n"])
+def test_total_energy_matches_oracle(torch, coeffs, name):
+    import latticemontecarlo_amd as L
+    path, els = coeffs[name]
+    gm = L.Model(path, els, L.ENERGY)
+    occs, ns = [], [4, 5, 7]
+    for i, n in enumerate(ns):
+        occs.append(random_alloc(n, gm.species, 0.1, seed=i))
+    batch = L.ReplicaBatch(ns, occs)
+    e = L.total_energy(gm, batch).cpu().numpy()
+    for i, n in enumerate(ns):
+        lat, (om,) = _oracle_models(path, els, n, [0])
+        assert abs(e[i] - om.total_energy(occs[i])) <= TOL_E * max(1.0, abs(e[i]))
+
+
+def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, **opts):
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs[name]
+    gb = L.Model(path, els, kind)
+    ge = L.Model(path, els, L.ENERGY)
+    species = gb.species
+    occs = [random_alloc(n, species, 0.05 + 0.03 * r, seed=500 + r) for r in range(R)]
+    seed = 987654321
+    tt = opts.pop("tt", None)
+    drv = L.KineticMcFirst(gb, ge, [n] * R, occs, temps, seed, maximum_steps=10 ** 9, **opts)
+    if tt is not None:
+        drv.set_schedule(*tt)
+    drv.set_trace(steps)
+    drv.run(steps)
+    gst = drv.state()
+    gtr = drv.trace()
+    gocc = drv.occupancy()
+    lat = O.Lattice(n)
+    ob = O.Model(lat, path, els, kind)
+    oe = O.Model(lat, path, els, 0)
+    refs = []
+    for r in range(R):
+        k = O.Kmc(ob, oe, lat, occs[r], float(np.broadcast_to(temps, (R,))[r]), seed, replica=r,
+                  tt=tt, **opts)
+        tr = k.run(steps, trace=True)
+        refs.append((k.state(), tr, k.occupancy()))
+    return gst, gtr, gocc, refs
+
+
+def _compare_kmc(gst, gtr, gocc, refs, steps):
+    for r, (st, tr, occ) in enumerate(refs):
+        assert np.array_equal(gtr["sel"][r], tr["sel"].astype(np.int64)), f"replica {r}"
+        np.testing.assert_allclose(gtr["ea"][r], tr["ea"], rtol=0, atol=TOL_E)
+        np.testing.assert_allclose(gtr["de"][r], tr["de"], rtol=0, atol=TOL_E)
+        np.testing.assert_allclose(gtr["time"][r], tr["time"], rtol=1e-12, atol=0)
+        assert np.array_equal(gocc[r], occ)
+        assert int(gst["steps"][r]) == st["steps"] == steps
+        assert int(gst["vacancy"][r]) == st["vacancy"]
+        assert np.array_equal(gst["disp"][r], st["disp"])
+        assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * steps
+        assert abs(gst["abs_energy"][r] - st["abs_energy"]) <= TOL_E * (steps + abs(st["abs_energy"]))
+        np.testing.assert_allclose(gst["time"][r], st["time"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name,kind", [("e0_3n", 1), ("q_3n", 2)])
+def test_kmc_trajectory_matches_oracle(torch, coeffs, name, kind):
+    steps = 400
+    gst, gtr, gocc, refs = _run_kmc_pair(coeffs, name, kind, n=6, R=4, steps=steps,
+                                         temps=np.array([450.0, 500.0, 600.0, 700.0]))
+    _compare_kmc(gst, gtr, gocc, refs, steps)
+
+
+def test_kmc_options_match_oracle(torch, coeffs):
+    steps = 300
+    tt = (np.array([0.0, 1e-9, 1e-6]), np.array([800.0, 600.0, 400.0]))
+    gst, gtr, gocc, refs = _run_kmc_pair(coeffs, "e0_4n", 1, n=5, R=3, steps=steps,
+                                         temps=550.0, rate_corrector=True, solute_disp=True,
+                                         tt=tt)
+    _compare_kmc(gst, gtr, gocc, refs, steps)
+    for r, (st, _, _) in enumerate(refs):
+        np.testing.assert_allclose(gst["com"][r], st["com"], rtol=0, atol=1e-9)
+        np.testing.assert_allclose(gst["temperature"][r], st["temperature"], rtol=1e-12)
+
+
+def test_cmc_trajectory_matches_oracle(torch, coeffs):
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs["e0_3n"]
+    n, R, trials = 5, 3, 600
+    gp = L.Model(path, els, L.PAIRSITE)
+    ge = L.Model(path, els, L.ENERGY)
+    occs = [random_alloc(n, gp.species, 0.05, seed=900 + r, vacancy=False) for r in range(R)]
+    seed = 42
+    drv = L.CanonicalMc(gp, ge, [n] * R, occs, 800.0, 300.0, seed, maximum_steps=trials)
+    drv.set_trace(trials)
+    drv.run(trials)
+    gst, gtr, gocc = drv.state(), drv.trace(), drv.occupancy()
+    lat = O.Lattice(n)
+    op = O.Model(lat, path, els, O.PAIRSITE)
+    oe = O.Model(lat, path, els, O.ENERGY)
+    for r in range(R):
+        c = O.Cmc(op, oe, lat, occs[r], 800.0, 300.0, seed, replica=r)
+        tr = c.run(trials, trials, trace=True)
+        st = c.state()
+        assert np.array_equal(gtr["p"][r], tr["p"]) and np.array_equal(gtr["q"][r], tr["q"])
+        np.testing.assert_allclose(gtr["de"][r], tr["de"], rtol=0, atol=TOL_E)
+        assert np.array_equal(gtr["acc"][r], tr["acc"])
+        assert np.array_equal(gocc[r], c.occupancy())
+        assert int(gst["accepted"][r]) == st["accepted"]
+        assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * trials
