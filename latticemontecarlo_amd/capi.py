@@ -99,6 +99,9 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with "
                               "`python -m latticemontecarlo_amd.build` (hipcc, gfx950)")
+        # torch ships its own libamdhip64.so.7; load it first so that liblmc.so binds to the
+        # same HIP runtime instance as the torch tensors it receives.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
