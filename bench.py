@@ -1,0 +1,195 @@
+"""Benchmark: plain batched KMC on one MI355X.
+
+Workload (the plain batched configuration): 1024 independent replicas of FCC Al with
+20 x 20 x 20 cells (32000 sites), one vacancy each, Mg and Zn fractions drawn uniformly in
+[1, 5] at.%, T uniform in [400, 700] K, synthetic cluster-expansion coefficients (E0 barrier
+model), FP64 energies, rates, prefix sums and clock. A bench "step" is one KMC step of every
+replica; the persistent kernel advances all replicas `--chunk` steps per launch.
+
+Prints one JSON line: KMC steps/s summed over all replicas, the roofline of the dominant kernel
+(algorithmic bytes: 677 B per replica-step, measured with HIP events on the launch stream) and
+the CPU baseline (the oracle's KineticMcFirst restatement on this host's cores, bounded sample
+of the single-replica 10^3-cell workload).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ALGO_BYTES_PER_STEP = 677  # 135 unique sites x (1 B species + 4 B index) + 2 B written
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(R, n, seed):
+    from latticemontecarlo_amd import synth
+    rng = np.random.default_rng(seed)
+    species = ["Al", "Mg", "X", "Zn"]
+    N = 4 * n ** 3
+    occs, temps = [], []
+    for r in range(R):
+        xm, xz = rng.uniform(0.01, 0.05, 2)
+        occs.append(synth.random_occupancy(n, species, {"Mg": round(xm * N), "Zn": round(xz * N)},
+                                           seed=seed * 100003 + r))
+        temps.append(rng.uniform(400.0, 700.0))
+    return occs, np.array(temps)
+
+
+def cpu_baseline(coeff_path, budget_s=12.0):
+    """Oracle KineticMcFirst on the host cores: single replica (n=10, 4000 sites, T=500 K)
+    with min(12, cores) OpenMP threads on the 12-event loop, and `cores` independent
+    single-thread replicas; returns the larger aggregate steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from latticemontecarlo_amd import synth
+    cores = min(16, os.cpu_count() or 1)
+    n = 10
+    els = ["Al", "Mg", "Zn"]
+    lat = O.Lattice(n)
+    bm = O.Model(lat, coeff_path, els, O.E0)
+    em = O.Model(lat, coeff_path, els, O.ENERGY)
+    occ = synth.random_occupancy(n, ["Al", "Mg", "X", "Zn"], {"Mg": 80, "Zn": 80}, seed=1)
+    # calibration (also builds every pair table once)
+    secs, _ = O.kmc_baseline(bm, em, lat, occ, 500.0, 7, 1, 1, 100000, 200)
+    per_step = max(secs / 200, 1e-6)
+    steps = int(max(200, budget_s / 2 / per_step))
+    secs1, _ = O.kmc_baseline(bm, em, lat, occ, 500.0, 7, 1, min(12, cores), 100000, steps)
+    single = steps / secs1
+    steps_t = int(max(100, steps / 2))
+    secs2, _ = O.kmc_baseline(bm, em, lat, occ, 500.0, 7, cores, 1, 100000, steps_t)
+    thru = cores * steps_t / secs2
+    log(f"cpu baseline: single replica {single:.1f} steps/s ({min(12, cores)} threads), "
+        f"throughput {thru:.1f} steps/s ({cores} replicas)")
+    best = max(single, thru)
+    return {"value": round(best, 2), "unit": "KMC steps/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle KineticMcFirst (E0, LRU 100000), 10^3 cells x 4000 sites, "
+                       f"T=500 K: 1 replica x {steps} steps with {min(12, cores)} threads "
+                       f"({single:.1f} steps/s) and {cores} replicas x {steps_t} steps "
+                       f"single-threaded ({thru:.1f} steps/s); value = larger")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100000, help="timed KMC steps per replica")
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--chunk", type=int, default=5000, help="KMC steps per kernel launch")
+    ap.add_argument("--replicas", type=int, default=1024)
+    ap.add_argument("--ncell", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); replicas are "
+              "independent, so N GPUs would each run their own share of them.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import synth
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    R, n = args.replicas, args.ncell
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0.json")
+    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "E0", seed=args.seed)
+    log(f"setup: {R} replicas x {4 * n ** 3} sites")
+    occs, temps = make_workload(R, n, args.seed)
+    bm = L.Model(coeff, ["Al", "Mg", "Zn"], L.E0)
+    em = L.Model(coeff, ["Al", "Mg", "Zn"], L.ENERGY)
+    total_steps = args.warmup + args.steps
+    drv = L.KineticMcFirst(bm, em, [n] * R, occs, temps, seed=args.seed,
+                           maximum_steps=total_steps + 10)
+    stream = torch.cuda.current_stream()
+    t0 = time.time()
+    left = args.warmup
+    while left > 0:
+        c = min(args.chunk, left)
+        drv.run(c, stream)
+        left -= c
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps: {time.time() - t0:.2f} s")
+
+    launches = []
+    left = args.steps
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    while left > 0:
+        c = min(args.chunk, left)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        drv.run(c, stream)
+        e1.record(stream)
+        launches.append((c, e0, e1))
+        left -= c
+    end = torch.cuda.Event(enable_timing=True)
+    end.record(stream)
+    for i, (c, e0, e1) in enumerate(launches):
+        e1.synchronize()
+        log(f"launch {i + 1}/{len(launches)}: {c} steps in {e0.elapsed_time(e1):.1f} ms")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    elapsed = start.elapsed_time(end) / 1e3
+    st = drv.state()
+    assert int(st["steps"].min()) == total_steps, "every replica must have run every step"
+    bm.check_errors()
+
+    replica_steps = R * args.steps
+    value = replica_steps / elapsed
+    kern = [(c, e0.elapsed_time(e1) / 1e3) for c, e0, e1 in launches]
+    bytes_per_launch = [ALGO_BYTES_PER_STEP * R * c for c, _ in kern]
+    achieved = sum(bytes_per_launch) / sum(t for _, t in kern) / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "kmc_e0_1024x32000_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "KMC steps/s over all replicas (plain GPU path)",
+        "value": round(value, 1),
+        "unit": "KMC steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain batched KMC, {R} replicas x {n}^3 FCC cells "
+                               f"({4 * n ** 3} sites), Al with 1-5 at.% Mg and Zn each, "
+                               "T 400-700 K, E0 barrier model, one vacancy per replica",
+                   "replicas": R, "sites_per_replica": 4 * n ** 3,
+                   "steps_per_replica": args.steps, "steps_per_launch": args.chunk,
+                   "event_evaluations_per_s": round(12 * value, 1),
+                   "wall_s": round(wall, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": traffic},
+    }
+    log(f"GPU: {value:.4g} KMC steps/s ({12 * value:.4g} event evaluations/s), "
+        f"{elapsed:.3f} s for {replica_steps} replica-steps")
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(coeff)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -55,6 +55,8 @@ void lmc_model_free(lmc_model *m);
 int32_t lmc_model_num_species(const lmc_model *m);
 /* Species code of an element string, -1 if not in the model. */
 int32_t lmc_model_species_code(const lmc_model *m, const char *element);
+/* Element string of a species code (NULL if out of range); valid while m lives. */
+const char *lmc_model_species_name(const lmc_model *m, int32_t code);
 /* Sizes: number of ECIs (Base.theta), MMM and MM2 feature lengths. */
 int lmc_model_layout(const lmc_model *m, int32_t *n_eci, int32_t *n_mmm, int32_t *n_mm2);
 /* ECI key `index`: label (0..7) and its sorted species codes (-1 padded to 3). */
@@ -64,6 +66,14 @@ int lmc_model_eci_key(const lmc_model *m, int32_t index, int32_t *label, int32_t
  * length. Returns the number of orbits in *n_orbits. */
 int lmc_model_orbits(const lmc_model *m, int32_t which, int32_t cap, int32_t *n_orbits,
                      int32_t *size, int32_t *kind, int32_t *shell, int32_t *base, int32_t *len);
+
+/* ---- Random numbers ------------------------------------------------------------------------
+ * The engine's Philox4x32-10 block for (seed, counter, replica, tag): key = seed (low, high
+ * words), counter words = (counter low, counter high, replica, tag). The reference draws from a
+ * clock-seeded mt19937_64 (lmc/mc/src/McAbstract.cpp:32); this replaces it so that every
+ * replica's stream is reproducible and resumable. */
+void lmc_philox4x32(uint64_t seed, uint64_t counter, uint32_t replica, uint32_t tag,
+                    uint32_t *out4);
 
 /* ---- Geometry (host) --------------------------------------------------------------------- */
 int64_t lmc_fcc_num_sites(int32_t n);

@@ -28,10 +28,9 @@ class Model:
         check(lib().lmc_model_load(str(json_path).encode(), arr, len(els), kind, C.byref(h)))
         self.h = h
         self.kind = kind
-        self.element_set = els
         n = lib().lmc_model_num_species(self.h)
-        self.species = sorted(els + ["X"])
-        assert len(self.species) == n
+        self.species = [lib().lmc_model_species_name(self.h, c).decode() for c in range(n)]
+        self.element_set = [e for e in self.species if e != "X"]
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -49,9 +49,11 @@ SIGNATURES = {
     "lmc_model_free": (None, [_vp]),
     "lmc_model_num_species": (_i32, [_vp]),
     "lmc_model_species_code": (_i32, [_vp, C.c_char_p]),
+    "lmc_model_species_name": (C.c_char_p, [_vp, _i32]),
     "lmc_model_layout": (C.c_int, [_vp, _i32p, _i32p, _i32p]),
     "lmc_model_eci_key": (C.c_int, [_vp, _i32, _i32p, _i32p]),
     "lmc_model_orbits": (C.c_int, [_vp, _i32, _i32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
+    "lmc_philox4x32": (None, [_u64, _u64, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
     "lmc_fcc_num_sites": (_i64, [_i32]),
     "lmc_fcc_neighbors": (_i32, [_i32, _i64, _i32, _i64p]),
     "lmc_jump_state_clusters": (_i32, [_i32, _i64, _i64, _i64p, _i32]),

@@ -16,6 +16,7 @@
 
 #include "lmc_device.h"
 #include "lmc_model.h"
+#include "lmc_rng.h"
 #include "lmc_tables.h"
 
 struct lmc_model {
@@ -156,8 +157,16 @@ const KTables &Upload(Model &m) {
   b.Add(site_cl, &k.site_cl);
   b.Add(site_off, &k.site_off);
   b.Add(m.w, &k.w);
-  b.Add(m.b_mmm, &k.b1);
-  b.Add(m.b_mm2, &k.b2);
+  std::vector<double> b1c, b2c;  // rows of the non-vacancy species, in code order
+  for (int c = 0; c < m.S; ++c) {
+    if (c == m.x_code) continue;
+    b1c.insert(b1c.end(), m.b_mmm.begin() + static_cast<size_t>(c) * m.F_mmm,
+               m.b_mmm.begin() + static_cast<size_t>(c + 1) * m.F_mmm);
+    b2c.insert(b2c.end(), m.b_mm2.begin() + static_cast<size_t>(c) * m.F_mm2,
+               m.b_mm2.begin() + static_cast<size_t>(c + 1) * m.F_mm2);
+  }
+  b.Add(b1c, &k.b1);
+  b.Add(b2c, &k.b2);
   b.Add(par, &k.par);
   b.Add(m.type_tab, &k.tt);
   b.Add(m.theta, &k.theta);
@@ -186,6 +195,7 @@ const KTables &Upload(Model &m) {
   k.n_slot = jt.n_slot;
   k.kind = m.kind;
   k.S = m.S;
+  k.ne = m.ne;
   k.x_code = m.x_code;
   k.F1 = m.F_mmm;
   k.F2 = m.F_mm2;
@@ -360,6 +370,10 @@ int32_t lmc_model_species_code(const lmc_model *m, const char *element) {
     if (m->m->eci.species[i] == element) return i;
   return -1;
 }
+const char *lmc_model_species_name(const lmc_model *m, int32_t code) {
+  if (!m || !m->m || code < 0 || code >= m->m->S) return nullptr;
+  return m->m->eci.species[code].c_str();
+}
 int lmc_model_layout(const lmc_model *m, int32_t *n_eci, int32_t *n_mmm, int32_t *n_mm2) {
   return Guard([&] {
     RequireModel(m);
@@ -401,6 +415,15 @@ int lmc_model_orbits(const lmc_model *m, int32_t which, int32_t cap, int32_t *n_
     }
     return LMC_OK;
   });
+}
+
+void lmc_philox4x32(uint64_t seed, uint64_t counter, uint32_t replica, uint32_t tag,
+                    uint32_t *out4) {
+  const lmc::U4 u = lmc::Philox4x32_10(seed, counter, replica, tag);
+  out4[0] = u.x;
+  out4[1] = u.y;
+  out4[2] = u.z;
+  out4[3] = u.w;
 }
 
 int64_t lmc_fcc_num_sites(int32_t n) { return n >= 4 ? 4ll * n * n * n : -1; }

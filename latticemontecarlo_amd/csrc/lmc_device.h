@@ -28,8 +28,8 @@ struct KTables {
   const uint32_t *site_cl;
   const int32_t *site_off;  // [43]
   const double *w;          // theta/N by (label, codes)
-  const double *b1;         // [S][F1]
-  const double *b2;         // [S][F2]
+  const double *b1;         // [ne][F1], row = one-hot rank of the migrating species
+  const double *b2;         // [ne][F2]
   const double *par;        // [S][6]
   const int8_t *tt;         // [3][S][S]
   const double *theta;      // [n_eci]
@@ -40,7 +40,7 @@ struct KTables {
   uint64_t vac_shell_mask;  // env slots within 3NN of the vacancy
   int n_state, n_mmm, n_mm2, n_site, n_eci;
   int v_slot, n_slot;
-  int kind, S, x_code, F1, F2, nw;
+  int kind, S, ne, x_code, F1, F2, nw;
 };
 
 struct DeviceTables {
