@@ -112,6 +112,9 @@ def main():
     drv = L.KineticMcFirst(bm, em, [n] * R, occs, temps, seed=args.seed,
                            maximum_steps=total_steps + 10)
     stream = torch.cuda.current_stream()
+    tune = drv.autotune(repeats=3)
+    log(f"autotune: dense {tune['ms_dense']:.3f} ms, solvent baseline {tune['ms_solvent']:.3f} ms "
+        f"per jump evaluation of the batch -> formulation {tune['formulation']}")
     t0 = time.time()
     left = args.warmup
     while left > 0:
@@ -176,6 +179,7 @@ def main():
                                "T 400-700 K, E0 barrier model, one vacancy per replica",
                    "replicas": R, "sites_per_replica": 4 * n ** 3,
                    "steps_per_replica": args.steps, "steps_per_launch": args.chunk,
+                   "formulation": ["dense", "solvent-baseline"][tune["formulation"]],
                    "event_evaluations_per_s": round(12 * value, 1),
                    "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,

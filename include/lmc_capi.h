@@ -55,6 +55,11 @@ void lmc_model_free(lmc_model *m);
 int32_t lmc_model_num_species(const lmc_model *m);
 /* Species code of an element string, -1 if not in the model. */
 int32_t lmc_model_species_code(const lmc_model *m, const char *element);
+/* Formulation of the jump kernels used by this model's stateless calls and as the default of
+ * new KMC drivers: 0 = dense (every cluster of every jump), 1 = solvent baseline (all-solvent
+ * sums precomputed per (solvent, migrating species); only clusters touching a non-solvent site
+ * are evaluated; default). Both are exact; they differ in FP64 summation order. */
+int lmc_model_set_formulation(lmc_model *m, int32_t formulation);
 /* Element string of a species code (NULL if out of range); valid while m lives. */
 const char *lmc_model_species_name(const lmc_model *m, int32_t code);
 /* Sizes: number of ECIs (Base.theta), MMM and MM2 feature lengths. */
@@ -148,6 +153,12 @@ void lmc_kmc_free(lmc_kmc *k);
 int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const double *temp);
 /* Per-replica trace of the next `capacity` steps (selected neighbour, Ea, dE, time). 0 = off. */
 int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity);
+/* Formulation of this driver's kernel (see lmc_model_set_formulation). */
+int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation);
+/* Autotuner: times both formulations of the jump kernel on the batch's current configurations
+ * (`repeats` launches each, HIP events) and keeps the faster; outputs may be NULL. */
+int lmc_kmc_autotune(lmc_kmc *k, int32_t repeats, int32_t *chosen, double *ms_dense,
+                     double *ms_solvent);
 /* Advance every replica by `steps` steps (fewer if it reaches maximum_steps or escapes).
  * Asynchronous on `stream`. */
 int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream);

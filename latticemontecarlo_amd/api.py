@@ -68,6 +68,10 @@ class Model:
     def check_errors(self):
         check(lib().lmc_check_errors(self.h))
 
+    def set_formulation(self, formulation):
+        """0 = dense, 1 = solvent baseline (default)."""
+        check(lib().lmc_model_set_formulation(self.h, formulation))
+
 
 def fcc_num_sites(n):
     return int(lib().lmc_fcc_num_sites(n))
@@ -200,6 +204,14 @@ class KineticMcFirst:
     def set_trace(self, capacity):
         check(lib().lmc_kmc_set_trace(self.h, capacity))
         self.trace_cap = capacity
+
+    def set_formulation(self, formulation):
+        check(lib().lmc_kmc_set_formulation(self.h, formulation))
+
+    def autotune(self, repeats=3):
+        ch, md, ms = C.c_int32(), C.c_double(), C.c_double()
+        check(lib().lmc_kmc_autotune(self.h, repeats, C.byref(ch), C.byref(md), C.byref(ms)))
+        return dict(formulation=ch.value, ms_dense=md.value, ms_solvent=ms.value)
 
     def run(self, steps, stream=None):
         s = capi.stream_ptr(stream) if stream is not None else None

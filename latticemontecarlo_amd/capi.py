@@ -50,6 +50,7 @@ SIGNATURES = {
     "lmc_model_num_species": (_i32, [_vp]),
     "lmc_model_species_code": (_i32, [_vp, C.c_char_p]),
     "lmc_model_species_name": (C.c_char_p, [_vp, _i32]),
+    "lmc_model_set_formulation": (C.c_int, [_vp, _i32]),
     "lmc_model_layout": (C.c_int, [_vp, _i32p, _i32p, _i32p]),
     "lmc_model_eci_key": (C.c_int, [_vp, _i32, _i32p, _i32p]),
     "lmc_model_orbits": (C.c_int, [_vp, _i32, _i32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
@@ -69,6 +70,8 @@ SIGNATURES = {
     "lmc_kmc_free": (None, [_vp]),
     "lmc_kmc_set_schedule": (C.c_int, [_vp, _i32, _f64p, _f64p]),
     "lmc_kmc_set_trace": (C.c_int, [_vp, _u64]),
+    "lmc_kmc_set_formulation": (C.c_int, [_vp, _i32]),
+    "lmc_kmc_autotune": (C.c_int, [_vp, _i32, _i32p, _f64p, _f64p]),
     "lmc_kmc_run": (C.c_int, [_vp, _u64, _vp]),
     "lmc_kmc_state": (C.c_int, [_vp, _u64p, _f64p, _f64p, _f64p, _f64p, _i64p, _i64p, _f64p,
                                 _i32p]),

@@ -112,6 +112,95 @@ Model::~Model() {
   }
 }
 
+// Solvent-baseline tables. With every non-endpoint site of a jump environment holding the
+// solvent s, the cluster sums of dE and of the barrier logits depend only on (s, M); a real
+// environment adds, for each cluster with a non-solvent site, value(cluster) - value(all s).
+// A cluster with several non-solvent sites is listed under each of them and evaluated only at
+// the lowest-numbered one (the kernel checks the entry's other slot).
+static void BuildSolventTables(const Model &m, const JumpTables &jt, std::vector<uint16_t> &off,
+                               std::vector<uint32_t> &inc, std::vector<double> &base) {
+  const int S = m.S;
+  std::vector<std::vector<uint32_t>> per(kEnvSlots);
+  auto state_entry = [](int label, bool nside, int other) {
+    return static_cast<uint32_t>(0 | (label << 2) | (nside ? 32 : 0) | (other << 6));
+  };
+  auto add_state = [&](const std::vector<StateCluster> &cl, int endpoint, bool nside) {
+    for (const auto &c : cl) {
+      int others[2] = {-1, -1}, n = 0;
+      for (int i = 0; i < c.size; ++i)
+        if (c.slots[i] != endpoint) others[n++] = c.slots[i];
+      for (int i = 0; i < n; ++i) {
+        const int other = n == 2 ? others[1 - i] : 63;
+        per[others[i]].push_back(state_entry(c.label, nside, other));
+      }
+    }
+  };
+  add_state(jt.state_v_only, jt.v_slot, false);
+  add_state(jt.state_n_only, jt.n_slot, true);
+  auto add_orbits = [&](const OrbitTable &ot, const std::vector<int> &bases, int type,
+                        bool backward) {
+    for (const auto &c : ot.clusters) {
+      const int kind = ot.orbit_kind[c.orbit];
+      const uint32_t b = static_cast<uint32_t>(bases[c.orbit]);
+      const int nsl = c.s1 < 0 ? 1 : 2;
+      for (int pos = 0; pos < nsl; ++pos) {
+        const int canon = pos == 0 ? c.s0 : c.s1;
+        const int canon_other = nsl == 1 ? -1 : (pos == 0 ? c.s1 : c.s0);
+        // backward: the species of canonical slot x is read from real slot mirror[x]
+        const int slot = backward ? jt.mirror[canon] : canon;
+        const int other = canon_other < 0 ? 63 : (backward ? jt.mirror[canon_other] : canon_other);
+        per[slot].push_back(static_cast<uint32_t>(type | (kind << 2) | (pos << 5) | (other << 6)) |
+                            (b << 12));
+      }
+    }
+  };
+  if (m.kind == kE0 || m.kind == kQuartic) add_orbits(jt.mmm, m.base_mmm, 1, false);
+  if (m.kind == kQuartic) {
+    add_orbits(jt.mm2, m.base_mm2, 2, false);
+    add_orbits(jt.mm2, m.base_mm2, 2, true);
+  }
+  off.assign(kEnvSlots + 1, 0);
+  for (int j = 0; j < kEnvSlots; ++j) {
+    off[j] = static_cast<uint16_t>(inc.size());
+    inc.insert(inc.end(), per[j].begin(), per[j].end());
+  }
+  off[kEnvSlots] = static_cast<uint16_t>(inc.size());
+  if (inc.size() > 65535) throw std::invalid_argument("incidence table too large");
+
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  base.assign(static_cast<size_t>(S) * S * 3, nan);
+  auto w_of = [&](int label, int a, int s) -> long double {
+    if (label == 0) return m.w[a];
+    if (label < 4) return m.w[S + (label - 1) * S * S + a * S + s];
+    return m.w[S + 3 * S * S + (label - 4) * S * S * S + (a * S + s) * S + s];
+  };
+  auto tt = [&](int kind, int a, int b) { return m.type_tab[(kind * S + a) * S + b]; };
+  for (int s = 0; s < S; ++s) {
+    if (s == m.x_code) continue;
+    for (int M = 0; M < S; ++M) {
+      if (M == m.x_code) continue;
+      long double de = 0, la = 0, lb = 0;
+      for (const auto &c : jt.state_v_only) de += w_of(c.label, M, s) - w_of(c.label, m.x_code, s);
+      for (const auto &c : jt.state_n_only) de -= w_of(c.label, M, s) - w_of(c.label, m.x_code, s);
+      if (m.kind == kE0 || m.kind == kQuartic)
+        for (const auto &c : jt.mmm.clusters) {
+          const int kind = jt.mmm.orbit_kind[c.orbit];
+          la += m.b_mmm[static_cast<size_t>(M) * m.F_mmm + m.base_mmm[c.orbit] + tt(kind, s, s)];
+        }
+      if (m.kind == kQuartic)
+        for (const auto &c : jt.mm2.clusters) {
+          const int kind = jt.mm2.orbit_kind[c.orbit];
+          lb += 2.0L * m.b_mm2[static_cast<size_t>(M) * m.F_mm2 + m.base_mm2[c.orbit] +
+                               tt(kind, s, s)];
+        }
+      double *o = &base[(static_cast<size_t>(s) * S + M) * 3];
+      o[0] = static_cast<double>(de);
+      o[1] = static_cast<double>(la);
+      o[2] = static_cast<double>(lb);
+    }
+  }
+}
+
 const KTables &Upload(Model &m) {
   const int device = CurrentDevice();
   if (m.dev) {
@@ -174,6 +263,13 @@ const KTables &Upload(Model &m) {
   b.Add(e1, &k.eci1);
   b.Add(e2, &k.eci2);
   b.Add(e3, &k.eci3);
+  std::vector<uint16_t> inc_off;
+  std::vector<uint32_t> inc;
+  std::vector<double> base;
+  BuildSolventTables(m, jt, inc_off, inc, base);
+  b.Add(inc_off, &k.inc_off);
+  b.Add(inc, &k.inc);
+  b.Add(base, &k.base);
   std::vector<int32_t> errflag(1, 0);
   const int32_t *err_ptr = nullptr;
   b.Add(errflag, &err_ptr);
@@ -200,6 +296,11 @@ const KTables &Upload(Model &m) {
   k.F1 = m.F_mmm;
   k.F2 = m.F_mm2;
   k.nw = static_cast<int>(m.w.size());
+  k.n_inc = static_cast<int>(inc.size());
+  k.solvent_default = 0;
+  for (int c = 0; c < m.S; ++c)
+    if (m.eci.species[c] == "Al") k.solvent_default = c;
+  if (k.solvent_default == m.x_code) k.solvent_default = m.x_code == 0 ? 1 : 0;
   dev->k = k;
   m.dev = dev.release();
   m.dev_err = const_cast<int32_t *>(err_ptr);
@@ -282,6 +383,7 @@ struct lmc_kmc {
   const lmc_model *bm = nullptr;
   lmc::KmcDev d;
   uint64_t trace_pos = 0;
+  int formulation = 1;
   ~lmc_kmc() {
     DevFree(d.occ);
     DevFree(d.off);
@@ -364,6 +466,11 @@ void lmc_model_free(lmc_model *m) {
   }
 }
 int32_t lmc_model_num_species(const lmc_model *m) { return m && m->m ? m->m->S : -1; }
+int lmc_model_set_formulation(lmc_model *m, int32_t formulation) {
+  if (!m || !m->m || formulation < 0 || formulation > 1) return Fail(LMC_ERR_ARG, "bad formulation");
+  m->m->formulation = formulation;
+  return LMC_OK;
+}
 int32_t lmc_model_species_code(const lmc_model *m, const char *element) {
   if (!m || !m->m || !element) return -1;
   for (int i = 0; i < m->m->S; ++i)
@@ -491,7 +598,7 @@ int lmc_eval_jumps(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_of
     if (n_replicas < 0) throw std::invalid_argument("n_replicas < 0");
     const auto &t = lmc::Upload(*m->m);
     Check(lmc::LaunchEvalJumps(t, d_occ, d_offset, d_ncell, d_vac, n_replicas, d_nb, d_ea, d_de,
-                               m->m->dev_err, S(stream)),
+                               m->m->dev_err, S(stream), m->m->formulation == 1),
           "eval jumps kernel");
     return LMC_OK;
   });
@@ -529,7 +636,7 @@ int lmc_eval_pair(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64
       Check(hipMemcpy(nc, &ncell, sizeof(ncell), hipMemcpyHostToDevice), "hipMemcpy");
       const auto &t = lmc::Upload(*m->m);
       Check(lmc::LaunchEvalJumps(t, d_occ, buf, nc, buf + 1, 1, buf + 2, fb, fb + 12,
-                                 m->m->dev_err, nullptr),
+                                 m->m->dev_err, nullptr, m->m->formulation == 1),
             "eval jumps kernel");
       int64_t nbs[12];
       double r[24];
@@ -678,12 +785,67 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
       RequireModel(energy);
       ComputeTotalEnergy(energy, d.occ, d.off, d.ncell, R, d.abs_energy, nullptr);
     }
+    k->formulation = barrier->m->formulation;
     *out = k.release();
     return LMC_OK;
   });
 }
 
 void lmc_kmc_free(lmc_kmc *k) { delete k; }
+
+int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation) {
+  if (!k || formulation < 0 || formulation > 1) return Fail(LMC_ERR_ARG, "bad formulation");
+  k->formulation = formulation;
+  return LMC_OK;
+}
+
+int lmc_kmc_autotune(lmc_kmc *k, int32_t repeats, int32_t *chosen, double *ms_dense,
+                     double *ms_solvent) {
+  return Guard([&] {
+    if (!k || repeats <= 0) throw std::invalid_argument("bad argument");
+    const auto &t = lmc::Upload(*k->bm->m);
+    const int32_t R = k->d.R;
+    int64_t *nb = DevAlloc<int64_t>(12 * static_cast<size_t>(R));
+    double *ea = DevAlloc<double>(24 * static_cast<size_t>(R));
+    hipEvent_t e0, e1;
+    Check(hipEventCreate(&e0), "hipEventCreate");
+    Check(hipEventCreate(&e1), "hipEventCreate");
+    float ms[2] = {0, 0};
+    try {
+      Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      for (int f = 0; f < 2; ++f) {
+        // one untimed launch per formulation, then `repeats` timed launches on the batch's
+        // current configurations (the stateless kernels do not change the state)
+        Check(lmc::LaunchEvalJumps(t, k->d.occ, k->d.off, k->d.ncell, k->d.vac, R, nb, ea,
+                                   ea + 12 * R, k->d.error, nullptr, f == 1),
+              "autotune launch");
+        Check(hipEventRecord(e0, nullptr), "hipEventRecord");
+        for (int i = 0; i < repeats; ++i)
+          Check(lmc::LaunchEvalJumps(t, k->d.occ, k->d.off, k->d.ncell, k->d.vac, R, nb, ea,
+                                     ea + 12 * R, k->d.error, nullptr, f == 1),
+                "autotune launch");
+        Check(hipEventRecord(e1, nullptr), "hipEventRecord");
+        Check(hipEventSynchronize(e1), "hipEventSynchronize");
+        Check(hipEventElapsedTime(&ms[f], e0, e1), "hipEventElapsedTime");
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipFree(nb);
+      (void)hipFree(ea);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(nb);
+    (void)hipFree(ea);
+    k->formulation = ms[1] <= ms[0] ? 1 : 0;
+    if (chosen) *chosen = k->formulation;
+    if (ms_dense) *ms_dense = ms[0] / repeats;
+    if (ms_solvent) *ms_solvent = ms[1] / repeats;
+    return LMC_OK;
+  });
+}
 
 int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const double *temp) {
   return Guard([&] {
@@ -735,7 +897,8 @@ int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream) {
   return Guard([&] {
     if (!k) throw std::invalid_argument("null driver");
     const auto &t = lmc::Upload(*k->bm->m);
-    Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream)), "KMC kernel");
+    Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream), k->formulation == 1),
+          "KMC kernel");
     k->trace_pos += steps;
     return LMC_OK;
   });

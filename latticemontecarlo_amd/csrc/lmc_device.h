@@ -37,6 +37,15 @@ struct KTables {
   const int32_t *eci1;      // [S]
   const int32_t *eci2;      // [3][S][S]
   const int32_t *eci3;      // [4][S][S][S]
+  // Solvent-baseline formulation: per env slot j, the clusters that change when slot j holds a
+  // non-solvent species (CSR: inc_off[61] -> inc entries), and for every (solvent s, migrating
+  // species M) the sums (dE, MMM logit, MM2 logit) of an environment that is all solvent.
+  //   entry: type[0:2] (0 state, 1 MMM, 2 MM2) | label or kind[2:5] | n-side or position[5]
+  //          | other slot[6:12] (63 = none) | feature base[12:24]
+  const uint16_t *inc_off;  // [61]
+  const uint32_t *inc;
+  const double *base;       // [S][S][3]
+  int n_inc, solvent_default;
   uint64_t vac_shell_mask;  // env slots within 3NN of the vacancy
   int n_state, n_mmm, n_mm2, n_site, n_eci;
   int v_slot, n_slot;
@@ -98,9 +107,9 @@ struct CmcDev {
 
 hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *off,
                            const int32_t *ncell, const int64_t *vac, int32_t R, int64_t *nb,
-                           double *ea, double *de, int32_t *err, hipStream_t s);
+                           double *ea, double *de, int32_t *err, hipStream_t s, bool solvent);
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
-                     hipStream_t s);
+                     hipStream_t s, bool solvent);
 hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off,
                         const int32_t *ncell, const int64_t *pairs, int32_t R, double *de,
                         hipStream_t s);

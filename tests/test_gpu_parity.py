@@ -26,14 +26,18 @@ def _oracle_models(path, els, n, kinds):
     return lat, [O.Model(lat, path, els, k) for k in kinds]
 
 
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("name,kind", [("e0_3", 1), ("e0_3n", 1), ("e0_4n", 1), ("q_3n", 2),
                                        ("q_4", 2)])
 @pytest.mark.parametrize("n", [5, 8])
-def test_eval_jumps_matches_oracle(torch, coeffs, name, kind, n):
+def test_eval_jumps_matches_oracle(torch, coeffs, name, kind, n, form):
+    """Both jump formulations (0 dense, 1 solvent baseline) against the oracle, solute
+    fractions 8-28 % per element, vacancies on box corners and edges."""
     import latticemontecarlo_amd as L
     path, els = coeffs[name]
     lat, (om,) = _oracle_models(path, els, n, [kind])
     gm = L.Model(path, els, kind)
+    gm.set_formulation(form)
     species = gm.species
     N = 4 * n ** 3
     occs, vacs = [], []
@@ -106,7 +110,7 @@ def test_total_energy_matches_oracle(torch, coeffs, name):
         assert abs(e[i] - om.total_energy(occs[i])) <= TOL_E * max(1.0, abs(e[i]))
 
 
-def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, **opts):
+def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, **opts):
     import latticemontecarlo_amd as L
     import oracle as O
     path, els = coeffs[name]
@@ -117,6 +121,7 @@ def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, **opts):
     seed = 987654321
     tt = opts.pop("tt", None)
     drv = L.KineticMcFirst(gb, ge, [n] * R, occs, temps, seed, maximum_steps=10 ** 9, **opts)
+    drv.set_formulation(form)
     if tt is not None:
         drv.set_schedule(*tt)
     drv.set_trace(steps)
@@ -151,12 +156,32 @@ def _compare_kmc(gst, gtr, gocc, refs, steps):
         np.testing.assert_allclose(gst["time"][r], st["time"], rtol=1e-12)
 
 
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("name,kind", [("e0_3n", 1), ("q_3n", 2)])
-def test_kmc_trajectory_matches_oracle(torch, coeffs, name, kind):
+def test_kmc_trajectory_matches_oracle(torch, coeffs, name, kind, form):
     steps = 400
     gst, gtr, gocc, refs = _run_kmc_pair(coeffs, name, kind, n=6, R=4, steps=steps,
-                                         temps=np.array([450.0, 500.0, 600.0, 700.0]))
+                                         temps=np.array([450.0, 500.0, 600.0, 700.0]), form=form)
     _compare_kmc(gst, gtr, gocc, refs, steps)
+
+
+def test_kmc_autotune_keeps_results(torch, coeffs):
+    """The autotuner times both formulations without changing the replicas' state."""
+    import latticemontecarlo_amd as L
+    path, els = coeffs["e0_3"]
+    gb = L.Model(path, els, L.E0)
+    n, R = 6, 8
+    occs = [random_alloc(n, gb.species, 0.03, seed=70 + r) for r in range(R)]
+    a = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
+    b = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
+    info = a.autotune(repeats=2)
+    assert info["formulation"] in (0, 1) and info["ms_dense"] > 0 and info["ms_solvent"] > 0
+    a.run(200)
+    b.run(200)
+    sa, sb = a.state(), b.state()
+    assert np.array_equal(sa["vacancy"], sb["vacancy"])
+    assert np.array_equal(sa["steps"], sb["steps"])
+    np.testing.assert_allclose(sa["time"], sb["time"], rtol=1e-12)
 
 
 def test_kmc_options_match_oracle(torch, coeffs):
