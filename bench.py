@@ -112,9 +112,9 @@ def main():
     drv = L.KineticMcFirst(bm, em, [n] * R, occs, temps, seed=args.seed,
                            maximum_steps=total_steps + 10)
     stream = torch.cuda.current_stream()
-    tune = drv.autotune(repeats=3)
+    tune = drv.autotune(steps=300)
     log(f"autotune: dense {tune['ms_dense']:.3f} ms, solvent baseline {tune['ms_solvent']:.3f} ms "
-        f"per jump evaluation of the batch -> formulation {tune['formulation']}")
+        f"per KMC step of the batch -> formulation {tune['formulation']}")
     t0 = time.time()
     left = args.warmup
     while left > 0:

@@ -155,9 +155,9 @@ int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const
 int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity);
 /* Formulation of this driver's kernel (see lmc_model_set_formulation). */
 int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation);
-/* Autotuner: times both formulations of the jump kernel on the batch's current configurations
- * (`repeats` launches each, HIP events) and keeps the faster; outputs may be NULL. */
-int lmc_kmc_autotune(lmc_kmc *k, int32_t repeats, int32_t *chosen, double *ms_dense,
+/* Autotuner: runs `steps` KMC steps with each formulation on a scratch copy of the batch state
+ * (HIP events), keeps the faster and leaves the replicas untouched; ms_* are per step. */
+int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dense,
                      double *ms_solvent);
 /* Advance every replica by `steps` steps (fewer if it reaches maximum_steps or escapes).
  * Asynchronous on `stream`. */

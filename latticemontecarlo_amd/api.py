@@ -208,9 +208,9 @@ class KineticMcFirst:
     def set_formulation(self, formulation):
         check(lib().lmc_kmc_set_formulation(self.h, formulation))
 
-    def autotune(self, repeats=3):
+    def autotune(self, steps=200):
         ch, md, ms = C.c_int32(), C.c_double(), C.c_double()
-        check(lib().lmc_kmc_autotune(self.h, repeats, C.byref(ch), C.byref(md), C.byref(ms)))
+        check(lib().lmc_kmc_autotune(self.h, steps, C.byref(ch), C.byref(md), C.byref(ms)))
         return dict(formulation=ch.value, ms_dense=md.value, ms_solvent=ms.value)
 
     def run(self, steps, stream=None):

@@ -12,6 +12,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lmc_device.h"
@@ -63,22 +64,44 @@ int32_t PackOff(const lmc::I3 &o) {
   return (o.x & 0xff) | ((o.y & 0xff) << 8) | ((o.z & 0xff) << 16);
 }
 
-uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side) {
+// Site-table entry: label | other1 << 4 | other2 << 10 (63 = none).
+uint32_t PackSite(const lmc::StateCluster &c, int centre) {
+  int other[2] = {63, 63}, k = 0;
+  for (int i = 0; i < c.size; ++i)
+    if (c.slots[i] != centre) other[k++] = c.slots[i];
+  return static_cast<uint32_t>(c.label) | (static_cast<uint32_t>(other[0]) << 4) |
+         (static_cast<uint32_t>(other[1]) << 10);
+}
+
+// Jump state-cluster entry: byte offset of the cluster's weight block [0:13] | other1 [13:19] |
+// other2 [19:25] | n-side [31]. Absent sites point at environment slot 63, which always holds
+// species 0, so singlets and pairs index their blocks like triplets.
+uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side, int S) {
   int other[2] = {63, 63}, k = 0;
   for (int i = 0; i < c.size; ++i)
     if (c.slots[i] != endpoint_slot) other[k++] = c.slots[i];
-  return static_cast<uint32_t>(c.label) | (n_side ? 8u : 0u) |
-         (static_cast<uint32_t>(other[0]) << 4) | (static_cast<uint32_t>(other[1]) << 10);
+  const int l = c.label;
+  const int block = l == 0 ? 0 : l < 4 ? S + (l - 1) * S * S : S + 3 * S * S + (l - 4) * S * S * S;
+  return static_cast<uint32_t>(block * 8) | (static_cast<uint32_t>(other[0]) << 13) |
+         (static_cast<uint32_t>(other[1]) << 19) | (n_side ? 0x80000000u : 0u);
 }
 
-std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<int> &base) {
+// Orbit-cluster entry: feature base [0:12] | kind * S * S [12:20] | slot0 [20:26] | slot1 [26:32]
+// (63 = none). Backward entries read the species of canonical slot x from slot mirror[x].
+std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<int> &base, int S,
+                                 const std::vector<int> *mirror) {
   std::vector<uint32_t> v;
   for (const auto &c : ot.clusters) {
     const int kind = ot.orbit_kind[c.orbit];
     const uint32_t b = static_cast<uint32_t>(base[c.orbit]);
     if (b > 4095) throw std::invalid_argument("feature vector too long for the packed tables");
-    v.push_back(b | (static_cast<uint32_t>(kind) << 12) | (static_cast<uint32_t>(c.s0) << 14) |
-                (static_cast<uint32_t>(c.s1 < 0 ? 63 : c.s1) << 20));
+    int s0 = c.s0, s1 = c.s1;
+    if (mirror) {
+      s0 = (*mirror)[s0];
+      s1 = s1 < 0 ? -1 : (*mirror)[s1];
+    }
+    v.push_back(b | (static_cast<uint32_t>(kind * S * S) << 12) |
+                (static_cast<uint32_t>(s0) << 20) | (static_cast<uint32_t>(s1 < 0 ? 63 : s1) << 26));
   }
   return v;
 }
@@ -216,12 +239,34 @@ const KTables &Upload(Model &m) {
     for (int s = 0; s < kEnvSlots; ++s) off.push_back(PackOff(jt.off[d][s]));
   std::vector<uint8_t> mirror(jt.mirror.begin(), jt.mirror.end());
   std::vector<uint32_t> state;
-  for (const auto &c : jt.state_v_only) state.push_back(PackState(c, jt.v_slot, false));
-  for (const auto &c : jt.state_n_only) state.push_back(PackState(c, jt.n_slot, true));
-  const auto mmm = PackOrbits(jt.mmm, m.base_mmm);
-  const auto mm2 = PackOrbits(jt.mm2, m.base_mm2);
+  for (const auto &c : jt.state_v_only) state.push_back(PackState(c, jt.v_slot, false, m.S));
+  for (const auto &c : jt.state_n_only) state.push_back(PackState(c, jt.n_slot, true, m.S));
+  const auto mmm = PackOrbits(jt.mmm, m.base_mmm, m.S, nullptr);
+  // MM2 clusters: forward entries, then the backward (mirrored) ones
+  auto mm2 = PackOrbits(jt.mm2, m.base_mm2, m.S, nullptr);
+  const std::vector<int> mir(jt.mirror.begin(), jt.mirror.end());
+  const auto mm2b = PackOrbits(jt.mm2, m.base_mm2, m.S, &mir);
+  mm2.insert(mm2.end(), mm2b.begin(), mm2b.end());
   std::vector<uint32_t> site_cl;
-  for (const auto &c : st.clusters) site_cl.push_back(PackState(c, 0, false));
+  for (const auto &c : st.clusters) site_cl.push_back(PackSite(c, 0));
+  // Jump dE weights by block and (a + S c1 + S^2 c2), migrating species a fastest.
+  const int S = m.S;
+  std::vector<double> wd(m.w.size());
+  for (int a = 0; a < S; ++a) {
+    wd[a] = m.w[a];
+    for (int c1 = 0; c1 < S; ++c1) {
+      for (int l = 0; l < 3; ++l) wd[S + l * S * S + a + S * c1] = m.w[S + l * S * S + a * S + c1];
+      for (int c2 = 0; c2 < S; ++c2)
+        for (int l = 0; l < 4; ++l)
+          wd[S + 3 * S * S + l * S * S * S + a + S * c1 + S * S * c2] =
+              m.w[S + 3 * S * S + l * S * S * S + (a * S + c1) * S + c2];
+    }
+  }
+  // A vacancy off the jump endpoints is flagged when the environment is gathered; its one-hot
+  // type maps to entry 0 so that every table index stays in range.
+  std::vector<int8_t> tt(m.type_tab.begin(), m.type_tab.end());
+  for (auto &x : tt)
+    if (x < 0) x = 0;
   std::vector<int32_t> site_off;
   for (int s = 0; s < kSiteSlots; ++s) site_off.push_back(PackOff(st.off[s]));
   std::vector<double> par = m.par;
@@ -246,6 +291,7 @@ const KTables &Upload(Model &m) {
   b.Add(site_cl, &k.site_cl);
   b.Add(site_off, &k.site_off);
   b.Add(m.w, &k.w);
+  b.Add(wd, &k.wd);
   std::vector<double> b1c, b2c;  // rows of the non-vacancy species, in code order
   for (int c = 0; c < m.S; ++c) {
     if (c == m.x_code) continue;
@@ -257,7 +303,7 @@ const KTables &Upload(Model &m) {
   b.Add(b1c, &k.b1);
   b.Add(b2c, &k.b2);
   b.Add(par, &k.par);
-  b.Add(m.type_tab, &k.tt);
+  b.Add(tt, &k.tt);
   b.Add(m.theta, &k.theta);
   b.Add(m.norm_total, &k.norm_tot);
   b.Add(e1, &k.eci1);
@@ -383,6 +429,7 @@ struct lmc_kmc {
   const lmc_model *bm = nullptr;
   lmc::KmcDev d;
   uint64_t trace_pos = 0;
+  int64_t total_sites = 0;
   int formulation = 1;
   ~lmc_kmc() {
     DevFree(d.occ);
@@ -786,6 +833,7 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
       ComputeTotalEnergy(energy, d.occ, d.off, d.ncell, R, d.abs_energy, nullptr);
     }
     k->formulation = barrier->m->formulation;
+    k->total_sites = b.total;
     *out = k.release();
     return LMC_OK;
   });
@@ -799,50 +847,63 @@ int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation) {
   return LMC_OK;
 }
 
-int lmc_kmc_autotune(lmc_kmc *k, int32_t repeats, int32_t *chosen, double *ms_dense,
+int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dense,
                      double *ms_solvent) {
   return Guard([&] {
-    if (!k || repeats <= 0) throw std::invalid_argument("bad argument");
+    if (!k || steps <= 0) throw std::invalid_argument("bad argument");
     const auto &t = lmc::Upload(*k->bm->m);
-    const int32_t R = k->d.R;
-    int64_t *nb = DevAlloc<int64_t>(12 * static_cast<size_t>(R));
-    double *ea = DevAlloc<double>(24 * static_cast<size_t>(R));
-    hipEvent_t e0, e1;
-    Check(hipEventCreate(&e0), "hipEventCreate");
-    Check(hipEventCreate(&e1), "hipEventCreate");
+    const size_t R = static_cast<size_t>(k->d.R);
+    // Scratch copy of every array the KMC kernel writes; read-only arrays are shared.
+    lmc::KmcDev s = k->d;
+    s.trace_cap = 0;
+    s.occ = nullptr;
+    std::vector<void *> owned;
+    auto clone = [&](auto *&dst, auto *src, size_t n) {
+      using T = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
+      dst = DevAlloc<T>(n);
+      owned.push_back(dst);
+      Check(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     float ms[2] = {0, 0};
     try {
       Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      Check(hipEventCreate(&e0), "hipEventCreate");
+      Check(hipEventCreate(&e1), "hipEventCreate");
       for (int f = 0; f < 2; ++f) {
-        // one untimed launch per formulation, then `repeats` timed launches on the batch's
-        // current configurations (the stateless kernels do not change the state)
-        Check(lmc::LaunchEvalJumps(t, k->d.occ, k->d.off, k->d.ncell, k->d.vac, R, nb, ea,
-                                   ea + 12 * R, k->d.error, nullptr, f == 1),
-              "autotune launch");
+        for (void *p : owned) (void)hipFree(p);
+        owned.clear();
+        clone(s.occ, k->d.occ, static_cast<size_t>(k->total_sites));
+        clone(s.vac, k->d.vac, R);
+        clone(s.steps, k->d.steps, R);
+        clone(s.time, k->d.time, R);
+        clone(s.energy, k->d.energy, R);
+        clone(s.abs_energy, k->d.abs_energy, R);
+        clone(s.temp, k->d.temp, R);
+        clone(s.escaped, k->d.escaped, R);
+        clone(s.disp, k->d.disp, 3 * R);
+        clone(s.com, k->d.com, 3 * R);
+        clone(s.error, k->d.error, 1);
         Check(hipEventRecord(e0, nullptr), "hipEventRecord");
-        for (int i = 0; i < repeats; ++i)
-          Check(lmc::LaunchEvalJumps(t, k->d.occ, k->d.off, k->d.ncell, k->d.vac, R, nb, ea,
-                                     ea + 12 * R, k->d.error, nullptr, f == 1),
-                "autotune launch");
+        Check(lmc::LaunchKmc(t, s, static_cast<uint64_t>(steps), 0, nullptr, f == 1),
+              "autotune KMC launch");
         Check(hipEventRecord(e1, nullptr), "hipEventRecord");
         Check(hipEventSynchronize(e1), "hipEventSynchronize");
         Check(hipEventElapsedTime(&ms[f], e0, e1), "hipEventElapsedTime");
       }
     } catch (...) {
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      (void)hipFree(nb);
-      (void)hipFree(ea);
+      for (void *p : owned) (void)hipFree(p);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
       throw;
     }
+    for (void *p : owned) (void)hipFree(p);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(nb);
-    (void)hipFree(ea);
     k->formulation = ms[1] <= ms[0] ? 1 : 0;
     if (chosen) *chosen = k->formulation;
-    if (ms_dense) *ms_dense = ms[0] / repeats;
-    if (ms_solvent) *ms_solvent = ms[1] / repeats;
+    if (ms_dense) *ms_dense = ms[0] / steps;
+    if (ms_solvent) *ms_solvent = ms[1] / steps;
     return LMC_OK;
   });
 }

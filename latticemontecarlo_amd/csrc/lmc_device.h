@@ -10,8 +10,10 @@ namespace lmc {
 
 // Packed table entries (all slots < 64, 63 = none):
 //   offset   : int8 x | int8 y << 8 | int8 z << 16
-//   state    : label[0:3] | n_endpoint[3] | other1[4:10] | other2[10:16]
-//   orbit    : feature base[0:12] | kind[12:14] | s0[14:20] | s1[20:26]
+//   state    : byte offset of the weight block in wd [0:13] | other1[13:19] | other2[19:25]
+//              | n_endpoint[31]
+//   orbit    : feature base[0:12] | kind * S * S [12:20] | s0[20:26] | s1[26:32]
+//   mm2      : the forward orbit entries, then the backward ones (slots mirrored)
 //   site     : label[0:3] | other1[4:10] | other2[10:16]
 constexpr int kMaxStateClusters = 512;
 constexpr int kMaxOrbitClusters = 640;
@@ -27,7 +29,8 @@ struct KTables {
   const uint32_t *mm2;
   const uint32_t *site_cl;
   const int32_t *site_off;  // [43]
-  const double *w;          // theta/N by (label, codes)
+  const double *w;          // theta/N by (label, codes): w1[S] | w2[3][S][S] | w3[4][S][S][S]
+  const double *wd;         // the same values by block and (a + S c1 + S^2 c2) (jump dE)
   const double *b1;         // [ne][F1], row = one-hot rank of the migrating species
   const double *b2;         // [ne][F2]
   const double *par;        // [S][6]
@@ -47,7 +50,7 @@ struct KTables {
   const double *base;       // [S][S][3]
   int n_inc, solvent_default;
   uint64_t vac_shell_mask;  // env slots within 3NN of the vacancy
-  int n_state, n_mmm, n_mm2, n_site, n_eci;
+  int n_state, n_mmm, n_mm2, n_site, n_eci;  // n_mm2 counts forward + backward entries
   int v_slot, n_slot;
   int kind, S, ne, x_code, F1, F2, nw;
 };

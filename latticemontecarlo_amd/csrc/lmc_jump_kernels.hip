@@ -14,8 +14,9 @@
 //                             changes of the clusters it touches;
 //   3. events   : 12 lanes form dE, Ea and the rate exp(-Ea beta) in a fixed order and store
 //                 them at the jump's rank (neighbour ids ascending);
-//   4. select   : one lane sums the 12 rates in that order (FP64), draws Philox u1/u2,
-//                 advances the residence-time clock and applies the jump
+//   4. select   : the 12 event lanes form the ordered cumulative probabilities (FP64); the lane
+//                 holding the chosen event advances the residence-time clock (Philox u1/u2
+//                 drawn by an idle lane during the gather) and applies the jump
 //                 (KineticMcFirstOmp.cpp:52-82, KineticMcAbstract.cpp:106-182).
 // Clusters containing both jump endpoints have the same species multiset before and after the
 // jump and are skipped. Tables live in dynamic LDS; the barrier weight tables go there when they
@@ -37,7 +38,6 @@ constexpr int kEnvPad = 768;    // 12 x 60 environment entries padded to whole w
 struct JumpLds {
   int32_t off[12 * kEnvSlots];
   double par[kMaxSpecies * 6];
-  uint8_t mirror[64];
   int8_t tt[3 * kMaxSpecies * kMaxSpecies];
   uint8_t env[12][64];
   int64_t nb[12];   // by direction
@@ -45,17 +45,21 @@ struct JumpLds {
   double ea[12], de[12], rate[12];
   int64_t snb[12];
   int dir[12];
-  // per-event sums (dE, logit MMM, logit MM2)
+  // per-event sums (dE, logit MMM, logit MM2) of the solvent path's earlier passes
   double acc[12][3];
+  // random numbers of the current step (drawn by an idle lane during the gather)
+  double u2, log_u1, corr;
+  // events whose environment holds a vacancy off the jump endpoints (no one-hot type)
+  int bad;
   // dense partials / solvent-baseline work list
   union {
     double part[12][kLanes][3];
     struct {
       uint16_t inc[kEnvPad];
       unsigned long long mask[12];
-      int prefix[13];
       int start[13];
-      double part[kPass][3];
+      int last_p0;
+      double part[kPass][kGroup][3];
     } sp;
   } u;
 };
@@ -87,12 +91,14 @@ template <bool kSolvent>
 __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const DynLayout &lay) {
   const int tid = threadIdx.x;
   for (int i = tid; i < 12 * kEnvSlots; i += kThreads) L.off[i] = t.off[i];
-  for (int i = tid; i < kEnvSlots; i += kThreads) L.mirror[i] = t.mirror[i];
   for (int i = tid; i < 3 * t.S * t.S; i += kThreads) L.tt[i] = t.tt[i];
   for (int i = tid; i < 6 * t.S; i += kThreads) L.par[i] = t.par[i];
+  if (tid < 12) L.env[tid][63] = 0;  // target of the absent sites of every packed cluster
+  if (tid == 0) L.bad = 0;
   Tabs T{};
   double *w = reinterpret_cast<double *>(dyn + lay.w);
-  for (int i = tid; i < t.nw; i += kThreads) w[i] = t.w[i];
+  const double *wsrc = kSolvent ? t.w : t.wd;
+  for (int i = tid; i < t.nw; i += kThreads) w[i] = wsrc[i];
   T.w = w;
   if (lay.b1_lds) {
     double *p = reinterpret_cast<double *>(dyn + lay.b1);
@@ -161,6 +167,7 @@ __device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const ui
         atomicOr(err, 4);
         x = 0;
       }
+      if (x == t.x_code && j != t.v_slot) atomicOr(&L.bad, 1 << k);
       L.env[k][j] = x;
       if (j == t.n_slot) L.nb[k] = s[u];
     }
@@ -173,101 +180,93 @@ __device__ __forceinline__ double WIndex(const double *w, int label, int a, int 
   return w[S + 3 * S * S + (label - 4) * S * S * S + (a * S + c1) * S + c2];
 }
 
-__device__ __forceinline__ double StateTerm(const double *w, const uint8_t *e, uint32_t d, int M,
-                                            int X, int S) {
-  const int label = d & 7;
-  const int o1 = (d >> 4) & 63, o2 = (d >> 10) & 63;
-  const int c1 = label ? e[o1] : 0;
-  const int c2 = label >= 4 ? e[o2] : 0;
-  const double v = WIndex(w, label, M, c1, c2, S) - WIndex(w, label, X, c1, c2, S);
-  return (d & 8) ? -v : v;
-}
-
-__device__ __forceinline__ double OrbitTerm(const JumpLds &L, const double *bm, const uint8_t *e,
-                                            uint32_t d, int S, bool backward) {
-  const int base = d & 4095, kind = (d >> 12) & 3;
-  int s0 = (d >> 14) & 63, s1 = (d >> 20) & 63;
-  if (backward) {
-    s0 = L.mirror[s0];
-    s1 = kind ? L.mirror[s1] : 0;
+// Sums of kU packed state clusters per lane: w(cluster | endpoint = M) - w(cluster | endpoint =
+// X), negated for the neighbour endpoint's clusters; wm, wx: the weight table shifted by M and X
+// entries. The loads of the kU clusters are issued level by level (entries, species, weights).
+template <int kU>
+__device__ __forceinline__ void StateBatch(const char *wm, const char *wx, const uint8_t *e,
+                                           const uint32_t *list, int c, int s8, int ss8,
+                                           double *acc) {
+  uint32_t d[kU];
+  int c1[kU], c2[kU];
+  double p[kU], q[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    c1[u] = e[(d[u] >> 13) & 63];
+    c2[u] = e[(d[u] >> 19) & 63];
   }
-  const int c0 = e[s0];
-  const int c1 = kind ? e[s1] : 0;
-  const int ty = L.tt[(kind * S + c0) * S + c1];
-  // A vacancy off the jump endpoints has no one-hot type (the reference throws).
-  return ty < 0 ? __builtin_nan("") : bm[base + ty];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int o = static_cast<int>(d[u] & 8191) + __umul24(c1[u], s8) + __umul24(c2[u], ss8);
+    p[u] = *reinterpret_cast<const double *>(wm + o);
+    q[u] = *reinterpret_cast<const double *>(wx + o);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const double v = p[u] - q[u];
+    acc[u] += __hiloint2double(__double2hiint(v) ^ static_cast<int>(d[u] & 0x80000000u),
+                               __double2loint(v));
+  }
 }
 
-// Phase 2, dense: per-lane partial sums of every cluster's weight, then per-event sums.
+// Sums of kU packed orbit clusters per lane: b[base + one-hot type(kind, c0, c1)].
+template <int kU>
+__device__ __forceinline__ void OrbitBatch(const int8_t *tt, const double *bm, const uint8_t *e,
+                                           const uint32_t *list, int c, int S, double *acc) {
+  uint32_t d[kU];
+  int c0[kU], c1[kU], ty[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    c0[u] = e[(d[u] >> 20) & 63];
+    c1[u] = e[d[u] >> 26];
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) ty[u] = tt[((d[u] >> 12) & 255) + __umul24(c0[u], S) + c1[u]];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) acc[u] += bm[(d[u] & 4095) + ty[u]];
+}
+
+// Phase 2, dense: per-lane partial sums of every cluster's weight (dE, MMM logit, MM2 logit).
 __device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T) {
   const int tid = threadIdx.x;
-  if (tid < 12 * kLanes) {
-    const int k = tid / kLanes, lane = tid - k * kLanes;
-    const uint8_t *e = L.env[k];
-    const int S = t.S, X = t.x_code, M = e[t.n_slot];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int c = lane;
-    for (; c + 3 * kLanes < t.n_state; c += 4 * kLanes) {
-      a0 += StateTerm(T.w, e, T.state[c], M, X, S);
-      a1 += StateTerm(T.w, e, T.state[c + kLanes], M, X, S);
-      a2 += StateTerm(T.w, e, T.state[c + 2 * kLanes], M, X, S);
-      a3 += StateTerm(T.w, e, T.state[c + 3 * kLanes], M, X, S);
-    }
-    for (; c < t.n_state; c += kLanes) a0 += StateTerm(T.w, e, T.state[c], M, X, S);
-    const double de = (a0 + a1) + (a2 + a3);
-    double la = 0.0, lb = 0.0;
-    const int row = L.tt[M * S];  // one-hot rank of the migrating species (-1 for X)
-    if (t.kind == kE0 || t.kind == kQuartic) {
-      if (row < 0) {
-        la = __builtin_nan("");
-      } else {
-        const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-        a0 = a1 = a2 = a3 = 0.0;
-        c = lane;
-        for (; c + 3 * kLanes < t.n_mmm; c += 4 * kLanes) {
-          a0 += OrbitTerm(L, bm, e, T.mmm[c], S, false);
-          a1 += OrbitTerm(L, bm, e, T.mmm[c + kLanes], S, false);
-          a2 += OrbitTerm(L, bm, e, T.mmm[c + 2 * kLanes], S, false);
-          a3 += OrbitTerm(L, bm, e, T.mmm[c + 3 * kLanes], S, false);
-        }
-        for (; c < t.n_mmm; c += kLanes) a0 += OrbitTerm(L, bm, e, T.mmm[c], S, false);
-        la = (a0 + a1) + (a2 + a3);
-      }
-    }
-    if (t.kind == kQuartic && row >= 0) {
-      const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-      a0 = a1 = a2 = a3 = 0.0;
-      c = lane;
-      for (; c + kLanes < t.n_mm2; c += 2 * kLanes) {
-        const uint32_t x = T.mm2[c], y = T.mm2[c + kLanes];
-        a0 += OrbitTerm(L, bm, e, x, S, false);
-        a1 += OrbitTerm(L, bm, e, x, S, true);
-        a2 += OrbitTerm(L, bm, e, y, S, false);
-        a3 += OrbitTerm(L, bm, e, y, S, true);
-      }
-      for (; c < t.n_mm2; c += kLanes) {
-        a0 += OrbitTerm(L, bm, e, T.mm2[c], S, false);
-        a1 += OrbitTerm(L, bm, e, T.mm2[c], S, true);
-      }
-      lb = (a0 + a1) + (a2 + a3);
-    }
-    L.u.part[k][lane][0] = de;
-    L.u.part[k][lane][1] = la;
-    L.u.part[k][lane][2] = lb;
+  if (tid >= 12 * kLanes) return;
+  const int k = tid / kLanes, lane = tid - k * kLanes;
+  const uint8_t *e = L.env[k];
+  const int S = t.S, M = e[t.n_slot];
+  const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
+  const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
+  const int s8 = 8 * S, ss8 = 8 * S * S;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = lane;
+  for (; c + 3 * kLanes < t.n_state; c += 4 * kLanes) StateBatch<4>(wm, wx, e, T.state, c, s8, ss8, a);
+  for (; c < t.n_state; c += kLanes) StateBatch<1>(wm, wx, e, T.state, c, s8, ss8, a);
+  const double de = (a[0] + a[1]) + (a[2] + a[3]);
+  double la = 0.0, lb = 0.0;
+  // one-hot rank of the migrating species (a vacancy is flagged in the gather)
+  const int row = L.tt[M * S];
+  if (t.kind == kE0 || t.kind == kQuartic) {
+    const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
+    a[0] = a[1] = a[2] = a[3] = 0.0;
+    c = lane;
+    for (; c + 3 * kLanes < t.n_mmm; c += 4 * kLanes) OrbitBatch<4>(L.tt, bm, e, T.mmm, c, S, a);
+    for (; c < t.n_mmm; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mmm, c, S, a);
+    la = (a[0] + a[1]) + (a[2] + a[3]);
   }
-  __syncthreads();
-  if (tid < 12) {
-    double de = 0.0, la = 0.0, lb = 0.0;
-#pragma unroll 7
-    for (int l = 0; l < kLanes; ++l) {
-      de += L.u.part[tid][l][0];
-      la += L.u.part[tid][l][1];
-      lb += L.u.part[tid][l][2];
-    }
-    L.acc[tid][0] = de;
-    L.acc[tid][1] = la;
-    L.acc[tid][2] = lb;
+  if (t.kind == kQuartic) {  // forward and backward MM2 entries in one list
+    const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
+    a[0] = a[1] = a[2] = a[3] = 0.0;
+    c = lane;
+    for (; c + 3 * kLanes < t.n_mm2; c += 4 * kLanes) OrbitBatch<4>(L.tt, bm, e, T.mm2, c, S, a);
+    for (; c < t.n_mm2; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mm2, c, S, a);
+    lb = (a[0] + a[1]) + (a[2] + a[3]);
   }
+  L.u.part[k][lane][0] = de;
+  L.u.part[k][lane][1] = la;
+  L.u.part[k][lane][2] = lb;
 }
 
 // Change of one touched cluster relative to the all-solvent environment.
@@ -294,15 +293,17 @@ __device__ __forceinline__ void SolventEntry(const JumpLds &L, const Tabs &T, co
   const int tp = L.tt[(kind * S + sv) * S + (kind ? sv : 0)];
   if (type == 1) {
     const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-    la += ty < 0 ? __builtin_nan("") : bm[base + ty] - bm[base + tp];
+    la += bm[base + ty] - bm[base + tp];
   } else {
     const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-    lb += ty < 0 ? __builtin_nan("") : bm[base + ty] - bm[base + tp];
+    lb += bm[base + ty] - bm[base + tp];
   }
 }
 
 // Phase 2, solvent baseline: list the environment entries (event k, slot j) that hold a
-// non-solvent species, then 8 lanes per entry add the changes of the clusters touching it.
+// non-solvent species, then 8 lanes per entry add the changes of the clusters touching it. Each
+// lane leaves its partial sums in LDS; the event lanes add them in a fixed order (passes before
+// the last here, the last pass in FinishEvents).
 __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int sv) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool flag[3];
@@ -318,34 +319,35 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
     const unsigned long long m = __ballot(f);
     if (lane == 0) L.u.sp.mask[wave + 4 * u] = m;
   }
+  if (tid < 12) L.acc[tid][0] = L.acc[tid][1] = L.acc[tid][2] = 0.0;
   __syncthreads();
-  if (tid == 0) {
-    int s = 0;
-    for (int c = 0; c < 12; ++c) {
-      L.u.sp.prefix[c] = s;
-      s += __popcll(L.u.sp.mask[c]);
-    }
-    L.u.sp.prefix[12] = s;
-  }
-  __syncthreads();
-  if (tid <= 12) {  // first list position of event k's entries
-    const int e0 = tid * kEnvSlots, c = e0 >> 6, b = e0 & 63;
-    L.u.sp.start[tid] = c >= 12 ? L.u.sp.prefix[12]
-                                : L.u.sp.prefix[c] + __popcll(L.u.sp.mask[c] & ((1ull << b) - 1ull));
-  }
+  int cnt[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) cnt[c] = __popcll(L.u.sp.mask[c]);
   const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
   for (int u = 0; u < 3; ++u)
     if (flag[u]) {
       const int c = wave + 4 * u;
-      L.u.sp.inc[L.u.sp.prefix[c] + __popcll(L.u.sp.mask[c] & below)] =
-          static_cast<uint16_t>(tid + u * kThreads);
+      int pre = 0;
+#pragma unroll
+      for (int q = 0; q < 12; ++q) pre += q < c ? cnt[q] : 0;
+      L.u.sp.inc[pre + __popcll(L.u.sp.mask[c] & below)] = static_cast<uint16_t>(tid + u * kThreads);
     }
-  if (tid < 12) L.acc[tid][0] = L.acc[tid][1] = L.acc[tid][2] = 0.0;
+  if (tid <= 12) {  // first list position of event k's entries (k = 12: the total)
+    const int e0 = tid * kEnvSlots, c = e0 >> 6, b = e0 & 63;
+    int pre = 0;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) pre += q < c ? cnt[q] : 0;
+    L.u.sp.start[tid] = pre + (c < 12 ? __popcll(L.u.sp.mask[c] & ((1ull << b) - 1ull)) : 0);
+  }
+  int total = 0;
+#pragma unroll
+  for (int q = 0; q < 12; ++q) total += cnt[q];
   __syncthreads();
-  const int total = L.u.sp.prefix[12];
   const int grp = tid / kGroup, g = tid - grp * kGroup;
-  for (int p0 = 0; p0 < total; p0 += kPass) {
+  int p0 = 0;
+  for (;; p0 += kPass) {
     double de = 0.0, la = 0.0, lb = 0.0;
     const int i = p0 + grp;
     if (i < total) {
@@ -355,52 +357,72 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
       const int M = e[t.n_slot];
       const int row = L.tt[M * t.S];
       const int cj = e[j];
-      if (row < 0) {
-        de = __builtin_nan("");
-      } else {
-        const int q1 = T.inc_off[j + 1];
-        for (int q = T.inc_off[j] + g; q < q1; q += kGroup)
-          SolventEntry(L, T, t, e, T.inc[q], j, cj, sv, M, row, de, la, lb);
-      }
+      const int q1 = T.inc_off[j + 1];
+      for (int q = T.inc_off[j] + g; q < q1; q += kGroup)
+        SolventEntry(L, T, t, e, T.inc[q], j, cj, sv, M, row, de, la, lb);
     }
-    // sum over the 8 lanes of the group (fixed butterfly order)
-#pragma unroll
-    for (int m = kGroup / 2; m > 0; m >>= 1) {
-      de += __shfl_xor(de, m);
-      la += __shfl_xor(la, m);
-      lb += __shfl_xor(lb, m);
-    }
-    if (g == 0) {
-      L.u.sp.part[grp][0] = de;
-      L.u.sp.part[grp][1] = la;
-      L.u.sp.part[grp][2] = lb;
-    }
+    L.u.sp.part[grp][g][0] = de;
+    L.u.sp.part[grp][g][1] = la;
+    L.u.sp.part[grp][g][2] = lb;
+    if (p0 + kPass >= total) break;
     __syncthreads();
     if (tid < 12) {
       const int lo = max(p0, L.u.sp.start[tid]), hi = min(p0 + kPass, L.u.sp.start[tid + 1]);
-      for (int q = lo; q < hi; ++q) {
-        L.acc[tid][0] += L.u.sp.part[q - p0][0];
-        L.acc[tid][1] += L.u.sp.part[q - p0][1];
-        L.acc[tid][2] += L.u.sp.part[q - p0][2];
-      }
+      for (int q = lo; q < hi; ++q)
+        for (int h = 0; h < kGroup; ++h) {
+          L.acc[tid][0] += L.u.sp.part[q - p0][h][0];
+          L.acc[tid][1] += L.u.sp.part[q - p0][h][1];
+          L.acc[tid][2] += L.u.sp.part[q - p0][h][2];
+        }
     }
     __syncthreads();
   }
-  if (tid < 12) {  // all-solvent sums of this (solvent, migrating species)
-    const int M = L.env[tid][t.n_slot];
-    const double *b = T.base + (static_cast<size_t>(sv) * t.S + M) * 3;
-    L.acc[tid][0] += b[0];
-    L.acc[tid][1] += b[1];
-    L.acc[tid][2] += b[2];
-  }
+  if (tid == 0) L.u.sp.last_p0 = p0;
+}
+
+// Per-event sums at the start of phase 3 (lanes k < 12, after the phase-2 barrier).
+__device__ __forceinline__ void SolventSums(const KTables &t, const JumpLds &L, const Tabs &T,
+                                            int k, int sv, double &de, double &la, double &lb) {
+  double a0 = L.acc[k][0], a1 = L.acc[k][1], a2 = L.acc[k][2];
+  const int p0 = L.u.sp.last_p0;
+  const int lo = max(p0, L.u.sp.start[k]), hi = min(p0 + kPass, L.u.sp.start[k + 1]);
+  for (int q = lo; q < hi; ++q)
+    for (int h = 0; h < kGroup; ++h) {
+      a0 += L.u.sp.part[q - p0][h][0];
+      a1 += L.u.sp.part[q - p0][h][1];
+      a2 += L.u.sp.part[q - p0][h][2];
+    }
+  // all-solvent sums of this (solvent, migrating species)
+  const int M = L.env[k][t.n_slot];
+  const double *b = T.base + (static_cast<size_t>(sv) * t.S + M) * 3;
+  de = a0 + b[0];
+  la = a1 + b[1];
+  lb = a2 + b[2];
 }
 
 // Phase 3: Ea, dE and rate per jump, stored at the jump's rank among the 12 neighbour ids.
-__device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, double beta,
-                                             int32_t *err) {
+template <bool kSolvent>
+__device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const Tabs &T, int sv,
+                                             double beta, int32_t *err) {
   const int k = threadIdx.x;
+  if (k >= 64) return;
+  double de, la, lb;
+  if (!kSolvent) {  // wave 0: lane 3q + c adds the kLanes partials of component c of event q
+    double s = 0.0;
+    if (k < 36) {
+      const int q = k / 3, c = k - 3 * q;
+#pragma unroll
+      for (int l = 0; l < kLanes; ++l) s += L.u.part[q][l][c];
+    }
+    const int q = k < 12 ? k : 0;
+    de = __shfl(s, 3 * q);
+    la = __shfl(s, 3 * q + 1);
+    lb = __shfl(s, 3 * q + 2);
+  }
   if (k >= 12) return;
-  const double de = L.acc[k][0], la = L.acc[k][1], lb = L.acc[k][2];
+  if (kSolvent) SolventSums(t, L, T, k, sv, de, la, lb);
+  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
+    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
   const int M = L.env[k][t.n_slot];
   const double *p = L.par + 6 * M;
   double ea = 0.0;
@@ -440,7 +462,7 @@ __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, cons
   if (kSolvent) SolventClusters(t, L, T, sv);
   else DenseClusters(t, L, T);
   __syncthreads();
-  FinishEvents(t, L, beta, err);
+  FinishEvents<kSolvent>(t, L, T, sv, beta, err);
   __syncthreads();
 }
 
@@ -479,28 +501,44 @@ __device__ double ScheduleTemperature(const double *tt, const double *tv, int n,
   return tv[i - 1] + ((time - tt[i - 1]) / (tt[i] - tt[i - 1])) * (tv[i] - tv[i - 1]);
 }
 
-// Phase 4 of one KMC step (lane 0): KineticMcFirstAbstract::OneStepSimulation after
-// BuildEventList (KineticMcAbstract.cpp:147-181).
+// Random numbers of step Q.steps (one lane, overlapping the gather): Philox u1, u2, log(u1) and
+// the RateCorrector factor (KineticMcFirstOmp.cpp:79-82, RateCorrector.hpp:17-24).
+__device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
+  const U4 u = Philox4x32_10(Q.seed, Q.steps, Q.rid, 0u);
+  L.u2 = UnitOpenRight(Join64(u.z, u.w));
+  L.log_u1 = log(UnitOpenLeft(Join64(u.x, u.y)));
+  L.corr = (Q.flags & 1) ? Q.cv / (1.64 * exp(-(0.66 / kBoltzmann / Q.T - 0.7))) / (1 - 13 * Q.cs)
+                         : 1.0;
+}
+
+// Phase 4 (lanes k < 12): lane j divides rate j by the total; every event lane then forms the
+// same ordered cumulative probabilities (KineticMcFirstOmp.cpp:69-77); the lane holding the first cumulative probability >= u2 (lane
+// 11 when there is none, KineticMcAbstract.cpp:106-123) advances the clock and applies the jump
+// (KineticMcAbstract.cpp:147-181).
 __device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, ReplicaLds &Q,
                                uint8_t *occ, int r, uint64_t trace_index) {
+  const int i = threadIdx.x;
+  if (i >= 12) return;
+  if (i == 0) L.bad = 0;  // every event lane has read it in phase 3
   double total = 0.0;
+#pragma unroll
   for (int j = 0; j < 12; ++j) total += L.rate[j];
-  const uint64_t steps = Q.steps;
-  const U4 u = Philox4x32_10(Q.seed, steps, Q.rid, 0u);
-  const double u1 = UnitOpenLeft(Join64(u.x, u.y));
-  const double u2 = UnitOpenRight(Join64(u.z, u.w));
-  double dt = -log(u1) / total / kPrefactor;  // KineticMcFirstOmp::CalculateTime (:79-82)
-  if (Q.flags & 1)  // RateCorrector::GetTimeCorrectionFactor (RateCorrector.hpp:17-24)
-    dt *= Q.cv / (1.64 * exp(-(0.66 / kBoltzmann / Q.T - 0.7))) / (1 - 13 * Q.cs);
-  int sel = 11;  // SelectEvent: first cumulative probability >= u2, else the last event
-  double cum = 0.0;
+  const double p = L.rate[i] / total;
+  const double u2 = L.u2;
+  double cum = 0.0, prev = 0.0;
+#pragma unroll
   for (int j = 0; j < 12; ++j) {
-    cum += L.rate[j] / total;
-    if (!(cum < u2)) {
-      sel = j;
-      break;
+    const double pj = __shfl(p, j);
+    if (j <= i) {
+      prev = cum;
+      cum += pj;
     }
   }
+  const bool mine = (!(cum < u2) && (i == 0 || prev < u2)) || (i == 11 && cum < u2);
+  if (!mine) return;
+  const int sel = i;
+  double dt = -L.log_u1 / total / kPrefactor;
+  if (Q.flags & 1) dt *= L.corr;
   if (Q.flags & 2) {  // IsEscaped (KineticMcAbstract.cpp:192-223)
     bool all = true;
     for (int j = 0; j < kEnvSlots; ++j)
@@ -543,7 +581,7 @@ __device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, Re
 }
 
 template <bool kSolvent>
-__global__ __launch_bounds__(kThreads) void KmcKernel(KTables t, DynLayout lay, KmcDev k,
+__global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout lay, KmcDev k,
                                                       uint64_t nsteps, uint64_t trace_base) {
   __shared__ JumpLds L;
   __shared__ ReplicaLds Q;
@@ -580,12 +618,16 @@ __global__ __launch_bounds__(kThreads) void KmcKernel(KTables t, DynLayout lay, 
   const int sv = Q.solvent;
   for (uint64_t it = 0; it < nsteps; ++it) {
     if (Q.done) break;
-    if (tid == 0 && k.n_tt > 0) {
-      Q.T = ScheduleTemperature(k.tt_time, k.tt_temp, k.n_tt, Q.time);
-      Q.beta = 1.0 / kBoltzmann / Q.T;
+    if (k.n_tt > 0) {  // UpdateTemperature (KineticMcAbstract.cpp:45-50), before any lane reads beta
+      if (tid == 0) {
+        Q.T = ScheduleTemperature(k.tt_time, k.tt_temp, k.n_tt, Q.time);
+        Q.beta = 1.0 / kBoltzmann / Q.T;
+      }
+      __syncthreads();
     }
+    if (tid == kThreads - 1) DrawStep(L, Q);
     EvaluateJumps<kSolvent>(t, L, T, occ, n, Q.vac, sv, Q.beta, k.error);
-    if (tid == 0) SelectAndApply(t, k, L, Q, occ, r, trace_base + it);
+    SelectAndApply(t, k, L, Q, occ, r, trace_base + it);
     __syncthreads();
   }
   if (tid == 0) {
@@ -639,6 +681,7 @@ DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
   } else {
     lay.lists = static_cast<int>(at);
     at = al(at + sizeof(uint32_t) * (t.n_state + t.n_mmm + (t.kind == kQuartic ? t.n_mm2 : 0)));
+    // (n_mm2 counts the forward and the backward entries)
   }
   const size_t b1 = sizeof(double) * static_cast<size_t>(t.ne) * t.F1;
   const size_t b2 = sizeof(double) * static_cast<size_t>(t.ne) * t.F2;

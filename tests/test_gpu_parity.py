@@ -174,7 +174,7 @@ def test_kmc_autotune_keeps_results(torch, coeffs):
     occs = [random_alloc(n, gb.species, 0.03, seed=70 + r) for r in range(R)]
     a = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
     b = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
-    info = a.autotune(repeats=2)
+    info = a.autotune(steps=50)
     assert info["formulation"] in (0, 1) and info["ms_dense"] > 0 and info["ms_solvent"] > 0
     a.run(200)
     b.run(200)
