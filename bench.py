@@ -31,6 +31,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(R, n, chunk):
+    """HBM bytes per launch of the KMC kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (copy-calibrated), scaled to `chunk` steps per launch; None when no
+    summary exists for this replica count and lattice size."""
+    path = os.path.join(ROOT, "profiles", f"kmc_e0_{R}x{4 * n ** 3}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_replica_step:"):
+                return round(float(line.split(":")[1]) * R * chunk)
+    return None
+
+
 def make_workload(R, n, seed):
     from latticemontecarlo_amd import synth
     rng = np.random.default_rng(seed)
@@ -156,11 +170,7 @@ def main():
     kern = [(c, e0.elapsed_time(e1) / 1e3) for c, e0, e1 in launches]
     bytes_per_launch = [ALGO_BYTES_PER_STEP * R * c for c, _ in kern]
     achieved = sum(bytes_per_launch) / sum(t for _, t in kern) / 1e9
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "kmc_e0_1024x32000_traffic.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = measured_traffic(R, n, args.chunk)
     line = {
         "metric": "KMC steps/s over all replicas (plain GPU path)",
         "value": round(value, 1),

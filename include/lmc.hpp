@@ -1,0 +1,407 @@
+// C++ mirror of the reference's predictor and driver interface over the C ABI (lmc_capi.h).
+//
+// Names, argument meanings and exceptions follow zhucongx/LatticeMonteCarlo so that host code
+// written against it compiles with few changes:
+//   cfg::Config, cfg::GenerateFCC                  lmc/cfg/include/Config.h:15-115
+//   pred::EnergyPredictor                          lmc/pred/include/EnergyPredictor.h:16-25
+//   pred::VacancyMigrationPredictorE0/Quartic      VacancyMigrationPredictorE0.h:15-24,
+//                                                  VacancyMigrationPredictorQuartic.h:18-27
+//   pred::EnergyChangePredictorPairSite            EnergyChangePredictorPairSite.h:16-25
+//   mc::KineticMcFirstBatch, mc::CanonicalMcBatch  KineticMcFirstOmp.h:11-25,
+//                                                  CanonicalMcSerial.h:11-20 (R replicas each)
+// Elements are strings ("Al", "X"); species codes are indices into sorted(element_set + "X").
+// Errors: LMC_ERR_IO -> std::runtime_error, LMC_ERR_MODEL -> std::out_of_range,
+// LMC_ERR_ARG -> std::invalid_argument, LMC_ERR_HIP -> std::runtime_error, each carrying
+// lmc_last_error(). Every computation runs on the GPU; there is no CPU path.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "lmc_capi.h"
+
+namespace lmc {
+
+inline void Check(int rc) {
+  if (rc == LMC_OK) return;
+  const std::string msg = lmc_last_error();
+  if (rc == LMC_ERR_ARG) throw std::invalid_argument(msg);
+  if (rc == LMC_ERR_MODEL) throw std::out_of_range(msg);
+  throw std::runtime_error(msg);
+}
+
+// Owning device allocation (lmc_device_alloc / lmc_device_free).
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(size_t bytes) { Check(lmc_device_alloc(bytes, &p_)); }
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  DeviceBuffer(DeviceBuffer &&o) noexcept : p_(o.p_) { o.p_ = nullptr; }
+  DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+    std::swap(p_, o.p_);
+    return *this;
+  }
+  ~DeviceBuffer() { (void)lmc_device_free(p_); }
+  template <typename T>
+  T *as() const { return static_cast<T *>(p_); }
+  void Upload(const void *src, size_t bytes) { Check(lmc_copy_to_device(p_, src, bytes)); }
+  void Download(void *dst, size_t bytes) const { Check(lmc_copy_to_host(dst, p_, bytes)); }
+
+ private:
+  void *p_ = nullptr;
+};
+
+// Model handle shared by the predictor classes.
+class ModelHandle {
+ public:
+  ModelHandle(const std::string &json, const std::set<std::string> &element_set, int kind) {
+    std::vector<const char *> els;
+    for (const auto &e : element_set)
+      if (e != "X") els.push_back(e.c_str());
+    lmc_model *m = nullptr;
+    Check(lmc_model_load(json.c_str(), els.data(), static_cast<int32_t>(els.size()), kind, &m));
+    m_.reset(m);
+  }
+  lmc_model *get() const { return m_.get(); }
+  int Code(const std::string &element) const {
+    const int c = lmc_model_species_code(m_.get(), element.c_str());
+    return c >= 0 ? c : lmc_model_species_code(m_.get(), "X");  // unknown strings are X
+  }
+
+ private:
+  struct Free {
+    void operator()(lmc_model *m) const { lmc_model_free(m); }
+  };
+  std::unique_ptr<lmc_model, Free> m_;
+};
+
+}  // namespace lmc
+
+namespace cfg {
+
+// An n x n x n-cell FCC box (a = 4.046 A) with one element per lattice site, ids in the
+// reference's x-major order (Config.cpp:466-470). The species codes of the element set index the
+// engine's tables; a device copy is made on first use and refreshed after every change.
+class Config {
+ public:
+  Config(int n, std::set<std::string> element_set_without_vacancy, const std::string &element)
+      : n_(n) {
+    element_set_without_vacancy.erase("X");
+    species_.assign(element_set_without_vacancy.begin(), element_set_without_vacancy.end());
+    species_.push_back("X");
+    std::sort(species_.begin(), species_.end());
+    if (n < 4) throw std::invalid_argument("FCC box needs at least 4 cells per side");
+    codes_.assign(static_cast<size_t>(lmc_fcc_num_sites(n)), CodeOf(element));
+  }
+  // Copies get their own device buffer.
+  Config(const Config &o) : n_(o.n_), species_(o.species_), codes_(o.codes_) {}
+  Config &operator=(const Config &o) {
+    n_ = o.n_;
+    species_ = o.species_;
+    codes_ = o.codes_;
+    dirty_ = true;
+    dev_.reset();
+    return *this;
+  }
+  Config(Config &&) = default;
+  Config &operator=(Config &&) = default;
+  [[nodiscard]] size_t GetNumAtoms() const { return codes_.size(); }
+  [[nodiscard]] int GetNumCells() const { return n_; }
+  [[nodiscard]] std::string GetElementAtLatticeId(size_t lattice_id) const {
+    return species_.at(codes_.at(lattice_id));
+  }
+  void SetAtomElementTypeAtLattice(size_t lattice_id, const std::string &element) {
+    codes_.at(lattice_id) = CodeOf(element);
+    dirty_ = true;
+  }
+  // Config::LatticeJump (Config.cpp:431-456): exchanges the two sites' atoms.
+  void LatticeJump(const std::pair<size_t, size_t> &lattice_id_jump_pair) {
+    std::swap(codes_.at(lattice_id_jump_pair.first), codes_.at(lattice_id_jump_pair.second));
+    dirty_ = true;
+  }
+  [[nodiscard]] size_t GetVacancyLatticeId() const {
+    const uint8_t x = CodeOf("X");
+    for (size_t i = 0; i < codes_.size(); ++i)
+      if (codes_[i] == x) return i;
+    throw std::runtime_error("vacancy not found");
+  }
+  [[nodiscard]] std::vector<size_t> GetNeighborLatticeIdVector(size_t lattice_id, int shell) const {
+    int64_t out[24];
+    const int32_t k = lmc_fcc_neighbors(n_, static_cast<int64_t>(lattice_id), shell, out);
+    if (k < 0) lmc::Check(k);
+    return std::vector<size_t>(out, out + k);
+  }
+  [[nodiscard]] std::set<std::string> GetElementSetWithoutVacancy() const {
+    std::set<std::string> s(species_.begin(), species_.end());
+    s.erase("X");
+    return s;
+  }
+  // Species codes by lattice id, in the order of sorted(element_set + "X").
+  [[nodiscard]] const std::vector<uint8_t> &GetCodes() const { return codes_; }
+  [[nodiscard]] const std::vector<std::string> &GetSpecies() const { return species_; }
+  [[nodiscard]] const uint8_t *DeviceOccupancy() const {
+    if (dirty_ || !dev_) {
+      if (!dev_) dev_ = std::make_unique<lmc::DeviceBuffer>(codes_.size());
+      dev_->Upload(codes_.data(), codes_.size());
+      dirty_ = false;
+    }
+    return dev_->as<uint8_t>();
+  }
+  [[nodiscard]] uint8_t CodeOf(const std::string &element) const {
+    for (size_t i = 0; i < species_.size(); ++i)
+      if (species_[i] == element) return static_cast<uint8_t>(i);
+    for (size_t i = 0; i < species_.size(); ++i)  // unknown strings are X (Element.hpp:25-26)
+      if (species_[i] == "X") return static_cast<uint8_t>(i);
+    return 0;
+  }
+
+ private:
+  int n_;
+  std::vector<std::string> species_;
+  std::vector<uint8_t> codes_;
+  mutable bool dirty_ = true;
+  mutable std::unique_ptr<lmc::DeviceBuffer> dev_;
+};
+
+// GenerateFCC (Config.cpp:1060-1095) for a cubic box of factors {n, n, n}.
+inline Config GenerateFCC(int n, const std::set<std::string> &element_set, const std::string &element) {
+  return Config(n, element_set, element);
+}
+
+}  // namespace cfg
+
+namespace pred {
+
+namespace detail {
+// Device copies of (offset 0, ncell) for single-replica calls.
+struct One {
+  lmc::DeviceBuffer off{sizeof(int64_t)}, ncell{sizeof(int32_t)}, idx{4 * sizeof(int64_t)},
+      out{sizeof(double)}, code{8};
+  explicit One(int n) {
+    const int64_t zero = 0;
+    off.Upload(&zero, sizeof(zero));
+    ncell.Upload(&n, sizeof(n));
+  }
+};
+}  // namespace detail
+
+class EnergyPredictor {
+ public:
+  EnergyPredictor(const std::string &predictor_filename, const std::set<std::string> &element_set)
+      : m_(predictor_filename, element_set, LMC_MODEL_ENERGY) {}
+  [[nodiscard]] double GetEnergy(const cfg::Config &config) const {
+    detail::One one(config.GetNumCells());
+    lmc::Check(lmc_total_energy(m_.get(), config.DeviceOccupancy(), one.off.as<int64_t>(),
+                                one.ncell.as<int32_t>(), 1, one.out.as<double>(), nullptr));
+    double e = 0;
+    one.out.Download(&e, sizeof(e));
+    lmc::Check(lmc_check_errors(m_.get()));
+    return e;
+  }
+  [[nodiscard]] lmc_model *handle() const { return m_.get(); }
+
+ private:
+  lmc::ModelHandle m_;
+};
+
+// Base of the two barrier models: GetBarrierAndDiffFromLatticeIdPair returns {Ea, dE} of the
+// jump of the vacancy at pair.first to its first neighbour pair.second.
+class VacancyMigrationPredictor {
+ public:
+  VacancyMigrationPredictor(const std::string &predictor_filename, const cfg::Config & /*reference_config*/,
+                            const std::set<std::string> &element_set, int kind)
+      : m_(predictor_filename, element_set, kind) {}
+  virtual ~VacancyMigrationPredictor() = default;
+  [[nodiscard]] virtual std::pair<double, double> GetBarrierAndDiffFromLatticeIdPair(
+      const cfg::Config &config, const std::pair<size_t, size_t> &lattice_id_jump_pair) const {
+    double ea = 0, de = 0;
+    lmc::Check(lmc_eval_pair(m_.get(), config.DeviceOccupancy(), config.GetNumCells(),
+                             static_cast<int64_t>(lattice_id_jump_pair.first),
+                             static_cast<int64_t>(lattice_id_jump_pair.second), &ea, &de));
+    lmc::Check(lmc_check_errors(m_.get()));
+    return {ea, de};
+  }
+  [[nodiscard]] lmc_model *handle() const { return m_.get(); }
+
+ private:
+  lmc::ModelHandle m_;
+};
+
+class VacancyMigrationPredictorE0 : public VacancyMigrationPredictor {
+ public:
+  VacancyMigrationPredictorE0(const std::string &predictor_filename, const cfg::Config &reference_config,
+                              const std::set<std::string> &element_set)
+      : VacancyMigrationPredictor(predictor_filename, reference_config, element_set, LMC_MODEL_E0) {}
+};
+
+class VacancyMigrationPredictorQuartic : public VacancyMigrationPredictor {
+ public:
+  VacancyMigrationPredictorQuartic(const std::string &predictor_filename,
+                                   const cfg::Config &reference_config,
+                                   const std::set<std::string> &element_set)
+      : VacancyMigrationPredictor(predictor_filename, reference_config, element_set,
+                                  LMC_MODEL_QUARTIC) {}
+};
+
+class EnergyChangePredictorPairSite {
+ public:
+  EnergyChangePredictorPairSite(const std::string &predictor_filename,
+                                const cfg::Config & /*reference_config*/,
+                                const std::set<std::string> &element_set)
+      : m_(predictor_filename, element_set, LMC_MODEL_PAIRSITE) {}
+  [[nodiscard]] double GetDeFromLatticeIdPair(const cfg::Config &config,
+                                              const std::pair<size_t, size_t> &lattice_id_jump_pair) const {
+    detail::One one(config.GetNumCells());
+    const int64_t p[2] = {static_cast<int64_t>(lattice_id_jump_pair.first),
+                          static_cast<int64_t>(lattice_id_jump_pair.second)};
+    one.idx.Upload(p, sizeof(p));
+    lmc::Check(lmc_swap_de(m_.get(), config.DeviceOccupancy(), one.off.as<int64_t>(),
+                           one.ncell.as<int32_t>(), one.idx.as<int64_t>(), 1, one.out.as<double>(),
+                           nullptr));
+    return Result(one);
+  }
+  [[nodiscard]] double GetDeFromLatticeIdSite(const cfg::Config &config, size_t lattice_id,
+                                              const std::string &new_element) const {
+    detail::One one(config.GetNumCells());
+    const int64_t s = static_cast<int64_t>(lattice_id);
+    const uint8_t c = config.CodeOf(new_element);
+    one.idx.Upload(&s, sizeof(s));
+    one.code.Upload(&c, 1);
+    lmc::Check(lmc_site_de(m_.get(), config.DeviceOccupancy(), one.off.as<int64_t>(),
+                           one.ncell.as<int32_t>(), one.idx.as<int64_t>(), one.code.as<uint8_t>(), 1,
+                           one.out.as<double>(), nullptr));
+    return Result(one);
+  }
+  [[nodiscard]] lmc_model *handle() const { return m_.get(); }
+
+ private:
+  double Result(const detail::One &one) const {
+    double de = 0;
+    one.out.Download(&de, sizeof(de));
+    lmc::Check(lmc_check_errors(m_.get()));
+    return de;
+  }
+  lmc::ModelHandle m_;
+};
+
+}  // namespace pred
+
+namespace mc {
+
+// KineticMcFirstOmp for R independent replicas advanced by one persistent kernel. Per-replica
+// parameters follow the reference constructor (temperature, restart_*, maximum_steps,
+// rate_corrector, early_stop, solute_disp); Simulate() runs every replica to maximum_steps.
+class KineticMcFirstBatch {
+ public:
+  KineticMcFirstBatch(const std::vector<cfg::Config> &configs, const std::vector<lmc_kmc_replica> &params,
+                      const pred::VacancyMigrationPredictor &barrier, const pred::EnergyPredictor *energy) {
+    if (configs.empty() || configs.size() != params.size())
+      throw std::invalid_argument("one parameter set per configuration");
+    std::vector<int32_t> ncell;
+    std::vector<uint8_t> occ;
+    for (const auto &c : configs) {
+      ncell.push_back(c.GetNumCells());
+      occ.insert(occ.end(), c.GetCodes().begin(), c.GetCodes().end());
+    }
+    lmc_kmc *k = nullptr;
+    lmc::Check(lmc_kmc_create(barrier.handle(), energy ? energy->handle() : nullptr,
+                              static_cast<int32_t>(configs.size()), ncell.data(), occ.data(),
+                              params.data(), &k));
+    k_.reset(k);
+    R_ = configs.size();
+    for (const auto &p : params) max_steps_ = std::max<uint64_t>(max_steps_, p.maximum_steps);
+  }
+  void SetTimeTemperature(const std::vector<double> &time, const std::vector<double> &temperature) {
+    lmc::Check(lmc_kmc_set_schedule(k_.get(), static_cast<int32_t>(time.size()), time.data(),
+                                    temperature.data()));
+  }
+  int Autotune(int steps = 200) {
+    int32_t chosen = 0;
+    double a = 0, b = 0;
+    lmc::Check(lmc_kmc_autotune(k_.get(), steps, &chosen, &a, &b));
+    return chosen;
+  }
+  void Run(uint64_t steps, void *stream = nullptr) { lmc::Check(lmc_kmc_run(k_.get(), steps, stream)); }
+  void Simulate() { Run(max_steps_ + 1); }
+  struct State {
+    std::vector<uint64_t> steps;
+    std::vector<double> time, energy, absolute_energy, temperature;
+    std::vector<int64_t> vacancy;
+  };
+  [[nodiscard]] State GetState() const {
+    State s;
+    s.steps.resize(R_);
+    s.time.resize(R_);
+    s.energy.resize(R_);
+    s.absolute_energy.resize(R_);
+    s.temperature.resize(R_);
+    s.vacancy.resize(R_);
+    lmc::Check(lmc_kmc_state(k_.get(), s.steps.data(), s.time.data(), s.energy.data(),
+                             s.absolute_energy.data(), s.temperature.data(), s.vacancy.data(),
+                             nullptr, nullptr, nullptr));
+    return s;
+  }
+  [[nodiscard]] std::vector<uint8_t> GetOccupancy(size_t total_sites) const {
+    std::vector<uint8_t> occ(total_sites);
+    lmc::Check(lmc_kmc_occupancy(k_.get(), occ.data()));
+    return occ;
+  }
+
+ private:
+  struct Free {
+    void operator()(lmc_kmc *k) const { lmc_kmc_free(k); }
+  };
+  std::unique_ptr<lmc_kmc, Free> k_;
+  size_t R_ = 0;
+  uint64_t max_steps_ = 0;
+};
+
+// CanonicalMcSerial for R independent replicas (swap trials, Metropolis acceptance).
+class CanonicalMcBatch {
+ public:
+  CanonicalMcBatch(const std::vector<cfg::Config> &configs, const std::vector<lmc_cmc_replica> &params,
+                   const pred::EnergyChangePredictorPairSite &pairsite,
+                   const pred::EnergyPredictor *energy) {
+    if (configs.empty() || configs.size() != params.size())
+      throw std::invalid_argument("one parameter set per configuration");
+    std::vector<int32_t> ncell;
+    std::vector<uint8_t> occ;
+    for (const auto &c : configs) {
+      ncell.push_back(c.GetNumCells());
+      occ.insert(occ.end(), c.GetCodes().begin(), c.GetCodes().end());
+    }
+    lmc_cmc *c = nullptr;
+    lmc::Check(lmc_cmc_create(pairsite.handle(), energy ? energy->handle() : nullptr,
+                              static_cast<int32_t>(configs.size()), ncell.data(), occ.data(),
+                              params.data(), &c));
+    c_.reset(c);
+    R_ = configs.size();
+  }
+  void Run(uint64_t trials, void *stream = nullptr) { lmc::Check(lmc_cmc_run(c_.get(), trials, stream)); }
+  [[nodiscard]] std::vector<double> GetEnergy() const {
+    std::vector<double> e(R_);
+    lmc::Check(lmc_cmc_state(c_.get(), nullptr, e.data(), nullptr, nullptr));
+    return e;
+  }
+  [[nodiscard]] std::vector<uint64_t> GetAccepted() const {
+    std::vector<uint64_t> a(R_);
+    lmc::Check(lmc_cmc_state(c_.get(), nullptr, nullptr, nullptr, a.data()));
+    return a;
+  }
+
+ private:
+  struct Free {
+    void operator()(lmc_cmc *c) const { lmc_cmc_free(c); }
+  };
+  std::unique_ptr<lmc_cmc, Free> c_;
+  size_t R_ = 0;
+};
+
+}  // namespace mc

@@ -40,6 +40,16 @@ typedef struct lmc_cmc lmc_cmc;
 const char *lmc_last_error(void);
 int lmc_version(void);
 
+/* ---- Device buffers -------------------------------------------------------------------------
+ * For host code that does not link the HIP runtime itself (the C++ mirror in lmc.hpp): plain
+ * device allocations and synchronous copies. No reference counterpart: the reference keeps its
+ * configuration in host memory (cfg::Config, lmc/cfg/include/Config.h:99-107). */
+int lmc_device_alloc(uint64_t bytes, void **d_ptr);
+int lmc_device_free(void *d_ptr);
+int lmc_copy_to_device(void *d_dst, const void *src, uint64_t bytes);
+int lmc_copy_to_host(void *dst, const void *d_src, uint64_t bytes);
+int lmc_synchronize(void *stream);
+
 /* ---- Models --------------------------------------------------------------------------------
  * Replaces the predictor constructors: EnergyPredictor(json, element_set)
  * (lmc/pred/include/EnergyPredictor.h:16-17), VacancyMigrationPredictorE0(json, reference_config,
@@ -56,9 +66,9 @@ int32_t lmc_model_num_species(const lmc_model *m);
 /* Species code of an element string, -1 if not in the model. */
 int32_t lmc_model_species_code(const lmc_model *m, const char *element);
 /* Formulation of the jump kernels used by this model's stateless calls and as the default of
- * new KMC drivers: 0 = dense (every cluster of every jump), 1 = solvent baseline (all-solvent
- * sums precomputed per (solvent, migrating species); only clusters touching a non-solvent site
- * are evaluated; default). Both are exact; they differ in FP64 summation order. */
+ * new KMC drivers: 0 = dense (every cluster of every jump; default), 1 = solvent baseline
+ * (all-solvent sums precomputed per (solvent, migrating species); only clusters touching a
+ * non-solvent site are evaluated). Both are exact; they differ in FP64 summation order. */
 int lmc_model_set_formulation(lmc_model *m, int32_t formulation);
 /* Element string of a species code (NULL if out of range); valid while m lives. */
 const char *lmc_model_species_name(const lmc_model *m, int32_t code);

@@ -44,6 +44,11 @@ class CmcReplica(C.Structure):
 SIGNATURES = {
     "lmc_last_error": (C.c_char_p, []),
     "lmc_version": (C.c_int, []),
+    "lmc_device_alloc": (C.c_int, [_u64, C.POINTER(_vp)]),
+    "lmc_device_free": (C.c_int, [_vp]),
+    "lmc_copy_to_device": (C.c_int, [_vp, _vp, _u64]),
+    "lmc_copy_to_host": (C.c_int, [_vp, _vp, _u64]),
+    "lmc_synchronize": (C.c_int, [_vp]),
     "lmc_model_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_char_p), _i32, _i32,
                                  C.POINTER(_vp)]),
     "lmc_model_free": (None, [_vp]),

@@ -430,7 +430,7 @@ struct lmc_kmc {
   lmc::KmcDev d;
   uint64_t trace_pos = 0;
   int64_t total_sites = 0;
-  int formulation = 1;
+  int formulation = 0;
   ~lmc_kmc() {
     DevFree(d.occ);
     DevFree(d.off);
@@ -491,6 +491,41 @@ extern "C" {
 
 const char *lmc_last_error(void) { return g_error.c_str(); }
 int lmc_version(void) { return 1; }
+
+int lmc_device_alloc(uint64_t bytes, void **d_ptr) {
+  return Guard([&] {
+    if (!d_ptr) throw std::invalid_argument("lmc_device_alloc: null argument");
+    *d_ptr = nullptr;
+    if (bytes) Check(hipMalloc(d_ptr, bytes), "hipMalloc");
+    return LMC_OK;
+  });
+}
+int lmc_device_free(void *d_ptr) {
+  return Guard([&] {
+    if (d_ptr) Check(hipFree(d_ptr), "hipFree");
+    return LMC_OK;
+  });
+}
+int lmc_copy_to_device(void *d_dst, const void *src, uint64_t bytes) {
+  return Guard([&] {
+    if (bytes && (!d_dst || !src)) throw std::invalid_argument("lmc_copy_to_device: null argument");
+    if (bytes) Check(hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    return LMC_OK;
+  });
+}
+int lmc_copy_to_host(void *dst, const void *d_src, uint64_t bytes) {
+  return Guard([&] {
+    if (bytes && (!dst || !d_src)) throw std::invalid_argument("lmc_copy_to_host: null argument");
+    if (bytes) Check(hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    return LMC_OK;
+  });
+}
+int lmc_synchronize(void *stream) {
+  return Guard([&] {
+    Check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return LMC_OK;
+  });
+}
 
 int lmc_model_load(const char *json_path, const char *const *elements, int32_t n_elements,
                    int32_t kind, lmc_model **out) {

@@ -42,7 +42,7 @@ struct Model {
 
   DeviceTables *dev = nullptr;        // lazily uploaded
   int32_t *dev_err = nullptr;         // device flag: unknown cluster / missing parameters
-  int formulation = 1;                // jump kernels: 0 dense, 1 solvent baseline
+  int formulation = 0;                // jump kernels: 0 dense, 1 solvent baseline
   ~Model();
 };
 

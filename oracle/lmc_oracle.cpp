@@ -11,6 +11,20 @@
 //  * the total rate is summed in event-index order instead of an OpenMP reduction
 //    (lmc/mc/src/KineticMcFirstOmp.cpp:55-68), and every dot product is summed left to right
 //    instead of Eigen's vectorised order.
+//
+// What pins each part (tests/test_oracle_pins.py; the reference ships no golden vectors):
+//  * Philox4x32-10: the published known-answer vectors;
+//  * lattice ids and neighbour shells: closed forms and brute-force distance enumeration;
+//  * cluster tables: the counts the reference states (EnergyUtility.cpp:751) and brute force;
+//  * GetDe / swap and site dE: equal to differences of the brute-force total CE energy;
+//  * E0 and Quartic barriers: the KRA identity Ea(i->j) - Ea(j->i) = dE, special cases with
+//    closed-form orbit averages, and the quartic closed form;
+//  * KMC/CMC drivers: invariants and the event-choice statistics.
+// Parity unpinned: the ORDER of the MMM/MM2 orbit features (GetAverageClusterParametersMapping*,
+// EnergyUtility.cpp:169-259). Their grouping and lengths are pinned; the order follows a reading
+// of IsClusterSmallerSymmetrically* (LatticeCluster.hpp:211-234) that no reference output
+// checks, so coefficient files trained by the reference may index the features differently.
+// The LRU key hash (boost::hash_combine) is not restated; it does not change results.
 #include "lmc_oracle.h"
 
 #include <algorithm>

@@ -1,0 +1,100 @@
+// Host program written against the C++ mirror (include/lmc.hpp) the way reference host code
+// uses pred::/mc:: objects. Driven by tests/test_cpp_mirror.py.
+//
+//   mirror_check cpu <json>                                   host-only checks, prints "ok"
+//   mirror_check gpu <n> <occ.bin> <e0.json> <pairsite.json>  one JSON line per result
+#include <cstdio>
+#include <fstream>
+#include <iterator>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "lmc.hpp"
+
+namespace {
+
+int Fail(const char *what) {
+  std::printf("FAIL %s\n", what);
+  return 1;
+}
+
+int CpuChecks(const std::string &json) {
+  const std::set<std::string> els{"Al", "Mg", "Zn"};
+  cfg::Config c = cfg::GenerateFCC(5, els, "Al");
+  if (c.GetNumAtoms() != 500) return Fail("site count");
+  if (c.GetNeighborLatticeIdVector(7, 1).size() != 12) return Fail("1NN count");
+  if (c.GetNeighborLatticeIdVector(7, 2).size() != 6) return Fail("2NN count");
+  if (c.GetNeighborLatticeIdVector(7, 3).size() != 24) return Fail("3NN count");
+  try {
+    (void)c.GetVacancyLatticeId();
+    return Fail("missing vacancy accepted");
+  } catch (const std::runtime_error &e) {
+    if (std::string(e.what()) != "vacancy not found") return Fail("vacancy message");
+  }
+  c.SetAtomElementTypeAtLattice(3, "X");
+  c.SetAtomElementTypeAtLattice(4, "Unobtainium");  // unknown strings are X (Element.hpp:25-26)
+  if (c.GetElementAtLatticeId(4) != "X" || c.GetVacancyLatticeId() != 3) return Fail("elements");
+  c.LatticeJump({3, 9});
+  if (c.GetVacancyLatticeId() != 4 || c.GetElementAtLatticeId(9) != "X") return Fail("jump");
+  try {
+    pred::EnergyPredictor p("/nonexistent/coefficients.json", els);
+    return Fail("missing file accepted");
+  } catch (const std::runtime_error &e) {
+    if (std::string(e.what()).find("Cannot open") == std::string::npos) return Fail("io message");
+  }
+  pred::VacancyMigrationPredictorE0 e0(json, c, els);  // loads and derives tables on the host
+  (void)e0;
+  std::printf("ok\n");
+  return 0;
+}
+
+int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const std::string &ps_json) {
+  const std::set<std::string> els{"Al", "Mg", "Zn"};
+  std::ifstream f(occ_path, std::ios::binary);
+  std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  cfg::Config c = cfg::GenerateFCC(n, els, "Al");
+  if (raw.size() != c.GetNumAtoms()) return Fail("occupancy size");
+  for (size_t i = 0; i < raw.size(); ++i)
+    c.SetAtomElementTypeAtLattice(i, c.GetSpecies().at(static_cast<uint8_t>(raw[i])));
+  const pred::VacancyMigrationPredictorE0 barrier(e0_json, c, els);
+  const pred::EnergyPredictor energy(e0_json, els);
+  const pred::EnergyChangePredictorPairSite pairsite(ps_json, c, els);
+  const size_t v = c.GetVacancyLatticeId();
+  for (size_t nb : c.GetNeighborLatticeIdVector(v, 1)) {
+    const auto [ea, de] = barrier.GetBarrierAndDiffFromLatticeIdPair(c, {v, nb});
+    std::printf("{\"jump\": [%zu, %zu], \"ea\": %.17g, \"de\": %.17g}\n", v, nb, ea, de);
+  }
+  std::printf("{\"energy\": %.17g}\n", energy.GetEnergy(c));
+  const size_t p = (v + 11) % c.GetNumAtoms(), q = (v + 137) % c.GetNumAtoms();
+  std::printf("{\"swap\": [%zu, %zu], \"de\": %.17g}\n", p, q,
+              pairsite.GetDeFromLatticeIdPair(c, {p, q}));
+  std::printf("{\"site\": %zu, \"element\": \"Mg\", \"de\": %.17g}\n", p,
+              pairsite.GetDeFromLatticeIdSite(c, p, "Mg"));
+  lmc_kmc_replica prm{};
+  prm.temperature = 600.0;
+  prm.seed = 99;
+  prm.replica_id = 0;
+  prm.maximum_steps = 200;
+  mc::KineticMcFirstBatch kmc({c}, {prm}, barrier, &energy);
+  kmc.Simulate();
+  const auto s = kmc.GetState();
+  std::printf("{\"kmc_steps\": %llu, \"kmc_time\": %.17g, \"kmc_vacancy\": %lld}\n",
+              static_cast<unsigned long long>(s.steps[0]), s.time[0], static_cast<long long>(s.vacancy[0]));
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  try {
+    const std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "cpu" && argc == 3) return CpuChecks(argv[2]);
+    if (mode == "gpu" && argc == 6) return GpuRun(std::stoi(argv[2]), argv[3], argv[4], argv[5]);
+    std::printf("usage: mirror_check cpu <json> | gpu <n> <occ.bin> <e0.json> <pairsite.json>\n");
+    return 2;
+  } catch (const std::exception &e) {
+    std::printf("FAIL exception: %s\n", e.what());
+    return 1;
+  }
+}

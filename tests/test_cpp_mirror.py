@@ -1,0 +1,66 @@
+"""The C++ mirror (include/lmc.hpp) compiled into a host program with g++ and linked against
+liblmc.so: host-side behaviour on the CPU, and its GPU results against the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, TERNARY, random_alloc
+
+LIBDIR = os.path.join(ROOT, "latticemontecarlo_amd")
+
+
+@pytest.fixture(scope="module")
+def exe(tmp_path_factory):
+    out = tmp_path_factory.mktemp("cpp") / "mirror_check"
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "mirror_check.cpp"), "-L", LIBDIR, "-llmc",
+           f"-Wl,-rpath,{LIBDIR}", "-o", str(out)]
+    subprocess.run(cmd, check=True)
+    return str(out)
+
+
+def test_mirror_host_behaviour(exe, coeffs):
+    path, _ = coeffs["e0_3"]
+    r = subprocess.run([exe, "cpu", path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
+    import oracle as O
+    path, els = coeffs["e0_3n"]
+    n = 6
+    species = sorted(TERNARY + ["X"])
+    occ = random_alloc(n, species, 0.1, seed=31)
+    occ_file = tmp_path / "occ.bin"
+    occ.astype(np.uint8).tofile(occ_file)
+    print(f"mirror_check gpu n={n} seed=31", flush=True)
+    r = subprocess.run([exe, "gpu", str(n), str(occ_file), path, path], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [json.loads(line) for line in r.stdout.splitlines()]
+    lat = O.Lattice(n)
+    ob = O.Model(lat, path, els, O.E0)
+    oe = O.Model(lat, path, els, O.ENERGY)
+    op = O.Model(lat, path, els, O.PAIRSITE)
+    jumps = [x for x in rows if "jump" in x]
+    assert len(jumps) == 12
+    for x in jumps:
+        ea, de = ob.barrier_de(occ, *x["jump"])
+        assert abs(x["ea"] - ea) <= 1e-9 and abs(x["de"] - de) <= 1e-9
+    e = next(x for x in rows if "energy" in x)["energy"]
+    assert abs(e - oe.total_energy(occ)) <= 1e-9 * max(1.0, abs(e))
+    sw = next(x for x in rows if "swap" in x)
+    assert abs(sw["de"] - op.swap_de(occ, *sw["swap"])) <= 1e-9
+    si = next(x for x in rows if "site" in x)
+    assert abs(si["de"] - op.site_de(occ, si["site"], species.index("Mg"))) <= 1e-9
+    k = next(x for x in rows if "kmc_steps" in x)
+    ref = O.Kmc(ob, oe, lat, occ, 600.0, 99, replica=0)
+    ref.run(201, maximum_steps=200)
+    st = ref.state()
+    assert k["kmc_steps"] == st["steps"] == 201
+    assert k["kmc_vacancy"] == st["vacancy"]
+    assert abs(k["kmc_time"] - st["time"]) <= 1e-12 * abs(st["time"])
