@@ -98,6 +98,15 @@ int32_t lmc_fcc_neighbors(int32_t n, int64_t site, int32_t shell, int64_t *out);
 /* Cluster tables of the jump v -> nb as lattice ids (for validation):
  * state clusters as (label, s0, s1, s2) and MMM/MM2 orbit clusters as (orbit, sym, s0, s1). */
 int32_t lmc_jump_state_clusters(int32_t n, int64_t v, int64_t nb, int64_t *out, int32_t cap);
+/* Lattice ids of the 60 environment slots of the jump v -> nb in the engine's slot order (the
+ * order of surrogate input rows); GetSortedLatticeVectorStateOfPair (EnergyUtility.cpp:261-287)
+ * gives the same site set. Returns 60 or a negative status. */
+int32_t lmc_jump_environment(int32_t n, int64_t v, int64_t nb, int64_t *out60);
+/* The four symmetries of a jump environment that fix both endpoints (identity and the three
+ * mirror/rotation operations of the bond's C2v group), as slot permutations perm[4][60]: the
+ * environment with species e'[j] = e[perm[g][j]] is the image of e under operation g and has
+ * the same E_kra and dE. No reference counterpart (used to augment surrogate training data). */
+int lmc_jump_symmetries(int32_t *perm);
 int32_t lmc_jump_orbit_clusters(int32_t n, int64_t v, int64_t nb, int32_t which, int64_t *out,
                                 int32_t cap);
 
@@ -209,6 +218,69 @@ int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energ
                   uint64_t *accepted);
 int lmc_cmc_occupancy(lmc_cmc *c, uint8_t *occ);
 int lmc_cmc_trace(lmc_cmc *c, int64_t *p, int64_t *q, double *de, int8_t *accepted);
+
+/* ---- Learned surrogate of the jump evaluation ------------------------------------------
+ * No reference counterpart: an ensemble of BF16 MLPs (one-hot jump environment -> 128 -> 128
+ * -> (E_kra, dE), SiLU, FP32 accumulation on the matrix cores) that stands in for
+ * GetBarrierAndDiffFromLatticeIdPair (VacancyMigrationPredictorE0.cpp:153-159,
+ * VacancyMigrationPredictorQuartic.cpp:247-276), with the cluster-expansion predictor as its
+ * fallback and judge. Input rows are the 60 environment slots of a jump in the engine's fixed
+ * per-direction slot order (one species code per byte, rows of 64 bytes). Ea = max(0, E_kra +
+ * dE/2); for the quartic model E_kra = Ea - dE/2. Supports 3 or 4 elements (plus vacancy). */
+typedef struct lmc_surrogate lmc_surrogate;
+int lmc_surrogate_create(const lmc_model *barrier, int32_t n_members, uint64_t seed,
+                         lmc_surrogate **out);
+void lmc_surrogate_free(lmc_surrogate *s);
+/* Members, padded input width, parameters per member, channels per slot and their species. */
+int lmc_surrogate_layout(const lmc_surrogate *s, int32_t *n_members, int32_t *n_inputs,
+                         int32_t *n_params, int32_t *n_channels, int32_t *channel_codes4,
+                         int32_t *log_ekra);
+/* FP32 parameters [n_members][n_params] (layout W1[K][128] b1 W2[128][128] b2 W3[128][2] b3). */
+int lmc_surrogate_get_params(const lmc_surrogate *s, float *params);
+int lmc_surrogate_set_params(lmc_surrogate *s, const float *params);
+/* Output normalisation: dE = y1 * sigma[1] + mu[1]; E_kra = y0 * sigma[0] + mu[0] for a
+ * quartic model and exp(y0 * sigma[0] + mu[0]) for an E0 model (whose E_kra is the exponential
+ * of its cluster sums, E0 = exp(logit * sigma_e0 + mu_e0), VacancyMigrationPredictorE0.cpp:147). */
+int lmc_surrogate_set_normalization(lmc_surrogate *s, const double *mu2, const double *sigma2);
+int lmc_surrogate_get_normalization(const lmc_surrogate *s, double *mu2, double *sigma2);
+/* Exact training samples: for each replica, the 12 jumps in event order: environment rows
+ * d_env[12 R][64], E_kra and dE (the plain predictor, FP64). */
+int lmc_jump_samples(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                     const int32_t *d_ncell, const int64_t *d_vac, int32_t n_replicas,
+                     uint8_t *d_env, double *d_ekra, double *d_de, void *stream);
+/* Normalised outputs of every member for n_rows environment rows: d_out[n_rows][M][2]. */
+int lmc_surrogate_forward(const lmc_surrogate *s, const uint8_t *d_env, int32_t n_rows,
+                          float *d_out, void *stream);
+/* Gradient of each member's loss sum_rows,outputs (y - t)^2 / (2 n_rows) with respect to its
+ * parameters (d_grad[M][n_params]) for normalised targets d_target[n_rows][2]; host loss[M]
+ * (optional) receives the sum of squared errors. Synchronous. */
+int lmc_surrogate_gradients(lmc_surrogate *s, const uint8_t *d_env, const float *d_target,
+                            int32_t n_rows, float *d_grad, double *loss);
+/* Adam training of every member on (environment, E_kra, dE) samples: sets the normalisation
+ * from the samples if none is set, shuffles each epoch (Philox, shuffle_seed), cosine-decays
+ * the learning rate from lr to lr/20. Host mse[M] (optional): mean squared error per output of
+ * the last epoch, in normalised units. Synchronous. */
+int lmc_surrogate_train(lmc_surrogate *s, const uint8_t *d_env, const double *d_ekra,
+                        const double *d_de, int32_t n_rows, int32_t epochs, int32_t batch,
+                        double lr, uint64_t shuffle_seed, double *mse);
+/* The surrogate's evaluation of the 12 jumps of each replica, in event order, exactly as the
+ * KMC step makes it: Ea = max(0, E_kra + dE/2) from the member means; an event falls back to the
+ * model m (exact, d_fallback = 1) when its environment is out of distribution (a vacancy off the
+ * endpoints, more solute slots than any training row) or when the members disagree on Ea or dE
+ * by more than `threshold` eV (< 0: every event falls back) while the event's lowest member
+ * barrier lies within `window` kT (T = d_temp[r]) of the replica's lowest barrier (window <= 0:
+ * every event counts). Needs 4 members. Asynchronous. */
+int lmc_surrogate_eval_jumps(const lmc_surrogate *s, const lmc_model *m, const uint8_t *d_occ,
+                             const int64_t *d_offset, const int32_t *d_ncell,
+                             const int64_t *d_vac, const double *d_temp, int32_t n_replicas,
+                             double threshold, double window, int64_t *d_nb, double *d_ea,
+                             double *d_de, int8_t *d_fallback, void *stream);
+/* Use the surrogate (NULL: the cluster expansion again) in every later lmc_kmc_run of k, with
+ * the fallback rule of lmc_surrogate_eval_jumps. The selected jump's dE is always recomputed
+ * exactly, so energies stay those of the cluster expansion. */
+int lmc_kmc_set_surrogate(lmc_kmc *k, const lmc_surrogate *s, double threshold, double window);
+/* Events evaluated since the surrogate was attached, and how many of them fell back. */
+int lmc_kmc_surrogate_stats(lmc_kmc *k, uint64_t *events, uint64_t *fallbacks);
 
 #ifdef __cplusplus
 }

@@ -5,5 +5,6 @@ include/lmc_capi.h); this package is the Python mirror of the reference's predic
 interface over that ABI.
 """
 from .api import (E0, ENERGY, PAIRSITE, QUARTIC, CanonicalMc, KineticMcFirst, Model,  # noqa: F401
-                  ReplicaBatch, eval_jumps, fcc_neighbors, fcc_num_sites, jump_orbit_clusters,
-                  jump_state_clusters, site_de, swap_de, total_energy)
+                  ReplicaBatch, Surrogate, augment, eval_jumps, fcc_neighbors, fcc_num_sites,
+                  jump_environment, jump_orbit_clusters, jump_samples, jump_state_clusters, jump_symmetries,
+                  site_de, swap_de, total_energy)

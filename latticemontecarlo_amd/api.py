@@ -34,7 +34,10 @@ class Model:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().lmc_model_free(self.h)
+            try:
+                lib().lmc_model_free(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
     def code(self, element):
@@ -99,6 +102,35 @@ def jump_orbit_clusters(n, v, nb, which):
     if k < 0:
         check(k)
     return out[:k].copy()
+
+
+def jump_environment(n, v, nb):
+    """Lattice ids of the 60 environment slots of the jump v -> nb (surrogate input order)."""
+    out = np.zeros(60, np.int64)
+    k = lib().lmc_jump_environment(n, v, nb, ptr(out, capi._i64p))
+    if k < 0:
+        check(k)
+    return out
+
+
+def jump_symmetries():
+    """Slot permutations [4, 60] of the jump symmetries that fix both endpoints."""
+    out = np.zeros((4, 60), np.int32)
+    check(lib().lmc_jump_symmetries(ptr(out, capi._i32p)))
+    return out
+
+
+def augment(env, ekra, de):
+    """Training rows under the four endpoint-fixing symmetries (exact: E_kra and dE are
+    invariant); torch tensors on any device."""
+    torch = _torch()
+    perm = torch.from_numpy(jump_symmetries().astype(np.int64)).to(env.device)
+    rows = []
+    for g in range(4):
+        e = env.clone()
+        e[:, :60] = env[:, :60][:, perm[g]]
+        rows.append(e)
+    return torch.cat(rows), ekra.repeat(4), de.repeat(4)
 
 
 class ReplicaBatch:
@@ -193,7 +225,10 @@ class KineticMcFirst:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().lmc_kmc_free(self.h)
+            try:
+                lib().lmc_kmc_free(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
     def set_schedule(self, times, temps):
@@ -237,6 +272,33 @@ class KineticMcFirst:
         sizes = 4 * self.ncell.astype(np.int64) ** 3
         return np.split(out, np.cumsum(sizes)[:-1])
 
+    def set_surrogate(self, surrogate, threshold=0.01, window=20.0):
+        """Evaluate jumps with `surrogate` (None: the cluster expansion) from the next run on."""
+        check(lib().lmc_kmc_set_surrogate(self.h, surrogate.h if surrogate else None, threshold,
+                                          window))
+        self._surrogate = surrogate
+
+    def surrogate_stats(self):
+        ev, fb = C.c_uint64(), C.c_uint64()
+        check(lib().lmc_kmc_surrogate_stats(self.h, C.byref(ev), C.byref(fb)))
+        return dict(events=ev.value, fallbacks=fb.value)
+
+    def samples(self, model=None, stream=None):
+        """Exact surrogate training samples (env rows, E_kra, dE) of the 12 jumps of every
+        replica in its current state, evaluated on the GPU with `model` (default: the driver's
+        barrier model)."""
+        torch = _torch()
+        occ, off, nc, vac = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().lmc_kmc_device_views(self.h, C.byref(occ), C.byref(off), C.byref(nc),
+                                         C.byref(vac)))
+        m = model if model is not None else self._models[0]
+        env = torch.empty((12 * self.R, 64), dtype=torch.uint8, device="cuda")
+        ekra = torch.empty(12 * self.R, dtype=torch.float64, device="cuda")
+        de = torch.empty(12 * self.R, dtype=torch.float64, device="cuda")
+        check(lib().lmc_jump_samples(m.h, occ, off, nc, vac, self.R, dptr(env), dptr(ekra),
+                                     dptr(de), capi.stream_ptr(stream)))
+        return env, ekra, de
+
     def trace(self):
         n = self.R * self.trace_cap
         sel, ea, de, t = np.zeros(n, np.int64), np.zeros(n), np.zeros(n), np.zeros(n)
@@ -278,7 +340,10 @@ class CanonicalMc:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().lmc_cmc_free(self.h)
+            try:
+                lib().lmc_cmc_free(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
     def set_trace(self, capacity):
@@ -305,6 +370,33 @@ class CanonicalMc:
         sizes = 4 * self.ncell.astype(np.int64) ** 3
         return np.split(out, np.cumsum(sizes)[:-1])
 
+    def set_surrogate(self, surrogate, threshold=0.01, window=20.0):
+        """Evaluate jumps with `surrogate` (None: the cluster expansion) from the next run on."""
+        check(lib().lmc_kmc_set_surrogate(self.h, surrogate.h if surrogate else None, threshold,
+                                          window))
+        self._surrogate = surrogate
+
+    def surrogate_stats(self):
+        ev, fb = C.c_uint64(), C.c_uint64()
+        check(lib().lmc_kmc_surrogate_stats(self.h, C.byref(ev), C.byref(fb)))
+        return dict(events=ev.value, fallbacks=fb.value)
+
+    def samples(self, model=None, stream=None):
+        """Exact surrogate training samples (env rows, E_kra, dE) of the 12 jumps of every
+        replica in its current state, evaluated on the GPU with `model` (default: the driver's
+        barrier model)."""
+        torch = _torch()
+        occ, off, nc, vac = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().lmc_kmc_device_views(self.h, C.byref(occ), C.byref(off), C.byref(nc),
+                                         C.byref(vac)))
+        m = model if model is not None else self._models[0]
+        env = torch.empty((12 * self.R, 64), dtype=torch.uint8, device="cuda")
+        ekra = torch.empty(12 * self.R, dtype=torch.float64, device="cuda")
+        de = torch.empty(12 * self.R, dtype=torch.float64, device="cuda")
+        check(lib().lmc_jump_samples(m.h, occ, off, nc, vac, self.R, dptr(env), dptr(ekra),
+                                     dptr(de), capi.stream_ptr(stream)))
+        return env, ekra, de
+
     def trace(self):
         n = self.R * self.trace_cap
         p, q = np.zeros(n, np.int64), np.zeros(n, np.int64)
@@ -314,3 +406,123 @@ class CanonicalMc:
         shape = (self.R, self.trace_cap)
         return dict(p=p.reshape(shape), q=q.reshape(shape), de=de.reshape(shape),
                     acc=acc.reshape(shape))
+
+
+def jump_samples(model, batch, vac, stream=None):
+    """Exact training samples of the 12 jumps per replica (event order): environment rows
+    (uint8 [12R, 64]), E_kra and dE (float64 [12R])."""
+    torch = _torch()
+    dev = batch.occ.device
+    R = batch.R
+    vac_t = torch.as_tensor(np.asarray(vac, np.int64)).to(dev)
+    env = torch.empty((12 * R, 64), dtype=torch.uint8, device=dev)
+    ekra = torch.empty(12 * R, dtype=torch.float64, device=dev)
+    de = torch.empty(12 * R, dtype=torch.float64, device=dev)
+    check(lib().lmc_jump_samples(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
+                                 dptr(vac_t), R, dptr(env), dptr(ekra), dptr(de),
+                                 capi.stream_ptr(stream)))
+    return env, ekra, de
+
+
+class Surrogate:
+    """Ensemble of BF16 MLPs standing in for a barrier model's jump evaluation (see
+    lmc_surrogate_create); the barrier model stays the fallback and the judge."""
+
+    def __init__(self, barrier_model, n_members=4, seed=0):
+        h = C.c_void_p()
+        check(lib().lmc_surrogate_create(barrier_model.h, n_members, seed, C.byref(h)))
+        self.h = h
+        self.model = barrier_model
+        m, k, p, c, lg = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        codes = np.zeros(4, np.int32)
+        check(lib().lmc_surrogate_layout(self.h, C.byref(m), C.byref(k), C.byref(p), C.byref(c),
+                                         ptr(codes, capi._i32p), C.byref(lg)))
+        self.members, self.inputs, self.n_params, self.channels = m.value, k.value, p.value, c.value
+        self.log_ekra = bool(lg.value)
+        self.channel_codes = [int(x) for x in codes[:c.value]]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                lib().lmc_surrogate_free(self.h)
+            except Exception:  # interpreter shutdown
+                pass
+            self.h = None
+
+    def get_params(self):
+        out = np.zeros((self.members, self.n_params), np.float32)
+        check(lib().lmc_surrogate_get_params(self.h, ptr(out, capi._f32p)))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (self.members, self.n_params)
+        check(lib().lmc_surrogate_set_params(self.h, ptr(p, capi._f32p)))
+
+    def normalization(self):
+        mu, sg = np.zeros(2), np.zeros(2)
+        check(lib().lmc_surrogate_get_normalization(self.h, ptr(mu, capi._f64p),
+                                                    ptr(sg, capi._f64p)))
+        return mu, sg
+
+    def set_normalization(self, mu, sigma):
+        mu = np.ascontiguousarray(mu, np.float64)
+        sg = np.ascontiguousarray(sigma, np.float64)
+        check(lib().lmc_surrogate_set_normalization(self.h, ptr(mu, capi._f64p),
+                                                    ptr(sg, capi._f64p)))
+
+    def forward(self, env, stream=None):
+        """Normalised outputs [n, members, 2] for environment rows env (uint8 [n, 64])."""
+        torch = _torch()
+        n = env.shape[0]
+        out = torch.empty((n, self.members, 2), dtype=torch.float32, device=env.device)
+        check(lib().lmc_surrogate_forward(self.h, dptr(env), n, dptr(out),
+                                          capi.stream_ptr(stream)))
+        return out
+
+    def gradients(self, env, target):
+        """Per-member gradients [members, n_params] of sum (y - t)^2 / (2n) and the sums of
+        squared errors [members] for normalised targets (float32 [n, 2])."""
+        torch = _torch()
+        n = env.shape[0]
+        g = torch.empty((self.members, self.n_params), dtype=torch.float32, device=env.device)
+        loss = np.zeros(self.members)
+        t = target.to(torch.float32).contiguous()
+        check(lib().lmc_surrogate_gradients(self.h, dptr(env), dptr(t), n, dptr(g),
+                                            ptr(loss, capi._f64p)))
+        return g, loss
+
+    def train(self, env, ekra, de, epochs, batch=1024, lr=2e-3, seed=0):
+        """Adam training on samples; returns the last epoch's mean squared error per member."""
+        mse = np.zeros(self.members)
+        check(lib().lmc_surrogate_train(self.h, dptr(env), dptr(ekra), dptr(de), env.shape[0],
+                                        epochs, batch, lr, seed, ptr(mse, capi._f64p)))
+        return mse
+
+    def eval_jumps(self, model, batch, vac, temps, threshold=0.01, window=20.0, stream=None):
+        """The 12 jumps per replica as the surrogate KMC step evaluates them (event order):
+        nb, Ea, dE [R, 12] and the fallback flags [R, 12] (1 = exact cluster expansion)."""
+        torch = _torch()
+        dev = batch.occ.device
+        R = batch.R
+        vac_t = torch.as_tensor(np.asarray(vac, np.int64)).to(dev)
+        T = torch.as_tensor(np.broadcast_to(np.asarray(temps, np.float64), (R,)).copy()).to(dev)
+        nb = torch.empty((R, 12), dtype=torch.int64, device=dev)
+        ea = torch.empty((R, 12), dtype=torch.float64, device=dev)
+        de = torch.empty((R, 12), dtype=torch.float64, device=dev)
+        fl = torch.empty((R, 12), dtype=torch.int8, device=dev)
+        check(lib().lmc_surrogate_eval_jumps(self.h, model.h, dptr(batch.occ), dptr(batch.offset),
+                                             dptr(batch.ncell), dptr(vac_t), dptr(T), R,
+                                             threshold, window, dptr(nb), dptr(ea), dptr(de),
+                                             dptr(fl), capi.stream_ptr(stream)))
+        return nb, ea, de, fl
+
+    def predict(self, env, stream=None):
+        """Denormalised (E_kra, dE) of every member: float64 [n, members, 2]."""
+        y = self.forward(env, stream).double()
+        mu, sg = self.normalization()
+        torch = _torch()
+        out = y * torch.tensor(sg, device=y.device) + torch.tensor(mu, device=y.device)
+        if self.log_ekra:
+            out[..., 0] = torch.exp(out[..., 0])
+        return out

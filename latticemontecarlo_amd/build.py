@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblmc.so")
 
 SOURCES = ["lmc_tables.cpp", "lmc_model.cpp", "lmc_capi.cpp", "lmc_jump_kernels.hip",
-           "lmc_site_kernels.hip"]
+           "lmc_site_kernels.hip", "lmc_surrogate.hip", "lmc_surrogate_kmc.hip"]
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code

@@ -22,6 +22,7 @@ _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _u64p = C.POINTER(C.c_uint64)
 _f64p = C.POINTER(C.c_double)
+_f32p = C.POINTER(C.c_float)
 _u8p = C.POINTER(C.c_uint8)
 _i8p = C.POINTER(C.c_int8)
 
@@ -63,6 +64,8 @@ SIGNATURES = {
     "lmc_fcc_num_sites": (_i64, [_i32]),
     "lmc_fcc_neighbors": (_i32, [_i32, _i64, _i32, _i64p]),
     "lmc_jump_state_clusters": (_i32, [_i32, _i64, _i64, _i64p, _i32]),
+    "lmc_jump_environment": (_i32, [_i32, _i64, _i64, _i64p]),
+    "lmc_jump_symmetries": (C.c_int, [_i32p]),
     "lmc_jump_orbit_clusters": (_i32, [_i32, _i64, _i64, _i32, _i64p, _i32]),
     "lmc_eval_jumps": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "lmc_eval_pair": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _f64p, _f64p]),
@@ -92,6 +95,22 @@ SIGNATURES = {
     "lmc_cmc_state": (C.c_int, [_vp, _u64p, _f64p, _f64p, _u64p]),
     "lmc_cmc_occupancy": (C.c_int, [_vp, _u8p]),
     "lmc_cmc_trace": (C.c_int, [_vp, _i64p, _i64p, _f64p, _i8p]),
+    "lmc_surrogate_create": (C.c_int, [_vp, _i32, _u64, C.POINTER(_vp)]),
+    "lmc_surrogate_free": (None, [_vp]),
+    "lmc_surrogate_layout": (C.c_int, [_vp, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),
+    "lmc_surrogate_get_params": (C.c_int, [_vp, _f32p]),
+    "lmc_surrogate_set_params": (C.c_int, [_vp, _f32p]),
+    "lmc_surrogate_set_normalization": (C.c_int, [_vp, _f64p, _f64p]),
+    "lmc_surrogate_get_normalization": (C.c_int, [_vp, _f64p, _f64p]),
+    "lmc_jump_samples": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "lmc_surrogate_forward": (C.c_int, [_vp, _vp, _i32, _vp, _vp]),
+    "lmc_surrogate_gradients": (C.c_int, [_vp, _vp, _vp, _i32, _vp, _f64p]),
+    "lmc_surrogate_train": (C.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, C.c_double, _u64,
+                                      _f64p]),
+    "lmc_surrogate_eval_jumps": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, C.c_double,
+                                           C.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "lmc_kmc_set_surrogate": (C.c_int, [_vp, _vp, C.c_double, C.c_double]),
+    "lmc_kmc_surrogate_stats": (C.c_int, [_vp, _u64p, _u64p]),
 }
 
 _lib = None

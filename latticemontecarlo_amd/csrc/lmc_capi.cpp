@@ -18,6 +18,7 @@
 #include "lmc_device.h"
 #include "lmc_model.h"
 #include "lmc_rng.h"
+#include "lmc_surrogate.h"
 #include "lmc_tables.h"
 
 struct lmc_model {
@@ -431,7 +432,13 @@ struct lmc_kmc {
   uint64_t trace_pos = 0;
   int64_t total_sites = 0;
   int formulation = 0;
+  // surrogate evaluation (optional)
+  const lmc_surrogate *sur = nullptr;
+  double sur_threshold = 0.01, sur_window = 20.0;
+  uint64_t *n_sur = nullptr, *n_fb = nullptr;
   ~lmc_kmc() {
+    DevFree(n_sur);
+    DevFree(n_fb);
     DevFree(d.occ);
     DevFree(d.off);
     DevFree(d.ncell);
@@ -648,6 +655,34 @@ int32_t lmc_jump_state_clusters(int32_t n, int64_t v, int64_t nb, int64_t *out, 
     ++cnt;
   }
   return cnt;
+}
+
+int32_t lmc_jump_environment(int32_t n, int64_t v, int64_t nb, int64_t *out60) {
+  if (n < 4) return Fail(LMC_ERR_ARG, "n must be >= 4");
+  if (!out60) return Fail(LMC_ERR_ARG, "null output");
+  const int k = FindDirection(n, v, nb);
+  if (k < 0) return Fail(LMC_ERR_ARG, "sites are not first neighbours");
+  const auto &jt = lmc::GetJumpTables();
+  for (int j = 0; j < lmc::kEnvSlots; ++j) out60[j] = lmc::SiteAdd(n, v, jt.off[k][j]);
+  return lmc::kEnvSlots;
+}
+
+int lmc_jump_symmetries(int32_t *perm) {
+  return Guard([&] {
+    if (!perm) throw std::invalid_argument("null output");
+    const auto &jt = lmc::GetJumpTables();
+    const int sy[4] = {1, 1, -1, -1}, sz[4] = {1, -1, 1, -1};
+    for (int g = 0; g < 4; ++g)
+      for (int j = 0; j < lmc::kEnvSlots; ++j) {
+        const lmc::I3 want{jt.key[j].x, sy[g] * jt.key[j].y, sz[g] * jt.key[j].z};
+        int found = -1;
+        for (int u = 0; u < lmc::kEnvSlots; ++u)
+          if (jt.key[u].x == want.x && jt.key[u].y == want.y && jt.key[u].z == want.z) found = u;
+        if (found < 0) throw std::logic_error("environment is not symmetric");
+        perm[g * lmc::kEnvSlots + j] = found;
+      }
+    return LMC_OK;
+  });
 }
 
 int32_t lmc_jump_orbit_clusters(int32_t n, int64_t v, int64_t nb, int32_t which, int64_t *out,
@@ -989,12 +1024,17 @@ int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity) {
   });
 }
 
+static hipError_t RunSurrogateKmc(lmc_kmc *k, uint64_t steps, hipStream_t s);
+
 int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream) {
   return Guard([&] {
     if (!k) throw std::invalid_argument("null driver");
     const auto &t = lmc::Upload(*k->bm->m);
-    Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream), k->formulation == 1),
-          "KMC kernel");
+    if (k->sur)
+      Check(RunSurrogateKmc(k, steps, S(stream)), "surrogate KMC kernel");
+    else
+      Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream), k->formulation == 1),
+            "KMC kernel");
     k->trace_pos += steps;
     return LMC_OK;
   });
@@ -1200,3 +1240,440 @@ int lmc_cmc_trace(lmc_cmc *c, int64_t *p, int64_t *q, double *de, int8_t *accept
 }
 
 }  // extern "C"
+
+// ---- Learned surrogate --------------------------------------------------------------------
+
+struct lmc_surrogate {
+  int M = 0, C = 0, Kp = 0, S = 0, x_code = 0, base_code = 0;
+  int log_ekra = 0;  // E0 models: output 0 is ln E_kra (the model's own logit scale)
+  int v_slot = 0;
+  int max_solutes = lmc::kEnvUsed;  // most non-base, non-vacancy slots seen in training
+  lmc::SurDev Dev(double threshold, double window, uint64_t *n_sur, uint64_t *n_fb) const {
+    lmc::SurDev d{};
+    d.w = View();
+    for (int i = 0; i < 2; ++i) {
+      d.mu[i] = mu[i];
+      d.sigma[i] = sigma[i];
+    }
+    d.log_ekra = log_ekra;
+    d.base_code = base_code;
+    d.max_solutes = max_solutes;
+    d.threshold = threshold;
+    d.window = window;
+    d.n_sur = n_sur;
+    d.n_fb = n_fb;
+    return d;
+  }
+  int chan[4] = {-1, -1, -1, -1};
+  lmc::MlpLayout lay{};
+  double mu[2] = {0.0, 0.0}, sigma[2] = {0.0, 0.0};
+  float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  uint16_t *w1t = nullptr, *w2t = nullptr, *w2k = nullptr, *w3 = nullptr;
+  uint64_t adam_t = 0;
+  ~lmc_surrogate() {
+    DevFree(master);
+    DevFree(adam_m);
+    DevFree(adam_v);
+    DevFree(w1t);
+    DevFree(w2t);
+    DevFree(w2k);
+    DevFree(w3);
+  }
+  lmc::MlpDev View() const {
+    lmc::MlpDev v{};
+    v.M = M;
+    v.C = C;
+    v.Kp = Kp;
+    v.lay = lay;
+    for (int i = 0; i < 4; ++i) v.chan[i] = chan[i];
+    v.master = master;
+    v.w1t = w1t;
+    v.w2t = w2t;
+    v.w2k = w2k;
+    v.w3 = w3;
+    return v;
+  }
+  lmc::MlpMut Mut() {
+    lmc::MlpMut v{};
+    v.M = M;
+    v.C = C;
+    v.Kp = Kp;
+    v.lay = lay;
+    v.master = master;
+    v.adam_m = adam_m;
+    v.adam_v = adam_v;
+    v.w1t = w1t;
+    v.w2t = w2t;
+    v.w2k = w2k;
+    v.w3 = w3;
+    return v;
+  }
+  void RefreshCopies() {  // bf16 copies from the master parameters
+    const lmc::AdamStep a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    Check(lmc::LaunchMlpReduceAdam(Mut(), nullptr, 0, nullptr, 0, a, nullptr), "surrogate copy");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  }
+};
+
+namespace {
+
+void RequireSurrogate(const lmc_surrogate *s) {
+  if (!s || !s->master) throw std::invalid_argument("null surrogate");
+}
+
+// Uniform (-a, a) from Philox (seed, counter i, member, tag).
+float InitUniform(uint64_t seed, uint64_t i, uint32_t member, float a) {
+  const lmc::U4 u = lmc::Philox4x32_10(seed, i, member, 0x5EEDu);
+  const double x = (u.x >> 8) * (1.0 / 16777216.0);  // [0, 1)
+  return static_cast<float>((2.0 * x - 1.0) * a);
+}
+
+}  // namespace
+
+int lmc_surrogate_create(const lmc_model *barrier, int32_t n_members, uint64_t seed,
+                         lmc_surrogate **out) {
+  return Guard([&] {
+    RequireModel(barrier);
+    if (!out) throw std::invalid_argument("null output");
+    if (barrier->m->kind != LMC_MODEL_E0 && barrier->m->kind != LMC_MODEL_QUARTIC)
+      throw std::invalid_argument("the surrogate stands in for an E0 or Quartic model");
+    if (n_members < 1 || n_members > 8) throw std::invalid_argument("n_members must be in [1, 8]");
+    const Model &m = *barrier->m;
+    std::unique_ptr<lmc_surrogate> s(new lmc_surrogate);
+    s->M = n_members;
+    s->S = m.S;
+    s->log_ekra = m.kind == LMC_MODEL_E0 ? 1 : 0;
+    s->v_slot = lmc::GetJumpTables().v_slot;
+    s->x_code = m.x_code;
+    s->base_code = lmc::Upload(*barrier->m).solvent_default;
+    for (int c = 0; c < m.S; ++c)
+      if (c != s->x_code && c != s->base_code) {
+        if (s->C == 3) throw std::invalid_argument("the surrogate supports at most 4 elements");
+        s->chan[s->C++] = c;
+      }
+    if (s->C < 2) throw std::invalid_argument("the surrogate needs at least 3 elements");
+    s->Kp = (lmc::kEnvUsed * s->C + 31) / 32 * 32;
+    s->lay = lmc::MakeMlpLayout(s->Kp);
+    const lmc::MlpLayout &L = s->lay;
+    const size_t total = static_cast<size_t>(s->M) * L.P;
+    std::vector<float> p(total, 0.f);
+    for (int mm = 0; mm < s->M; ++mm) {
+      float *q = p.data() + static_cast<size_t>(mm) * L.P;
+      for (int k = 0; k < lmc::kEnvUsed * s->C; ++k)  // rows past 60 C stay zero
+        for (int n = 0; n < lmc::kHidden; ++n)
+          q[L.oW1 + k * lmc::kHidden + n] = InitUniform(seed, L.oW1 + k * lmc::kHidden + n, mm, 0.6f);
+      for (int i = 0; i < lmc::kHidden * lmc::kHidden; ++i)
+        q[L.oW2 + i] = InitUniform(seed, L.oW2 + i, mm, 0.153f);
+      for (int i = 0; i < lmc::kHidden * lmc::kMlpOut; ++i)
+        q[L.oW3 + i] = InitUniform(seed, L.oW3 + i, mm, 0.21f);
+    }
+    s->master = DevCopy(p);
+    s->adam_m = DevCopy(std::vector<float>(total, 0.f));
+    s->adam_v = DevCopy(std::vector<float>(total, 0.f));
+    s->w1t = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * s->Kp);
+    s->w2t = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kHidden);
+    s->w2k = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kHidden);
+    s->w3 = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kMlpOut);
+    s->RefreshCopies();
+    *out = s.release();
+    return LMC_OK;
+  });
+}
+
+void lmc_surrogate_free(lmc_surrogate *s) { delete s; }
+
+int lmc_surrogate_layout(const lmc_surrogate *s, int32_t *n_members, int32_t *n_inputs,
+                         int32_t *n_params, int32_t *n_channels, int32_t *channel_codes4,
+                         int32_t *log_ekra) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (log_ekra) *log_ekra = s->log_ekra;
+    if (n_members) *n_members = s->M;
+    if (n_inputs) *n_inputs = s->Kp;
+    if (n_params) *n_params = s->lay.P;
+    if (n_channels) *n_channels = s->C;
+    if (channel_codes4)
+      for (int i = 0; i < 4; ++i) channel_codes4[i] = s->chan[i];
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_get_params(const lmc_surrogate *s, float *params) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (!params) throw std::invalid_argument("null output");
+    ToHost(params, s->master, static_cast<size_t>(s->M) * s->lay.P);
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_set_params(lmc_surrogate *s, const float *params) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (!params) throw std::invalid_argument("null input");
+    Check(hipMemcpy(s->master, params, sizeof(float) * s->M * s->lay.P, hipMemcpyHostToDevice),
+          "hipMemcpy");
+    s->RefreshCopies();
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_set_normalization(lmc_surrogate *s, const double *mu2, const double *sigma2) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (!mu2 || !sigma2 || !(sigma2[0] > 0) || !(sigma2[1] > 0))
+      throw std::invalid_argument("normalisation needs positive sigmas");
+    for (int i = 0; i < 2; ++i) {
+      s->mu[i] = mu2[i];
+      s->sigma[i] = sigma2[i];
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_get_normalization(const lmc_surrogate *s, double *mu2, double *sigma2) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    for (int i = 0; i < 2; ++i) {
+      if (mu2) mu2[i] = s->mu[i];
+      if (sigma2) sigma2[i] = s->sigma[i];
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_jump_samples(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                     const int32_t *d_ncell, const int64_t *d_vac, int32_t n_replicas,
+                     uint8_t *d_env, double *d_ekra, double *d_de, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    if (m->m->kind != LMC_MODEL_E0 && m->m->kind != LMC_MODEL_QUARTIC)
+      throw std::invalid_argument("lmc_jump_samples needs an E0 or Quartic model");
+    if (n_replicas < 0 || !d_env || !d_ekra || !d_de) throw std::invalid_argument("bad arguments");
+    if (n_replicas == 0) return LMC_OK;
+    const auto &t = lmc::Upload(*m->m);
+    int64_t *nb = DevAlloc<int64_t>(12 * static_cast<size_t>(n_replicas));
+    double *ea = DevAlloc<double>(12 * static_cast<size_t>(n_replicas));
+    const hipError_t e = lmc::LaunchEvalJumps(t, d_occ, d_offset, d_ncell, d_vac, n_replicas, nb,
+                                              ea, d_de, m->m->dev_err, S(stream),
+                                              m->m->formulation == 1, d_env, d_ekra);
+    (void)hipStreamSynchronize(S(stream));
+    DevFree(nb);
+    DevFree(ea);
+    Check(e, "jump samples kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_forward(const lmc_surrogate *s, const uint8_t *d_env, int32_t n_rows,
+                          float *d_out, void *stream) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (n_rows < 0 || (n_rows && (!d_env || !d_out))) throw std::invalid_argument("bad arguments");
+    if (n_rows) Check(lmc::LaunchMlpForward(s->View(), d_env, n_rows, d_out, S(stream)), "forward");
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_gradients(lmc_surrogate *s, const uint8_t *d_env, const float *d_target,
+                            int32_t n_rows, float *d_grad, double *loss) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (n_rows <= 0 || !d_env || !d_target || !d_grad) throw std::invalid_argument("bad arguments");
+    const int tiles = (n_rows + lmc::MlpTileRows() - 1) / lmc::MlpTileRows();
+    float *part = DevAlloc<float>(static_cast<size_t>(tiles) * s->M * s->lay.P);
+    float *lp = DevAlloc<float>(static_cast<size_t>(tiles) * s->M);
+    hipError_t e = lmc::LaunchMlpTrain(s->View(), d_env, nullptr, 0, n_rows, d_target, part, lp, nullptr);
+    const lmc::AdamStep a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (e == hipSuccess) e = lmc::LaunchMlpReduceAdam(s->Mut(), part, tiles, d_grad, 0, a, nullptr);
+    std::vector<float> hl(static_cast<size_t>(tiles) * s->M);
+    if (e == hipSuccess) e = hipMemcpy(hl.data(), lp, hl.size() * sizeof(float), hipMemcpyDeviceToHost);
+    DevFree(part);
+    DevFree(lp);
+    Check(e, "surrogate gradients");
+    if (loss)
+      for (int mm = 0; mm < s->M; ++mm) {
+        double acc = 0.0;
+        for (int b = 0; b < tiles; ++b) acc += hl[static_cast<size_t>(b) * s->M + mm];
+        loss[mm] = acc;
+      }
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_train(lmc_surrogate *s, const uint8_t *d_env, const double *d_ekra,
+                        const double *d_de, int32_t n_rows, int32_t epochs, int32_t batch,
+                        double lr, uint64_t shuffle_seed, double *mse) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    if (n_rows <= 0 || epochs <= 0 || batch <= 0 || !d_env || !d_ekra || !d_de || !(lr > 0))
+      throw std::invalid_argument("bad arguments");
+    const size_t n = static_cast<size_t>(n_rows);
+    std::vector<double> ek(n), de(n);
+    ToHost(ek.data(), d_ekra, n);
+    ToHost(de.data(), d_de, n);
+    if (s->log_ekra)
+      for (double &x : ek) {
+        if (!(x > 0)) throw std::invalid_argument("E_kra samples of an E0 model must be positive");
+        x = std::log(x);
+      }
+    if (!(s->sigma[0] > 0) || !(s->sigma[1] > 0)) {  // normalisation from the samples
+      const std::vector<double> *v[2] = {&ek, &de};
+      for (int o = 0; o < 2; ++o) {
+        long double sum = 0, sq = 0;
+        for (double x : *v[o]) sum += x;
+        const double mean = static_cast<double>(sum / n);
+        for (double x : *v[o]) sq += (x - mean) * (x - mean);
+        s->mu[o] = mean;
+        s->sigma[o] = std::max(1e-6, static_cast<double>(std::sqrt(sq / n)));
+      }
+    }
+    {  // out-of-distribution bound: the most solute slots of any training row
+      std::vector<uint8_t> rows(n * lmc::kEnvRow);
+      ToHost(rows.data(), d_env, rows.size());
+      int most = 0;
+      for (size_t i = 0; i < n; ++i) {
+        int c = 0;
+        for (int j = 0; j < lmc::kEnvUsed; ++j)
+          c += (j != s->v_slot && rows[i * lmc::kEnvRow + j] != s->base_code) ? 1 : 0;
+        most = std::max(most, c);
+      }
+      s->max_solutes = most;
+    }
+    std::vector<float> tg(2 * n);
+    for (size_t i = 0; i < n; ++i) {
+      tg[2 * i] = static_cast<float>((ek[i] - s->mu[0]) / s->sigma[0]);
+      tg[2 * i + 1] = static_cast<float>((de[i] - s->mu[1]) / s->sigma[1]);
+    }
+    const int rows = lmc::MlpTileRows();
+    const int bsz = std::min<int>(batch, n_rows);
+    const int tiles_max = (bsz + rows - 1) / rows;
+    const int nbatch = (n_rows + bsz - 1) / bsz;
+    float *dt = DevCopy(tg);
+    int32_t *order = DevAlloc<int32_t>(n);
+    float *part = DevAlloc<float>(static_cast<size_t>(tiles_max) * s->M * s->lay.P);
+    float *lp = DevAlloc<float>(static_cast<size_t>(nbatch) * tiles_max * s->M);
+    std::vector<int32_t> perm(n);
+    std::vector<float> hl(static_cast<size_t>(nbatch) * tiles_max * s->M);
+    const double total_steps = static_cast<double>(epochs) * nbatch;
+    hipError_t e = hipSuccess;
+    double step = 0;
+    for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
+      for (size_t i = 0; i < n; ++i) perm[i] = static_cast<int32_t>(i);
+      for (size_t i = n - 1; i > 0; --i) {  // Fisher-Yates on Philox (shuffle_seed, epoch)
+        const lmc::U4 u = lmc::Philox4x32_10(shuffle_seed, i, static_cast<uint32_t>(ep), 0x5A5Au);
+        const uint64_t bits = (static_cast<uint64_t>(u.x) << 32) | u.y;
+        const size_t j = static_cast<size_t>((static_cast<unsigned __int128>(bits) * (i + 1)) >> 64);
+        std::swap(perm[i], perm[j]);
+      }
+      e = hipMemcpy(order, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemset(lp, 0, hl.size() * sizeof(float));
+      for (int b = 0; b < nbatch && e == hipSuccess; ++b, step += 1) {
+        const int row0 = b * bsz, nb = std::min(bsz, n_rows - row0);
+        e = lmc::LaunchMlpTrain(s->View(), d_env, order, row0, nb, dt, part,
+                                lp + static_cast<size_t>(b) * tiles_max * s->M, nullptr);
+        if (e != hipSuccess) break;
+        ++s->adam_t;
+        const double frac = step / std::max(1.0, total_steps - 1);
+        const double lr_t = lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * frac)));
+        const float b1 = 0.9f, b2 = 0.999f;
+        const lmc::AdamStep a{static_cast<float>(lr_t), b1, b2, 1e-8f,
+                              static_cast<float>(1.0 / (1.0 - std::pow(b1, s->adam_t))),
+                              static_cast<float>(1.0 / (1.0 - std::pow(b2, s->adam_t)))};
+        e = lmc::LaunchMlpReduceAdam(s->Mut(), part, (nb + rows - 1) / rows, nullptr, 1, a, nullptr);
+      }
+      if (e == hipSuccess && ep == epochs - 1)
+        e = hipMemcpy(hl.data(), lp, hl.size() * sizeof(float), hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    DevFree(dt);
+    DevFree(order);
+    DevFree(part);
+    DevFree(lp);
+    Check(e, "surrogate training");
+    if (mse)
+      for (int mm = 0; mm < s->M; ++mm) {
+        double acc = 0.0;
+        for (int b = 0; b < nbatch * tiles_max; ++b) acc += hl[static_cast<size_t>(b) * s->M + mm];
+        mse[mm] = acc / (2.0 * n_rows);
+      }
+    return LMC_OK;
+  });
+}
+
+static hipError_t RunSurrogateKmc(lmc_kmc *k, uint64_t steps, hipStream_t s) {
+  const auto &t = lmc::Upload(*k->bm->m);
+  const lmc::SurDev sd = k->sur->Dev(k->sur_threshold, k->sur_window, k->n_sur, k->n_fb);
+  const hipError_t e = lmc::LaunchSurrogateKmc(t, k->d, sd, steps, k->trace_pos, s);
+  return e;
+}
+
+int lmc_kmc_set_surrogate(lmc_kmc *k, const lmc_surrogate *s, double threshold, double window) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    if (s && s->M != 4) throw std::invalid_argument("the KMC surrogate path needs 4 members");
+    if (s && s->S != k->bm->m->S) throw std::invalid_argument("surrogate built for another model");
+    k->sur = s;
+    k->sur_threshold = threshold;
+    k->sur_window = window;
+    if (s && !k->n_sur) {
+      k->n_sur = DevCopy(std::vector<uint64_t>(k->d.R, 0));
+      k->n_fb = DevCopy(std::vector<uint64_t>(k->d.R, 0));
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_surrogate_stats(lmc_kmc *k, uint64_t *events, uint64_t *fallbacks) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t R = static_cast<size_t>(k->d.R);
+    std::vector<uint64_t> a(R, 0), b(R, 0);
+    if (k->n_sur) {
+      ToHost(a.data(), k->n_sur, R);
+      ToHost(b.data(), k->n_fb, R);
+    }
+    uint64_t sa = 0, sb = 0;
+    for (size_t i = 0; i < R; ++i) {
+      sa += a[i];
+      sb += b[i];
+    }
+    if (events) *events = sa + sb;
+    if (fallbacks) *fallbacks = sb;
+    return LMC_OK;
+  });
+}
+
+int lmc_surrogate_eval_jumps(const lmc_surrogate *s, const lmc_model *m, const uint8_t *d_occ,
+                             const int64_t *d_offset, const int32_t *d_ncell,
+                             const int64_t *d_vac, const double *d_temp, int32_t n_replicas,
+                             double threshold, double window, int64_t *d_nb, double *d_ea,
+                             double *d_de, int8_t *d_fallback, void *stream) {
+  return Guard([&] {
+    RequireSurrogate(s);
+    RequireModel(m);
+    if (s->M != 4) throw std::invalid_argument("the surrogate evaluation needs 4 members");
+    if (m->m->kind != LMC_MODEL_E0 && m->m->kind != LMC_MODEL_QUARTIC)
+      throw std::invalid_argument("the surrogate stands in for an E0 or Quartic model");
+    if (m->m->S != s->S) throw std::invalid_argument("surrogate built for another model");
+    if (n_replicas < 0 || (n_replicas && (!d_occ || !d_offset || !d_ncell || !d_vac || !d_temp ||
+                                           !d_nb || !d_ea || !d_de || !d_fallback)))
+      throw std::invalid_argument("bad arguments");
+    if (n_replicas == 0) return LMC_OK;
+    const auto &t = lmc::Upload(*m->m);
+    lmc::KmcDev kd{};
+    kd.R = n_replicas;
+    kd.occ = const_cast<uint8_t *>(d_occ);
+    kd.off = const_cast<int64_t *>(d_offset);
+    kd.ncell = const_cast<int32_t *>(d_ncell);
+    kd.vac = const_cast<int64_t *>(d_vac);
+    kd.temp = const_cast<double *>(d_temp);
+    kd.error = m->m->dev_err;
+    const lmc::SurOut out{d_nb, d_ea, d_de, d_fallback};
+    Check(lmc::LaunchSurrogateEval(t, kd, s->Dev(threshold, window, nullptr, nullptr), out,
+                                   S(stream)),
+          "surrogate evaluation kernel");
+    return LMC_OK;
+  });
+}

@@ -108,9 +108,12 @@ struct CmcDev {
   int8_t *tr_acc = nullptr;
 };
 
+// env_out (optional): [12 R][64] environment rows of the events in rank order; ekra_out
+// (optional): [12 R] kinetically resolved barrier (E0 model: E0; quartic: Ea - dE/2).
 hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *off,
                            const int32_t *ncell, const int64_t *vac, int32_t R, int64_t *nb,
-                           double *ea, double *de, int32_t *err, hipStream_t s, bool solvent);
+                           double *ea, double *de, int32_t *err, hipStream_t s, bool solvent,
+                           uint8_t *env_out = nullptr, double *ekra_out = nullptr);
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
                      hipStream_t s, bool solvent);
 hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off,

@@ -23,448 +23,10 @@
 // fit the workgroup's share of the CU's LDS, otherwise they are read from global memory.
 #include <hip/hip_runtime.h>
 
-#include "lmc_dev_common.h"
-#include "lmc_device.h"
-#include "lmc_rng.h"
+#include "lmc_jump_device.h"
 
 namespace lmc {
 namespace {
-
-constexpr int kLanes = 21;      // dense: lanes per jump event (12 * 21 = 252 <= 256)
-constexpr int kGroup = 8;       // solvent: lanes per non-solvent site
-constexpr int kPass = kThreads / kGroup;  // sites handled per pass
-constexpr int kEnvPad = 768;    // 12 x 60 environment entries padded to whole waves
-
-struct JumpLds {
-  int32_t off[12 * kEnvSlots];
-  double par[kMaxSpecies * 6];
-  int8_t tt[3 * kMaxSpecies * kMaxSpecies];
-  uint8_t env[12][64];
-  int64_t nb[12];   // by direction
-  // by rank (neighbour-id order)
-  double ea[12], de[12], rate[12];
-  int64_t snb[12];
-  int dir[12];
-  // per-event sums (dE, logit MMM, logit MM2) of the solvent path's earlier passes
-  double acc[12][3];
-  // random numbers of the current step (drawn by an idle lane during the gather)
-  double u2, log_u1, corr;
-  // events whose environment holds a vacancy off the jump endpoints (no one-hot type)
-  int bad;
-  // dense partials / solvent-baseline work list
-  union {
-    double part[12][kLanes][3];
-    struct {
-      uint16_t inc[kEnvPad];
-      unsigned long long mask[12];
-      int start[13];
-      int last_p0;
-      double part[kPass][kGroup][3];
-    } sp;
-  } u;
-};
-
-// Replica state of the persistent KMC kernel (lane 0 reads and writes it in phase 4).
-struct ReplicaLds {
-  uint64_t steps, max_steps, seed;
-  double time, energy, abse, T, beta, cv, cs, smass;
-  double com[3];
-  int64_t vac, disp[3];
-  uint32_t rid;
-  int flags, esc, solvent, done;
-};
-
-// Byte offsets of the tables in dynamic LDS (computed on the host).
-struct DynLayout {
-  int w, b1, b2, base, lists, offs, bytes;
-  int b1_lds, b2_lds;
-};
-
-struct Tabs {  // resolved table pointers (LDS or global)
-  const double *w, *b1, *b2, *base;
-  const uint32_t *state, *mmm, *mm2;  // dense lists
-  const uint32_t *inc;                // solvent-baseline entries
-  const uint16_t *inc_off;
-};
-
-template <bool kSolvent>
-__device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const DynLayout &lay) {
-  const int tid = threadIdx.x;
-  for (int i = tid; i < 12 * kEnvSlots; i += kThreads) L.off[i] = t.off[i];
-  for (int i = tid; i < 3 * t.S * t.S; i += kThreads) L.tt[i] = t.tt[i];
-  for (int i = tid; i < 6 * t.S; i += kThreads) L.par[i] = t.par[i];
-  if (tid < 12) L.env[tid][63] = 0;  // target of the absent sites of every packed cluster
-  if (tid == 0) L.bad = 0;
-  Tabs T{};
-  double *w = reinterpret_cast<double *>(dyn + lay.w);
-  const double *wsrc = kSolvent ? t.w : t.wd;
-  for (int i = tid; i < t.nw; i += kThreads) w[i] = wsrc[i];
-  T.w = w;
-  if (lay.b1_lds) {
-    double *p = reinterpret_cast<double *>(dyn + lay.b1);
-    for (int i = tid; i < t.ne * t.F1; i += kThreads) p[i] = t.b1[i];
-    T.b1 = p;
-  } else {
-    T.b1 = t.b1;
-  }
-  if (lay.b2_lds) {
-    double *p = reinterpret_cast<double *>(dyn + lay.b2);
-    for (int i = tid; i < t.ne * t.F2; i += kThreads) p[i] = t.b2[i];
-    T.b2 = p;
-  } else {
-    T.b2 = t.b2;
-  }
-  if (kSolvent) {
-    double *bs = reinterpret_cast<double *>(dyn + lay.base);
-    for (int i = tid; i < 3 * t.S * t.S; i += kThreads) bs[i] = t.base[i];
-    T.base = bs;
-    uint32_t *in = reinterpret_cast<uint32_t *>(dyn + lay.lists);
-    for (int i = tid; i < t.n_inc; i += kThreads) in[i] = t.inc[i];
-    T.inc = in;
-    uint16_t *io = reinterpret_cast<uint16_t *>(dyn + lay.offs);
-    for (int i = tid; i <= kEnvSlots; i += kThreads) io[i] = t.inc_off[i];
-    T.inc_off = io;
-  } else {
-    uint32_t *p = reinterpret_cast<uint32_t *>(dyn + lay.lists);
-    for (int i = tid; i < t.n_state; i += kThreads) p[i] = t.state[i];
-    T.state = p;
-    p += t.n_state;
-    for (int i = tid; i < t.n_mmm; i += kThreads) p[i] = t.mmm[i];
-    T.mmm = p;
-    p += t.n_mmm;
-    if (t.kind == kQuartic)
-      for (int i = tid; i < t.n_mm2; i += kThreads) p[i] = t.mm2[i];
-    T.mm2 = p;
-  }
-  return T;
-}
-
-// Phase 1: species of the 12 environments; neighbour ids of the 12 jumps. 720 sites over 256
-// threads: three independent loads per thread, issued together.
-__device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const uint8_t *occ, int n,
-                                          int64_t vac, int32_t *err) {
-  const XYZ c = ToXYZ(vac, n);
-  constexpr int kTotal = 12 * kEnvSlots;
-  int64_t s[3];
-  uint8_t v[3];
-#pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int i = threadIdx.x + u * kThreads;
-    if (i < kTotal) {
-      int dx, dy, dz;
-      Unpack(L.off[i], dx, dy, dz);
-      s[u] = SiteAt(c, n, dx, dy, dz);
-      v[u] = occ[s[u]];
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int i = threadIdx.x + u * kThreads;
-    if (i < kTotal) {
-      const int k = i / kEnvSlots, j = i - k * kEnvSlots;
-      uint8_t x = v[u];
-      if (x >= t.S) {  // corrupt occupancy: flag it and keep every table index in range
-        atomicOr(err, 4);
-        x = 0;
-      }
-      if (x == t.x_code && j != t.v_slot) atomicOr(&L.bad, 1 << k);
-      L.env[k][j] = x;
-      if (j == t.n_slot) L.nb[k] = s[u];
-    }
-  }
-}
-
-__device__ __forceinline__ double WIndex(const double *w, int label, int a, int c1, int c2, int S) {
-  if (label == 0) return w[a];
-  if (label < 4) return w[S + (label - 1) * S * S + a * S + c1];
-  return w[S + 3 * S * S + (label - 4) * S * S * S + (a * S + c1) * S + c2];
-}
-
-// Sums of kU packed state clusters per lane: w(cluster | endpoint = M) - w(cluster | endpoint =
-// X), negated for the neighbour endpoint's clusters; wm, wx: the weight table shifted by M and X
-// entries. The loads of the kU clusters are issued level by level (entries, species, weights).
-template <int kU>
-__device__ __forceinline__ void StateBatch(const char *wm, const char *wx, const uint8_t *e,
-                                           const uint32_t *list, int c, int s8, int ss8,
-                                           double *acc) {
-  uint32_t d[kU];
-  int c1[kU], c2[kU];
-  double p[kU], q[kU];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    c1[u] = e[(d[u] >> 13) & 63];
-    c2[u] = e[(d[u] >> 19) & 63];
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const int o = static_cast<int>(d[u] & 8191) + __umul24(c1[u], s8) + __umul24(c2[u], ss8);
-    p[u] = *reinterpret_cast<const double *>(wm + o);
-    q[u] = *reinterpret_cast<const double *>(wx + o);
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const double v = p[u] - q[u];
-    acc[u] += __hiloint2double(__double2hiint(v) ^ static_cast<int>(d[u] & 0x80000000u),
-                               __double2loint(v));
-  }
-}
-
-// Sums of kU packed orbit clusters per lane: b[base + one-hot type(kind, c0, c1)].
-template <int kU>
-__device__ __forceinline__ void OrbitBatch(const int8_t *tt, const double *bm, const uint8_t *e,
-                                           const uint32_t *list, int c, int S, double *acc) {
-  uint32_t d[kU];
-  int c0[kU], c1[kU], ty[kU];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    c0[u] = e[(d[u] >> 20) & 63];
-    c1[u] = e[d[u] >> 26];
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) ty[u] = tt[((d[u] >> 12) & 255) + __umul24(c0[u], S) + c1[u]];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) acc[u] += bm[(d[u] & 4095) + ty[u]];
-}
-
-// Phase 2, dense: per-lane partial sums of every cluster's weight (dE, MMM logit, MM2 logit).
-__device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T) {
-  const int tid = threadIdx.x;
-  if (tid >= 12 * kLanes) return;
-  const int k = tid / kLanes, lane = tid - k * kLanes;
-  const uint8_t *e = L.env[k];
-  const int S = t.S, M = e[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
-  const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
-  const int s8 = 8 * S, ss8 = 8 * S * S;
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
-  int c = lane;
-  for (; c + 3 * kLanes < t.n_state; c += 4 * kLanes) StateBatch<4>(wm, wx, e, T.state, c, s8, ss8, a);
-  for (; c < t.n_state; c += kLanes) StateBatch<1>(wm, wx, e, T.state, c, s8, ss8, a);
-  const double de = (a[0] + a[1]) + (a[2] + a[3]);
-  double la = 0.0, lb = 0.0;
-  // one-hot rank of the migrating species (a vacancy is flagged in the gather)
-  const int row = L.tt[M * S];
-  if (t.kind == kE0 || t.kind == kQuartic) {
-    const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-    a[0] = a[1] = a[2] = a[3] = 0.0;
-    c = lane;
-    for (; c + 3 * kLanes < t.n_mmm; c += 4 * kLanes) OrbitBatch<4>(L.tt, bm, e, T.mmm, c, S, a);
-    for (; c < t.n_mmm; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mmm, c, S, a);
-    la = (a[0] + a[1]) + (a[2] + a[3]);
-  }
-  if (t.kind == kQuartic) {  // forward and backward MM2 entries in one list
-    const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-    a[0] = a[1] = a[2] = a[3] = 0.0;
-    c = lane;
-    for (; c + 3 * kLanes < t.n_mm2; c += 4 * kLanes) OrbitBatch<4>(L.tt, bm, e, T.mm2, c, S, a);
-    for (; c < t.n_mm2; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mm2, c, S, a);
-    lb = (a[0] + a[1]) + (a[2] + a[3]);
-  }
-  L.u.part[k][lane][0] = de;
-  L.u.part[k][lane][1] = la;
-  L.u.part[k][lane][2] = lb;
-}
-
-// Change of one touched cluster relative to the all-solvent environment.
-__device__ __forceinline__ void SolventEntry(const JumpLds &L, const Tabs &T, const KTables &t,
-                                             const uint8_t *e, uint32_t d, int j, int cj, int sv,
-                                             int M, int row, double &de, double &la, double &lb) {
-  const int S = t.S;
-  const int type = d & 3, other = (d >> 6) & 63;
-  const int co = other == 63 ? sv : e[other];
-  if (other != 63 && other < j && co != sv) return;  // evaluated at the lower non-solvent slot
-  if (type == 0) {
-    const int label = (d >> 2) & 7;
-    const int X = t.x_code;
-    const double v = (WIndex(T.w, label, M, cj, co, S) - WIndex(T.w, label, X, cj, co, S)) -
-                     (WIndex(T.w, label, M, sv, sv, S) - WIndex(T.w, label, X, sv, sv, S));
-    de += (d & 32) ? -v : v;
-    return;
-  }
-  const int kind = (d >> 2) & 3;
-  const int base = d >> 12;
-  const bool second = (d & 32) != 0;
-  const int c0 = second ? co : cj, c1 = kind ? (second ? cj : co) : 0;
-  const int ty = L.tt[(kind * S + c0) * S + c1];
-  const int tp = L.tt[(kind * S + sv) * S + (kind ? sv : 0)];
-  if (type == 1) {
-    const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-    la += bm[base + ty] - bm[base + tp];
-  } else {
-    const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-    lb += bm[base + ty] - bm[base + tp];
-  }
-}
-
-// Phase 2, solvent baseline: list the environment entries (event k, slot j) that hold a
-// non-solvent species, then 8 lanes per entry add the changes of the clusters touching it. Each
-// lane leaves its partial sums in LDS; the event lanes add them in a fixed order (passes before
-// the last here, the last pass in FinishEvents).
-__device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int sv) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  bool flag[3];
-#pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int idx = tid + u * kThreads;
-    bool f = false;
-    if (idx < 12 * kEnvSlots) {
-      const int k = idx / kEnvSlots, j = idx - k * kEnvSlots;
-      f = j != t.v_slot && j != t.n_slot && L.env[k][j] != sv;
-    }
-    flag[u] = f;
-    const unsigned long long m = __ballot(f);
-    if (lane == 0) L.u.sp.mask[wave + 4 * u] = m;
-  }
-  if (tid < 12) L.acc[tid][0] = L.acc[tid][1] = L.acc[tid][2] = 0.0;
-  __syncthreads();
-  int cnt[12];
-#pragma unroll
-  for (int c = 0; c < 12; ++c) cnt[c] = __popcll(L.u.sp.mask[c]);
-  const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int u = 0; u < 3; ++u)
-    if (flag[u]) {
-      const int c = wave + 4 * u;
-      int pre = 0;
-#pragma unroll
-      for (int q = 0; q < 12; ++q) pre += q < c ? cnt[q] : 0;
-      L.u.sp.inc[pre + __popcll(L.u.sp.mask[c] & below)] = static_cast<uint16_t>(tid + u * kThreads);
-    }
-  if (tid <= 12) {  // first list position of event k's entries (k = 12: the total)
-    const int e0 = tid * kEnvSlots, c = e0 >> 6, b = e0 & 63;
-    int pre = 0;
-#pragma unroll
-    for (int q = 0; q < 12; ++q) pre += q < c ? cnt[q] : 0;
-    L.u.sp.start[tid] = pre + (c < 12 ? __popcll(L.u.sp.mask[c] & ((1ull << b) - 1ull)) : 0);
-  }
-  int total = 0;
-#pragma unroll
-  for (int q = 0; q < 12; ++q) total += cnt[q];
-  __syncthreads();
-  const int grp = tid / kGroup, g = tid - grp * kGroup;
-  int p0 = 0;
-  for (;; p0 += kPass) {
-    double de = 0.0, la = 0.0, lb = 0.0;
-    const int i = p0 + grp;
-    if (i < total) {
-      const int idx = L.u.sp.inc[i];
-      const int k = idx / kEnvSlots, j = idx - k * kEnvSlots;
-      const uint8_t *e = L.env[k];
-      const int M = e[t.n_slot];
-      const int row = L.tt[M * t.S];
-      const int cj = e[j];
-      const int q1 = T.inc_off[j + 1];
-      for (int q = T.inc_off[j] + g; q < q1; q += kGroup)
-        SolventEntry(L, T, t, e, T.inc[q], j, cj, sv, M, row, de, la, lb);
-    }
-    L.u.sp.part[grp][g][0] = de;
-    L.u.sp.part[grp][g][1] = la;
-    L.u.sp.part[grp][g][2] = lb;
-    if (p0 + kPass >= total) break;
-    __syncthreads();
-    if (tid < 12) {
-      const int lo = max(p0, L.u.sp.start[tid]), hi = min(p0 + kPass, L.u.sp.start[tid + 1]);
-      for (int q = lo; q < hi; ++q)
-        for (int h = 0; h < kGroup; ++h) {
-          L.acc[tid][0] += L.u.sp.part[q - p0][h][0];
-          L.acc[tid][1] += L.u.sp.part[q - p0][h][1];
-          L.acc[tid][2] += L.u.sp.part[q - p0][h][2];
-        }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) L.u.sp.last_p0 = p0;
-}
-
-// Per-event sums at the start of phase 3 (lanes k < 12, after the phase-2 barrier).
-__device__ __forceinline__ void SolventSums(const KTables &t, const JumpLds &L, const Tabs &T,
-                                            int k, int sv, double &de, double &la, double &lb) {
-  double a0 = L.acc[k][0], a1 = L.acc[k][1], a2 = L.acc[k][2];
-  const int p0 = L.u.sp.last_p0;
-  const int lo = max(p0, L.u.sp.start[k]), hi = min(p0 + kPass, L.u.sp.start[k + 1]);
-  for (int q = lo; q < hi; ++q)
-    for (int h = 0; h < kGroup; ++h) {
-      a0 += L.u.sp.part[q - p0][h][0];
-      a1 += L.u.sp.part[q - p0][h][1];
-      a2 += L.u.sp.part[q - p0][h][2];
-    }
-  // all-solvent sums of this (solvent, migrating species)
-  const int M = L.env[k][t.n_slot];
-  const double *b = T.base + (static_cast<size_t>(sv) * t.S + M) * 3;
-  de = a0 + b[0];
-  la = a1 + b[1];
-  lb = a2 + b[2];
-}
-
-// Phase 3: Ea, dE and rate per jump, stored at the jump's rank among the 12 neighbour ids.
-template <bool kSolvent>
-__device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const Tabs &T, int sv,
-                                             double beta, int32_t *err) {
-  const int k = threadIdx.x;
-  if (k >= 64) return;
-  double de, la, lb;
-  if (!kSolvent) {  // wave 0: lane 3q + c adds the kLanes partials of component c of event q
-    double s = 0.0;
-    if (k < 36) {
-      const int q = k / 3, c = k - 3 * q;
-#pragma unroll
-      for (int l = 0; l < kLanes; ++l) s += L.u.part[q][l][c];
-    }
-    const int q = k < 12 ? k : 0;
-    de = __shfl(s, 3 * q);
-    la = __shfl(s, 3 * q + 1);
-    lb = __shfl(s, 3 * q + 2);
-  }
-  if (k >= 12) return;
-  if (kSolvent) SolventSums(t, L, T, k, sv, de, la, lb);
-  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
-    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
-  const int M = L.env[k][t.n_slot];
-  const double *p = L.par + 6 * M;
-  double ea = 0.0;
-  if (t.kind == kE0) {
-    // E0 = exp(logit * sigma_e0 + mu_e0); Ea = max(0, E0 + dE/2) (...E0.cpp:147-158)
-    const double e0 = exp((p[0] + la) * p[2] + p[1]);
-    ea = fmax(0.0, e0 + de / 2);
-  } else if (t.kind == kQuartic) {
-    // Quartic barrier (...Quartic.cpp:266-275)
-    const double D = exp((p[0] + la) * p[2] + p[1]);
-    const double Ks = exp((p[3] + lb) * p[5] + p[4]);
-    const double b = 4 * de / (D * D * D);
-    const double a = Ks / (4 * D * D);
-    const double c = (9 * b * b - 16 * a * a * D * D) / (32 * a);
-    const double delta = sqrt(fabs(9 * b * b - 32 * a * c));
-    ea = (3 * b + delta) * (3 * b + delta) * (3 * b * b - 16 * a * c + b * delta) / (a * a * a) /
-         2048;
-  }
-  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);  // unknown cluster type
-  int r = 0;
-  const int64_t mine = L.nb[k];
-#pragma unroll
-  for (int j = 0; j < 12; ++j) r += L.nb[j] < mine;
-  L.ea[r] = ea;
-  L.de[r] = de;
-  L.rate[r] = exp(-ea * beta);  // JumpEvent forward rate (JumpEvent.cpp:6-13)
-  L.snb[r] = mine;
-  L.dir[r] = k;
-}
-
-template <bool kSolvent>
-__device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, const Tabs &T,
-                                              const uint8_t *occ, int n, int64_t vac, int sv,
-                                              double beta, int32_t *err) {
-  GatherEnv(t, L, occ, n, vac, err);
-  __syncthreads();
-  if (kSolvent) SolventClusters(t, L, T, sv);
-  else DenseClusters(t, L, T);
-  __syncthreads();
-  FinishEvents<kSolvent>(t, L, T, sv, beta, err);
-  __syncthreads();
-}
 
 template <bool kSolvent>
 __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout lay,
@@ -472,7 +34,8 @@ __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout
                                                             const int64_t *off,
                                                             const int32_t *ncell,
                                                             const int64_t *vac, int64_t *nb,
-                                                            double *ea, double *de, int32_t *err) {
+                                                            double *ea, double *de, int32_t *err,
+                                                            uint8_t *env_out, double *ekra_out) {
   __shared__ JumpLds L;
   extern __shared__ char dyn[];
   const int r = blockIdx.x;
@@ -480,104 +43,25 @@ __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout
     if (threadIdx.x == 0) atomicOr(err, 8);
     return;
   }
-  const Tabs T = LoadJumpTables<kSolvent>(t, L, dyn, lay);
+  const Tabs T =
+      LoadJumpTables<kSolvent>(t, L, dyn, lay, threadIdx.x, kThreads, threadIdx.x, kThreads);
   __syncthreads();
-  EvaluateJumps<kSolvent>(t, L, T, occ + off[r], ncell[r], vac[r], t.solvent_default, 0.0, err);
+  EvaluateJumps<kSolvent>(t, L, T, occ + off[r], ncell[r], vac[r], t.solvent_default, 0.0, err,
+                          threadIdx.x);
   const int k = threadIdx.x;
   if (k < 12) {
     const int64_t o = static_cast<int64_t>(r) * 12 + k;
     nb[o] = L.snb[k];
     ea[o] = L.ea[k];
     de[o] = L.de[k];
+    if (ekra_out) ekra_out[o] = L.ekra[k];
   }
-}
-
-// TimeTemperatureInterpolator::GetTemperature (TimeTemperatureInterpolator.cpp:44-65)
-__device__ double ScheduleTemperature(const double *tt, const double *tv, int n, double time) {
-  int i = 0;
-  while (i < n && tt[i] < time) ++i;
-  if (i == n) return tv[n - 1];
-  if (i == 0 && time <= tt[0]) return tv[0];
-  return tv[i - 1] + ((time - tt[i - 1]) / (tt[i] - tt[i - 1])) * (tv[i] - tv[i - 1]);
-}
-
-// Random numbers of step Q.steps (one lane, overlapping the gather): Philox u1, u2, log(u1) and
-// the RateCorrector factor (KineticMcFirstOmp.cpp:79-82, RateCorrector.hpp:17-24).
-__device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
-  const U4 u = Philox4x32_10(Q.seed, Q.steps, Q.rid, 0u);
-  L.u2 = UnitOpenRight(Join64(u.z, u.w));
-  L.log_u1 = log(UnitOpenLeft(Join64(u.x, u.y)));
-  L.corr = (Q.flags & 1) ? Q.cv / (1.64 * exp(-(0.66 / kBoltzmann / Q.T - 0.7))) / (1 - 13 * Q.cs)
-                         : 1.0;
-}
-
-// Phase 4 (lanes k < 12): lane j divides rate j by the total; every event lane then forms the
-// same ordered cumulative probabilities (KineticMcFirstOmp.cpp:69-77); the lane holding the first cumulative probability >= u2 (lane
-// 11 when there is none, KineticMcAbstract.cpp:106-123) advances the clock and applies the jump
-// (KineticMcAbstract.cpp:147-181).
-__device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, ReplicaLds &Q,
-                               uint8_t *occ, int r, uint64_t trace_index) {
-  const int i = threadIdx.x;
-  if (i >= 12) return;
-  if (i == 0) L.bad = 0;  // every event lane has read it in phase 3
-  double total = 0.0;
-#pragma unroll
-  for (int j = 0; j < 12; ++j) total += L.rate[j];
-  const double p = L.rate[i] / total;
-  const double u2 = L.u2;
-  double cum = 0.0, prev = 0.0;
-#pragma unroll
-  for (int j = 0; j < 12; ++j) {
-    const double pj = __shfl(p, j);
-    if (j <= i) {
-      prev = cum;
-      cum += pj;
+  if (env_out) {  // environment rows of the events in rank order (60 slot codes, zero padded)
+    for (int i = threadIdx.x; i < 12 * 64; i += kThreads) {
+      const int q = i >> 6, j = i & 63;
+      env_out[(static_cast<int64_t>(r) * 12 + q) * 64 + j] = j < kEnvSlots ? L.env[L.dir[q]][j] : 0;
     }
   }
-  const bool mine = (!(cum < u2) && (i == 0 || prev < u2)) || (i == 11 && cum < u2);
-  if (!mine) return;
-  const int sel = i;
-  double dt = -L.log_u1 / total / kPrefactor;
-  if (Q.flags & 1) dt *= L.corr;
-  if (Q.flags & 2) {  // IsEscaped (KineticMcAbstract.cpp:192-223)
-    bool all = true;
-    for (int j = 0; j < kEnvSlots; ++j)
-      if (((t.vac_shell_mask >> j) & 1ull) && L.env[0][j] != Q.solvent) all = false;
-    if (all) {
-      Q.esc = 1;
-      Q.steps = Q.max_steps + 1;
-    }
-  }
-  const int dir = L.dir[sel];
-  const int64_t jump = L.snb[sel];
-  const double de = L.de[sel];
-  if (trace_index < k.trace_cap) {
-    const uint64_t o = static_cast<uint64_t>(r) * k.trace_cap + trace_index;
-    k.tr_sel[o] = jump;
-    k.tr_ea[o] = L.ea[sel];
-    k.tr_de[o] = de;
-    k.tr_time[o] = Q.time + dt;
-  }
-  Q.time += dt;
-  Q.energy += de;
-  Q.abse += de;
-  int dx, dy, dz;
-  Unpack(L.off[dir * kEnvSlots + t.n_slot], dx, dy, dz);
-  const int mig = L.env[dir][t.n_slot];
-  if ((Q.flags & 4) && mig != Q.solvent) {  // solute COM (KineticMcAbstract.cpp:163-177)
-    const double wgt = k.mass[mig] / Q.smass;
-    Q.com[0] -= dx * kHalfLattice * wgt;
-    Q.com[1] -= dy * kHalfLattice * wgt;
-    Q.com[2] -= dz * kHalfLattice * wgt;
-  }
-  Q.disp[0] += dx;
-  Q.disp[1] += dy;
-  Q.disp[2] += dz;
-  occ[Q.vac] = static_cast<uint8_t>(mig);  // Config::LatticeJump (Config.cpp:431-456)
-  occ[jump] = static_cast<uint8_t>(t.x_code);
-  Q.steps += 1;
-  Q.vac = jump;
-  if (Q.steps > Q.max_steps || Q.esc) Q.done = 1;
 }
 
 template <bool kSolvent>
@@ -588,7 +72,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
   extern __shared__ char dyn[];
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  const Tabs T = LoadJumpTables<kSolvent>(t, L, dyn, lay);
+  const Tabs T = LoadJumpTables<kSolvent>(t, L, dyn, lay, tid, kThreads, tid, kThreads);
   uint8_t *occ = k.occ + k.off[r];
   const int n = k.ncell[r];
   if (tid == 0) {
@@ -626,8 +110,8 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
       __syncthreads();
     }
     if (tid == kThreads - 1) DrawStep(L, Q);
-    EvaluateJumps<kSolvent>(t, L, T, occ, n, Q.vac, sv, Q.beta, k.error);
-    SelectAndApply(t, k, L, Q, occ, r, trace_base + it);
+    EvaluateJumps<kSolvent>(t, L, T, occ, n, Q.vac, sv, Q.beta, k.error, tid);
+    SelectAndApply(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }
   if (tid == 0) {
@@ -710,14 +194,15 @@ hipError_t AllowLds(K kernel, size_t bytes) {
 
 hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *off,
                            const int32_t *ncell, const int64_t *vac, int32_t R, int64_t *nb,
-                           double *ea, double *de, int32_t *err, hipStream_t s, bool solvent) {
+                           double *ea, double *de, int32_t *err, hipStream_t s, bool solvent,
+                           uint8_t *env_out, double *ekra_out) {
   if (R <= 0) return hipSuccess;
   const DynLayout lay = PlanLds(t, R, solvent);
   auto go = [&](auto kernel) {
     hipError_t e = AllowLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kernel, dim3(R), dim3(kThreads), lay.bytes, s, t, lay, occ, off, ncell,
-                       vac, nb, ea, de, err);
+                       vac, nb, ea, de, err, env_out, ekra_out);
     return hipGetLastError();
   };
   return solvent ? go(EvalJumpsKernel<true>) : go(EvalJumpsKernel<false>);
