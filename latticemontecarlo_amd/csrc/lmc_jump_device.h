@@ -123,42 +123,62 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
 }
 
 // Phase 1: species of the 12 environments; neighbour ids of the 12 jumps. 720 sites over the
-// kBT threads of the replica's block: independent loads per thread, issued together.
+// kBT threads of the replica's block: independent loads per thread, issued together. Split in
+// two so that a caller can issue other loads between them.
 template <int kBT = kThreads>
-__device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const uint8_t *occ, int n,
-                                          int64_t vac, int32_t *err, int tid) {
-  const XYZ c = ToXYZ(vac, n);
-  constexpr int kTotal = 12 * kEnvSlots;
-  constexpr int kU = (kTotal + kBT - 1) / kBT;
+struct Gathered {
+  static constexpr int kU = (12 * kEnvSlots + kBT - 1) / kBT;
   int64_t s[kU];
   uint8_t v[kU];
+};
+
+template <int kBT = kThreads>
+__device__ __forceinline__ void GatherIssue(const JumpLds &L, const uint8_t *occ, int n,
+                                            int64_t vac, int tid, Gathered<kBT> &G) {
+  const XYZ c = ToXYZ(vac, n);
+  constexpr int kTotal = 12 * kEnvSlots;
 #pragma unroll
-  for (int u = 0; u < kU; ++u) {
+  for (int u = 0; u < Gathered<kBT>::kU; ++u) {
     const int i = tid + u * kBT;
     if (i < kTotal) {
       int dx, dy, dz;
       Unpack(L.off[i], dx, dy, dz);
-      s[u] = SiteAt(c, n, dx, dy, dz);
-      v[u] = occ[s[u]];
+      G.s[u] = SiteAt(c, n, dx, dy, dz);
+      G.v[u] = occ[G.s[u]];
     }
   }
+}
+
+template <int kBT = kThreads>
+__device__ __forceinline__ void GatherStore(const KTables &t, JumpLds &L, int32_t *err, int tid,
+                                            const Gathered<kBT> &G) {
+  constexpr int kTotal = 12 * kEnvSlots;
 #pragma unroll
-  for (int u = 0; u < kU; ++u) {
+  for (int u = 0; u < Gathered<kBT>::kU; ++u) {
     const int i = tid + u * kBT;
     if (i < kTotal) {
       const int k = i / kEnvSlots, j = i - k * kEnvSlots;
-      uint8_t x = v[u];
+      uint8_t x = G.v[u];
       if (x >= t.S) {  // corrupt occupancy: flag it and keep every table index in range
         atomicOr(err, 4);
         x = 0;
       }
       if (x == t.x_code && j != t.v_slot) atomicOr(&L.bad, 1 << k);
       L.env[k][j] = x;
-      if (j == t.n_slot) L.nb[k] = s[u];
+      if (j == t.n_slot) L.nb[k] = G.s[u];
     }
   }
 }
 
+template <int kBT = kThreads>
+__device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const uint8_t *occ, int n,
+                                          int64_t vac, int32_t *err, int tid) {
+  Gathered<kBT> G;
+  GatherIssue<kBT>(L, occ, n, vac, tid, G);
+  GatherStore<kBT>(t, L, err, tid, G);
+}
+
+// w(cluster) of the solvent-baseline entries: singlet, pair or triplet table by label.
 __device__ __forceinline__ double WIndex(const double *w, int label, int a, int c1, int c2, int S) {
   if (label == 0) return w[a];
   if (label < 4) return w[S + (label - 1) * S * S + a * S + c1];

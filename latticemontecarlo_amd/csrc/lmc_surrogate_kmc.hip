@@ -35,8 +35,10 @@ constexpr int kRowsS = 12 * kG;      // event rows per workgroup step
 constexpr int kMT = kRowsS / 16;     // 16-row tiles
 constexpr int kMembers = 4;
 
+constexpr int kH1Stride = kHidden + 8;  // bf16 per h1 row: 272 B, so 16 rows hit distinct banks
+
 struct SurLds {
-  uint16_t h1[kMembers][kRowsS][kHidden];
+  uint16_t h1[kMembers][kRowsS][kH1Stride];
   float yp[kMembers][2][kRowsS][kMlpOut];
   uint8_t none[kEnvRow];  // input row of an absent replica (no channel matches)
   double dsum[kG][2];     // exact dE: per-wave sums
@@ -44,86 +46,120 @@ struct SurLds {
   double dt[kG];
 };
 
-// Phase 2: ensemble outputs (normalised) of the 48 rows into S.yp (all 8 waves).
+// First-layer weight fragments of wave (m, h) for one workgroup step (4 column tiles x KS
+// K-steps), loaded before the gather so that their latency overlaps it.
 template <int C>
-__device__ void EnsembleForward(const SurDev &sd, const JumpLds *Ls, const bool *act, SurLds &S) {
+struct WaveWeights {
+  static constexpr int KS = (kEnvUsed * C + 31) / 32;
+  u32x4 b1[4][KS];
+};
+
+template <int C>
+__device__ __forceinline__ void LoadWaveWeights(const SurDev &sd, WaveWeights<C> &ww) {
   constexpr int Kp = (kEnvUsed * C + 31) / 32 * 32, KS = Kp / 32;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15, grp = lane >> 4;
   const int m = wv >> 1, q = wv & 1;
   const MlpDev &w = sd.w;
-  const float *par = w.master + static_cast<size_t>(m) * w.lay.P;
   const uint16_t *W1t = w.w1t + static_cast<size_t>(m) * kHidden * Kp;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const int n = 64 * q + 16 * nt + col;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) ww.b1[nt][s] = Load8(W1t + static_cast<size_t>(n) * Kp + 32 * s + 8 * grp);
+  }
+}
+
+// Phase 2: ensemble outputs (normalised) of the 48 rows into S.yp (all 8 waves).
+template <int C>
+__device__ void EnsembleForward(const SurDev &sd, const WaveWeights<C> &ww, const JumpLds *Ls,
+                                const bool *act, SurLds &S) {
+  constexpr int KS = WaveWeights<C>::KS;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int m = wv >> 1, q = wv & 1;
+  const MlpDev &w = sd.w;
+  const float *par = w.master + static_cast<size_t>(m) * w.lay.P;
   const uint16_t *W2t = w.w2t + static_cast<size_t>(m) * kHidden * kHidden;
   const uint16_t *W3 = w.w3 + static_cast<size_t>(m) * kHidden * kMlpOut;
   const int chan[4] = {w.chan[0], w.chan[1], w.chan[2], w.chan[3]};
-  const uint8_t *erow[kMT];
+  u32x4 a[kMT][KS];  // one-hot fragments of the wave's rows, built once per step
 #pragma unroll
   for (int mt = 0; mt < kMT; ++mt) {
     const int row = 16 * mt + col, g = row / 12, k = row - 12 * g;
-    erow[mt] = act[g] ? Ls[g].env[k] : S.none;
+    const uint8_t *e = act[g] ? Ls[g].env[k] : S.none;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) a[mt][s] = OneHotFragment<C>(e, 32 * s + 8 * grp, chan);
   }
-#pragma unroll 1
+#pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    const int n0 = 64 * q + 16 * nt;
+    const int n = 64 * q + 16 * nt + col;
     f32x4 acc[kMT];
 #pragma unroll
     for (int mt = 0; mt < kMT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint16_t *bp = W1t + static_cast<size_t>(n0 + col) * Kp + 8 * grp;
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const u32x4 b = Load8(bp + 32 * s);
+    for (int s = 0; s < KS; ++s)
 #pragma unroll
-      for (int mt = 0; mt < kMT; ++mt)
-        acc[mt] = Mfma(OneHotFragment<C>(erow[mt], 32 * s + 8 * grp, chan), b, acc[mt]);
-    }
-    const float bias = par[w.lay.ob1 + n0 + col];
+      for (int mt = 0; mt < kMT; ++mt) acc[mt] = Mfma(a[mt][s], ww.b1[nt][s], acc[mt]);
+    const float bias = par[w.lay.ob1 + n];
 #pragma unroll
     for (int mt = 0; mt < kMT; ++mt)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float z = acc[mt][i] + bias;
-        S.h1[m][16 * mt + 4 * grp + i][n0 + col] = static_cast<uint16_t>(Bf16Bits(z * Sigmoid(z)));
+        S.h1[m][16 * mt + 4 * grp + i][n] = static_cast<uint16_t>(Bf16Bits(z * Sigmoid(z)));
       }
   }
+  u32x4 b2[4][4];  // second-layer fragments, in flight across the barrier
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      b2[nt][s] = Load8(W2t + static_cast<size_t>(64 * q + 16 * nt + col) * kHidden + 32 * s + 8 * grp);
   __syncthreads();
-  // second layer and this wave's share of the output layer: partial sums over its 64 columns,
-  // reduced over the 16 lanes of each row group and accumulated in S.yp per column tile
-#pragma unroll 1
+  // second layer and this wave's share of the output layer: per-lane partial sums over the
+  // wave's 4 column tiles, then one reduction over the 16 lanes of each row group (each level's
+  // 24 independent cross-lane moves are issued together)
+  float p[kMT * 4 * kMlpOut] = {};
+#pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
-    const int n0 = 64 * q + 16 * nt;
+    const int n = 64 * q + 16 * nt + col;
     f32x4 acc[kMT];
 #pragma unroll
     for (int mt = 0; mt < kMT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint16_t *bp = W2t + static_cast<size_t>(n0 + col) * kHidden + 8 * grp;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const u32x4 b = Load8(bp + 32 * s);
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
       for (int mt = 0; mt < kMT; ++mt)
-        acc[mt] = Mfma(Load8(&S.h1[m][16 * mt + col][32 * s + 8 * grp]), b, acc[mt]);
-    }
-    const float bias = par[w.lay.ob2 + n0 + col];
-    const float w30 = Bf16Float(W3[(n0 + col) * kMlpOut]);
-    const float w31 = Bf16Float(W3[(n0 + col) * kMlpOut + 1]);
+        acc[mt] = Mfma(Load8(&S.h1[m][16 * mt + col][32 * s + 8 * grp]), b2[nt][s], acc[mt]);
+    const float bias = par[w.lay.ob2 + n];
+    const float w30 = Bf16Float(W3[n * kMlpOut]);
+    const float w31 = Bf16Float(W3[n * kMlpOut + 1]);
 #pragma unroll
     for (int mt = 0; mt < kMT; ++mt)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float z = acc[mt][i] + bias;
         const float h = Bf16Round(z * Sigmoid(z));
-        float v0 = h * w30, v1 = h * w31;
-#pragma unroll
-        for (int off = 1; off < 16; off <<= 1) {
-          v0 += __shfl_xor(v0, off);
-          v1 += __shfl_xor(v1, off);
-        }
-        if (col == 0) {
-          float *y = S.yp[m][q][16 * mt + 4 * grp + i];
-          y[0] = nt ? y[0] + v0 : v0;
-          y[1] = nt ? y[1] + v1 : v1;
-        }
+        p[(mt * 4 + i) * kMlpOut] += h * w30;
+        p[(mt * 4 + i) * kMlpOut + 1] += h * w31;
       }
   }
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) {
+    float o[kMT * 4 * kMlpOut];
+#pragma unroll
+    for (int j = 0; j < kMT * 4 * kMlpOut; ++j) o[j] = __shfl_xor(p[j], off);
+#pragma unroll
+    for (int j = 0; j < kMT * 4 * kMlpOut; ++j) p[j] += o[j];
+  }
+  if (col == 0)
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float *y = S.yp[m][q][16 * mt + 4 * grp + i];
+        y[0] = p[(mt * 4 + i) * kMlpOut];
+        y[1] = p[(mt * 4 + i) * kMlpOut + 1];
+      }
 }
 
 // Phase 3 (lanes k < 12 of block g): surrogate events, fallback flags. Returns the flag.
@@ -159,9 +195,17 @@ __device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, co
 #pragma unroll
   for (int j = 0; j < 12; ++j) emin = fmin(emin, __shfl(ea, j));
   int solutes = 0;  // environment slots holding neither the base species nor the vacancy
-  for (int j = 0; j < kEnvUsed; ++j) {
-    const int c = L.env[k][j];
-    solutes += (c != sd.base_code && j != t.v_slot) ? 1 : 0;
+  {
+    const uint32_t *words = reinterpret_cast<const uint32_t *>(L.env[k]);
+    uint32_t wv[kEnvUsed / 4];
+#pragma unroll
+    for (int i = 0; i < kEnvUsed / 4; ++i) wv[i] = words[i];
+#pragma unroll
+    for (int i = 0; i < kEnvUsed / 4; ++i)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        solutes += (((wv[i] >> (8 * b)) & 0xFF) != static_cast<uint32_t>(sd.base_code)) ? 1 : 0;
+    solutes -= (L.env[k][t.v_slot] != sd.base_code) ? 1 : 0;
   }
   const double spread = fmax(hi - lo, dhi - dlo);
   const bool relevant = sd.window <= 0.0 || lo < emin + sd.window / beta;
@@ -307,10 +351,14 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
     }
     __syncthreads();
     const bool on = act[g];
+    Gathered<kSub> gat;
+    if (on) GatherIssue<kSub>(L, occ, n, Q.vac, lt, gat);
+    WaveWeights<C> ww;  // issued after the occupancy loads, consumed after the gather
+    LoadWaveWeights<C>(sd, ww);
     if (kKmc && on && lt == kSub - 1) DrawStep(L, Q);
-    if (on) GatherEnv<kSub>(t, L, occ, n, Q.vac, k.error, lt);
+    if (on) GatherStore<kSub>(t, L, k.error, lt, gat);
     __syncthreads();
-    EnsembleForward<C>(sd, Ls, act, S);
+    EnsembleForward<C>(sd, ww, Ls, act, S);
     __syncthreads();
     if (on && lt < 12) {
       const bool f = EnsembleEvent(t, sd, L, S, g, lt, Q.beta);
