@@ -144,37 +144,42 @@ Model::~Model() {
 static void BuildSolventTables(const Model &m, const JumpTables &jt, std::vector<uint16_t> &off,
                                std::vector<uint32_t> &inc, std::vector<double> &base) {
   const int S = m.S;
-  std::vector<std::vector<uint32_t>> per(kEnvSlots);
-  auto state_entry = [](int label, bool nside, int other) {
-    return static_cast<uint32_t>(0 | (label << 2) | (nside ? 32 : 0) | (other << 6));
-  };
+  // per slot j: the clusters containing j (j not a jump endpoint), by list type
+  std::vector<std::vector<uint32_t>> per[3];
+  for (auto &v : per) v.assign(kEnvSlots, {});
+  // state entry: weight-block byte offset [0:13] | j is the second other site [13] | the other
+  // site o [14:20] (63 = none) | neighbour-endpoint side [31]
   auto add_state = [&](const std::vector<StateCluster> &cl, int endpoint, bool nside) {
     for (const auto &c : cl) {
       int others[2] = {-1, -1}, n = 0;
       for (int i = 0; i < c.size; ++i)
         if (c.slots[i] != endpoint) others[n++] = c.slots[i];
+      const int l = c.label;
+      const int block = l == 0 ? 0 : l < 4 ? S + (l - 1) * S * S : S + 3 * S * S + (l - 4) * S * S * S;
       for (int i = 0; i < n; ++i) {
         const int other = n == 2 ? others[1 - i] : 63;
-        per[others[i]].push_back(state_entry(c.label, nside, other));
+        per[0][others[i]].push_back(static_cast<uint32_t>(block * 8) | (static_cast<uint32_t>(i) << 13) |
+                                    (static_cast<uint32_t>(other) << 14) | (nside ? 0x80000000u : 0u));
       }
     }
   };
   add_state(jt.state_v_only, jt.v_slot, false);
   add_state(jt.state_n_only, jt.n_slot, true);
-  auto add_orbits = [&](const OrbitTable &ot, const std::vector<int> &bases, int type,
-                        bool backward) {
+  // orbit entry: feature base [0:12] | kind * S * S [12:20] | j is the cluster's second site [20]
+  // | the other site o [21:27] (63 = none); backward entries read canonical slot x at mirror[x]
+  auto add_orbits = [&](const OrbitTable &ot, const std::vector<int> &bases, int list, bool backward) {
     for (const auto &c : ot.clusters) {
       const int kind = ot.orbit_kind[c.orbit];
-      const uint32_t b = static_cast<uint32_t>(bases[c.orbit]);
+      const uint32_t bse = static_cast<uint32_t>(bases[c.orbit]);
+      if (bse > 4095) throw std::invalid_argument("feature vector too long for the packed tables");
       const int nsl = c.s1 < 0 ? 1 : 2;
       for (int pos = 0; pos < nsl; ++pos) {
         const int canon = pos == 0 ? c.s0 : c.s1;
         const int canon_other = nsl == 1 ? -1 : (pos == 0 ? c.s1 : c.s0);
-        // backward: the species of canonical slot x is read from real slot mirror[x]
         const int slot = backward ? jt.mirror[canon] : canon;
         const int other = canon_other < 0 ? 63 : (backward ? jt.mirror[canon_other] : canon_other);
-        per[slot].push_back(static_cast<uint32_t>(type | (kind << 2) | (pos << 5) | (other << 6)) |
-                            (b << 12));
+        per[list][slot].push_back(bse | (static_cast<uint32_t>(kind * S * S) << 12) |
+                                  (static_cast<uint32_t>(pos) << 20) | (static_cast<uint32_t>(other) << 21));
       }
     }
   };
@@ -183,12 +188,14 @@ static void BuildSolventTables(const Model &m, const JumpTables &jt, std::vector
     add_orbits(jt.mm2, m.base_mm2, 2, false);
     add_orbits(jt.mm2, m.base_mm2, 2, true);
   }
-  off.assign(kEnvSlots + 1, 0);
-  for (int j = 0; j < kEnvSlots; ++j) {
-    off[j] = static_cast<uint16_t>(inc.size());
-    inc.insert(inc.end(), per[j].begin(), per[j].end());
+  off.assign(3 * (kEnvSlots + 1), 0);
+  for (int t = 0; t < 3; ++t) {
+    for (int j = 0; j < kEnvSlots; ++j) {
+      off[t * (kEnvSlots + 1) + j] = static_cast<uint16_t>(inc.size());
+      inc.insert(inc.end(), per[t][j].begin(), per[t][j].end());
+    }
+    off[t * (kEnvSlots + 1) + kEnvSlots] = static_cast<uint16_t>(inc.size());
   }
-  off[kEnvSlots] = static_cast<uint16_t>(inc.size());
   if (inc.size() > 65535) throw std::invalid_argument("incidence table too large");
 
   const double nan = std::numeric_limits<double>::quiet_NaN();

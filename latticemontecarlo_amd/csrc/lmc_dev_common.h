@@ -40,6 +40,12 @@ __device__ __forceinline__ void Unpack(int32_t v, int &x, int &y, int &z) {
   y = static_cast<int8_t>((v >> 8) & 0xff);
   z = static_cast<int8_t>((v >> 16) & 0xff);
 }
+// Offsets of the jump environments (|component| <= 4 in a/2 units), 4 bits each, biased by 8.
+__device__ __forceinline__ void Unpack16(uint32_t v, int &x, int &y, int &z) {
+  x = static_cast<int>(v & 15) - 8;
+  y = static_cast<int>((v >> 4) & 15) - 8;
+  z = static_cast<int>((v >> 8) & 15) - 8;
+}
 // A replica's lattice size and a site id are usable (uniform over a block).
 __device__ __forceinline__ bool ValidSite(int n, int64_t site) {
   return n >= 4 && site >= 0 && site < 4ll * n * n * n;

@@ -40,12 +40,16 @@ struct KTables {
   const int32_t *eci1;      // [S]
   const int32_t *eci2;      // [3][S][S]
   const int32_t *eci3;      // [4][S][S][S]
-  // Solvent-baseline formulation: per env slot j, the clusters that change when slot j holds a
-  // non-solvent species (CSR: inc_off[61] -> inc entries), and for every (solvent s, migrating
-  // species M) the sums (dE, MMM logit, MM2 logit) of an environment that is all solvent.
-  //   entry: type[0:2] (0 state, 1 MMM, 2 MM2) | label or kind[2:5] | n-side or position[5]
-  //          | other slot[6:12] (63 = none) | feature base[12:24]
-  const uint16_t *inc_off;  // [61]
+  // Solvent-baseline formulation: for every (solvent s, migrating species M), the sums (dE, MMM
+  // logit, MM2 logit) of an environment that is all solvent (base), and per environment slot j
+  // the clusters containing j (CSR inc_off[3][61] -> inc, lists: state, MMM, MM2 forward and
+  // backward); a real environment adds value - value(all solvent) for each cluster touching a
+  // non-solvent slot, once (at its lowest such slot).
+  //   state entry: weight-block byte offset [0:13] | j is the second other site [13]
+  //                | other site [14:20] (63 = none) | neighbour-endpoint side [31]
+  //   orbit entry: feature base [0:12] | kind * S * S [12:20] | j is the second site [20]
+  //                | other site [21:27] (63 = none)
+  const uint16_t *inc_off;  // [3][61]
   const uint32_t *inc;
   const double *base;       // [S][S][3]
   int n_inc, solvent_default;

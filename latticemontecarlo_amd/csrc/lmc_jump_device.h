@@ -12,12 +12,10 @@ namespace lmc {
 namespace {
 
 constexpr int kLanes = 21;      // dense: lanes per jump event (12 * 21 = 252 <= 256)
-constexpr int kGroup = 8;       // solvent: lanes per non-solvent site
-constexpr int kPass = kThreads / kGroup;  // sites handled per pass
-constexpr int kEnvPad = 768;    // 12 x 60 environment entries padded to whole waves
+constexpr int kSLanes = 16;     // solvent baseline: lanes per jump event (4 events per wave)
 
 struct JumpLds {
-  int32_t off[12 * kEnvSlots];
+  uint16_t off[12 * kEnvSlots];  // relative offsets, 4 bits per component (+8)
   double par[kMaxSpecies * 6];
   int8_t tt[3 * kMaxSpecies * kMaxSpecies];
   uint8_t env[12][64];
@@ -26,23 +24,13 @@ struct JumpLds {
   double ea[12], de[12], rate[12], ekra[12];
   int64_t snb[12];
   int dir[12];
-  // per-event sums (dE, logit MMM, logit MM2) of the solvent path's earlier passes
-  double acc[12][3];
   // random numbers of the current step (drawn by an idle lane during the gather)
   double u2, log_u1, corr;
   // events whose environment holds a vacancy off the jump endpoints (no one-hot type)
   int bad;
-  // dense partials / solvent-baseline work list
-  union {
-    double part[kThreads][3];  // dense: [event * kLanes + lane]; surrogate fallback: per thread
-    struct {
-      uint16_t inc[kEnvPad];
-      unsigned long long mask[12];
-      int start[13];
-      int last_p0;
-      double part[kPass][kGroup][3];
-    } sp;
-  } u;
+  // per-lane partial sums (dE, MMM logit, MM2 logit): dense [event * kLanes + lane], solvent
+  // baseline [event * kSLanes + lane], surrogate fallback per thread
+  double part[kThreads][3];
 };
 
 // Replica state of the persistent KMC kernel (lane 0 reads and writes it in phase 4).
@@ -73,15 +61,18 @@ struct Tabs {  // resolved table pointers (LDS or global)
 template <bool kSolvent>
 __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const DynLayout &lay,
                                int tid, int lsize, int gtid, int gsize) {
-  for (int i = tid; i < 12 * kEnvSlots; i += lsize) L.off[i] = t.off[i];
+  for (int i = tid; i < 12 * kEnvSlots; i += lsize) {
+    int dx, dy, dz;
+    Unpack(t.off[i], dx, dy, dz);
+    L.off[i] = static_cast<uint16_t>((dx + 8) | ((dy + 8) << 4) | ((dz + 8) << 8));
+  }
   for (int i = tid; i < 3 * t.S * t.S; i += lsize) L.tt[i] = t.tt[i];
   for (int i = tid; i < 6 * t.S; i += lsize) L.par[i] = t.par[i];
   if (tid < 12) L.env[tid][63] = 0;  // target of the absent sites of every packed cluster
   if (tid == 0) L.bad = 0;
   Tabs T{};
   double *w = reinterpret_cast<double *>(dyn + lay.w);
-  const double *wsrc = kSolvent ? t.w : t.wd;
-  for (int i = gtid; i < t.nw; i += gsize) w[i] = wsrc[i];
+  for (int i = gtid; i < t.nw; i += gsize) w[i] = t.wd[i];
   T.w = w;
   if (lay.b1_lds) {
     double *p = reinterpret_cast<double *>(dyn + lay.b1);
@@ -105,7 +96,7 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     for (int i = gtid; i < t.n_inc; i += gsize) in[i] = t.inc[i];
     T.inc = in;
     uint16_t *io = reinterpret_cast<uint16_t *>(dyn + lay.offs);
-    for (int i = gtid; i <= kEnvSlots; i += gsize) io[i] = t.inc_off[i];
+    for (int i = gtid; i < 3 * (kEnvSlots + 1); i += gsize) io[i] = t.inc_off[i];
     T.inc_off = io;
   } else {
     uint32_t *p = reinterpret_cast<uint32_t *>(dyn + lay.lists);
@@ -142,7 +133,7 @@ __device__ __forceinline__ void GatherIssue(const JumpLds &L, const uint8_t *occ
     const int i = tid + u * kBT;
     if (i < kTotal) {
       int dx, dy, dz;
-      Unpack(L.off[i], dx, dy, dz);
+      Unpack16(L.off[i], dx, dy, dz);
       G.s[u] = SiteAt(c, n, dx, dy, dz);
       G.v[u] = occ[G.s[u]];
     }
@@ -176,13 +167,6 @@ __device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const ui
   Gathered<kBT> G;
   GatherIssue<kBT>(L, occ, n, vac, tid, G);
   GatherStore<kBT>(t, L, err, tid, G);
-}
-
-// w(cluster) of the solvent-baseline entries: singlet, pair or triplet table by label.
-__device__ __forceinline__ double WIndex(const double *w, int label, int a, int c1, int c2, int S) {
-  if (label == 0) return w[a];
-  if (label < 4) return w[S + (label - 1) * S * S + a * S + c1];
-  return w[S + 3 * S * S + (label - 4) * S * S * S + (a * S + c1) * S + c2];
 }
 
 // Sums of kU packed state clusters per lane: w(cluster | endpoint = M) - w(cluster | endpoint =
@@ -268,140 +252,108 @@ __device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T, int t
     for (; c < t.n_mm2; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mm2, c, S, a);
     lb = (a[0] + a[1]) + (a[2] + a[3]);
   }
-  L.u.part[k * kLanes + lane][0] = de;
-  L.u.part[k * kLanes + lane][1] = la;
-  L.u.part[k * kLanes + lane][2] = lb;
+  L.part[k * kLanes + lane][0] = de;
+  L.part[k * kLanes + lane][1] = la;
+  L.part[k * kLanes + lane][2] = lb;
 }
 
-// Change of one touched cluster relative to the all-solvent environment.
-__device__ __forceinline__ void SolventEntry(const JumpLds &L, const Tabs &T, const KTables &t,
-                                             const uint8_t *e, uint32_t d, int j, int cj, int sv,
-                                             int M, int row, double &de, double &la, double &lb) {
-  const int S = t.S;
-  const int type = d & 3, other = (d >> 6) & 63;
-  const int co = other == 63 ? sv : e[other];
-  if (other != 63 && other < j && co != sv) return;  // evaluated at the lower non-solvent slot
-  if (type == 0) {
-    const int label = (d >> 2) & 7;
-    const int X = t.x_code;
-    const double v = (WIndex(T.w, label, M, cj, co, S) - WIndex(T.w, label, X, cj, co, S)) -
-                     (WIndex(T.w, label, M, sv, sv, S) - WIndex(T.w, label, X, sv, sv, S));
-    de += (d & 32) ? -v : v;
-    return;
-  }
-  const int kind = (d >> 2) & 3;
-  const int base = d >> 12;
-  const bool second = (d & 32) != 0;
-  const int c0 = second ? co : cj, c1 = kind ? (second ? cj : co) : 0;
-  const int ty = L.tt[(kind * S + c0) * S + c1];
-  const int tp = L.tt[(kind * S + sv) * S + (kind ? sv : 0)];
-  if (type == 1) {
-    const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-    la += bm[base + ty] - bm[base + tp];
-  } else {
-    const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-    lb += bm[base + ty] - bm[base + tp];
-  }
+// Solvent baseline: change of one state cluster touching slot j (species cj) relative to the
+// all-solvent environment; zero when the cluster's other site o is a lower non-solvent slot
+// (the cluster is counted there).
+__device__ __forceinline__ double SolventState(const char *wm, const char *wx, const uint8_t *e,
+                                               uint32_t d, int j, int cj, int sv, int s8, int ss8) {
+  const int o = (d >> 14) & 63, co = e[o];
+  const bool second = (d >> 13) & 1;
+  const int mj = second ? ss8 : s8, mo = second ? s8 : ss8;
+  const int blk = static_cast<int>(d & 8191);
+  const int oa = blk + __umul24(cj, mj) + __umul24(co, mo);
+  const int ob = blk + __umul24(sv, mj) + __umul24(o == 63 ? 0 : sv, mo);
+  const double v = (*reinterpret_cast<const double *>(wm + oa) - *reinterpret_cast<const double *>(wx + oa)) -
+                   (*reinterpret_cast<const double *>(wm + ob) - *reinterpret_cast<const double *>(wx + ob));
+  const double sv_ = __hiloint2double(__double2hiint(v) ^ static_cast<int>(d & 0x80000000u), __double2loint(v));
+  return (o != 63 && o < j && co != sv) ? 0.0 : sv_;
 }
 
-// Phase 2, solvent baseline: list the environment entries (event k, slot j) that hold a
-// non-solvent species, then 8 lanes per entry add the changes of the clusters touching it. Each
-// lane leaves its partial sums in LDS; the event lanes add them in a fixed order (passes before
-// the last here, the last pass in FinishEvents).
+// Solvent baseline: change of one orbit cluster touching slot j.
+__device__ __forceinline__ double SolventOrbit(const int8_t *tt, const double *bm, const uint8_t *e,
+                                               uint32_t d, int j, int cj, int sv, int S) {
+  const int o = (d >> 21) & 63, co = e[o];
+  const bool second = (d >> 20) & 1;
+  const int kS2 = (d >> 12) & 255, base = d & 4095;
+  const int c0 = second ? co : cj, c1 = second ? cj : co;
+  const int ty = tt[kS2 + __umul24(c0, S) + c1];
+  const int tb = tt[kS2 + __umul24(sv, S) + (o == 63 ? 0 : sv)];
+  const double v = bm[base + ty] - bm[base + tb];
+  return (o != 63 && o < j && co != sv) ? 0.0 : v;
+}
+
+// Phase 2, solvent baseline: 16 lanes per event (threads 0..191). The event's non-solvent
+// environment slots (endpoints excluded) come from wave ballots; the lanes stride over each such
+// slot's incidence lists. Partial sums go to L.part[event * 16 + lane].
 __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int sv, int tid) {
-  const int lane = tid & 63, wave = tid >> 6;
-  bool flag[3];
+  const int k = tid >> 4, lane = tid & 15;
+  const bool active = tid < 12 * kSLanes;
+  const uint8_t *e = L.env[active ? k : 0];
+  uint64_t mask = 0;
 #pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int idx = tid + u * kThreads;
-    bool f = false;
-    if (idx < 12 * kEnvSlots) {
-      const int k = idx / kEnvSlots, j = idx - k * kEnvSlots;
-      f = j != t.v_slot && j != t.n_slot && L.env[k][j] != sv;
-    }
-    flag[u] = f;
-    const unsigned long long m = __ballot(f);
-    if (lane == 0) L.u.sp.mask[wave + 4 * u] = m;
+  for (int i = 0; i < 4; ++i) {
+    const int j = lane + 16 * i;
+    const bool f = active && j < kEnvSlots && j != t.v_slot && j != t.n_slot && e[j] != sv;
+    const unsigned long long b = __ballot(f);
+    mask |= ((b >> (16 * (k & 3))) & 0xFFFFull) << (16 * i);
   }
-  if (tid < 12) L.acc[tid][0] = L.acc[tid][1] = L.acc[tid][2] = 0.0;
-  __syncthreads();
-  int cnt[12];
-#pragma unroll
-  for (int c = 0; c < 12; ++c) cnt[c] = __popcll(L.u.sp.mask[c]);
-  const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-  for (int u = 0; u < 3; ++u)
-    if (flag[u]) {
-      const int c = wave + 4 * u;
-      int pre = 0;
-#pragma unroll
-      for (int q = 0; q < 12; ++q) pre += q < c ? cnt[q] : 0;
-      L.u.sp.inc[pre + __popcll(L.u.sp.mask[c] & below)] = static_cast<uint16_t>(tid + u * kThreads);
+  if (!active) return;
+  const int S = t.S, M = e[t.n_slot];
+  const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
+  const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
+  const int s8 = 8 * S, ss8 = 8 * S * S;
+  const int row = L.tt[M * S];
+  const double *b1 = T.b1 + static_cast<size_t>(row) * t.F1;
+  const double *b2 = T.b2 + static_cast<size_t>(row) * t.F2;
+  const bool orbits = t.kind == kE0 || t.kind == kQuartic, mm2 = t.kind == kQuartic;
+  double de = 0.0, la = 0.0, lb = 0.0;
+  const uint16_t *offS = T.inc_off, *offM = T.inc_off + 61, *offQ = T.inc_off + 122;
+  // two slots at a time; their state, MMM (and MM2) lists advance together, branch-free (an
+  // exhausted list reads entry 0 and contributes zero), so several load chains are in flight
+  for (uint64_t mm = mask; mm;) {
+    const int j0 = __ffsll(static_cast<unsigned long long>(mm)) - 1;
+    mm &= mm - 1;
+    const bool two = mm != 0;
+    const int j1 = two ? __ffsll(static_cast<unsigned long long>(mm)) - 1 : j0;
+    if (two) mm &= mm - 1;
+    const int c0 = e[j0], c1 = e[j1];
+    int s0 = offS[j0] + lane, s1 = offS[j1] + lane, m0 = offM[j0] + lane, m1 = offM[j1] + lane;
+    const int s0e = offS[j0 + 1], m0e = orbits ? offM[j0 + 1] : 0;
+    const int s1e = two ? offS[j1 + 1] : 0, m1e = two && orbits ? offM[j1 + 1] : 0;
+    while (s0 < s0e || s1 < s1e || m0 < m0e || m1 < m1e) {
+      const bool bs0 = s0 < s0e, bs1 = s1 < s1e, bm0 = m0 < m0e, bm1 = m1 < m1e;
+      const double vs0 = SolventState(wm, wx, e, T.inc[bs0 ? s0 : 0], j0, c0, sv, s8, ss8);
+      const double vs1 = SolventState(wm, wx, e, T.inc[bs1 ? s1 : 0], j1, c1, sv, s8, ss8);
+      const double vm0 = SolventOrbit(L.tt, b1, e, T.inc[bm0 ? m0 : offM[0]], j0, c0, sv, S);
+      const double vm1 = SolventOrbit(L.tt, b1, e, T.inc[bm1 ? m1 : offM[0]], j1, c1, sv, S);
+      de += (bs0 ? vs0 : 0.0) + (bs1 ? vs1 : 0.0);
+      la += (bm0 ? vm0 : 0.0) + (bm1 ? vm1 : 0.0);
+      s0 += kSLanes;
+      s1 += kSLanes;
+      m0 += kSLanes;
+      m1 += kSLanes;
     }
-  if (tid <= 12) {  // first list position of event k's entries (k = 12: the total)
-    const int e0 = tid * kEnvSlots, c = e0 >> 6, b = e0 & 63;
-    int pre = 0;
-#pragma unroll
-    for (int q = 0; q < 12; ++q) pre += q < c ? cnt[q] : 0;
-    L.u.sp.start[tid] = pre + (c < 12 ? __popcll(L.u.sp.mask[c] & ((1ull << b) - 1ull)) : 0);
+    if (mm2) {
+      int q0 = offQ[j0] + lane, q1 = offQ[j1] + lane;
+      const int q0e = offQ[j0 + 1], q1e = two ? offQ[j1 + 1] : 0;
+      while (q0 < q0e || q1 < q1e) {
+        const bool bq0 = q0 < q0e, bq1 = q1 < q1e;
+        const double vq0 = SolventOrbit(L.tt, b2, e, T.inc[bq0 ? q0 : offQ[0]], j0, c0, sv, S);
+        const double vq1 = SolventOrbit(L.tt, b2, e, T.inc[bq1 ? q1 : offQ[0]], j1, c1, sv, S);
+        lb += (bq0 ? vq0 : 0.0) + (bq1 ? vq1 : 0.0);
+        q0 += kSLanes;
+        q1 += kSLanes;
+      }
+    }
   }
-  int total = 0;
-#pragma unroll
-  for (int q = 0; q < 12; ++q) total += cnt[q];
-  __syncthreads();
-  const int grp = tid / kGroup, g = tid - grp * kGroup;
-  int p0 = 0;
-  for (;; p0 += kPass) {
-    double de = 0.0, la = 0.0, lb = 0.0;
-    const int i = p0 + grp;
-    if (i < total) {
-      const int idx = L.u.sp.inc[i];
-      const int k = idx / kEnvSlots, j = idx - k * kEnvSlots;
-      const uint8_t *e = L.env[k];
-      const int M = e[t.n_slot];
-      const int row = L.tt[M * t.S];
-      const int cj = e[j];
-      const int q1 = T.inc_off[j + 1];
-      for (int q = T.inc_off[j] + g; q < q1; q += kGroup)
-        SolventEntry(L, T, t, e, T.inc[q], j, cj, sv, M, row, de, la, lb);
-    }
-    L.u.sp.part[grp][g][0] = de;
-    L.u.sp.part[grp][g][1] = la;
-    L.u.sp.part[grp][g][2] = lb;
-    if (p0 + kPass >= total) break;
-    __syncthreads();
-    if (tid < 12) {
-      const int lo = max(p0, L.u.sp.start[tid]), hi = min(p0 + kPass, L.u.sp.start[tid + 1]);
-      for (int q = lo; q < hi; ++q)
-        for (int h = 0; h < kGroup; ++h) {
-          L.acc[tid][0] += L.u.sp.part[q - p0][h][0];
-          L.acc[tid][1] += L.u.sp.part[q - p0][h][1];
-          L.acc[tid][2] += L.u.sp.part[q - p0][h][2];
-        }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) L.u.sp.last_p0 = p0;
-}
-
-// Per-event sums at the start of phase 3 (lanes k < 12, after the phase-2 barrier).
-__device__ __forceinline__ void SolventSums(const KTables &t, const JumpLds &L, const Tabs &T,
-                                            int k, int sv, double &de, double &la, double &lb) {
-  double a0 = L.acc[k][0], a1 = L.acc[k][1], a2 = L.acc[k][2];
-  const int p0 = L.u.sp.last_p0;
-  const int lo = max(p0, L.u.sp.start[k]), hi = min(p0 + kPass, L.u.sp.start[k + 1]);
-  for (int q = lo; q < hi; ++q)
-    for (int h = 0; h < kGroup; ++h) {
-      a0 += L.u.sp.part[q - p0][h][0];
-      a1 += L.u.sp.part[q - p0][h][1];
-      a2 += L.u.sp.part[q - p0][h][2];
-    }
-  // all-solvent sums of this (solvent, migrating species)
-  const int M = L.env[k][t.n_slot];
-  const double *b = T.base + (static_cast<size_t>(sv) * t.S + M) * 3;
-  de = a0 + b[0];
-  la = a1 + b[1];
-  lb = a2 + b[2];
+  L.part[tid][0] = de;
+  L.part[tid][1] = la;
+  L.part[tid][2] = lb;
 }
 
 // Ea and E_kra of event k from its cluster sums: dE, MMM logit la and MM2 logit lb.
@@ -450,23 +402,27 @@ __device__ __forceinline__ void StoreEvent(JumpLds &L, int k, double ea, double 
 template <bool kSolvent>
 __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const Tabs &T, int sv,
                                              double beta, int32_t *err, int tid) {
+  constexpr int kL = kSolvent ? kSLanes : kLanes;
   const int k = tid;
   if (k >= 64) return;
-  double de, la, lb;
-  if (!kSolvent) {  // wave 0: lane 3q + c adds the kLanes partials of component c of event q
-    double s = 0.0;
-    if (k < 36) {
-      const int q = k / 3, c = k - 3 * q;
+  // wave 0: lane 3q + c adds the kL partials of component c of event q; lane q collects them
+  double s = 0.0;
+  if (k < 36) {
+    const int q = k / 3, c = k - 3 * q;
 #pragma unroll
-      for (int l = 0; l < kLanes; ++l) s += L.u.part[q * kLanes + l][c];
-    }
-    const int q = k < 12 ? k : 0;
-    de = __shfl(s, 3 * q);
-    la = __shfl(s, 3 * q + 1);
-    lb = __shfl(s, 3 * q + 2);
+    for (int l = 0; l < kL; ++l) s += L.part[q * kL + l][c];
   }
+  const int q = k < 12 ? k : 0;
+  double de = __shfl(s, 3 * q);
+  double la = __shfl(s, 3 * q + 1);
+  double lb = __shfl(s, 3 * q + 2);
   if (k >= 12) return;
-  if (kSolvent) SolventSums(t, L, T, k, sv, de, la, lb);
+  if (kSolvent) {  // all-solvent sums of this (solvent, migrating species)
+    const double *b = T.base + (static_cast<size_t>(sv) * t.S + L.env[k][t.n_slot]) * 3;
+    de += b[0];
+    la += b[1];
+    lb += b[2];
+  }
   if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
     la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
   double ea, ekra;
@@ -562,7 +518,7 @@ __device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, 
   Q.energy += de;
   Q.abse += de;
   int dx, dy, dz;
-  Unpack(L.off[dir * kEnvSlots + t.n_slot], dx, dy, dz);
+  Unpack16(L.off[dir * kEnvSlots + t.n_slot], dx, dy, dz);
   const int mig = L.env[dir][t.n_slot];
   if ((Q.flags & 4) && mig != Q.solvent) {  // solute COM (KineticMcAbstract.cpp:163-177)
     const double wgt = k.mass[mig] / Q.smass;

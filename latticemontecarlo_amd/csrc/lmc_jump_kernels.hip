@@ -9,9 +9,9 @@
 //                 of two exact formulations:
 //                   dense   - 21 lanes per jump sum every cluster (4 accumulators per lane);
 //                   solvent - per (solvent, migrating species) sums of an all-solvent
-//                             environment are precomputed; the block lists the environment
-//                             sites that hold another species and 8 lanes per such site add the
-//                             changes of the clusters it touches;
+//                             environment are precomputed; 16 lanes per jump add, for each
+//                             environment site holding another species, the changes of the
+//                             clusters it touches;
 //   3. events   : 12 lanes form dE, Ea and the rate exp(-Ea beta) in a fixed order and store
 //                 them at the jump's rank (neighbour ids ascending);
 //   4. select   : the 12 event lanes form the ordered cumulative probabilities (FP64); the lane
@@ -161,7 +161,7 @@ DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
     lay.lists = static_cast<int>(at);
     at = al(at + sizeof(uint32_t) * t.n_inc);
     lay.offs = static_cast<int>(at);
-    at = al(at + sizeof(uint16_t) * (kEnvSlots + 1));
+    at = al(at + sizeof(uint16_t) * 3 * (kEnvSlots + 1));
   } else {
     lay.lists = static_cast<int>(at);
     at = al(at + sizeof(uint32_t) * (t.n_state + t.n_mmm + (t.kind == kQuartic ? t.n_mm2 : 0)));

@@ -216,7 +216,7 @@ __device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, co
 }
 
 // Phase 4a (all lanes of block g): exact cluster sums of the flagged events, kSub / nf lanes
-// per event; partials in L.u.part.
+// per event; partials in L.part.
 __device__ void FallbackClusters(const KTables &t, JumpLds &L, const Tabs &T, const SurLds &S,
                                  int g, int lt) {
   const int mask = S.fbmask[g];
@@ -249,9 +249,9 @@ __device__ void FallbackClusters(const KTables &t, JumpLds &L, const Tabs &T, co
       lb = a[0];
     }
   }
-  L.u.part[lt][0] = de;
-  L.u.part[lt][1] = la;
-  L.u.part[lt][2] = lb;
+  L.part[lt][0] = de;
+  L.part[lt][1] = la;
+  L.part[lt][2] = lb;
 }
 
 // Phase 4b (lane e < nf of block g): exact event e from its lanes' partials (in lane order).
@@ -266,9 +266,9 @@ __device__ void FallbackFinish(const KTables &t, JumpLds &L, const SurLds &S, in
   const int k = __ffs(mm) - 1;
   double de = 0.0, la = 0.0, lb = 0.0;
   for (int i = e * le; i < (e + 1) * le; ++i) {
-    de += L.u.part[i][0];
-    la += L.u.part[i][1];
-    lb += L.u.part[i][2];
+    de += L.part[i][0];
+    la += L.part[i][1];
+    lb += L.part[i][2];
   }
   if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1)) la = __builtin_nan("");
   double ea, ekra;
