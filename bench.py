@@ -92,6 +92,81 @@ def cpu_baseline(coeff_path, budget_s=12.0):
                        f"single-threaded ({thru:.1f} steps/s); value = larger")}
 
 
+def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_value):
+    """Learned surrogate on the same workload: 4 BF16 MLP members trained on the GPU on exact
+    samples of KMC trajectories (1024 replicas of 10^3 cells, same composition range, every 12
+    steps, x4 by the jump symmetries), then the surrogate KMC path timed on the bench replicas
+    (same seeds), its fallback rate, and its barrier error against the exact evaluation of the
+    final states."""
+    import torch
+    t0 = time.time()
+    s_occs, s_temps = make_workload(1024, 10, seed + 1)
+    drv = L.KineticMcFirst(bm, None, [10] * 1024, s_occs, s_temps, seed=seed + 1,
+                           maximum_steps=10 ** 9)
+    drv.run(20)
+    E, K, D = [], [], []
+    for _ in range(160):
+        drv.run(12)
+        e, k, d = drv.samples()
+        E.append(e)
+        K.append(k)
+        D.append(d)
+    env, ek, de = L.augment(torch.cat(E), torch.cat(K), torch.cat(D))
+    del drv, E, K, D
+    n_samples = env.shape[0] // 4
+    sur = L.Surrogate(bm, n_members=4, seed=seed)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    mse = sur.train(env, ek, de, epochs=40, batch=4096, lr=3e-3, seed=seed)
+    torch.cuda.synchronize()
+    train_s = time.time() - t1
+    del env, ek, de
+    log(f"surrogate: {n_samples} samples ({time.time() - t0 - train_s:.1f} s), trained in "
+        f"{train_s:.1f} s, mse {np.round(mse, 5)}")
+    threshold, window = 0.010, 20.0
+    drv = L.KineticMcFirst(bm, None, [n] * R, occs, temps, seed=seed, maximum_steps=10 ** 9)
+    drv.set_surrogate(sur, threshold=threshold, window=window)
+    stream = torch.cuda.current_stream()
+    drv.run(warmup, stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    left = steps
+    while left > 0:
+        c = min(chunk, left)
+        drv.run(c, stream)
+        left -= c
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    value = R * steps / (ms / 1e3)
+    st = drv.surrogate_stats()
+    occ_now, state = drv.occupancy(), drv.state()
+    batch = L.ReplicaBatch([n] * R, occ_now)
+    _, ea, _, fl = [x.cpu().numpy() for x in sur.eval_jumps(bm, batch, state["vacancy"], temps,
+                                                                threshold, window)]
+    _, ea0, _ = [x.cpu().numpy() for x in L.eval_jumps(bm, batch, state["vacancy"])]
+    f = fl.astype(bool)
+    kT = 8.617333262145e-5 * np.asarray(temps)[:, None]
+    rel = ea0 < ea0.min(1, keepdims=True) + window * kT
+    err = np.abs(ea - ea0)
+    out = {"value": round(value, 1), "unit": "KMC steps/s", "steps": steps,
+           "speedup_vs_plain": round(value / plain_value, 3),
+           "fallback_rate": round(st["fallbacks"] / max(1, st["events"]), 4),
+           "mae_ea_meV_all_events": round(float(err.mean()) * 1e3, 3),
+           "mae_ea_meV_surrogate_events": round(float(err[~f].mean()) * 1e3, 3) if (~f).any() else None,
+           "mae_ea_meV_relevant_surrogate_events":
+               round(float(err[~f & rel].mean()) * 1e3, 3) if (~f & rel).any() else None,
+           "threshold_meV": threshold * 1e3, "relevance_window_kT": window, "members": 4,
+           "train_samples": n_samples, "train_rows_augmented": 4 * n_samples, "train_s": round(train_s, 2),
+           "dtype": "bf16 weights and activations, fp32 accumulation"}
+    log(f"surrogate KMC: {value:.4g} steps/s ({value / plain_value:.2f}x plain), fallback "
+        f"{out['fallback_rate'] * 100:.1f}%, MAE Ea {out['mae_ea_meV_all_events']} meV (all), "
+        f"{out['mae_ea_meV_relevant_surrogate_events']} meV (relevant surrogate events)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +177,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=20)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-steps", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); replicas are "
@@ -198,6 +275,10 @@ def main():
     }
     log(f"GPU: {value:.4g} KMC steps/s ({12 * value:.4g} event evaluations/s), "
         f"{elapsed:.3f} s for {replica_steps} replica-steps")
+    if not args.no_surrogate:
+        line["surrogate"] = surrogate_leg(L, bm, R, n, occs, temps, args.seed,
+                                          args.surrogate_steps, min(args.warmup, 1000),
+                                          args.chunk, value)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(coeff)
     else:

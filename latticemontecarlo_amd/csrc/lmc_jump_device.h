@@ -26,6 +26,8 @@ struct JumpLds {
   int dir[12];
   // random numbers of the current step (drawn by an idle lane during the gather)
   double u2, log_u1, corr;
+  // selection probabilities by rank; per-event sums (dE, MMM logit, MM2 logit)
+  double prob[12], sums[12][3];
   // events whose environment holds a vacancy off the jump endpoints (no one-hot type)
   int bad;
   // per-lane partial sums (dE, MMM logit, MM2 logit): dense [event * kLanes + lane], solvent
@@ -406,17 +408,19 @@ __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const
   const int k = tid;
   if (k >= 64) return;
   // wave 0: lane 3q + c adds the kL partials of component c of event q; lane q collects them
-  double s = 0.0;
+  // (LDS traffic of one wave completes in program order)
   if (k < 36) {
     const int q = k / 3, c = k - 3 * q;
+    double s = 0.0;
 #pragma unroll
     for (int l = 0; l < kL; ++l) s += L.part[q * kL + l][c];
+    L.sums[q][c] = s;
   }
-  const int q = k < 12 ? k : 0;
-  double de = __shfl(s, 3 * q);
-  double la = __shfl(s, 3 * q + 1);
-  double lb = __shfl(s, 3 * q + 2);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (k >= 12) return;
+  double de = L.sums[k][0], la = L.sums[k][1], lb = L.sums[k][2];
   if (kSolvent) {  // all-solvent sums of this (solvent, migrating species)
     const double *b = T.base + (static_cast<size_t>(sv) * t.S + L.env[k][t.n_slot]) * 3;
     de += b[0];
@@ -467,17 +471,20 @@ __device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
 // same ordered cumulative probabilities (KineticMcFirstOmp.cpp:69-77); the lane holding the first
 // cumulative probability >= u2 (lane 11 when there is none, KineticMcAbstract.cpp:106-123)
 // returns true with the event's rank and the residence time (KineticMcFirstOmp.cpp:79-82).
-__device__ __forceinline__ bool ChooseEvent(const JumpLds &L, const ReplicaLds &Q, int i, int &sel,
+__device__ __forceinline__ bool ChooseEvent(JumpLds &L, const ReplicaLds &Q, int i, int &sel,
                                             double &dt) {
   double total = 0.0;
 #pragma unroll
   for (int j = 0; j < 12; ++j) total += L.rate[j];
-  const double p = L.rate[i] / total;
+  L.prob[i] = L.rate[i] / total;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const double u2 = L.u2;
   double cum = 0.0, prev = 0.0;
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
-    const double pj = __shfl(p, j);
+    const double pj = L.prob[j];
     if (j <= i) {
       prev = cum;
       cum += pj;
