@@ -74,16 +74,21 @@ uint32_t PackSite(const lmc::StateCluster &c, int centre) {
          (static_cast<uint32_t>(other[1]) << 10);
 }
 
-// Jump state-cluster entry: byte offset of the cluster's weight block [0:13] | other1 [13:19] |
-// other2 [19:25] | n-side [31]. Absent sites point at environment slot 63, which always holds
-// species 0, so singlets and pairs index their blocks like triplets.
+// Reduced dE-weight block of a cluster label: singlet 0, pairs 1 + (l - 1) S, triplets
+// 1 + 3 S + (l - 4) S^2; entry r = block + c1 + S c2 (codes of the two other sites).
+int ReducedBlock(int l, int S) {
+  return l == 0 ? 0 : l < 4 ? 1 + (l - 1) * S : 1 + 3 * S + (l - 4) * S * S;
+}
+
+// Jump state-cluster entry: byte offset of its reduced weight block [0:13] | other1 [13:19] |
+// other2 [19:25] | neighbour-endpoint side [31] (enters with the opposite sign). Absent sites
+// point at environment slot 63, which always holds species 0, so singlets and pairs index their
+// blocks like triplets.
 uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side, int S) {
   int other[2] = {63, 63}, k = 0;
   for (int i = 0; i < c.size; ++i)
     if (c.slots[i] != endpoint_slot) other[k++] = c.slots[i];
-  const int l = c.label;
-  const int block = l == 0 ? 0 : l < 4 ? S + (l - 1) * S * S : S + 3 * S * S + (l - 4) * S * S * S;
-  return static_cast<uint32_t>(block * 8) | (static_cast<uint32_t>(other[0]) << 13) |
+  return static_cast<uint32_t>(ReducedBlock(c.label, S) * 8) | (static_cast<uint32_t>(other[0]) << 13) |
          (static_cast<uint32_t>(other[1]) << 19) | (n_side ? 0x80000000u : 0u);
 }
 
@@ -147,15 +152,14 @@ static void BuildSolventTables(const Model &m, const JumpTables &jt, std::vector
   // per slot j: the clusters containing j (j not a jump endpoint), by list type
   std::vector<std::vector<uint32_t>> per[3];
   for (auto &v : per) v.assign(kEnvSlots, {});
-  // state entry: weight-block byte offset [0:13] | j is the second other site [13] | the other
-  // site o [14:20] (63 = none) | neighbour-endpoint side [31]
+  // state entry: reduced weight-block byte offset [0:13] | j is the second other site [13] | the
+  // other site o [14:20] (63 = none) | neighbour-endpoint side [31]
   auto add_state = [&](const std::vector<StateCluster> &cl, int endpoint, bool nside) {
     for (const auto &c : cl) {
       int others[2] = {-1, -1}, n = 0;
       for (int i = 0; i < c.size; ++i)
         if (c.slots[i] != endpoint) others[n++] = c.slots[i];
-      const int l = c.label;
-      const int block = l == 0 ? 0 : l < 4 ? S + (l - 1) * S * S : S + 3 * S * S + (l - 4) * S * S * S;
+      const int block = ReducedBlock(c.label, S);
       for (int i = 0; i < n; ++i) {
         const int other = n == 2 ? others[1 - i] : 63;
         per[0][others[i]].push_back(static_cast<uint32_t>(block * 8) | (static_cast<uint32_t>(i) << 13) |
@@ -257,19 +261,24 @@ const KTables &Upload(Model &m) {
   mm2.insert(mm2.end(), mm2b.begin(), mm2b.end());
   std::vector<uint32_t> site_cl;
   for (const auto &c : st.clusters) site_cl.push_back(PackSite(c, 0));
-  // Jump dE weights by block and (a + S c1 + S^2 c2), migrating species a fastest.
+  // Jump dE weights by migrating species M and reduced index r (see ReducedBlock):
+  // wd[M][r] = w(cluster | endpoint M) - w(cluster | endpoint X); the vacancy's row is zero.
   const int S = m.S;
-  std::vector<double> wd(m.w.size());
-  for (int a = 0; a < S; ++a) {
-    wd[a] = m.w[a];
-    for (int c1 = 0; c1 < S; ++c1) {
-      for (int l = 0; l < 3; ++l) wd[S + l * S * S + a + S * c1] = m.w[S + l * S * S + a * S + c1];
-      for (int c2 = 0; c2 < S; ++c2)
-        for (int l = 0; l < 4; ++l)
-          wd[S + 3 * S * S + l * S * S * S + a + S * c1 + S * S * c2] =
-              m.w[S + 3 * S * S + l * S * S * S + (a * S + c1) * S + c2];
+  const int nr = 1 + 3 * S + 4 * S * S;
+  auto wval = [&](int l, int a, int c1, int c2) {
+    if (l == 0) return m.w[a];
+    if (l < 4) return m.w[S + (l - 1) * S * S + a * S + c1];
+    return m.w[S + 3 * S * S + (l - 4) * S * S * S + (a * S + c1) * S + c2];
+  };
+  std::vector<double> wd(static_cast<size_t>(S) * nr, 0.0);
+  for (int M = 0; M < S; ++M)
+    for (int l = 0; l < 8; ++l) {
+      const int n1 = l == 0 ? 1 : S, n2 = l < 4 ? 1 : S;
+      for (int c1 = 0; c1 < n1; ++c1)
+        for (int c2 = 0; c2 < n2; ++c2)
+          wd[static_cast<size_t>(M) * nr + ReducedBlock(l, S) + c1 + S * c2] =
+              wval(l, M, c1, c2) - wval(l, m.x_code, c1, c2);
     }
-  }
   // A vacancy off the jump endpoints is flagged when the environment is gathered; its one-hot
   // type maps to entry 0 so that every table index stays in range.
   std::vector<int8_t> tt(m.type_tab.begin(), m.type_tab.end());
@@ -349,7 +358,9 @@ const KTables &Upload(Model &m) {
   k.x_code = m.x_code;
   k.F1 = m.F_mmm;
   k.F2 = m.F_mm2;
-  k.nw = static_cast<int>(m.w.size());
+  k.nw = static_cast<int>(m.w.size());  // wd has the same size: S (1 + 3S + 4S^2)
+  if (wd.size() != m.w.size()) throw std::logic_error("dE weight table size");
+  k.n_state_v = static_cast<int>(jt.state_v_only.size());
   k.n_inc = static_cast<int>(inc.size());
   k.solvent_default = 0;
   for (int c = 0; c < m.S; ++c)

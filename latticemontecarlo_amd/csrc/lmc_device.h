@@ -10,8 +10,8 @@ namespace lmc {
 
 // Packed table entries (all slots < 64, 63 = none):
 //   offset   : int8 x | int8 y << 8 | int8 z << 16
-//   state    : byte offset of the weight block in wd [0:13] | other1[13:19] | other2[19:25]
-//              | n_endpoint[31]
+//   state    : byte offset of the reduced weight block in wd [0:13] | other1[13:19]
+//              | other2[19:25] | n_endpoint[31]
 //   orbit    : feature base[0:12] | kind * S * S [12:20] | s0[20:26] | s1[26:32]
 //   mm2      : the forward orbit entries, then the backward ones (slots mirrored)
 //   site     : label[0:3] | other1[4:10] | other2[10:16]
@@ -30,7 +30,7 @@ struct KTables {
   const uint32_t *site_cl;
   const int32_t *site_off;  // [43]
   const double *w;          // theta/N by (label, codes): w1[S] | w2[3][S][S] | w3[4][S][S][S]
-  const double *wd;         // the same values by block and (a + S c1 + S^2 c2) (jump dE)
+  const double *wd;         // jump dE: [M][1 + 3S + 4S^2] w(.|M) - w(.|X), nw entries
   const double *b1;         // [ne][F1], row = one-hot rank of the migrating species
   const double *b2;         // [ne][F2]
   const double *par;        // [S][6]
@@ -45,7 +45,7 @@ struct KTables {
   // the clusters containing j (CSR inc_off[3][61] -> inc, lists: state, MMM, MM2 forward and
   // backward); a real environment adds value - value(all solvent) for each cluster touching a
   // non-solvent slot, once (at its lowest such slot).
-  //   state entry: weight-block byte offset [0:13] | j is the second other site [13]
+  //   state entry: reduced weight-block byte offset [0:13] | j is the second other site [13]
   //                | other site [14:20] (63 = none) | neighbour-endpoint side [31]
   //   orbit entry: feature base [0:12] | kind * S * S [12:20] | j is the second site [20]
   //                | other site [21:27] (63 = none)
@@ -55,6 +55,7 @@ struct KTables {
   int n_inc, solvent_default;
   uint64_t vac_shell_mask;  // env slots within 3NN of the vacancy
   int n_state, n_mmm, n_mm2, n_site, n_eci;  // n_mm2 counts forward + backward entries
+  int n_state_v;            // state entries of the vacancy endpoint (the rest: neighbour's)
   int v_slot, n_slot;
   int kind, S, ne, x_code, F1, F2, nw;
 };

@@ -172,17 +172,16 @@ __device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const ui
 }
 
 // Sums of kU packed state clusters per lane: w(cluster | endpoint = M) - w(cluster | endpoint =
-// X), negated for the neighbour endpoint's clusters; wm, wx: the weight table shifted by M and X
-// entries. The loads of the kU clusters are issued level by level (entries, species, weights).
+// X) from wm, the difference table's row of M (s8 = 8 S), negated for the neighbour endpoint's
+// clusters. The loads of the kU clusters are issued
+// level by level (entries, species, weights).
 template <int kU>
-__device__ __forceinline__ void StateBatch(const char *wm, const char *wx, const uint8_t *e,
-                                           const uint32_t *list, int c, int s8, int ss8,
-                                           double *acc) {
+__device__ __forceinline__ void StateBatch(const char *wm, const uint8_t *e, const uint32_t *list,
+                                           int c, int stride, int s8, double *acc) {
   uint32_t d[kU];
   int c1[kU], c2[kU];
-  double p[kU], q[kU];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
+  for (int u = 0; u < kU; ++u) d[u] = list[c + u * stride];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     c1[u] = e[(d[u] >> 13) & 63];
@@ -190,16 +189,21 @@ __device__ __forceinline__ void StateBatch(const char *wm, const char *wx, const
   }
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    const int o = static_cast<int>(d[u] & 8191) + __umul24(c1[u], s8) + __umul24(c2[u], ss8);
-    p[u] = *reinterpret_cast<const double *>(wm + o);
-    q[u] = *reinterpret_cast<const double *>(wx + o);
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const double v = p[u] - q[u];
+    const int o = static_cast<int>(d[u] & 8191) + (c1[u] << 3) + __umul24(c2[u], s8);
+    const double v = *reinterpret_cast<const double *>(wm + o);
     acc[u] += __hiloint2double(__double2hiint(v) ^ static_cast<int>(d[u] & 0x80000000u),
                                __double2loint(v));
   }
+}
+
+// Sum of the state clusters [c0, c1) of an event, lanes lane, lane + stride, ...
+__device__ __forceinline__ double StateRange(const char *wm, const uint8_t *e, const uint32_t *list,
+                                             int c0, int c1, int lane, int stride, int s8) {
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = c0 + lane;
+  for (; c + 3 * stride < c1; c += 4 * stride) StateBatch<4>(wm, e, list, c, stride, s8, a);
+  for (; c < c1; c += stride) StateBatch<1>(wm, e, list, c, stride, s8, a);
+  return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
 // Sums of kU packed orbit clusters per lane: b[base + one-hot type(kind, c0, c1)].
@@ -227,14 +231,10 @@ __device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T, int t
   const int k = tid / kLanes, lane = tid - k * kLanes;
   const uint8_t *e = L.env[k];
   const int S = t.S, M = e[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
-  const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
-  const int s8 = 8 * S, ss8 = 8 * S * S;
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
-  int c = lane;
-  for (; c + 3 * kLanes < t.n_state; c += 4 * kLanes) StateBatch<4>(wm, wx, e, T.state, c, s8, ss8, a);
-  for (; c < t.n_state; c += kLanes) StateBatch<1>(wm, wx, e, T.state, c, s8, ss8, a);
-  const double de = (a[0] + a[1]) + (a[2] + a[3]);
+  const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / S * 8;
+  const double de = StateRange(wm, e, T.state, 0, t.n_state, lane, kLanes, 8 * S);
+  double a[4];
+  int c;
   double la = 0.0, lb = 0.0;
   // one-hot rank of the migrating species (a vacancy is flagged in the gather)
   const int row = L.tt[M * S];
@@ -262,16 +262,15 @@ __device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T, int t
 // Solvent baseline: change of one state cluster touching slot j (species cj) relative to the
 // all-solvent environment; zero when the cluster's other site o is a lower non-solvent slot
 // (the cluster is counted there).
-__device__ __forceinline__ double SolventState(const char *wm, const char *wx, const uint8_t *e,
-                                               uint32_t d, int j, int cj, int sv, int s8, int ss8) {
+__device__ __forceinline__ double SolventState(const char *wm, const uint8_t *e, uint32_t d, int j,
+                                               int cj, int sv, int s8) {
   const int o = (d >> 14) & 63, co = e[o];
   const bool second = (d >> 13) & 1;
-  const int mj = second ? ss8 : s8, mo = second ? s8 : ss8;
+  const int mj = second ? s8 : 8, mo = second ? 8 : s8;
   const int blk = static_cast<int>(d & 8191);
   const int oa = blk + __umul24(cj, mj) + __umul24(co, mo);
   const int ob = blk + __umul24(sv, mj) + __umul24(o == 63 ? 0 : sv, mo);
-  const double v = (*reinterpret_cast<const double *>(wm + oa) - *reinterpret_cast<const double *>(wx + oa)) -
-                   (*reinterpret_cast<const double *>(wm + ob) - *reinterpret_cast<const double *>(wx + ob));
+  const double v = *reinterpret_cast<const double *>(wm + oa) - *reinterpret_cast<const double *>(wm + ob);
   const double sv_ = __hiloint2double(__double2hiint(v) ^ static_cast<int>(d & 0x80000000u), __double2loint(v));
   return (o != 63 && o < j && co != sv) ? 0.0 : sv_;
 }
@@ -306,9 +305,8 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
   }
   if (!active) return;
   const int S = t.S, M = e[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
-  const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
-  const int s8 = 8 * S, ss8 = 8 * S * S;
+  const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / S * 8;
+  const int s8 = 8 * S;
   const int row = L.tt[M * S];
   const double *b1 = T.b1 + static_cast<size_t>(row) * t.F1;
   const double *b2 = T.b2 + static_cast<size_t>(row) * t.F2;
@@ -329,8 +327,8 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
     const int s1e = two ? offS[j1 + 1] : 0, m1e = two && orbits ? offM[j1 + 1] : 0;
     while (s0 < s0e || s1 < s1e || m0 < m0e || m1 < m1e) {
       const bool bs0 = s0 < s0e, bs1 = s1 < s1e, bm0 = m0 < m0e, bm1 = m1 < m1e;
-      const double vs0 = SolventState(wm, wx, e, T.inc[bs0 ? s0 : 0], j0, c0, sv, s8, ss8);
-      const double vs1 = SolventState(wm, wx, e, T.inc[bs1 ? s1 : 0], j1, c1, sv, s8, ss8);
+      const double vs0 = SolventState(wm, e, T.inc[bs0 ? s0 : 0], j0, c0, sv, s8);
+      const double vs1 = SolventState(wm, e, T.inc[bs1 ? s1 : 0], j1, c1, sv, s8);
       const double vm0 = SolventOrbit(L.tt, b1, e, T.inc[bm0 ? m0 : offM[0]], j0, c0, sv, S);
       const double vm1 = SolventOrbit(L.tt, b1, e, T.inc[bm1 ? m1 : offM[0]], j1, c1, sv, S);
       de += (bs0 ? vs0 : 0.0) + (bs1 ? vs1 : 0.0);

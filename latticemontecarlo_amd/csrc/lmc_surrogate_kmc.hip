@@ -230,11 +230,9 @@ __device__ void FallbackClusters(const KTables &t, JumpLds &L, const Tabs &T, co
     const int k = __ffs(mm) - 1;
     const uint8_t *env = L.env[k];
     const int S_ = t.S, M = env[t.n_slot];
-    const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
-    const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
-    double a[1] = {0.0};
-    for (int c = lane; c < t.n_state; c += le) StateBatch<1>(wm, wx, env, T.state, c, 8 * S_, 8 * S_ * S_, a);
-    de = a[0];
+    const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / S_ * 8;
+    de = StateRange(wm, env, T.state, 0, t.n_state, lane, le, 8 * S_);
+    double a[1];
     const int row = L.tt[M * S_];
     if (t.kind == kE0 || t.kind == kQuartic) {
       const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
@@ -282,11 +280,8 @@ __device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLd
                         int k) {
   const uint8_t *env = L.env[k];
   const int M = env[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + 8 * M;
-  const char *wx = reinterpret_cast<const char *>(T.w) + 8 * t.x_code;
-  double a[1] = {0.0};
-  for (int c = lt; c < t.n_state; c += kSub) StateBatch<1>(wm, wx, env, T.state, c, 8 * t.S, 8 * t.S * t.S, a);
-  double v = a[0];
+  const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / t.S * 8;
+  double v = StateRange(wm, env, T.state, 0, t.n_state, lt, kSub, 8 * t.S);
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
   if ((lt & 63) == 0) S.dsum[g][lt >> 6] = v;
