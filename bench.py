@@ -167,6 +167,77 @@ def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_va
     return out
 
 
+def uneven_workload(args):
+    """Uneven-size batched KMC alongside canonical MC: 256 KMC replicas of n^3 cells with n drawn
+    from 8..40 (2048-256000 sites; Mg, Zn 1-5 at.% each, T 400-700 K) on one stream and 64
+    CanonicalMc replicas of 30^3 cells (108000 sites, Al/Mg/Zn 90/5/5, T 800 -> 300 K linear
+    over the run's trials) on another, concurrently. Prints one JSON line."""
+    import torch
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import synth
+    rng = np.random.default_rng(args.seed)
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0.json")
+    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "E0", seed=args.seed)
+    bm = L.Model(coeff, ["Al", "Mg", "Zn"], L.E0)
+    ps = L.Model(coeff, ["Al", "Mg", "Zn"], L.PAIRSITE)
+    species = ["Al", "Mg", "X", "Zn"]
+    Rk, Rc, nc = 256, 64, 30
+    sizes = rng.integers(8, 41, Rk)
+    kocc, ktemp = [], rng.uniform(400.0, 700.0, Rk)
+    for r, n in enumerate(sizes):
+        N = 4 * int(n) ** 3
+        xm, xz = rng.uniform(0.01, 0.05, 2)
+        kocc.append(synth.random_occupancy(int(n), species, {"Mg": round(xm * N), "Zn": round(xz * N)},
+                                           seed=args.seed * 7 + r))
+    Nc = 4 * nc ** 3
+    cocc = [synth.random_occupancy(nc, species, {"Mg": round(0.05 * Nc), "Zn": round(0.05 * Nc)},
+                                   seed=args.seed * 11 + r, vacancy=False) for r in range(Rc)]
+    ksteps, trials = args.steps, args.cmc_trials
+    log(f"uneven: {Rk} KMC replicas of {int(sizes.min())}^3..{int(sizes.max())}^3 cells "
+        f"({int((4 * sizes.astype(np.int64) ** 3).sum())} sites), {Rc} CMC replicas of {nc}^3")
+    kmc = L.KineticMcFirst(bm, None, [int(n) for n in sizes], kocc, ktemp, seed=args.seed,
+                           maximum_steps=10 ** 9)
+    cmc = L.CanonicalMc(ps, None, [nc] * Rc, cocc, 800.0, 300.0, args.seed, maximum_steps=trials)
+    tune = kmc.autotune(steps=200)
+    sk, sc = torch.cuda.Stream(), torch.cuda.Stream()  # default priority
+    kmc.run(500, sk)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(sk)
+    ev[2].record(sc)
+    for _ in range(max(1, ksteps // args.chunk)):
+        kmc.run(args.chunk, sk)
+    cchunk = max(1, trials // 20)
+    for _ in range(max(1, trials // cchunk)):
+        cmc.run(cchunk, sc)
+    ev[1].record(sk)
+    ev[3].record(sc)
+    torch.cuda.synchronize()
+    tk = ev[0].elapsed_time(ev[1]) / 1e3
+    tc = ev[2].elapsed_time(ev[3]) / 1e3
+    kst = kmc.state()
+    cst = cmc.state()
+    bm.check_errors()
+    kdone = int(kst["steps"].sum()) - 500 * Rk
+    cdone = int(cst["steps"].sum())
+    kval, cval = kdone / tk, cdone / tc
+    log(f"KMC {kval:.4g} steps/s over {Rk} uneven replicas ({tk:.2f} s), CMC {cval:.4g} trials/s "
+        f"over {Rc} replicas ({tc:.2f} s), acceptance {cst['accepted'].sum() / max(1, cdone):.3f}")
+    line = {"metric": "KMC steps/s over all replicas (uneven sizes, with CanonicalMc alongside)",
+            "value": round(kval, 1), "unit": "KMC steps/s", "n_gpus": 1, "steps": ksteps,
+            "warmup": 500, "ms_per_step": round(tk / max(1, ksteps) * 1e3, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "uneven-size batched KMC alongside CanonicalMc", "kmc_replicas": Rk,
+                       "kmc_cells": [int(sizes.min()), int(sizes.max())],
+                       "kmc_formulation": ["dense", "solvent-baseline"][tune["formulation"]],
+                       "cmc_replicas": Rc, "cmc_sites": Nc, "cmc_trials_per_replica": trials,
+                       "cmc_trials_per_s": round(cval, 1),
+                       "cmc_acceptance": round(float(cst["accepted"].sum()) / max(1, cdone), 4)}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +249,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch eagerly instead of a HIP graph")
+    ap.add_argument("--workload", choices=["batched", "uneven"], default="batched")
+    ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -191,6 +265,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.workload == "uneven":
+        uneven_workload(args)
+        return
     R, n = args.replicas, args.ncell
     tmp = tempfile.mkdtemp()
     coeff = os.path.join(tmp, "coeff_e0.json")
@@ -215,38 +292,37 @@ def main():
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} steps: {time.time() - t0:.2f} s")
 
-    launches = []
-    left = args.steps
+    # timed region: args.steps / chunk launches of `chunk` steps, captured once in a HIP graph
+    # (outside the timed region) and replayed
+    n_launch = max(1, args.steps // args.chunk)
+    assert n_launch * args.chunk == args.steps, "--steps must be a multiple of --chunk"
+    if not args.eager:
+        drv.build_graph(args.chunk, n_launch)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    while left > 0:
-        c = min(args.chunk, left)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        drv.run(c, stream)
-        e1.record(stream)
-        launches.append((c, e0, e1))
-        left -= c
     end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    if args.eager:
+        for _ in range(n_launch):
+            drv.run(args.chunk, stream)
+    else:
+        drv.run_graph(args.chunk, n_launch, stream)
     end.record(stream)
-    for i, (c, e0, e1) in enumerate(launches):
-        e1.synchronize()
-        log(f"launch {i + 1}/{len(launches)}: {c} steps in {e0.elapsed_time(e1):.1f} ms")
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     elapsed = start.elapsed_time(end) / 1e3
+    log(f"{n_launch} launches of {args.chunk} steps ({'eager' if args.eager else 'HIP graph'}): "
+        f"{elapsed * 1e3:.1f} ms, {elapsed / n_launch * 1e3:.2f} ms per launch")
     st = drv.state()
     assert int(st["steps"].min()) == total_steps, "every replica must have run every step"
     bm.check_errors()
 
     replica_steps = R * args.steps
     value = replica_steps / elapsed
-    kern = [(c, e0.elapsed_time(e1) / 1e3) for c, e0, e1 in launches]
-    bytes_per_launch = [ALGO_BYTES_PER_STEP * R * c for c, _ in kern]
-    achieved = sum(bytes_per_launch) / sum(t for _, t in kern) / 1e9
+    # algorithmic bytes of one launch over its average duration (the launches run back to back
+    # on one stream, so the region's time over the launch count is the average launch duration)
+    achieved = ALGO_BYTES_PER_STEP * R * args.chunk / (elapsed / n_launch) / 1e9
     traffic = measured_traffic(R, n, args.chunk)
     line = {
         "metric": "KMC steps/s over all replicas (plain GPU path)",
@@ -266,6 +342,7 @@ def main():
                                "T 400-700 K, E0 barrier model, one vacancy per replica",
                    "replicas": R, "sites_per_replica": 4 * n ** 3,
                    "steps_per_replica": args.steps, "steps_per_launch": args.chunk,
+                   "launch": "eager" if args.eager else "hip graph",
                    "formulation": ["dense", "solvent-baseline"][tune["formulation"]],
                    "event_evaluations_per_s": round(12 * value, 1),
                    "wall_s": round(wall, 3)},

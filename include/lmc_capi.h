@@ -181,6 +181,12 @@ int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dens
 /* Advance every replica by `steps` steps (fewer if it reaches maximum_steps or escapes).
  * Asynchronous on `stream`. */
 int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream);
+/* `launches` launches of `steps_per_launch` steps each, captured once in a HIP graph (rebuilt when
+ * the arguments, the formulation or the surrogate change) and replayed on `stream`; the same
+ * work as that many lmc_kmc_run calls. Needs tracing off. Asynchronous. */
+int lmc_kmc_run_graph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches, void *stream);
+/* Captures that graph without running it (lmc_kmc_run_graph with the same arguments replays it). */
+int lmc_kmc_build_graph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches);
 /* Host copies of the state arrays (each of length R, or 3R for disp/com); any may be NULL.
  * disp: vacancy displacement in units of a/2; com: solute COM trajectory (A). Synchronous. */
 int lmc_kmc_state(lmc_kmc *k, uint64_t *steps, double *time, double *energy, double *abs_energy,

@@ -252,6 +252,14 @@ class KineticMcFirst:
         s = capi.stream_ptr(stream) if stream is not None else None
         check(lib().lmc_kmc_run(self.h, steps, s))
 
+    def build_graph(self, steps_per_launch, launches):
+        check(lib().lmc_kmc_build_graph(self.h, steps_per_launch, launches))
+
+    def run_graph(self, steps_per_launch, launches, stream=None):
+        """`launches` x `steps_per_launch` steps replayed from a captured HIP graph."""
+        s = capi.stream_ptr(stream) if stream is not None else None
+        check(lib().lmc_kmc_run_graph(self.h, steps_per_launch, launches, s))
+
     def state(self):
         R = self.R
         out = dict(steps=np.zeros(R, np.uint64), time=np.zeros(R), energy=np.zeros(R),

@@ -111,6 +111,8 @@ SIGNATURES = {
                                            C.c_double, _vp, _vp, _vp, _vp, _vp]),
     "lmc_kmc_set_surrogate": (C.c_int, [_vp, _vp, C.c_double, C.c_double]),
     "lmc_kmc_surrogate_stats": (C.c_int, [_vp, _u64p, _u64p]),
+    "lmc_kmc_run_graph": (C.c_int, [_vp, _u64, _i32, _vp]),
+    "lmc_kmc_build_graph": (C.c_int, [_vp, _u64, _i32]),
 }
 
 _lib = None

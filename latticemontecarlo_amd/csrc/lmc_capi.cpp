@@ -80,20 +80,20 @@ int ReducedBlock(int l, int S) {
   return l == 0 ? 0 : l < 4 ? 1 + (l - 1) * S : 1 + 3 * S + (l - 4) * S * S;
 }
 
-// Jump state-cluster entry: byte offset of its reduced weight block [0:13] | other1 [13:19] |
-// other2 [19:25] | neighbour-endpoint side [31] (enters with the opposite sign). Absent sites
-// point at environment slot 63, which always holds species 0, so singlets and pairs index their
-// blocks like triplets.
+// Jump state-cluster entry: other1 [0:8] | other2 [8:16] | byte offset of its reduced weight
+// block [16:29] | neighbour-endpoint side [31] (enters with the opposite sign). The slots are
+// whole bytes so that reading them costs no extraction. Absent sites point at environment slot
+// 63, which always holds species 0, so singlets and pairs index their blocks like triplets.
 uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side, int S) {
   int other[2] = {63, 63}, k = 0;
   for (int i = 0; i < c.size; ++i)
     if (c.slots[i] != endpoint_slot) other[k++] = c.slots[i];
-  return static_cast<uint32_t>(ReducedBlock(c.label, S) * 8) | (static_cast<uint32_t>(other[0]) << 13) |
-         (static_cast<uint32_t>(other[1]) << 19) | (n_side ? 0x80000000u : 0u);
+  return static_cast<uint32_t>(other[0]) | (static_cast<uint32_t>(other[1]) << 8) |
+         (static_cast<uint32_t>(ReducedBlock(c.label, S) * 8) << 16) | (n_side ? 0x80000000u : 0u);
 }
 
-// Orbit-cluster entry: feature base [0:12] | kind * S * S [12:20] | slot0 [20:26] | slot1 [26:32]
-// (63 = none). Backward entries read the species of canonical slot x from slot mirror[x].
+// Orbit-cluster entry: slot0 [0:8] | slot1 [8:16] (63 = none) | feature base [16:28] | kind
+// [28:30]. Backward entries read the species of canonical slot x from slot mirror[x].
 std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<int> &base, int S,
                                  const std::vector<int> *mirror) {
   std::vector<uint32_t> v;
@@ -106,8 +106,9 @@ std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<in
       s0 = (*mirror)[s0];
       s1 = s1 < 0 ? -1 : (*mirror)[s1];
     }
-    v.push_back(b | (static_cast<uint32_t>(kind * S * S) << 12) |
-                (static_cast<uint32_t>(s0) << 20) | (static_cast<uint32_t>(s1 < 0 ? 63 : s1) << 26));
+    (void)S;
+    v.push_back(static_cast<uint32_t>(s0) | (static_cast<uint32_t>(s1 < 0 ? 63 : s1) << 8) |
+                (b << 16) | (static_cast<uint32_t>(kind) << 28));
   }
   return v;
 }
@@ -454,7 +455,19 @@ struct lmc_kmc {
   const lmc_surrogate *sur = nullptr;
   double sur_threshold = 0.01, sur_window = 20.0;
   uint64_t *n_sur = nullptr, *n_fb = nullptr;
+  // captured launch sequence (lmc_kmc_run_graph)
+  hipGraphExec_t graph = nullptr;
+  hipStream_t capture = nullptr;
+  uint64_t graph_steps = 0;
+  int32_t graph_launches = 0, graph_form = -1;
+  const void *graph_sur = nullptr;
+  void DropGraph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
   ~lmc_kmc() {
+    DropGraph();
+    if (capture) (void)hipStreamDestroy(capture);
     DevFree(n_sur);
     DevFree(n_fb);
     DevFree(d.occ);
@@ -1043,6 +1056,59 @@ int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity) {
 }
 
 static hipError_t RunSurrogateKmc(lmc_kmc *k, uint64_t steps, hipStream_t s);
+static hipError_t PrepareSurrogate(lmc_kmc *k);
+
+static void BuildGraph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches);
+
+int lmc_kmc_build_graph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    BuildGraph(k, steps_per_launch, launches);
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_run_graph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches, void *stream) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    BuildGraph(k, steps_per_launch, launches);
+    Check(hipGraphLaunch(k->graph, S(stream)), "hipGraphLaunch");
+    return LMC_OK;
+  });
+}
+
+static void BuildGraph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches) {
+  {
+    if (steps_per_launch == 0 || launches <= 0) throw std::invalid_argument("nothing to run");
+    if (k->d.trace_cap) throw std::invalid_argument("graph replay needs tracing off");
+    const auto &t = lmc::Upload(*k->bm->m);
+    if (!k->graph || k->graph_steps != steps_per_launch || k->graph_launches != launches ||
+        k->graph_form != k->formulation || k->graph_sur != static_cast<const void *>(k->sur)) {
+      k->DropGraph();
+      if (!k->capture) Check(hipStreamCreateWithFlags(&k->capture, hipStreamNonBlocking), "hipStreamCreate");
+      if (k->sur)
+        Check(PrepareSurrogate(k), "surrogate KMC set-up");
+      else
+        Check(lmc::PrepareKmc(t, k->d.R, k->formulation == 1), "KMC set-up");
+      Check(hipStreamBeginCapture(k->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      hipError_t e = hipSuccess;
+      for (int32_t i = 0; i < launches && e == hipSuccess; ++i)
+        e = k->sur ? RunSurrogateKmc(k, steps_per_launch, k->capture)
+                   : lmc::LaunchKmc(t, k->d, steps_per_launch, 0, k->capture, k->formulation == 1);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(k->capture, &g);
+      Check(e, "KMC launch under capture");
+      Check(e2, "hipStreamEndCapture");
+      const hipError_t e3 = hipGraphInstantiate(&k->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      Check(e3, "hipGraphInstantiate");
+      k->graph_steps = steps_per_launch;
+      k->graph_launches = launches;
+      k->graph_form = k->formulation;
+      k->graph_sur = k->sur;
+    }
+  }
+}
 
 int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream) {
   return Guard([&] {
@@ -1617,6 +1683,10 @@ int lmc_surrogate_train(lmc_surrogate *s, const uint8_t *d_env, const double *d_
       }
     return LMC_OK;
   });
+}
+
+static hipError_t PrepareSurrogate(lmc_kmc *k) {
+  return lmc::PrepareSurrogateKmc(lmc::Upload(*k->bm->m), k->sur->C);
 }
 
 static hipError_t RunSurrogateKmc(lmc_kmc *k, uint64_t steps, hipStream_t s) {

@@ -10,9 +10,9 @@ namespace lmc {
 
 // Packed table entries (all slots < 64, 63 = none):
 //   offset   : int8 x | int8 y << 8 | int8 z << 16
-//   state    : byte offset of the reduced weight block in wd [0:13] | other1[13:19]
-//              | other2[19:25] | n_endpoint[31]
-//   orbit    : feature base[0:12] | kind * S * S [12:20] | s0[20:26] | s1[26:32]
+//   state    : other1[0:8] | other2[8:16] | byte offset of the reduced weight block in wd
+//              [16:29] | n_endpoint[31]
+//   orbit    : s0[0:8] | s1[8:16] | feature base[16:28] | kind[28:30]
 //   mm2      : the forward orbit entries, then the backward ones (slots mirrored)
 //   site     : label[0:3] | other1[4:10] | other2[10:16]
 constexpr int kMaxStateClusters = 512;
@@ -121,6 +121,9 @@ hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *
                            uint8_t *env_out = nullptr, double *ekra_out = nullptr);
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
                      hipStream_t s, bool solvent);
+// Everything LaunchKmc sets up outside the stream (LDS plan, kernel attributes), so that its
+// launches can then be captured in a graph.
+hipError_t PrepareKmc(const KTables &t, int32_t R, bool solvent);
 hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off,
                         const int32_t *ncell, const int64_t *pairs, int32_t R, double *de,
                         hipStream_t s);

@@ -60,7 +60,9 @@ struct Tabs {  // resolved table pointers (LDS or global)
 
 // Per-replica LDS tables (threads tid of this replica's block) and the workgroup's shared
 // dynamic tables (threads gtid of gsize); returns the resolved table pointers.
-template <bool kSolvent>
+// kBL: the barrier weight tables live in LDS (lay.b1_lds, and lay.b2_lds for the quartic model),
+// known at compile time so that their reads are LDS instructions.
+template <bool kSolvent, bool kBL>
 __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const DynLayout &lay,
                                int tid, int lsize, int gtid, int gsize) {
   for (int i = tid; i < 12 * kEnvSlots; i += lsize) {
@@ -76,18 +78,16 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
   double *w = reinterpret_cast<double *>(dyn + lay.w);
   for (int i = gtid; i < t.nw; i += gsize) w[i] = t.wd[i];
   T.w = w;
-  if (lay.b1_lds) {
+  if (kBL) {
     double *p = reinterpret_cast<double *>(dyn + lay.b1);
     for (int i = gtid; i < t.ne * t.F1; i += gsize) p[i] = t.b1[i];
     T.b1 = p;
+    double *q = reinterpret_cast<double *>(dyn + lay.b2);
+    if (t.kind == kQuartic)
+      for (int i = gtid; i < t.ne * t.F2; i += gsize) q[i] = t.b2[i];
+    T.b2 = q;
   } else {
     T.b1 = t.b1;
-  }
-  if (lay.b2_lds) {
-    double *p = reinterpret_cast<double *>(dyn + lay.b2);
-    for (int i = gtid; i < t.ne * t.F2; i += gsize) p[i] = t.b2[i];
-    T.b2 = p;
-  } else {
     T.b2 = t.b2;
   }
   if (kSolvent) {
@@ -184,12 +184,12 @@ __device__ __forceinline__ void StateBatch(const char *wm, const uint8_t *e, con
   for (int u = 0; u < kU; ++u) d[u] = list[c + u * stride];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    c1[u] = e[(d[u] >> 13) & 63];
-    c2[u] = e[(d[u] >> 19) & 63];
+    c1[u] = e[d[u] & 0xFF];
+    c2[u] = e[(d[u] >> 8) & 0xFF];
   }
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    const int o = static_cast<int>(d[u] & 8191) + (c1[u] << 3) + __umul24(c2[u], s8);
+    const int o = static_cast<int>((d[u] >> 16) & 8191) + (c1[u] << 3) + __umul24(c2[u], s8);
     const double v = *reinterpret_cast<const double *>(wm + o);
     acc[u] += __hiloint2double(__double2hiint(v) ^ static_cast<int>(d[u] & 0x80000000u),
                                __double2loint(v));
@@ -206,23 +206,25 @@ __device__ __forceinline__ double StateRange(const char *wm, const uint8_t *e, c
   return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
-// Sums of kU packed orbit clusters per lane: b[base + one-hot type(kind, c0, c1)].
+// Sums of kU packed orbit clusters per lane: b[base + one-hot type(kind, c0, c1)]; ss = S * S.
 template <int kU>
 __device__ __forceinline__ void OrbitBatch(const int8_t *tt, const double *bm, const uint8_t *e,
                                            const uint32_t *list, int c, int S, double *acc) {
+  const int ss = S * S;
   uint32_t d[kU];
   int c0[kU], c1[kU], ty[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    c0[u] = e[(d[u] >> 20) & 63];
-    c1[u] = e[d[u] >> 26];
+    c0[u] = e[d[u] & 0xFF];
+    c1[u] = e[(d[u] >> 8) & 0xFF];
   }
 #pragma unroll
-  for (int u = 0; u < kU; ++u) ty[u] = tt[((d[u] >> 12) & 255) + __umul24(c0[u], S) + c1[u]];
+  for (int u = 0; u < kU; ++u)
+    ty[u] = tt[__umul24(d[u] >> 28, ss) + __umul24(c0[u], S) + c1[u]];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) acc[u] += bm[(d[u] & 4095) + ty[u]];
+  for (int u = 0; u < kU; ++u) acc[u] += bm[((d[u] >> 16) & 4095) + ty[u]];
 }
 
 // Phase 2, dense: per-lane partial sums of every cluster's weight (dE, MMM logit, MM2 logit).

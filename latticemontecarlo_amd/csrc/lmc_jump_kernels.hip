@@ -28,7 +28,7 @@
 namespace lmc {
 namespace {
 
-template <bool kSolvent>
+template <bool kSolvent, bool kBL>
 __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout lay,
                                                             const uint8_t *occ,
                                                             const int64_t *off,
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout
     return;
   }
   const Tabs T =
-      LoadJumpTables<kSolvent>(t, L, dyn, lay, threadIdx.x, kThreads, threadIdx.x, kThreads);
+      LoadJumpTables<kSolvent, kBL>(t, L, dyn, lay, threadIdx.x, kThreads, threadIdx.x, kThreads);
   __syncthreads();
   EvaluateJumps<kSolvent>(t, L, T, occ + off[r], ncell[r], vac[r], t.solvent_default, 0.0, err,
                           threadIdx.x);
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout
   }
 }
 
-template <bool kSolvent>
+template <bool kSolvent, bool kBL>
 __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout lay, KmcDev k,
                                                       uint64_t nsteps, uint64_t trace_base) {
   __shared__ JumpLds L;
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
   extern __shared__ char dyn[];
   const int r = blockIdx.x;
   const int tid = threadIdx.x;
-  const Tabs T = LoadJumpTables<kSolvent>(t, L, dyn, lay, tid, kThreads, tid, kThreads);
+  const Tabs T = LoadJumpTables<kSolvent, kBL>(t, L, dyn, lay, tid, kThreads, tid, kThreads);
   uint8_t *occ = k.occ + k.off[r];
   const int n = k.ncell[r];
   if (tid == 0) {
@@ -174,10 +174,15 @@ DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
     lay.b1 = static_cast<int>(at);
     at = al(at + b1);
   }
-  if (t.kind == kQuartic && lay.b1_lds && stat + at + b2 <= share) {
-    lay.b2_lds = 1;
-    lay.b2 = static_cast<int>(at);
-    at = al(at + b2);
+  if (t.kind == kQuartic && lay.b1_lds) {
+    if (stat + at + b2 <= share) {
+      lay.b2_lds = 1;
+      lay.b2 = static_cast<int>(at);
+      at = al(at + b2);
+    } else {  // both barrier tables in LDS or neither
+      at = lay.b1;
+      lay.b1_lds = 0;
+    }
   }
   lay.bytes = static_cast<int>(at);
   return lay;
@@ -205,7 +210,16 @@ hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *
                        vac, nb, ea, de, err, env_out, ekra_out);
     return hipGetLastError();
   };
-  return solvent ? go(EvalJumpsKernel<true>) : go(EvalJumpsKernel<false>);
+  if (lay.b1_lds) return solvent ? go(EvalJumpsKernel<true, true>) : go(EvalJumpsKernel<false, true>);
+  return solvent ? go(EvalJumpsKernel<true, false>) : go(EvalJumpsKernel<false, false>);
+}
+
+hipError_t PrepareKmc(const KTables &t, int32_t R, bool solvent) {
+  if (R <= 0) return hipSuccess;
+  const DynLayout lay = PlanLds(t, R, solvent);
+  if (lay.b1_lds)
+    return solvent ? AllowLds(KmcKernel<true, true>, lay.bytes) : AllowLds(KmcKernel<false, true>, lay.bytes);
+  return solvent ? AllowLds(KmcKernel<true, false>, lay.bytes) : AllowLds(KmcKernel<false, false>, lay.bytes);
 }
 
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
@@ -219,7 +233,8 @@ hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t
                        trace_base);
     return hipGetLastError();
   };
-  return solvent ? go(KmcKernel<true>) : go(KmcKernel<false>);
+  if (lay.b1_lds) return solvent ? go(KmcKernel<true, true>) : go(KmcKernel<false, true>);
+  return solvent ? go(KmcKernel<true, false>) : go(KmcKernel<false, false>);
 }
 
 }  // namespace lmc

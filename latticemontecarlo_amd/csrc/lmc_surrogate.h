@@ -51,6 +51,7 @@ struct SurOut {  // stateless evaluation, event (rank) order
 };
 hipError_t LaunchSurrogateKmc(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
                               uint64_t trace_base, hipStream_t s);
+hipError_t PrepareSurrogateKmc(const KTables &t, int C);
 hipError_t LaunchSurrogateEval(const KTables &t, const KmcDev &k, const SurDev &sd,
                                const SurOut &out, hipStream_t s);
 

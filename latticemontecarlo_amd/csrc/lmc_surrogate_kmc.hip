@@ -287,7 +287,7 @@ __device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLd
   if ((lt & 63) == 0) S.dsum[g][lt >> 6] = v;
 }
 
-template <int C, bool kKmc>
+template <int C, bool kKmc, bool kBL>
 __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout lay, KmcDev k,
                                                           SurDev sd, uint64_t nsteps,
                                                           uint64_t trace_base, SurOut out) {
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
   const bool live = r < k.R;
   JumpLds &L = Ls[g];
   ReplicaLds &Q = Qs[g];
-  const Tabs T = LoadJumpTables<false>(t, L, dyn, lay, lt, kSub, tid, kBlock);
+  const Tabs T = LoadJumpTables<false, kBL>(t, L, dyn, lay, lt, kSub, tid, kBlock);
   uint8_t *occ = live ? k.occ + k.off[r] : nullptr;
   const int n = live ? k.ncell[r] : 0;
   if (tid < kEnvRow) S.none[tid] = 0xFF;
@@ -452,10 +452,15 @@ DynLayout PlanSurrogateLds(const KTables &t) {
     lay.b1 = static_cast<int>(at);
     at = al(at + b1);
   }
-  if (t.kind == kQuartic && lay.b1_lds && stat + at + b2 <= share) {
-    lay.b2_lds = 1;
-    lay.b2 = static_cast<int>(at);
-    at = al(at + b2);
+  if (t.kind == kQuartic && lay.b1_lds) {
+    if (stat + at + b2 <= share) {
+      lay.b2_lds = 1;
+      lay.b2 = static_cast<int>(at);
+      at = al(at + b2);
+    } else {  // both barrier tables in LDS or neither
+      at = lay.b1;
+      lay.b1_lds = 0;
+    }
   }
   lay.bytes = static_cast<int>(at);
   return lay;
@@ -463,8 +468,13 @@ DynLayout PlanSurrogateLds(const KTables &t) {
 
 template <typename K>
 hipError_t AllowSurrogateLds(K kernel, size_t bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+  static int allowed = -1;  // per kernel instance: the largest size already allowed
+  if (static_cast<int>(bytes) <= allowed) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           static_cast<int>(bytes));
+  if (e == hipSuccess) allowed = static_cast<int>(bytes);
+  return e;
 }
 
 template <bool kKmc>
@@ -481,12 +491,29 @@ hipError_t LaunchSurrogate(const KTables &t, const KmcDev &k, const SurDev &sd, 
                        trace_base, out);
     return hipGetLastError();
   };
-  if (sd.w.C == 2) return go(SurrogateKernel<2, kKmc>);
-  if (sd.w.C == 3) return go(SurrogateKernel<3, kKmc>);
+  if (lay.b1_lds) {
+    if (sd.w.C == 2) return go(SurrogateKernel<2, kKmc, true>);
+    if (sd.w.C == 3) return go(SurrogateKernel<3, kKmc, true>);
+  } else {
+    if (sd.w.C == 2) return go(SurrogateKernel<2, kKmc, false>);
+    if (sd.w.C == 3) return go(SurrogateKernel<3, kKmc, false>);
+  }
   return hipErrorInvalidValue;
 }
 
 }  // namespace
+
+hipError_t PrepareSurrogateKmc(const KTables &t, int C) {
+  const DynLayout lay = PlanSurrogateLds(t);
+  if (lay.b1_lds) {
+    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, true, true>, lay.bytes);
+    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, true, true>, lay.bytes);
+  } else {
+    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, true, false>, lay.bytes);
+    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, true, false>, lay.bytes);
+  }
+  return hipErrorInvalidValue;
+}
 
 hipError_t LaunchSurrogateKmc(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
                               uint64_t trace_base, hipStream_t s) {
