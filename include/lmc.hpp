@@ -9,6 +9,7 @@
 //   pred::EnergyChangePredictorPairSite            EnergyChangePredictorPairSite.h:16-25
 //   mc::KineticMcFirstBatch, mc::CanonicalMcBatch  KineticMcFirstOmp.h:11-25,
 //                                                  CanonicalMcSerial.h:11-20 (R replicas each)
+//   pred::Surrogate                                no counterpart (learned jump evaluation)
 // Elements are strings ("Al", "X"); species codes are indices into sorted(element_set + "X").
 // Errors: LMC_ERR_IO -> std::runtime_error, LMC_ERR_MODEL -> std::out_of_range,
 // LMC_ERR_ARG -> std::invalid_argument, LMC_ERR_HIP -> std::runtime_error, each carrying
@@ -291,6 +292,37 @@ class EnergyChangePredictorPairSite {
   lmc::ModelHandle m_;
 };
 
+// Learned stand-in for a barrier predictor's jump evaluation (no reference counterpart): an
+// ensemble of BF16 MLPs trained on the GPU on exact samples, with the predictor as fallback.
+// Device pointers are from DeviceBuffer, hipMalloc or any HIP allocation.
+class Surrogate {
+ public:
+  Surrogate(const VacancyMigrationPredictor &barrier, int members = 4, uint64_t seed = 0) {
+    lmc_surrogate *s = nullptr;
+    lmc::Check(lmc_surrogate_create(barrier.handle(), members, seed, &s));
+    s_.reset(s);
+  }
+  // Adam on (environment rows [n][64], E_kra, dE) device arrays; returns the last epoch's mean
+  // squared error per member and output (normalised units).
+  std::vector<double> Train(const uint8_t *d_env, const double *d_ekra, const double *d_de,
+                            int32_t n_rows, int32_t epochs, int32_t batch = 2048,
+                            double lr = 3e-3, uint64_t shuffle_seed = 0) {
+    int32_t m = 0, k = 0, p = 0, c = 0, codes[4], log_ekra = 0;
+    lmc::Check(lmc_surrogate_layout(s_.get(), &m, &k, &p, &c, codes, &log_ekra));
+    std::vector<double> mse(static_cast<size_t>(m) * 2);
+    lmc::Check(lmc_surrogate_train(s_.get(), d_env, d_ekra, d_de, n_rows, epochs, batch, lr,
+                                   shuffle_seed, mse.data()));
+    return mse;
+  }
+  [[nodiscard]] lmc_surrogate *handle() const { return s_.get(); }
+
+ private:
+  struct Free {
+    void operator()(lmc_surrogate *s) const { lmc_surrogate_free(s); }
+  };
+  std::unique_ptr<lmc_surrogate, Free> s_;
+};
+
 }  // namespace pred
 
 namespace mc {
@@ -329,7 +361,32 @@ class KineticMcFirstBatch {
     return chosen;
   }
   void Run(uint64_t steps, void *stream = nullptr) { lmc::Check(lmc_kmc_run(k_.get(), steps, stream)); }
+  // `launches` launches of `steps_per_launch` steps replayed from one captured HIP graph.
+  void RunGraph(uint64_t steps_per_launch, int32_t launches, void *stream = nullptr) {
+    lmc::Check(lmc_kmc_run_graph(k_.get(), steps_per_launch, launches, stream));
+  }
   void Simulate() { Run(max_steps_ + 1); }
+  // Jump evaluation by `s` (nullptr: the cluster expansion) from the next run on.
+  void SetSurrogate(const pred::Surrogate *s, double threshold = 0.01, double window = 20.0) {
+    lmc::Check(lmc_kmc_set_surrogate(k_.get(), s ? s->handle() : nullptr, threshold, window));
+  }
+  // (events evaluated, events that fell back to the cluster expansion)
+  [[nodiscard]] std::pair<uint64_t, uint64_t> SurrogateStats() const {
+    uint64_t ev = 0, fb = 0;
+    lmc::Check(lmc_kmc_surrogate_stats(k_.get(), &ev, &fb));
+    return {ev, fb};
+  }
+  // Exact training samples of the 12 jumps of every replica in its current state: device
+  // arrays d_env[12 R][64], d_ekra[12 R], d_de[12 R].
+  void Samples(const pred::VacancyMigrationPredictor &barrier, uint8_t *d_env, double *d_ekra,
+               double *d_de, void *stream = nullptr) {
+    uint8_t *occ = nullptr;
+    const int64_t *off = nullptr, *vac = nullptr;
+    const int32_t *nc = nullptr;
+    lmc::Check(lmc_kmc_device_views(k_.get(), &occ, &off, &nc, &vac));
+    lmc::Check(lmc_jump_samples(barrier.handle(), occ, off, nc, vac, static_cast<int32_t>(R_),
+                                d_env, d_ekra, d_de, stream));
+  }
   struct State {
     std::vector<uint64_t> steps;
     std::vector<double> time, energy, absolute_energy, temperature;

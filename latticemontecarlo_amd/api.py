@@ -378,33 +378,6 @@ class CanonicalMc:
         sizes = 4 * self.ncell.astype(np.int64) ** 3
         return np.split(out, np.cumsum(sizes)[:-1])
 
-    def set_surrogate(self, surrogate, threshold=0.01, window=20.0):
-        """Evaluate jumps with `surrogate` (None: the cluster expansion) from the next run on."""
-        check(lib().lmc_kmc_set_surrogate(self.h, surrogate.h if surrogate else None, threshold,
-                                          window))
-        self._surrogate = surrogate
-
-    def surrogate_stats(self):
-        ev, fb = C.c_uint64(), C.c_uint64()
-        check(lib().lmc_kmc_surrogate_stats(self.h, C.byref(ev), C.byref(fb)))
-        return dict(events=ev.value, fallbacks=fb.value)
-
-    def samples(self, model=None, stream=None):
-        """Exact surrogate training samples (env rows, E_kra, dE) of the 12 jumps of every
-        replica in its current state, evaluated on the GPU with `model` (default: the driver's
-        barrier model)."""
-        torch = _torch()
-        occ, off, nc, vac = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
-        check(lib().lmc_kmc_device_views(self.h, C.byref(occ), C.byref(off), C.byref(nc),
-                                         C.byref(vac)))
-        m = model if model is not None else self._models[0]
-        env = torch.empty((12 * self.R, 64), dtype=torch.uint8, device="cuda")
-        ekra = torch.empty(12 * self.R, dtype=torch.float64, device="cuda")
-        de = torch.empty(12 * self.R, dtype=torch.float64, device="cuda")
-        check(lib().lmc_jump_samples(m.h, occ, off, nc, vac, self.R, dptr(env), dptr(ekra),
-                                     dptr(de), capi.stream_ptr(stream)))
-        return env, ekra, de
-
     def trace(self):
         n = self.R * self.trace_cap
         p, q = np.zeros(n, np.int64), np.zeros(n, np.int64)

@@ -74,41 +74,90 @@ uint32_t PackSite(const lmc::StateCluster &c, int centre) {
          (static_cast<uint32_t>(other[1]) << 10);
 }
 
-// Reduced dE-weight block of a cluster label: singlet 0, pairs 1 + (l - 1) S, triplets
-// 1 + 3 S + (l - 4) S^2; entry r = block + c1 + S c2 (codes of the two other sites).
-int ReducedBlock(int l, int S) {
-  return l == 0 ? 0 : l < 4 ? 1 + (l - 1) * S : 1 + 3 * S + (l - 4) * S * S;
+// Reduced dE-weight block of a cluster label over element ranks (n = number of elements):
+// singlet 0, pairs 1 + (l - 1) n, triplets 1 + 3 n + (l - 4) n^2; entry r = block + r1 + n r2
+// (ranks of the two other sites).
+int ReducedBlock(int l, int n) {
+  return l == 0 ? 0 : l < 4 ? 1 + (l - 1) * n : 1 + 3 * n + (l - 4) * n * n;
 }
 
-// Jump state-cluster entry: other1 [0:8] | other2 [8:16] | byte offset of its reduced weight
-// block [16:29] | neighbour-endpoint side [31] (enters with the opposite sign). The slots are
-// whole bytes so that reading them costs no extraction. Absent sites point at environment slot
-// 63, which always holds species 0, so singlets and pairs index their blocks like triplets.
-uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side, int S) {
+// Jump state-cluster entry: other1 [0:8] | 2 * other2 [8:16] | byte offset of its weight
+// [16:32]. The kernel reads other1 from the event's rank*8 array (bytes) and other2 from its
+// rank*8*ne array (16-bit words), so the two bytes are the array offsets themselves. The
+// neighbour endpoint's clusters (opposite sign) index the negated copy of the table, at
+// neg_bytes. Absent sites point at slot 63, whose rank entries are 0, so singlets and pairs
+// index their blocks like triplets.
+uint32_t PackState(const lmc::StateCluster &c, int endpoint_slot, bool n_side, int ne,
+                   int neg_bytes) {
   int other[2] = {63, 63}, k = 0;
   for (int i = 0; i < c.size; ++i)
     if (c.slots[i] != endpoint_slot) other[k++] = c.slots[i];
-  return static_cast<uint32_t>(other[0]) | (static_cast<uint32_t>(other[1]) << 8) |
-         (static_cast<uint32_t>(ReducedBlock(c.label, S) * 8) << 16) | (n_side ? 0x80000000u : 0u);
+  const int at = ReducedBlock(c.label, ne) * 8 + (n_side ? neg_bytes : 0);
+  if (at > 65535) throw std::invalid_argument("dE weight table too large for the packed tables");
+  return static_cast<uint32_t>(other[0]) | (static_cast<uint32_t>(2 * other[1]) << 8) |
+         (static_cast<uint32_t>(at) << 16);
 }
 
-// Orbit-cluster entry: slot0 [0:8] | slot1 [8:16] (63 = none) | feature base [16:28] | kind
-// [28:30]. Backward entries read the species of canonical slot x from slot mirror[x].
-std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<int> &base, int S,
+// Expanded barrier-feature layout: orbit o owns ne (singlet) or ne^2 (pair) consecutive
+// weights, indexed by the element ranks (r0 ne + r1 for a pair, r for a singlet), so that the
+// kernel needs no one-hot type lookup. Returns the bases and sets the total length.
+std::vector<int> ExpandedBases(const lmc::OrbitTable &ot, int ne, int &total) {
+  std::vector<int> b(ot.orbit_kind.size());
+  total = 0;
+  for (size_t o = 0; o < b.size(); ++o) {
+    b[o] = total;
+    total += ot.orbit_kind[o] == 0 ? ne : ne * ne;
+  }
+  return b;
+}
+
+// Expanded weights [ne][total] of one model's features b[S][F] (rows by element rank).
+std::vector<double> ExpandWeights(const lmc::Model &m, const lmc::OrbitTable &ot,
+                                  const std::vector<int> &base, const std::vector<double> &b,
+                                  int F, const std::vector<int> &ebase, int total) {
+  const int S = m.S, ne = m.ne;
+  std::vector<int> code;  // element rank -> species code
+  for (int c = 0; c < S; ++c)
+    if (c != m.x_code) code.push_back(c);
+  std::vector<double> v(static_cast<size_t>(ne) * total, 0.0);
+  for (int r = 0; r < ne; ++r) {
+    const double *row = b.data() + static_cast<size_t>(code[r]) * F;
+    double *out = v.data() + static_cast<size_t>(r) * total;
+    for (size_t o = 0; o < ebase.size(); ++o) {
+      const int kind = ot.orbit_kind[o];
+      const int n1 = kind == 0 ? 1 : ne;
+      for (int r0 = 0; r0 < n1; ++r0)
+        for (int r1 = 0; r1 < ne; ++r1) {
+          const int a = kind == 0 ? code[r1] : code[r0], bb = kind == 0 ? code[0] : code[r1];
+          const int ty = m.type_tab[(kind * S + a) * S + bb];
+          if (ty < 0) throw std::logic_error("one-hot type of an element pair");
+          out[ebase[o] + (kind == 0 ? r1 : r0 * ne + r1)] = row[base[o] + ty];
+        }
+    }
+  }
+  return v;
+}
+
+// Orbit-cluster entry: 2 * slot0 [0:8] | slot1 [8:16] | byte offset of its expanded weights
+// [16:32]. slot0 is read from the event's rank*8*ne array, slot1 from its rank*8 array; a
+// singlet's site is slot1 and its slot0 is 63 (rank entries 0). Backward entries read the
+// species of canonical slot x from slot mirror[x].
+std::vector<uint32_t> PackOrbits(const lmc::OrbitTable &ot, const std::vector<int> &ebase,
                                  const std::vector<int> *mirror) {
   std::vector<uint32_t> v;
   for (const auto &c : ot.clusters) {
-    const int kind = ot.orbit_kind[c.orbit];
-    const uint32_t b = static_cast<uint32_t>(base[c.orbit]);
-    if (b > 4095) throw std::invalid_argument("feature vector too long for the packed tables");
+    const uint32_t b = static_cast<uint32_t>(ebase[c.orbit]) * 8;
+    if (b > 65535) throw std::invalid_argument("feature vector too long for the packed tables");
     int s0 = c.s0, s1 = c.s1;
     if (mirror) {
       s0 = (*mirror)[s0];
       s1 = s1 < 0 ? -1 : (*mirror)[s1];
     }
-    (void)S;
-    v.push_back(static_cast<uint32_t>(s0) | (static_cast<uint32_t>(s1 < 0 ? 63 : s1) << 8) |
-                (b << 16) | (static_cast<uint32_t>(kind) << 28));
+    if (s1 < 0) {  // singlet: its site takes the rank*8 role
+      s1 = s0;
+      s0 = 63;
+    }
+    v.push_back(static_cast<uint32_t>(2 * s0) | (static_cast<uint32_t>(s1) << 8) | (b << 16));
   }
   return v;
 }
@@ -147,20 +196,21 @@ Model::~Model() {
 // environment adds, for each cluster with a non-solvent site, value(cluster) - value(all s).
 // A cluster with several non-solvent sites is listed under each of them and evaluated only at
 // the lowest-numbered one (the kernel checks the entry's other slot).
-static void BuildSolventTables(const Model &m, const JumpTables &jt, std::vector<uint16_t> &off,
+static void BuildSolventTables(const Model &m, const JumpTables &jt, const std::vector<int> &eb1,
+                               const std::vector<int> &eb2, std::vector<uint16_t> &off,
                                std::vector<uint32_t> &inc, std::vector<double> &base) {
   const int S = m.S;
   // per slot j: the clusters containing j (j not a jump endpoint), by list type
   std::vector<std::vector<uint32_t>> per[3];
   for (auto &v : per) v.assign(kEnvSlots, {});
-  // state entry: reduced weight-block byte offset [0:13] | j is the second other site [13] | the
-  // other site o [14:20] (63 = none) | neighbour-endpoint side [31]
+  // state entry: reduced weight-block byte offset [0:13] (rank layout, positive rows) | j is the
+  // second other site [13] | the other site o [14:20] (63 = none) | neighbour-endpoint side [31]
   auto add_state = [&](const std::vector<StateCluster> &cl, int endpoint, bool nside) {
     for (const auto &c : cl) {
       int others[2] = {-1, -1}, n = 0;
       for (int i = 0; i < c.size; ++i)
         if (c.slots[i] != endpoint) others[n++] = c.slots[i];
-      const int block = ReducedBlock(c.label, S);
+      const int block = ReducedBlock(c.label, m.ne);
       for (int i = 0; i < n; ++i) {
         const int other = n == 2 ? others[1 - i] : 63;
         per[0][others[i]].push_back(static_cast<uint32_t>(block * 8) | (static_cast<uint32_t>(i) << 13) |
@@ -170,28 +220,28 @@ static void BuildSolventTables(const Model &m, const JumpTables &jt, std::vector
   };
   add_state(jt.state_v_only, jt.v_slot, false);
   add_state(jt.state_n_only, jt.n_slot, true);
-  // orbit entry: feature base [0:12] | kind * S * S [12:20] | j is the cluster's second site [20]
-  // | the other site o [21:27] (63 = none); backward entries read canonical slot x at mirror[x]
-  auto add_orbits = [&](const OrbitTable &ot, const std::vector<int> &bases, int list, bool backward) {
+  // orbit entry: byte offset of the orbit's expanded weights [0:16] | j takes the rank*8 role
+  // (second site, or a singlet's site) [16] | the other site o [17:23] (63 = none); backward
+  // entries read canonical slot x at mirror[x]
+  auto add_orbits = [&](const OrbitTable &ot, const std::vector<int> &eb, int list, bool backward) {
     for (const auto &c : ot.clusters) {
-      const int kind = ot.orbit_kind[c.orbit];
-      const uint32_t bse = static_cast<uint32_t>(bases[c.orbit]);
-      if (bse > 4095) throw std::invalid_argument("feature vector too long for the packed tables");
+      const uint32_t bse = static_cast<uint32_t>(eb[c.orbit]) * 8;
+      if (bse > 65535) throw std::invalid_argument("feature vector too long for the packed tables");
       const int nsl = c.s1 < 0 ? 1 : 2;
       for (int pos = 0; pos < nsl; ++pos) {
         const int canon = pos == 0 ? c.s0 : c.s1;
         const int canon_other = nsl == 1 ? -1 : (pos == 0 ? c.s1 : c.s0);
         const int slot = backward ? jt.mirror[canon] : canon;
         const int other = canon_other < 0 ? 63 : (backward ? jt.mirror[canon_other] : canon_other);
-        per[list][slot].push_back(bse | (static_cast<uint32_t>(kind * S * S) << 12) |
-                                  (static_cast<uint32_t>(pos) << 20) | (static_cast<uint32_t>(other) << 21));
+        const uint32_t role = (nsl == 1 || pos == 1) ? 1u : 0u;
+        per[list][slot].push_back(bse | (role << 16) | (static_cast<uint32_t>(other) << 17));
       }
     }
   };
-  if (m.kind == kE0 || m.kind == kQuartic) add_orbits(jt.mmm, m.base_mmm, 1, false);
+  if (m.kind == kE0 || m.kind == kQuartic) add_orbits(jt.mmm, eb1, 1, false);
   if (m.kind == kQuartic) {
-    add_orbits(jt.mm2, m.base_mm2, 2, false);
-    add_orbits(jt.mm2, m.base_mm2, 2, true);
+    add_orbits(jt.mm2, eb2, 2, false);
+    add_orbits(jt.mm2, eb2, 2, true);
   }
   off.assign(3 * (kEnvSlots + 1), 0);
   for (int t = 0; t < 3; ++t) {
@@ -251,40 +301,49 @@ const KTables &Upload(Model &m) {
   for (int d = 0; d < 12; ++d)
     for (int s = 0; s < kEnvSlots; ++s) off.push_back(PackOff(jt.off[d][s]));
   std::vector<uint8_t> mirror(jt.mirror.begin(), jt.mirror.end());
-  std::vector<uint32_t> state;
-  for (const auto &c : jt.state_v_only) state.push_back(PackState(c, jt.v_slot, false, m.S));
-  for (const auto &c : jt.state_n_only) state.push_back(PackState(c, jt.n_slot, true, m.S));
-  const auto mmm = PackOrbits(jt.mmm, m.base_mmm, m.S, nullptr);
-  // MM2 clusters: forward entries, then the backward (mirrored) ones
-  auto mm2 = PackOrbits(jt.mm2, m.base_mm2, m.S, nullptr);
-  const std::vector<int> mir(jt.mirror.begin(), jt.mirror.end());
-  const auto mm2b = PackOrbits(jt.mm2, m.base_mm2, m.S, &mir);
-  mm2.insert(mm2.end(), mm2b.begin(), mm2b.end());
-  std::vector<uint32_t> site_cl;
-  for (const auto &c : st.clusters) site_cl.push_back(PackSite(c, 0));
-  // Jump dE weights by migrating species M and reduced index r (see ReducedBlock):
-  // wd[M][r] = w(cluster | endpoint M) - w(cluster | endpoint X); the vacancy's row is zero.
-  const int S = m.S;
-  const int nr = 1 + 3 * S + 4 * S * S;
+  // Jump dE weights by migrating species M and reduced index r over element ranks (see
+  // ReducedBlock): wd[rank M][r] = w(cluster | endpoint M) - w(cluster | endpoint X); rows
+  // ne .. 2 ne - 1 hold the same values negated (the neighbour endpoint's clusters).
+  const int S = m.S, ne = m.ne;
+  const int nr = 1 + 3 * ne + 4 * ne * ne;
+  std::vector<int> code;  // element rank -> species code
+  for (int c = 0; c < S; ++c)
+    if (c != m.x_code) code.push_back(c);
   auto wval = [&](int l, int a, int c1, int c2) {
     if (l == 0) return m.w[a];
     if (l < 4) return m.w[S + (l - 1) * S * S + a * S + c1];
     return m.w[S + 3 * S * S + (l - 4) * S * S * S + (a * S + c1) * S + c2];
   };
-  std::vector<double> wd(static_cast<size_t>(S) * nr, 0.0);
-  for (int M = 0; M < S; ++M)
+  std::vector<double> wd(static_cast<size_t>(2 * ne) * nr, 0.0);
+  for (int rm = 0; rm < ne; ++rm)
     for (int l = 0; l < 8; ++l) {
-      const int n1 = l == 0 ? 1 : S, n2 = l < 4 ? 1 : S;
-      for (int c1 = 0; c1 < n1; ++c1)
-        for (int c2 = 0; c2 < n2; ++c2)
-          wd[static_cast<size_t>(M) * nr + ReducedBlock(l, S) + c1 + S * c2] =
-              wval(l, M, c1, c2) - wval(l, m.x_code, c1, c2);
+      const int n1 = l == 0 ? 1 : ne, n2 = l < 4 ? 1 : ne;
+      for (int r1 = 0; r1 < n1; ++r1)
+        for (int r2 = 0; r2 < n2; ++r2) {
+          const size_t i = static_cast<size_t>(rm) * nr + ReducedBlock(l, ne) + r1 + ne * r2;
+          wd[i] = wval(l, code[rm], code[r1], code[r2]) - wval(l, m.x_code, code[r1], code[r2]);
+          wd[i + static_cast<size_t>(ne) * nr] = -wd[i];
+        }
     }
-  // A vacancy off the jump endpoints is flagged when the environment is gathered; its one-hot
-  // type maps to entry 0 so that every table index stays in range.
-  std::vector<int8_t> tt(m.type_tab.begin(), m.type_tab.end());
-  for (auto &x : tt)
-    if (x < 0) x = 0;
+  const int neg_bytes = ne * nr * 8;
+  std::vector<uint32_t> state;
+  for (const auto &c : jt.state_v_only) state.push_back(PackState(c, jt.v_slot, false, ne, neg_bytes));
+  for (const auto &c : jt.state_n_only) state.push_back(PackState(c, jt.n_slot, true, ne, neg_bytes));
+  int F1e = 0, F2e = 0;
+  const std::vector<int> eb1 = ExpandedBases(jt.mmm, ne, F1e), eb2 = ExpandedBases(jt.mm2, ne, F2e);
+  const auto mmm = PackOrbits(jt.mmm, eb1, nullptr);
+  // MM2 clusters: forward entries, then the backward (mirrored) ones
+  auto mm2 = PackOrbits(jt.mm2, eb2, nullptr);
+  const std::vector<int> mir(jt.mirror.begin(), jt.mirror.end());
+  const auto mm2b = PackOrbits(jt.mm2, eb2, &mir);
+  mm2.insert(mm2.end(), mm2b.begin(), mm2b.end());
+  std::vector<uint32_t> site_cl;
+  for (const auto &c : st.clusters) site_cl.push_back(PackSite(c, 0));
+  // species code -> element rank, 4 bits each (a vacancy maps to rank 0: it is flagged when
+  // the environment is gathered, and every table index stays in range)
+  uint32_t rank_lut = 0;
+  for (int c = 0; c < S; ++c)
+    if (c != m.x_code) rank_lut |= static_cast<uint32_t>(m.onehot[c]) << (4 * c);
   std::vector<int32_t> site_off;
   for (int s = 0; s < kSiteSlots; ++s) site_off.push_back(PackOff(st.off[s]));
   std::vector<double> par = m.par;
@@ -310,18 +369,11 @@ const KTables &Upload(Model &m) {
   b.Add(site_off, &k.site_off);
   b.Add(m.w, &k.w);
   b.Add(wd, &k.wd);
-  std::vector<double> b1c, b2c;  // rows of the non-vacancy species, in code order
-  for (int c = 0; c < m.S; ++c) {
-    if (c == m.x_code) continue;
-    b1c.insert(b1c.end(), m.b_mmm.begin() + static_cast<size_t>(c) * m.F_mmm,
-               m.b_mmm.begin() + static_cast<size_t>(c + 1) * m.F_mmm);
-    b2c.insert(b2c.end(), m.b_mm2.begin() + static_cast<size_t>(c) * m.F_mm2,
-               m.b_mm2.begin() + static_cast<size_t>(c + 1) * m.F_mm2);
-  }
+  const std::vector<double> b1c = ExpandWeights(m, jt.mmm, m.base_mmm, m.b_mmm, m.F_mmm, eb1, F1e);
+  const std::vector<double> b2c = ExpandWeights(m, jt.mm2, m.base_mm2, m.b_mm2, m.F_mm2, eb2, F2e);
   b.Add(b1c, &k.b1);
   b.Add(b2c, &k.b2);
   b.Add(par, &k.par);
-  b.Add(tt, &k.tt);
   b.Add(m.theta, &k.theta);
   b.Add(m.norm_total, &k.norm_tot);
   b.Add(e1, &k.eci1);
@@ -330,7 +382,7 @@ const KTables &Upload(Model &m) {
   std::vector<uint16_t> inc_off;
   std::vector<uint32_t> inc;
   std::vector<double> base;
-  BuildSolventTables(m, jt, inc_off, inc, base);
+  BuildSolventTables(m, jt, eb1, eb2, inc_off, inc, base);
   b.Add(inc_off, &k.inc_off);
   b.Add(inc, &k.inc);
   b.Add(base, &k.base);
@@ -357,10 +409,12 @@ const KTables &Upload(Model &m) {
   k.S = m.S;
   k.ne = m.ne;
   k.x_code = m.x_code;
-  k.F1 = m.F_mmm;
-  k.F2 = m.F_mm2;
-  k.nw = static_cast<int>(m.w.size());  // wd has the same size: S (1 + 3S + 4S^2)
-  if (wd.size() != m.w.size()) throw std::logic_error("dE weight table size");
+  k.F1 = F1e;
+  k.F2 = F2e;
+  k.nw = static_cast<int>(m.w.size());
+  k.nwd = static_cast<int>(wd.size());
+  k.wrow = nr;
+  k.rank_lut = rank_lut;
   k.n_state_v = static_cast<int>(jt.state_v_only.size());
   k.n_inc = static_cast<int>(inc.size());
   k.solvent_default = 0;

@@ -30,11 +30,12 @@ struct KTables {
   const uint32_t *site_cl;
   const int32_t *site_off;  // [43]
   const double *w;          // theta/N by (label, codes): w1[S] | w2[3][S][S] | w3[4][S][S][S]
-  const double *wd;         // jump dE: [M][1 + 3S + 4S^2] w(.|M) - w(.|X), nw entries
-  const double *b1;         // [ne][F1], row = one-hot rank of the migrating species
+  // jump dE over element ranks: [2 ne][wrow] w(.|M) - w(.|X) by the migrating species' rank,
+  // rows ne .. 2 ne - 1 negated; wrow = 1 + 3 ne + 4 ne^2, nwd entries
+  const double *wd;
+  const double *b1;         // [ne][F1] expanded barrier weights, row = rank of the migrating species
   const double *b2;         // [ne][F2]
   const double *par;        // [S][6]
-  const int8_t *tt;         // [3][S][S]
   const double *theta;      // [n_eci]
   const double *norm_tot;   // [n_eci]
   const int32_t *eci1;      // [S]
@@ -47,8 +48,8 @@ struct KTables {
   // non-solvent slot, once (at its lowest such slot).
   //   state entry: reduced weight-block byte offset [0:13] | j is the second other site [13]
   //                | other site [14:20] (63 = none) | neighbour-endpoint side [31]
-  //   orbit entry: feature base [0:12] | kind * S * S [12:20] | j is the second site [20]
-  //                | other site [21:27] (63 = none)
+  //   orbit entry: expanded-weight byte offset [0:16] | j takes the rank*8 role [16]
+  //                | other site [17:23] (63 = none)
   const uint16_t *inc_off;  // [3][61]
   const uint32_t *inc;
   const double *base;       // [S][S][3]
@@ -58,6 +59,8 @@ struct KTables {
   int n_state_v;            // state entries of the vacancy endpoint (the rest: neighbour's)
   int v_slot, n_slot;
   int kind, S, ne, x_code, F1, F2, nw;
+  int nwd, wrow;
+  uint32_t rank_lut;        // species code -> element rank, 4 bits per code (vacancy: 0)
 };
 
 struct DeviceTables {

@@ -14,11 +14,18 @@ namespace {
 constexpr int kLanes = 21;      // dense: lanes per jump event (12 * 21 = 252 <= 256)
 constexpr int kSLanes = 16;     // solvent baseline: lanes per jump event (4 events per wave)
 
+// Element ranks of one event's environment slots, pre-scaled for the cluster tables: rank * 8
+// (bytes) and rank * 8 * ne (16-bit); slots 60..63 hold 0 (slot 63 is the absent site).
+struct EventRanks {
+  uint8_t r8[64];
+  uint16_t rn8[64];
+};
+
 struct JumpLds {
   uint16_t off[12 * kEnvSlots];  // relative offsets, 4 bits per component (+8)
   double par[kMaxSpecies * 6];
-  int8_t tt[3 * kMaxSpecies * kMaxSpecies];
   uint8_t env[12][64];
+  EventRanks er[12];
   int64_t nb[12];   // by direction
   // by rank (neighbour-id order)
   double ea[12], de[12], rate[12], ekra[12];
@@ -70,13 +77,17 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     Unpack(t.off[i], dx, dy, dz);
     L.off[i] = static_cast<uint16_t>((dx + 8) | ((dy + 8) << 4) | ((dz + 8) << 8));
   }
-  for (int i = tid; i < 3 * t.S * t.S; i += lsize) L.tt[i] = t.tt[i];
   for (int i = tid; i < 6 * t.S; i += lsize) L.par[i] = t.par[i];
-  if (tid < 12) L.env[tid][63] = 0;  // target of the absent sites of every packed cluster
+  if (tid < 12 * 4) {  // slots 60..63; slot 63 is the target of every packed cluster's absent sites
+    const int k = tid >> 2, j = kEnvSlots + (tid & 3);
+    L.env[k][j] = 0;
+    L.er[k].r8[j] = 0;
+    L.er[k].rn8[j] = 0;
+  }
   if (tid == 0) L.bad = 0;
   Tabs T{};
   double *w = reinterpret_cast<double *>(dyn + lay.w);
-  for (int i = gtid; i < t.nw; i += gsize) w[i] = t.wd[i];
+  for (int i = gtid; i < t.nwd; i += gsize) w[i] = t.wd[i];
   T.w = w;
   if (kBL) {
     double *p = reinterpret_cast<double *>(dyn + lay.b1);
@@ -113,6 +124,20 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     T.mm2 = p;
   }
   return T;
+}
+
+// Element rank of a species code (a vacancy gives 0).
+__device__ __forceinline__ uint32_t RankOf(const KTables &t, int code) {
+  return (t.rank_lut >> (4 * code)) & 15u;
+}
+
+// Row of the dE difference table (bytes) and of an expanded barrier table for migrating
+// species code M.
+__device__ __forceinline__ const char *WRow(const KTables &t, const Tabs &T, int M) {
+  return reinterpret_cast<const char *>(T.w) + __umul24(RankOf(t, M), 8 * t.wrow);
+}
+__device__ __forceinline__ const char *BRow(const double *b, int F, const KTables &t, int M) {
+  return reinterpret_cast<const char *>(b) + __umul24(RankOf(t, M), 8 * F);
 }
 
 // Phase 1: species of the 12 environments; neighbour ids of the 12 jumps. 720 sites over the
@@ -158,6 +183,9 @@ __device__ __forceinline__ void GatherStore(const KTables &t, JumpLds &L, int32_
       }
       if (x == t.x_code && j != t.v_slot) atomicOr(&L.bad, 1 << k);
       L.env[k][j] = x;
+      const uint32_t rk = RankOf(t, x);
+      L.er[k].r8[j] = static_cast<uint8_t>(rk << 3);
+      L.er[k].rn8[j] = static_cast<uint16_t>(__umul24(rk, 8 * t.ne));
       if (j == t.n_slot) L.nb[k] = G.s[u];
     }
   }
@@ -171,123 +199,117 @@ __device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const ui
   GatherStore<kBT>(t, L, err, tid, G);
 }
 
+// Byte offset of a packed entry's rank*8 slot (byte 1 of a state entry is the other one).
+__device__ __forceinline__ uint32_t R8At(const EventRanks &er, uint32_t byte_off) {
+  return er.r8[byte_off];
+}
+__device__ __forceinline__ uint32_t Rn8At(const EventRanks &er, uint32_t byte_off) {
+  return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(er.rn8) + byte_off);
+}
+
 // Sums of kU packed state clusters per lane: w(cluster | endpoint = M) - w(cluster | endpoint =
-// X) from wm, the difference table's row of M (s8 = 8 S), negated for the neighbour endpoint's
-// clusters. The loads of the kU clusters are issued
-// level by level (entries, species, weights).
+// X) from wm, the difference table's row of M (the entry's offset selects the negated copy for
+// the neighbour endpoint's clusters). The loads of the kU clusters are issued level by level
+// (entries, ranks, weights).
 template <int kU>
-__device__ __forceinline__ void StateBatch(const char *wm, const uint8_t *e, const uint32_t *list,
-                                           int c, int stride, int s8, double *acc) {
-  uint32_t d[kU];
-  int c1[kU], c2[kU];
+__device__ __forceinline__ void StateBatch(const char *wm, const EventRanks &er,
+                                           const uint32_t *list, int c, int stride, double *acc) {
+  uint32_t d[kU], x1[kU], x2[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) d[u] = list[c + u * stride];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    c1[u] = e[d[u] & 0xFF];
-    c2[u] = e[(d[u] >> 8) & 0xFF];
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const int o = static_cast<int>((d[u] >> 16) & 8191) + (c1[u] << 3) + __umul24(c2[u], s8);
-    const double v = *reinterpret_cast<const double *>(wm + o);
-    acc[u] += __hiloint2double(__double2hiint(v) ^ static_cast<int>(d[u] & 0x80000000u),
-                               __double2loint(v));
-  }
-}
-
-// Sum of the state clusters [c0, c1) of an event, lanes lane, lane + stride, ...
-__device__ __forceinline__ double StateRange(const char *wm, const uint8_t *e, const uint32_t *list,
-                                             int c0, int c1, int lane, int stride, int s8) {
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
-  int c = c0 + lane;
-  for (; c + 3 * stride < c1; c += 4 * stride) StateBatch<4>(wm, e, list, c, stride, s8, a);
-  for (; c < c1; c += stride) StateBatch<1>(wm, e, list, c, stride, s8, a);
-  return (a[0] + a[1]) + (a[2] + a[3]);
-}
-
-// Sums of kU packed orbit clusters per lane: b[base + one-hot type(kind, c0, c1)]; ss = S * S.
-template <int kU>
-__device__ __forceinline__ void OrbitBatch(const int8_t *tt, const double *bm, const uint8_t *e,
-                                           const uint32_t *list, int c, int S, double *acc) {
-  const int ss = S * S;
-  uint32_t d[kU];
-  int c0[kU], c1[kU], ty[kU];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) d[u] = list[c + u * kLanes];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    c0[u] = e[d[u] & 0xFF];
-    c1[u] = e[(d[u] >> 8) & 0xFF];
+    x1[u] = R8At(er, d[u] & 0xFF);
+    x2[u] = Rn8At(er, (d[u] >> 8) & 0xFF);
   }
 #pragma unroll
   for (int u = 0; u < kU; ++u)
-    ty[u] = tt[__umul24(d[u] >> 28, ss) + __umul24(c0[u], S) + c1[u]];
+    acc[u] += *reinterpret_cast<const double *>(wm + (x1[u] + x2[u] + (d[u] >> 16)));
+}
+
+// Sum of the state clusters [c0, c1) of an event, lanes lane, lane + stride, ...
+__device__ __forceinline__ double StateRange(const char *wm, const EventRanks &er,
+                                             const uint32_t *list, int c0, int c1, int lane,
+                                             int stride) {
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = c0 + lane;
+  for (; c + 3 * stride < c1; c += 4 * stride) StateBatch<4>(wm, er, list, c, stride, a);
+  for (; c < c1; c += stride) StateBatch<1>(wm, er, list, c, stride, a);
+  return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+// Sums of kU packed orbit clusters per lane: bm[rank0 * ne + rank1] of the cluster's expanded
+// weights (bm: the migrating species' row, bytes).
+template <int kU>
+__device__ __forceinline__ void OrbitBatch(const char *bm, const EventRanks &er,
+                                           const uint32_t *list, int c, int stride, double *acc) {
+  uint32_t d[kU], x0[kU], x1[kU];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) acc[u] += bm[((d[u] >> 16) & 4095) + ty[u]];
+  for (int u = 0; u < kU; ++u) d[u] = list[c + u * stride];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    x0[u] = Rn8At(er, d[u] & 0xFF);
+    x1[u] = R8At(er, (d[u] >> 8) & 0xFF);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u)
+    acc[u] += *reinterpret_cast<const double *>(bm + (x0[u] + x1[u] + (d[u] >> 16)));
+}
+
+// Sum of the orbit clusters [0, n) of an event, lanes lane, lane + stride, ...
+__device__ __forceinline__ double OrbitRange(const char *bm, const EventRanks &er,
+                                             const uint32_t *list, int n, int lane, int stride) {
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = lane;
+  for (; c + 3 * stride < n; c += 4 * stride) OrbitBatch<4>(bm, er, list, c, stride, a);
+  for (; c < n; c += stride) OrbitBatch<1>(bm, er, list, c, stride, a);
+  return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
 // Phase 2, dense: per-lane partial sums of every cluster's weight (dE, MMM logit, MM2 logit).
 __device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T, int tid) {
   if (tid >= 12 * kLanes) return;
   const int k = tid / kLanes, lane = tid - k * kLanes;
-  const uint8_t *e = L.env[k];
-  const int S = t.S, M = e[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / S * 8;
-  const double de = StateRange(wm, e, T.state, 0, t.n_state, lane, kLanes, 8 * S);
-  double a[4];
-  int c;
+  const EventRanks &er = L.er[k];
+  const int M = L.env[k][t.n_slot];
+  const double de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, kLanes);
   double la = 0.0, lb = 0.0;
-  // one-hot rank of the migrating species (a vacancy is flagged in the gather)
-  const int row = L.tt[M * S];
-  if (t.kind == kE0 || t.kind == kQuartic) {
-    const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-    a[0] = a[1] = a[2] = a[3] = 0.0;
-    c = lane;
-    for (; c + 3 * kLanes < t.n_mmm; c += 4 * kLanes) OrbitBatch<4>(L.tt, bm, e, T.mmm, c, S, a);
-    for (; c < t.n_mmm; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mmm, c, S, a);
-    la = (a[0] + a[1]) + (a[2] + a[3]);
-  }
-  if (t.kind == kQuartic) {  // forward and backward MM2 entries in one list
-    const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-    a[0] = a[1] = a[2] = a[3] = 0.0;
-    c = lane;
-    for (; c + 3 * kLanes < t.n_mm2; c += 4 * kLanes) OrbitBatch<4>(L.tt, bm, e, T.mm2, c, S, a);
-    for (; c < t.n_mm2; c += kLanes) OrbitBatch<1>(L.tt, bm, e, T.mm2, c, S, a);
-    lb = (a[0] + a[1]) + (a[2] + a[3]);
-  }
+  if (t.kind == kE0 || t.kind == kQuartic)
+    la = OrbitRange(BRow(T.b1, t.F1, t, M), er, T.mmm, t.n_mmm, lane, kLanes);
+  if (t.kind == kQuartic)  // forward and backward MM2 entries in one list
+    lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2, lane, kLanes);
   L.part[k * kLanes + lane][0] = de;
   L.part[k * kLanes + lane][1] = la;
   L.part[k * kLanes + lane][2] = lb;
 }
 
-// Solvent baseline: change of one state cluster touching slot j (species cj) relative to the
-// all-solvent environment; zero when the cluster's other site o is a lower non-solvent slot
-// (the cluster is counted there).
-__device__ __forceinline__ double SolventState(const char *wm, const uint8_t *e, uint32_t d, int j,
-                                               int cj, int sv, int s8) {
+// Solvent baseline: change of one state cluster touching slot j relative to the all-solvent
+// environment; zero when the cluster's other site o is a lower non-solvent slot (the cluster is
+// counted there). rj, rnj: slot j's rank entries; sr, srn: the solvent's.
+__device__ __forceinline__ double SolventState(const char *wm, const EventRanks &er,
+                                               const uint8_t *e, uint32_t d, int j, uint32_t rj,
+                                               uint32_t rnj, uint32_t sr, uint32_t srn, int sv) {
   const int o = (d >> 14) & 63, co = e[o];
-  const bool second = (d >> 13) & 1;
-  const int mj = second ? s8 : 8, mo = second ? 8 : s8;
-  const int blk = static_cast<int>(d & 8191);
-  const int oa = blk + __umul24(cj, mj) + __umul24(co, mo);
-  const int ob = blk + __umul24(sv, mj) + __umul24(o == 63 ? 0 : sv, mo);
+  const bool second = (d >> 13) & 1, none = o == 63;
+  const uint32_t blk = d & 8191;
+  const uint32_t oa = blk + (second ? rnj + er.r8[o] : rj + er.rn8[o]);
+  const uint32_t ob = blk + (second ? srn + (none ? 0u : sr) : sr + (none ? 0u : srn));
   const double v = *reinterpret_cast<const double *>(wm + oa) - *reinterpret_cast<const double *>(wm + ob);
   const double sv_ = __hiloint2double(__double2hiint(v) ^ static_cast<int>(d & 0x80000000u), __double2loint(v));
-  return (o != 63 && o < j && co != sv) ? 0.0 : sv_;
+  return (!none && o < j && co != sv) ? 0.0 : sv_;
 }
 
 // Solvent baseline: change of one orbit cluster touching slot j.
-__device__ __forceinline__ double SolventOrbit(const int8_t *tt, const double *bm, const uint8_t *e,
-                                               uint32_t d, int j, int cj, int sv, int S) {
-  const int o = (d >> 21) & 63, co = e[o];
-  const bool second = (d >> 20) & 1;
-  const int kS2 = (d >> 12) & 255, base = d & 4095;
-  const int c0 = second ? co : cj, c1 = second ? cj : co;
-  const int ty = tt[kS2 + __umul24(c0, S) + c1];
-  const int tb = tt[kS2 + __umul24(sv, S) + (o == 63 ? 0 : sv)];
-  const double v = bm[base + ty] - bm[base + tb];
-  return (o != 63 && o < j && co != sv) ? 0.0 : v;
+__device__ __forceinline__ double SolventOrbit(const char *bm, const EventRanks &er,
+                                               const uint8_t *e, uint32_t d, int j, uint32_t rj,
+                                               uint32_t rnj, uint32_t sr, uint32_t srn, int sv) {
+  const int o = (d >> 17) & 63, co = e[o];
+  const bool role1 = (d >> 16) & 1, none = o == 63;
+  const uint32_t base = d & 0xFFFF;
+  const uint32_t ia = base + (role1 ? rj + er.rn8[o] : rnj + er.r8[o]);
+  const uint32_t ib = base + (role1 ? sr + (none ? 0u : srn) : srn + (none ? 0u : sr));
+  const double v = *reinterpret_cast<const double *>(bm + ia) - *reinterpret_cast<const double *>(bm + ib);
+  return (!none && o < j && co != sv) ? 0.0 : v;
 }
 
 // Phase 2, solvent baseline: 16 lanes per event (threads 0..191). The event's non-solvent
@@ -306,12 +328,11 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
     mask |= ((b >> (16 * (k & 3))) & 0xFFFFull) << (16 * i);
   }
   if (!active) return;
-  const int S = t.S, M = e[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / S * 8;
-  const int s8 = 8 * S;
-  const int row = L.tt[M * S];
-  const double *b1 = T.b1 + static_cast<size_t>(row) * t.F1;
-  const double *b2 = T.b2 + static_cast<size_t>(row) * t.F2;
+  const EventRanks &er = L.er[k];
+  const int M = e[t.n_slot];
+  const char *wm = WRow(t, T, M);
+  const char *b1 = BRow(T.b1, t.F1, t, M), *b2 = BRow(T.b2, t.F2, t, M);
+  const uint32_t sr = RankOf(t, sv) << 3, srn = __umul24(RankOf(t, sv), 8 * t.ne);
   const bool orbits = t.kind == kE0 || t.kind == kQuartic, mm2 = t.kind == kQuartic;
   double de = 0.0, la = 0.0, lb = 0.0;
   const uint16_t *offS = T.inc_off, *offM = T.inc_off + 61, *offQ = T.inc_off + 122;
@@ -323,16 +344,16 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
     const bool two = mm != 0;
     const int j1 = two ? __ffsll(static_cast<unsigned long long>(mm)) - 1 : j0;
     if (two) mm &= mm - 1;
-    const int c0 = e[j0], c1 = e[j1];
+    const uint32_t r0 = er.r8[j0], rn0 = er.rn8[j0], r1 = er.r8[j1], rn1 = er.rn8[j1];
     int s0 = offS[j0] + lane, s1 = offS[j1] + lane, m0 = offM[j0] + lane, m1 = offM[j1] + lane;
     const int s0e = offS[j0 + 1], m0e = orbits ? offM[j0 + 1] : 0;
     const int s1e = two ? offS[j1 + 1] : 0, m1e = two && orbits ? offM[j1 + 1] : 0;
     while (s0 < s0e || s1 < s1e || m0 < m0e || m1 < m1e) {
       const bool bs0 = s0 < s0e, bs1 = s1 < s1e, bm0 = m0 < m0e, bm1 = m1 < m1e;
-      const double vs0 = SolventState(wm, e, T.inc[bs0 ? s0 : 0], j0, c0, sv, s8);
-      const double vs1 = SolventState(wm, e, T.inc[bs1 ? s1 : 0], j1, c1, sv, s8);
-      const double vm0 = SolventOrbit(L.tt, b1, e, T.inc[bm0 ? m0 : offM[0]], j0, c0, sv, S);
-      const double vm1 = SolventOrbit(L.tt, b1, e, T.inc[bm1 ? m1 : offM[0]], j1, c1, sv, S);
+      const double vs0 = SolventState(wm, er, e, T.inc[bs0 ? s0 : 0], j0, r0, rn0, sr, srn, sv);
+      const double vs1 = SolventState(wm, er, e, T.inc[bs1 ? s1 : 0], j1, r1, rn1, sr, srn, sv);
+      const double vm0 = SolventOrbit(b1, er, e, T.inc[bm0 ? m0 : offM[0]], j0, r0, rn0, sr, srn, sv);
+      const double vm1 = SolventOrbit(b1, er, e, T.inc[bm1 ? m1 : offM[0]], j1, r1, rn1, sr, srn, sv);
       de += (bs0 ? vs0 : 0.0) + (bs1 ? vs1 : 0.0);
       la += (bm0 ? vm0 : 0.0) + (bm1 ? vm1 : 0.0);
       s0 += kSLanes;
@@ -345,8 +366,8 @@ __device__ void SolventClusters(const KTables &t, JumpLds &L, const Tabs &T, int
       const int q0e = offQ[j0 + 1], q1e = two ? offQ[j1 + 1] : 0;
       while (q0 < q0e || q1 < q1e) {
         const bool bq0 = q0 < q0e, bq1 = q1 < q1e;
-        const double vq0 = SolventOrbit(L.tt, b2, e, T.inc[bq0 ? q0 : offQ[0]], j0, c0, sv, S);
-        const double vq1 = SolventOrbit(L.tt, b2, e, T.inc[bq1 ? q1 : offQ[0]], j1, c1, sv, S);
+        const double vq0 = SolventOrbit(b2, er, e, T.inc[bq0 ? q0 : offQ[0]], j0, r0, rn0, sr, srn, sv);
+        const double vq1 = SolventOrbit(b2, er, e, T.inc[bq1 ? q1 : offQ[0]], j1, r1, rn1, sr, srn, sv);
         lb += (bq0 ? vq0 : 0.0) + (bq1 ? vq1 : 0.0);
         q0 += kSLanes;
         q1 += kSLanes;

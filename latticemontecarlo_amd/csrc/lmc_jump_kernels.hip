@@ -154,7 +154,7 @@ DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
   DynLayout lay{};
   size_t at = 0;
   lay.w = static_cast<int>(at);
-  at = al(at + sizeof(double) * t.nw);
+  at = al(at + sizeof(double) * t.nwd);
   if (solvent) {
     lay.base = static_cast<int>(at);
     at = al(at + sizeof(double) * 3 * t.S * t.S);

@@ -228,24 +228,13 @@ __device__ void FallbackClusters(const KTables &t, JumpLds &L, const Tabs &T, co
     int mm = mask;
     for (int i = 0; i < e; ++i) mm &= mm - 1;
     const int k = __ffs(mm) - 1;
-    const uint8_t *env = L.env[k];
-    const int S_ = t.S, M = env[t.n_slot];
-    const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / S_ * 8;
-    de = StateRange(wm, env, T.state, 0, t.n_state, lane, le, 8 * S_);
-    double a[1];
-    const int row = L.tt[M * S_];
-    if (t.kind == kE0 || t.kind == kQuartic) {
-      const double *bm = T.b1 + static_cast<size_t>(row) * t.F1;
-      a[0] = 0.0;
-      for (int c = lane; c < t.n_mmm; c += le) OrbitBatch<1>(L.tt, bm, env, T.mmm, c, S_, a);
-      la = a[0];
-    }
-    if (t.kind == kQuartic) {
-      const double *bm = T.b2 + static_cast<size_t>(row) * t.F2;
-      a[0] = 0.0;
-      for (int c = lane; c < t.n_mm2; c += le) OrbitBatch<1>(L.tt, bm, env, T.mm2, c, S_, a);
-      lb = a[0];
-    }
+    const EventRanks &er = L.er[k];
+    const int M = L.env[k][t.n_slot];
+    de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, le);
+    if (t.kind == kE0 || t.kind == kQuartic)
+      la = OrbitRange(BRow(T.b1, t.F1, t, M), er, T.mmm, t.n_mmm, lane, le);
+    if (t.kind == kQuartic)
+      lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2, lane, le);
   }
   L.part[lt][0] = de;
   L.part[lt][1] = la;
@@ -278,10 +267,8 @@ __device__ void FallbackFinish(const KTables &t, JumpLds &L, const SurLds &S, in
 // Phase 6 (all lanes of block g): exact dE of direction k from the state clusters.
 __device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLds &S, int g, int lt,
                         int k) {
-  const uint8_t *env = L.env[k];
-  const int M = env[t.n_slot];
-  const char *wm = reinterpret_cast<const char *>(T.w) + static_cast<size_t>(M) * t.nw / t.S * 8;
-  double v = StateRange(wm, env, T.state, 0, t.n_state, lt, kSub, 8 * t.S);
+  const int M = L.env[k][t.n_slot];
+  double v = StateRange(WRow(t, T, M), L.er[k], T.state, 0, t.n_state, lt, kSub);
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
   if ((lt & 63) == 0) S.dsum[g][lt >> 6] = v;
@@ -442,7 +429,7 @@ DynLayout PlanSurrogateLds(const KTables &t) {
   DynLayout lay{};
   size_t at = 0;
   lay.w = 0;
-  at = al(sizeof(double) * t.nw);
+  at = al(sizeof(double) * t.nwd);
   lay.lists = static_cast<int>(at);
   at = al(at + sizeof(uint32_t) * (t.n_state + t.n_mmm + (t.kind == kQuartic ? t.n_mm2 : 0)));
   const size_t b1 = sizeof(double) * static_cast<size_t>(t.ne) * t.F1;
