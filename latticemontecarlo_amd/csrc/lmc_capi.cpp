@@ -303,9 +303,10 @@ const KTables &Upload(Model &m) {
   std::vector<uint8_t> mirror(jt.mirror.begin(), jt.mirror.end());
   // Jump dE weights by migrating species M and reduced index r over element ranks (see
   // ReducedBlock): wd[rank M][r] = w(cluster | endpoint M) - w(cluster | endpoint X); rows
-  // ne .. 2 ne - 1 hold the same values negated (the neighbour endpoint's clusters).
+  // ne .. 2 ne - 1 hold the same values negated (the neighbour endpoint's clusters). Each row
+  // ends with one zero, the weight of the padding entries.
   const int S = m.S, ne = m.ne;
-  const int nr = 1 + 3 * ne + 4 * ne * ne;
+  const int nr = 1 + 3 * ne + 4 * ne * ne, wrow = nr + 1;
   std::vector<int> code;  // element rank -> species code
   for (int c = 0; c < S; ++c)
     if (c != m.x_code) code.push_back(c);
@@ -314,29 +315,47 @@ const KTables &Upload(Model &m) {
     if (l < 4) return m.w[S + (l - 1) * S * S + a * S + c1];
     return m.w[S + 3 * S * S + (l - 4) * S * S * S + (a * S + c1) * S + c2];
   };
-  std::vector<double> wd(static_cast<size_t>(2 * ne) * nr, 0.0);
+  std::vector<double> wd(static_cast<size_t>(2 * ne) * wrow, 0.0);
   for (int rm = 0; rm < ne; ++rm)
     for (int l = 0; l < 8; ++l) {
       const int n1 = l == 0 ? 1 : ne, n2 = l < 4 ? 1 : ne;
       for (int r1 = 0; r1 < n1; ++r1)
         for (int r2 = 0; r2 < n2; ++r2) {
-          const size_t i = static_cast<size_t>(rm) * nr + ReducedBlock(l, ne) + r1 + ne * r2;
+          const size_t i = static_cast<size_t>(rm) * wrow + ReducedBlock(l, ne) + r1 + ne * r2;
           wd[i] = wval(l, code[rm], code[r1], code[r2]) - wval(l, m.x_code, code[r1], code[r2]);
-          wd[i + static_cast<size_t>(ne) * nr] = -wd[i];
+          wd[i + static_cast<size_t>(ne) * wrow] = -wd[i];
         }
     }
-  const int neg_bytes = ne * nr * 8;
+  const int neg_bytes = ne * wrow * 8;
   std::vector<uint32_t> state;
   for (const auto &c : jt.state_v_only) state.push_back(PackState(c, jt.v_slot, false, ne, neg_bytes));
   for (const auto &c : jt.state_n_only) state.push_back(PackState(c, jt.n_slot, true, ne, neg_bytes));
+  // expanded barrier rows: F1e / F2e weights, then one zero (the padding entries' weight)
   int F1e = 0, F2e = 0;
   const std::vector<int> eb1 = ExpandedBases(jt.mmm, ne, F1e), eb2 = ExpandedBases(jt.mm2, ne, F2e);
-  const auto mmm = PackOrbits(jt.mmm, eb1, nullptr);
+  auto mmm = PackOrbits(jt.mmm, eb1, nullptr);
   // MM2 clusters: forward entries, then the backward (mirrored) ones
   auto mm2 = PackOrbits(jt.mm2, eb2, nullptr);
   const std::vector<int> mir(jt.mirror.begin(), jt.mirror.end());
   const auto mm2b = PackOrbits(jt.mm2, eb2, &mir);
   mm2.insert(mm2.end(), mm2b.begin(), mm2b.end());
+  // Lists padded to whole batches of the dense kernel (kClusterPad entries) with entries that
+  // read the zero at the end of every weight row. Their sites are slot 63 (rank entries 0), in
+  // each list's own byte roles: a state entry's byte 0 indexes the rank*8 bytes and byte 1 the
+  // 16-bit rank*8*ne words (offset 2*63); an orbit entry the other way round.
+  auto pad = [](std::vector<uint32_t> &v, uint32_t roles, uint32_t zero_at) {
+    if (zero_at > 65535) throw std::invalid_argument("weight table too large for the packed tables");
+    while (v.size() % kClusterPad) v.push_back(roles | (zero_at << 16));
+  };
+  pad(state, 63u | (126u << 8), static_cast<uint32_t>(nr * 8));
+  pad(mmm, 126u | (63u << 8), static_cast<uint32_t>(F1e * 8));
+  pad(mm2, 126u | (63u << 8), static_cast<uint32_t>(F2e * 8));
+  // the 16-bit rank words are read at even byte offsets only
+  for (uint32_t e : state)
+    if ((e >> 8) & 1) throw std::logic_error("state entry with an odd rank-word offset");
+  for (const auto *v : {&mmm, &mm2})
+    for (uint32_t e : *v)
+      if (e & 1) throw std::logic_error("orbit entry with an odd rank-word offset");
   std::vector<uint32_t> site_cl;
   for (const auto &c : st.clusters) site_cl.push_back(PackSite(c, 0));
   // species code -> element rank, 4 bits each (a vacancy maps to rank 0: it is flagged when
@@ -369,8 +388,8 @@ const KTables &Upload(Model &m) {
   b.Add(site_off, &k.site_off);
   b.Add(m.w, &k.w);
   b.Add(wd, &k.wd);
-  const std::vector<double> b1c = ExpandWeights(m, jt.mmm, m.base_mmm, m.b_mmm, m.F_mmm, eb1, F1e);
-  const std::vector<double> b2c = ExpandWeights(m, jt.mm2, m.base_mm2, m.b_mm2, m.F_mm2, eb2, F2e);
+  const std::vector<double> b1c = ExpandWeights(m, jt.mmm, m.base_mmm, m.b_mmm, m.F_mmm, eb1, F1e + 1);
+  const std::vector<double> b2c = ExpandWeights(m, jt.mm2, m.base_mm2, m.b_mm2, m.F_mm2, eb2, F2e + 1);
   b.Add(b1c, &k.b1);
   b.Add(b2c, &k.b2);
   b.Add(par, &k.par);
@@ -409,11 +428,11 @@ const KTables &Upload(Model &m) {
   k.S = m.S;
   k.ne = m.ne;
   k.x_code = m.x_code;
-  k.F1 = F1e;
-  k.F2 = F2e;
+  k.F1 = F1e + 1;
+  k.F2 = F2e + 1;
   k.nw = static_cast<int>(m.w.size());
   k.nwd = static_cast<int>(wd.size());
-  k.wrow = nr;
+  k.wrow = wrow;
   k.rank_lut = rank_lut;
   k.n_state_v = static_cast<int>(jt.state_v_only.size());
   k.n_inc = static_cast<int>(inc.size());

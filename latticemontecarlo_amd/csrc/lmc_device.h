@@ -16,7 +16,9 @@ namespace lmc {
 //   mm2      : the forward orbit entries, then the backward ones (slots mirrored)
 //   site     : label[0:3] | other1[4:10] | other2[10:16]
 constexpr int kMaxStateClusters = 512;
-constexpr int kMaxOrbitClusters = 640;
+constexpr int kMaxOrbitClusters = 768;
+// Cluster lists are padded to multiples of this (4 entries per lane of an event's 21 lanes).
+constexpr int kClusterPad = 84;
 constexpr int kMaxSiteClusters = 256;
 constexpr int kMaxW = 1024;  // w1 | w2 | w3 for S <= 6
 
@@ -31,9 +33,9 @@ struct KTables {
   const int32_t *site_off;  // [43]
   const double *w;          // theta/N by (label, codes): w1[S] | w2[3][S][S] | w3[4][S][S][S]
   // jump dE over element ranks: [2 ne][wrow] w(.|M) - w(.|X) by the migrating species' rank,
-  // rows ne .. 2 ne - 1 negated; wrow = 1 + 3 ne + 4 ne^2, nwd entries
+  // rows ne .. 2 ne - 1 negated; wrow = 2 + 3 ne + 4 ne^2 (last: 0), nwd entries
   const double *wd;
-  const double *b1;         // [ne][F1] expanded barrier weights, row = rank of the migrating species
+  const double *b1;         // [ne][F1] expanded barrier weights (last: 0), row = rank of the migrating species
   const double *b2;         // [ne][F2]
   const double *par;        // [S][6]
   const double *theta;      // [n_eci]

@@ -11,7 +11,8 @@
 namespace lmc {
 namespace {
 
-constexpr int kLanes = 21;      // dense: lanes per jump event (12 * 21 = 252 <= 256)
+constexpr int kLanes = 21;      // dense: lanes per jump event (3 events per wave)
+static_assert(kClusterPad == 4 * kLanes, "cluster lists are padded to whole dense batches");
 constexpr int kSLanes = 16;     // solvent baseline: lanes per jump event (4 events per wave)
 
 // Element ranks of one event's environment slots, pre-scaled for the cluster tables: rank * 8
@@ -50,6 +51,7 @@ struct ReplicaLds {
   int64_t vac, disp[3];
   uint32_t rid;
   int flags, esc, solvent, done;
+  int vx, vy, vz, m2;  // vacancy coordinates (a/2 units) and the box edge 2n, advanced per jump
 };
 
 // Byte offsets of the tables in dynamic LDS (computed on the host).
@@ -140,6 +142,16 @@ __device__ __forceinline__ const char *BRow(const double *b, int F, const KTable
   return reinterpret_cast<const char *>(b) + __umul24(RankOf(t, M), 8 * F);
 }
 
+// Vacancy coordinates of the replica state from its site id (once per launch).
+__device__ __forceinline__ void SetVacancyXYZ(ReplicaLds &Q, int n) {
+  const XYZ c = ToXYZ(Q.vac, n);
+  Q.vx = c.X;
+  Q.vy = c.Y;
+  Q.vz = c.Z;
+  Q.m2 = 2 * n;
+}
+__device__ __forceinline__ XYZ VacancyXYZ(const ReplicaLds &Q) { return {Q.vx, Q.vy, Q.vz}; }
+
 // Phase 1: species of the 12 environments; neighbour ids of the 12 jumps. 720 sites over the
 // kBT threads of the replica's block: independent loads per thread, issued together. Split in
 // two so that a caller can issue other loads between them.
@@ -152,8 +164,7 @@ struct Gathered {
 
 template <int kBT = kThreads>
 __device__ __forceinline__ void GatherIssue(const JumpLds &L, const uint8_t *occ, int n,
-                                            int64_t vac, int tid, Gathered<kBT> &G) {
-  const XYZ c = ToXYZ(vac, n);
+                                            const XYZ &c, int tid, Gathered<kBT> &G) {
   constexpr int kTotal = 12 * kEnvSlots;
 #pragma unroll
   for (int u = 0; u < Gathered<kBT>::kU; ++u) {
@@ -193,7 +204,7 @@ __device__ __forceinline__ void GatherStore(const KTables &t, JumpLds &L, int32_
 
 template <int kBT = kThreads>
 __device__ __forceinline__ void GatherEnv(const KTables &t, JumpLds &L, const uint8_t *occ, int n,
-                                          int64_t vac, int32_t *err, int tid) {
+                                          const XYZ &vac, int32_t *err, int tid) {
   Gathered<kBT> G;
   GatherIssue<kBT>(L, occ, n, vac, tid, G);
   GatherStore<kBT>(t, L, err, tid, G);
@@ -256,6 +267,56 @@ __device__ __forceinline__ void OrbitBatch(const char *bm, const EventRanks &er,
     acc[u] += *reinterpret_cast<const double *>(bm + (x0[u] + x1[u] + (d[u] >> 16)));
 }
 
+// Four state and four orbit clusters per lane at once (entries c + u kLanes, u < 4): eight
+// independent load chains, issued level by level.
+__device__ __forceinline__ void StateOrbitBatch(const char *wm, const char *bm, const EventRanks &er,
+                                                const uint32_t *sl, const uint32_t *ol, int c,
+                                                double *a, double *b) {
+  uint32_t ds[4], dq[4], x1[4], x2[4], y0[4], y1[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    ds[u] = sl[c + u * kLanes];
+    dq[u] = ol[c + u * kLanes];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    x1[u] = R8At(er, ds[u] & 0xFF);
+    x2[u] = Rn8At(er, (ds[u] >> 8) & 0xFF);
+    y0[u] = Rn8At(er, dq[u] & 0xFF);
+    y1[u] = R8At(er, (dq[u] >> 8) & 0xFF);
+  }
+  const char *p[8];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    p[u] = wm + (x1[u] + x2[u] + (ds[u] >> 16));
+    p[4 + u] = bm + (y0[u] + y1[u] + (dq[u] >> 16));
+  }
+  double v[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double *>(p[u]);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    a[u] += v[u];
+    b[u] += v[4 + u];
+  }
+}
+
+// Dense sums of an event's state list (dE, row wm) and orbit list (logit, row bm), lanes lane,
+// lane + kLanes, ...: both lists are padded to whole batches (multiples of 4 kLanes entries)
+// and advance together.
+__device__ __forceinline__ void StateOrbitRange(const char *wm, const char *bm, const EventRanks &er,
+                                                const uint32_t *sl, int ns, const uint32_t *ol,
+                                                int no, int lane, double &de, double &la) {
+  double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+  const int both = ns < no ? ns : no;
+  int c = lane;
+  for (; c < both; c += 4 * kLanes) StateOrbitBatch(wm, bm, er, sl, ol, c, a, b);
+  for (; c < ns; c += 4 * kLanes) StateBatch<4>(wm, er, sl, c, kLanes, a);
+  for (; c < no; c += 4 * kLanes) OrbitBatch<4>(bm, er, ol, c, kLanes, b);
+  de = (a[0] + a[1]) + (a[2] + a[3]);
+  la = (b[0] + b[1]) + (b[2] + b[3]);
+}
+
 // Sum of the orbit clusters [0, n) of an event, lanes lane, lane + stride, ...
 __device__ __forceinline__ double OrbitRange(const char *bm, const EventRanks &er,
                                              const uint32_t *list, int n, int lane, int stride) {
@@ -267,15 +328,19 @@ __device__ __forceinline__ double OrbitRange(const char *bm, const EventRanks &e
 }
 
 // Phase 2, dense: per-lane partial sums of every cluster's weight (dE, MMM logit, MM2 logit).
+// Wave w evaluates events 3w .. 3w + 2 (lanes 0..62), so that it can finish them alone.
 __device__ void DenseClusters(const KTables &t, JumpLds &L, const Tabs &T, int tid) {
-  if (tid >= 12 * kLanes) return;
-  const int k = tid / kLanes, lane = tid - k * kLanes;
+  const int w = tid >> 6, l = tid & 63;
+  if (l >= 3 * kLanes) return;
+  const int g = l / kLanes, k = 3 * w + g, lane = l - g * kLanes;
   const EventRanks &er = L.er[k];
   const int M = L.env[k][t.n_slot];
-  const double de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, kLanes);
-  double la = 0.0, lb = 0.0;
+  double de, la = 0.0, lb = 0.0;
   if (t.kind == kE0 || t.kind == kQuartic)
-    la = OrbitRange(BRow(T.b1, t.F1, t, M), er, T.mmm, t.n_mmm, lane, kLanes);
+    StateOrbitRange(WRow(t, T, M), BRow(T.b1, t.F1, t, M), er, T.state, t.n_state, T.mmm, t.n_mmm,
+                    lane, de, la);
+  else
+    de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, kLanes);
   if (t.kind == kQuartic)  // forward and backward MM2 entries in one list
     lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2, lane, kLanes);
   L.part[k * kLanes + lane][0] = de;
@@ -421,26 +486,31 @@ __device__ __forceinline__ void StoreEvent(JumpLds &L, int k, double ea, double 
   L.dir[r] = k;
 }
 
-// Phase 3: Ea, dE and rate per jump, stored at the jump's rank among the 12 neighbour ids.
+// Phase 3: Ea, dE and rate per jump, stored at the jump's rank among the 12 neighbour ids. Each
+// wave finishes the events whose partials its own lanes wrote (3 dense, 4 solvent-baseline), so
+// no workgroup barrier separates this phase from the cluster sums.
 template <bool kSolvent>
 __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const Tabs &T, int sv,
                                              double beta, int32_t *err, int tid) {
   constexpr int kL = kSolvent ? kSLanes : kLanes;
-  const int k = tid;
-  if (k >= 64) return;
-  // wave 0: lane 3q + c adds the kL partials of component c of event q; lane q collects them
-  // (LDS traffic of one wave completes in program order)
-  if (k < 36) {
-    const int q = k / 3, c = k - 3 * q;
+  constexpr int kE = 64 / kL;  // events per wave
+  const int w = tid >> 6, l = tid & 63;
+  if (w * kE >= 12) return;
+  // lane 3q + c adds the kL partials of component c of the wave's event q (LDS traffic of one
+  // wave completes in program order)
+  if (l < 3 * kE) {
+    const int q = l / 3, c = l - 3 * q;
+    const int e = w * kE + q;
     double s = 0.0;
 #pragma unroll
-    for (int l = 0; l < kL; ++l) s += L.part[q * kL + l][c];
-    L.sums[q][c] = s;
+    for (int i = 0; i < kL; ++i) s += L.part[e * kL + i][c];
+    L.sums[e][c] = s;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (k >= 12) return;
+  if (l >= kE) return;
+  const int k = w * kE + l;
   double de = L.sums[k][0], la = L.sums[k][1], lb = L.sums[k][2];
   if (kSolvent) {  // all-solvent sums of this (solvent, migrating species)
     const double *b = T.base + (static_cast<size_t>(sv) * t.S + L.env[k][t.n_slot]) * 3;
@@ -456,15 +526,30 @@ __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const
   StoreEvent(L, k, ea, de, ekra, beta);
 }
 
+// Random numbers of step Q.steps (one lane, overlapping the gather): Philox u1, u2, log(u1) and
+// the RateCorrector factor (KineticMcFirstOmp.cpp:79-82, RateCorrector.hpp:17-24).
+__device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
+  const U4 u = Philox4x32_10(Q.seed, Q.steps, Q.rid, 0u);
+  L.u2 = UnitOpenRight(Join64(u.z, u.w));
+  L.log_u1 = log(UnitOpenLeft(Join64(u.x, u.y)));
+  L.corr = (Q.flags & 1) ? Q.cv / (1.64 * exp(-(0.66 / kBoltzmann / Q.T - 0.7))) / (1 - 13 * Q.cs)
+                         : 1.0;
+}
+
+// `draw`: the replica whose next random numbers the last thread draws while the environment loads
+// are in flight (null: none).
 template <bool kSolvent>
 __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, const Tabs &T,
-                                              const uint8_t *occ, int n, int64_t vac, int sv,
-                                              double beta, int32_t *err, int tid) {
-  GatherEnv(t, L, occ, n, vac, err, tid);
+                                              const uint8_t *occ, int n, const XYZ &vac, int sv,
+                                              double beta, int32_t *err, int tid,
+                                              const ReplicaLds *draw = nullptr) {
+  Gathered<> G;
+  GatherIssue(L, occ, n, vac, tid, G);
+  if (draw && tid == kThreads - 1) DrawStep(L, *draw);
+  GatherStore(t, L, err, tid, G);
   __syncthreads();
   if (kSolvent) SolventClusters(t, L, T, sv, tid);
   else DenseClusters(t, L, T, tid);
-  __syncthreads();
   FinishEvents<kSolvent>(t, L, T, sv, beta, err, tid);
   __syncthreads();
 }
@@ -478,20 +563,11 @@ __device__ double ScheduleTemperature(const double *tt, const double *tv, int n,
   return tv[i - 1] + ((time - tt[i - 1]) / (tt[i] - tt[i - 1])) * (tv[i] - tv[i - 1]);
 }
 
-// Random numbers of step Q.steps (one lane, overlapping the gather): Philox u1, u2, log(u1) and
-// the RateCorrector factor (KineticMcFirstOmp.cpp:79-82, RateCorrector.hpp:17-24).
-__device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
-  const U4 u = Philox4x32_10(Q.seed, Q.steps, Q.rid, 0u);
-  L.u2 = UnitOpenRight(Join64(u.z, u.w));
-  L.log_u1 = log(UnitOpenLeft(Join64(u.x, u.y)));
-  L.corr = (Q.flags & 1) ? Q.cv / (1.64 * exp(-(0.66 / kBoltzmann / Q.T - 0.7))) / (1 - 13 * Q.cs)
-                         : 1.0;
-}
 
-// Phase 4 (lanes k < 12): lane j divides rate j by the total; every event lane then forms the
-// same ordered cumulative probabilities (KineticMcFirstOmp.cpp:69-77); the lane holding the first
-// cumulative probability >= u2 (lane 11 when there is none, KineticMcAbstract.cpp:106-123)
-// returns true with the event's rank and the residence time (KineticMcFirstOmp.cpp:79-82).
+// Phase 4 (lanes i < 12): lane i divides rate i by the total (KineticMcFirstOmp.cpp:69-77);
+// lane 0 then forms the ordered cumulative probabilities and takes the first one >= u2, or event
+// 11 when there is none (KineticMcAbstract.cpp:106-123), and the residence time
+// (KineticMcFirstOmp.cpp:79-82). Returns true on lane 0 with the event's rank.
 __device__ __forceinline__ bool ChooseEvent(JumpLds &L, const ReplicaLds &Q, int i, int &sel,
                                             double &dt) {
   double total = 0.0;
@@ -501,19 +577,15 @@ __device__ __forceinline__ bool ChooseEvent(JumpLds &L, const ReplicaLds &Q, int
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (i != 0) return false;
   const double u2 = L.u2;
-  double cum = 0.0, prev = 0.0;
+  double cum = 0.0;
+  sel = 11;
 #pragma unroll
-  for (int j = 0; j < 12; ++j) {
-    const double pj = L.prob[j];
-    if (j <= i) {
-      prev = cum;
-      cum += pj;
-    }
+  for (int j = 0; j < 11; ++j) {
+    cum += L.prob[j];
+    if (sel == 11 && !(cum < u2)) sel = j;
   }
-  const bool mine = (!(cum < u2) && (i == 0 || prev < u2)) || (i == 11 && cum < u2);
-  if (!mine) return false;
-  sel = i;
   dt = -L.log_u1 / total / kPrefactor;
   if (Q.flags & 1) dt *= L.corr;
   return true;
@@ -525,9 +597,16 @@ __device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, 
                            uint8_t *occ, int r, uint64_t trace_index, int sel, double dt,
                            double de) {
   if (Q.flags & 2) {  // IsEscaped (KineticMcAbstract.cpp:192-223)
+    // walk the set bits of the shell mask without unrolling: an unrolled loop keeps one scalar
+    // predicate per slot live through the whole kernel
     bool all = true;
-    for (int j = 0; j < kEnvSlots; ++j)
-      if (((t.vac_shell_mask >> j) & 1ull) && L.env[0][j] != Q.solvent) all = false;
+    uint64_t m = t.vac_shell_mask;
+#pragma unroll 1
+    while (m && all) {
+      const int j = __ffsll(static_cast<unsigned long long>(m)) - 1;
+      m &= m - 1;
+      all = L.env[0][j] == Q.solvent;
+    }
     if (all) {
       Q.esc = 1;
       Q.steps = Q.max_steps + 1;
@@ -557,6 +636,9 @@ __device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, 
   Q.disp[0] += dx;
   Q.disp[1] += dy;
   Q.disp[2] += dz;
+  Q.vx = Wrap(Q.vx + dx, Q.m2);
+  Q.vy = Wrap(Q.vy + dy, Q.m2);
+  Q.vz = Wrap(Q.vz + dz, Q.m2);
   occ[Q.vac] = static_cast<uint8_t>(mig);  // Config::LatticeJump (Config.cpp:431-456)
   occ[jump] = static_cast<uint8_t>(t.x_code);
   Q.steps += 1;

@@ -46,8 +46,8 @@ __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout
   const Tabs T =
       LoadJumpTables<kSolvent, kBL>(t, L, dyn, lay, threadIdx.x, kThreads, threadIdx.x, kThreads);
   __syncthreads();
-  EvaluateJumps<kSolvent>(t, L, T, occ + off[r], ncell[r], vac[r], t.solvent_default, 0.0, err,
-                          threadIdx.x);
+  EvaluateJumps<kSolvent>(t, L, T, occ + off[r], ncell[r], ToXYZ(vac[r], ncell[r]),
+                          t.solvent_default, 0.0, err, threadIdx.x);
   const int k = threadIdx.x;
   if (k < 12) {
     const int64_t o = static_cast<int64_t>(r) * 12 + k;
@@ -97,6 +97,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
     }
     Q.done = (Q.steps > Q.max_steps || Q.esc || !ValidSite(n, Q.vac)) ? 1 : 0;
     Q.beta = 1.0 / kBoltzmann / Q.T;
+    SetVacancyXYZ(Q, n);
   }
   __syncthreads();
   const int sv = Q.solvent;
@@ -109,8 +110,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
       }
       __syncthreads();
     }
-    if (tid == kThreads - 1) DrawStep(L, Q);
-    EvaluateJumps<kSolvent>(t, L, T, occ, n, Q.vac, sv, Q.beta, k.error, tid);
+    EvaluateJumps<kSolvent>(t, L, T, occ, n, VacancyXYZ(Q), sv, Q.beta, k.error, tid, &Q);
     SelectAndApply(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }

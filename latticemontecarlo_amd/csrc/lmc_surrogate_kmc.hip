@@ -320,6 +320,7 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
                     : (ValidSite(n, Q.vac) ? 0 : 1);
       if (!ValidSite(n, Q.vac)) atomicOr(k.error, 8);
       Q.beta = 1.0 / kBoltzmann / Q.T;
+      SetVacancyXYZ(Q, n);
     }
   }
   __syncthreads();
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
     __syncthreads();
     const bool on = act[g];
     Gathered<kSub> gat;
-    if (on) GatherIssue<kSub>(L, occ, n, Q.vac, lt, gat);
+    if (on) GatherIssue<kSub>(L, occ, n, VacancyXYZ(Q), lt, gat);
     WaveWeights<C> ww;  // issued after the occupancy loads, consumed after the gather
     LoadWaveWeights<C>(sd, ww);
     if (kKmc && on && lt == kSub - 1) DrawStep(L, Q);

@@ -81,6 +81,27 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
   const auto s = kmc.GetState();
   std::printf("{\"kmc_steps\": %llu, \"kmc_time\": %.17g, \"kmc_vacancy\": %lld}\n",
               static_cast<unsigned long long>(s.steps[0]), s.time[0], static_cast<long long>(s.vacancy[0]));
+  // the same 201 steps replayed from a captured graph of three launches
+  mc::KineticMcFirstBatch g({c}, {prm}, barrier, &energy);
+  g.RunGraph(67, 3);
+  const auto sg = g.GetState();
+  std::printf("{\"graph_steps\": %llu, \"graph_time\": %.17g, \"graph_vacancy\": %lld}\n",
+              static_cast<unsigned long long>(sg.steps[0]), sg.time[0], static_cast<long long>(sg.vacancy[0]));
+  // a surrogate trained briefly on the batch's own samples, with every event falling back
+  mc::KineticMcFirstBatch u({c}, {prm}, barrier, &energy);
+  pred::Surrogate sur(barrier, 4, 1);
+  lmc::DeviceBuffer env(12 * 64), ekra(12 * sizeof(double)), dde(12 * sizeof(double));
+  u.Samples(barrier, env.as<uint8_t>(), ekra.as<double>(), dde.as<double>());
+  lmc::Check(lmc_synchronize(nullptr));
+  const auto mse = sur.Train(env.as<uint8_t>(), ekra.as<double>(), dde.as<double>(), 12, 2, 12);
+  u.SetSurrogate(&sur, -1.0, 20.0);
+  u.Simulate();
+  const auto su = u.GetState();
+  const auto [events, fallbacks] = u.SurrogateStats();
+  std::printf("{\"sur_steps\": %llu, \"sur_time\": %.17g, \"sur_vacancy\": %lld, \"sur_events\": %llu, "
+              "\"sur_fallbacks\": %llu, \"sur_mse_size\": %zu}\n",
+              static_cast<unsigned long long>(su.steps[0]), su.time[0], static_cast<long long>(su.vacancy[0]),
+              static_cast<unsigned long long>(events), static_cast<unsigned long long>(fallbacks), mse.size());
   return 0;
 }
 

@@ -64,3 +64,12 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
     assert k["kmc_steps"] == st["steps"] == 201
     assert k["kmc_vacancy"] == st["vacancy"]
     assert abs(k["kmc_time"] - st["time"]) <= 1e-12 * abs(st["time"])
+    g = next(x for x in rows if "graph_steps" in x)  # graph replay: bit-identical to eager
+    assert (g["graph_steps"], g["graph_vacancy"], g["graph_time"]) == (
+        k["kmc_steps"], k["kmc_vacancy"], k["kmc_time"])
+    # surrogate with full fallback: the plain run (its exact sums take another lane order, so
+    # the clock agrees to rounding)
+    u = next(x for x in rows if "sur_steps" in x)
+    assert (u["sur_steps"], u["sur_vacancy"]) == (k["kmc_steps"], k["kmc_vacancy"])
+    assert abs(u["sur_time"] - k["kmc_time"]) <= 1e-12 * abs(k["kmc_time"])
+    assert u["sur_events"] == u["sur_fallbacks"] == 12 * 201 and u["sur_mse_size"] == 8
