@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 ALGO_BYTES_PER_STEP = 677  # 135 unique sites x (1 B species + 4 B index) + 2 B written
+METRIC = "KMC steps/s over all replicas, surrogate vs plain GPU at \u22645 meV barrier MAE"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -92,17 +93,14 @@ def cpu_baseline(coeff_path, budget_s=12.0):
                        f"single-threaded ({thru:.1f} steps/s); value = larger")}
 
 
-def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_value):
-    """Learned surrogate on the same workload: 4 BF16 MLP members trained on the GPU on exact
-    samples of KMC trajectories (1024 replicas of 10^3 cells, same composition range, every 12
-    steps, x4 by the jump symmetries), then the surrogate KMC path timed on the bench replicas
-    (same seeds), its fallback rate, and its barrier error against the exact evaluation of the
-    final states."""
+def train_surrogate(L, bm, s_occs, s_temps, n, seed):
+    """Four BF16 MLP members trained on the GPU on exact samples of KMC trajectories of the
+    replicas `s_occs` (n^3 cells; 160 samples of the 12 jumps of every replica, 12 steps apart,
+    x4 by the jump symmetries). Returns the surrogate and a summary."""
     import torch
     t0 = time.time()
-    s_occs, s_temps = make_workload(1024, 10, seed + 1)
-    drv = L.KineticMcFirst(bm, None, [10] * 1024, s_occs, s_temps, seed=seed + 1,
-                           maximum_steps=10 ** 9)
+    R = len(s_occs)
+    drv = L.KineticMcFirst(bm, None, [n] * R, s_occs, s_temps, seed=seed + 1, maximum_steps=10 ** 9)
     drv.run(20)
     E, K, D = [], [], []
     for _ in range(160):
@@ -123,6 +121,14 @@ def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_va
     del env, ek, de
     log(f"surrogate: {n_samples} samples ({time.time() - t0 - train_s:.1f} s), trained in "
         f"{train_s:.1f} s, mse {np.round(mse, 5)}")
+    return sur, {"train_samples": n_samples, "train_rows_augmented": 4 * n_samples,
+                 "train_s": round(train_s, 2)}
+
+
+def time_surrogate(L, bm, sur, R, n, occs, temps, seed, steps, warmup, chunk, plain_value):
+    """The surrogate KMC path on the given replicas (same seeds as the plain run): throughput,
+    fallback rate, and barrier error of the final states against the exact evaluation."""
+    import torch
     threshold, window = 0.010, 20.0
     drv = L.KineticMcFirst(bm, None, [n] * R, occs, temps, seed=seed, maximum_steps=10 ** 9)
     drv.set_surrogate(sur, threshold=threshold, window=window)
@@ -159,12 +165,94 @@ def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_va
            "mae_ea_meV_relevant_surrogate_events":
                round(float(err[~f & rel].mean()) * 1e3, 3) if (~f & rel).any() else None,
            "threshold_meV": threshold * 1e3, "relevance_window_kT": window, "members": 4,
-           "train_samples": n_samples, "train_rows_augmented": 4 * n_samples, "train_s": round(train_s, 2),
            "dtype": "bf16 weights and activations, fp32 accumulation"}
     log(f"surrogate KMC: {value:.4g} steps/s ({value / plain_value:.2f}x plain), fallback "
         f"{out['fallback_rate'] * 100:.1f}%, MAE Ea {out['mae_ea_meV_all_events']} meV (all), "
         f"{out['mae_ea_meV_relevant_surrogate_events']} meV (relevant surrogate events)")
     return out
+
+
+def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_value):
+    """Learned surrogate on the bench workload: trained on trajectories of 1024 replicas of
+    10^3 cells of the same composition range, then timed on the bench replicas."""
+    s_occs, s_temps = make_workload(1024, 10, seed + 1)
+    sur, info = train_surrogate(L, bm, s_occs, s_temps, 10, seed)
+    out = time_surrogate(L, bm, sur, R, n, occs, temps, seed, steps, warmup, chunk, plain_value)
+    out.update(info)
+    return out
+
+
+def timed_plain(L, drv, R, steps, warmup, chunk):
+    """Warm-up launches, then `steps` KMC steps of every replica replayed from one HIP graph of
+    steps / chunk launches; returns the seconds measured with HIP events on the stream."""
+    import torch
+    stream = torch.cuda.current_stream()
+    left = warmup
+    while left > 0:
+        c = min(chunk, left)
+        drv.run(c, stream)
+        left -= c
+    n_launch = max(1, steps // chunk)
+    assert n_launch * chunk == steps, "--steps must be a multiple of --chunk"
+    drv.build_graph(chunk, n_launch)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    drv.run_graph(chunk, n_launch, stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1e3
+
+
+def large_workload(args):
+    """Large surrogate run: 64 replicas of 50^3 cells (500000 sites), Al/Cu/Mg/Zn 94/2/2/2 at.%,
+    T = 450 K, quaternary E0 model. The plain path and the surrogate path (trained on exact
+    samples of 1024 trajectories of 10^3 cells of the same composition) run on the same seeds;
+    the surrogate's fallback rate and barrier error are reported. Prints one JSON line."""
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import synth
+    els = ["Al", "Cu", "Mg", "Zn"]
+    species = sorted(els + ["X"])
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0_q.json")
+    synth.write_coefficients(coeff, els, "E0", seed=args.seed)
+    bm = L.Model(coeff, els, L.E0)
+
+    def alloy(R, n, seed):
+        N = 4 * n ** 3
+        c = round(0.02 * N)
+        return [synth.random_occupancy(n, species, {"Cu": c, "Mg": c, "Zn": c}, seed=seed * 100003 + r)
+                for r in range(R)]
+
+    R, n = 64, 50
+    occs, temps = alloy(R, n, args.seed), np.full(R, 450.0)
+    log(f"large: {R} replicas x {4 * n ** 3} sites, Al/Cu/Mg/Zn 94/2/2/2, T 450 K")
+    drv = L.KineticMcFirst(bm, None, [n] * R, occs, temps, seed=args.seed, maximum_steps=10 ** 9)
+    tune = drv.autotune(steps=200)
+    elapsed = timed_plain(L, drv, R, args.steps, args.warmup, args.chunk)
+    bm.check_errors()
+    value = R * args.steps / elapsed
+    log(f"plain: {value:.4g} KMC steps/s over {R} replicas ({elapsed:.2f} s)")
+    del drv
+    sur, info = train_surrogate(L, bm, alloy(1024, 10, args.seed + 1), np.full(1024, 450.0), 10,
+                                args.seed)
+    s = time_surrogate(L, bm, sur, R, n, occs, temps, args.seed, args.steps, args.warmup,
+                       args.chunk, value)
+    s.update(info)
+    line = {"metric": METRIC, "value": round(value, 1), "unit": "KMC steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed / args.steps * 1e3, 6), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"large surrogate run, {R} replicas x {n}^3 FCC cells "
+                                   f"({4 * n ** 3} sites), Al/Cu/Mg/Zn 94/2/2/2 at.%, T 450 K, "
+                                   "quaternary E0 barrier model",
+                       "value_path": "plain GPU path (exact); the surrogate path is reported "
+                                     "under surrogate",
+                       "formulation": ["dense", "solvent-baseline"][tune["formulation"]],
+                       "event_evaluations_per_s": round(12 * value, 1)},
+            "surrogate": s}
+    print(json.dumps(line), flush=True)
 
 
 def uneven_workload(args):
@@ -250,7 +338,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch eagerly instead of a HIP graph")
-    ap.add_argument("--workload", choices=["batched", "uneven"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "uneven", "large"], default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
     args = ap.parse_args()
@@ -267,6 +355,9 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == "uneven":
         uneven_workload(args)
+        return
+    if args.workload == "large":
+        large_workload(args)
         return
     R, n = args.replicas, args.ncell
     tmp = tempfile.mkdtemp()
@@ -325,7 +416,7 @@ def main():
     achieved = ALGO_BYTES_PER_STEP * R * args.chunk / (elapsed / n_launch) / 1e9
     traffic = measured_traffic(R, n, args.chunk)
     line = {
-        "metric": "KMC steps/s over all replicas (plain GPU path)",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "KMC steps/s",
         "n_gpus": 1,
@@ -340,6 +431,8 @@ def main():
         "config": {"workload": f"plain batched KMC, {R} replicas x {n}^3 FCC cells "
                                f"({4 * n ** 3} sites), Al with 1-5 at.% Mg and Zn each, "
                                "T 400-700 K, E0 barrier model, one vacancy per replica",
+                   "value_path": "plain GPU path (exact); the surrogate path is reported under "
+                                 "surrogate",
                    "replicas": R, "sites_per_replica": 4 * n ** 3,
                    "steps_per_replica": args.steps, "steps_per_launch": args.chunk,
                    "launch": "eager" if args.eager else "hip graph",

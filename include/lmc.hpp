@@ -10,6 +10,8 @@
 //   mc::KineticMcFirstBatch, mc::CanonicalMcBatch  KineticMcFirstOmp.h:11-25,
 //                                                  CanonicalMcSerial.h:11-20 (R replicas each)
 //   pred::Surrogate                                no counterpart (learned jump evaluation)
+//   ansys::ExitTime                                lmc/ansys/include/ExitTime.h:12-63 (barrier
+//                                                  lists and exit times)
 // Elements are strings ("Al", "X"); species codes are indices into sorted(element_set + "X").
 // Errors: LMC_ERR_IO -> std::runtime_error, LMC_ERR_MODEL -> std::out_of_range,
 // LMC_ERR_ARG -> std::invalid_argument, LMC_ERR_HIP -> std::runtime_error, each carrying
@@ -17,11 +19,13 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -462,3 +466,65 @@ class CanonicalMcBatch {
 };
 
 }  // namespace mc
+
+namespace ansys {
+
+// ExitTime over one configuration (lmc/ansys/include/ExitTime.h:12-63): every migration barrier
+// of a vacancy placed at a site is evaluated on the GPU (lmc_vacancy_scan) in the configuration
+// whose own vacancy holds the solvent; the derived quantities are host arithmetic.
+class ExitTime {
+ public:
+  ExitTime(const cfg::Config &config, const std::string &solvent_element, double escape_temperature,
+           const pred::VacancyMigrationPredictor &vacancy_migration_predictor)
+      : config_(config),
+        solvent_(config.CodeOf(solvent_element)),
+        beta_escape_(1.0 / 8.617333262145e-5 / escape_temperature),
+        predictor_(vacancy_migration_predictor) {}
+
+  // GetBarrierListAndExitTime (ExitTime.cpp:216-244): per lattice site, the 12 migration
+  // barriers of a vacancy placed there (ascending), their mean, and the exit time
+  // 1 / (nu sum_j exp(-Ea_j beta_escape)).
+  [[nodiscard]] std::tuple<std::vector<std::vector<double>>, std::vector<double>, std::vector<double>>
+  GetBarrierListAndExitTime() const {
+    const size_t N = config_.GetNumAtoms();
+    std::vector<int64_t> sites(N);
+    for (size_t i = 0; i < N; ++i) sites[i] = static_cast<int64_t>(i);
+    int64_t vac = -1;
+    const uint8_t x = config_.CodeOf("X");
+    for (size_t i = 0; i < N; ++i)
+      if (config_.GetCodes()[i] == x) vac = static_cast<int64_t>(i);
+    lmc::DeviceBuffer d_sites(N * sizeof(int64_t)), nb(12 * N * sizeof(int64_t)),
+        ea(12 * N * sizeof(double)), de(12 * N * sizeof(double));
+    d_sites.Upload(sites.data(), N * sizeof(int64_t));
+    lmc::Check(lmc_vacancy_scan(predictor_.handle(), config_.DeviceOccupancy(),
+                                config_.GetNumCells(), vac, solvent_, d_sites.as<int64_t>(),
+                                static_cast<int32_t>(N), nb.as<int64_t>(), ea.as<double>(),
+                                de.as<double>(), nullptr));
+    std::vector<double> h(12 * N);
+    ea.Download(h.data(), h.size() * sizeof(double));
+    lmc::Check(lmc_check_errors(predictor_.handle()));
+    std::vector<std::vector<double>> lists(N);
+    std::vector<double> mean(N), exit_time(N);
+    for (size_t i = 0; i < N; ++i) {
+      lists[i].assign(h.begin() + static_cast<std::ptrdiff_t>(12 * i),
+                      h.begin() + static_cast<std::ptrdiff_t>(12 * i + 12));
+      std::sort(lists[i].begin(), lists[i].end());
+      double sum = 0.0, rate = 0.0;
+      for (double b : lists[i]) {
+        sum += b;
+        rate += std::exp(-b * beta_escape_);
+      }
+      mean[i] = sum / 12.0;
+      exit_time[i] = 1.0 / rate / 1e13;
+    }
+    return {lists, mean, exit_time};
+  }
+
+ private:
+  const cfg::Config &config_;
+  int32_t solvent_;
+  double beta_escape_;
+  const pred::VacancyMigrationPredictor &predictor_;
+};
+
+}  // namespace ansys

@@ -121,6 +121,16 @@ int32_t lmc_jump_orbit_clusters(int32_t n, int64_t v, int64_t nb, int32_t which,
 int lmc_eval_jumps(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
                    const int32_t *d_ncell, const int64_t *d_vac, int32_t n_replicas,
                    int64_t *d_nb, double *d_ea, double *d_de, void *stream);
+/* Vacancy scan of the ExitTime analysis: in the configuration d_occ (one replica of ncell^3
+ * cells) with its vacancy `vacancy` replaced by species `solvent` (-1: no vacancy), a vacancy
+ * placed at each of d_sites[n_sites] in turn, and its 12 jumps in neighbour-id order:
+ * d_nb/d_ea/d_de of length 12 n_sites. Replaces the loops over sites of
+ * ExitTime::GetBarrierListAndExitTime and ExitTime::GetJumpEnergetics
+ * (lmc/ansys/src/ExitTime.cpp:216-311), each jump a GetBarrierAndDiffFromLatticeIdPair call on
+ * the modified configuration. Asynchronous; an invalid site is reported by lmc_check_errors(). */
+int lmc_vacancy_scan(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64_t vacancy,
+                     int32_t solvent, const int64_t *d_sites, int32_t n_sites, int64_t *d_nb,
+                     double *d_ea, double *d_de, void *stream);
 /* One jump v -> nb of one replica, synchronous, host outputs: replaces
  * GetBarrierAndDiffFromLatticeIdPair(config, {v, nb}). */
 int lmc_eval_pair(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64_t v, int64_t nb,

@@ -7,4 +7,5 @@ interface over that ABI.
 from .api import (E0, ENERGY, PAIRSITE, QUARTIC, CanonicalMc, KineticMcFirst, Model,  # noqa: F401
                   ReplicaBatch, Surrogate, augment, eval_jumps, fcc_neighbors, fcc_num_sites,
                   jump_environment, jump_orbit_clusters, jump_samples, jump_state_clusters, jump_symmetries,
-                  site_de, swap_de, total_energy)
+                  site_de, swap_de, total_energy, vacancy_scan)
+from . import ansys  # noqa: F401,E402

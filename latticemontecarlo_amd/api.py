@@ -162,6 +162,23 @@ def eval_jumps(model, batch, vac, stream=None):
     return nb, ea, de
 
 
+def vacancy_scan(model, occ, n, vacancy, solvent, sites, stream=None):
+    """The 12 jumps (neighbour-id order) of a vacancy placed at each of `sites` in the
+    configuration `occ` (uint8 device tensor, one replica of n^3 cells) with its vacancy site
+    `vacancy` (-1: none) holding species `solvent` instead: (nb, Ea, dE) [K, 12]
+    (ExitTime::GetJumpEnergetics, lmc/ansys/src/ExitTime.cpp:272-311)."""
+    torch = _torch()
+    dev = occ.device
+    s = torch.as_tensor(np.asarray(sites, np.int64).reshape(-1)).to(dev)
+    K = int(s.numel())
+    nb = torch.empty((K, 12), dtype=torch.int64, device=dev)
+    ea = torch.empty((K, 12), dtype=torch.float64, device=dev)
+    de = torch.empty((K, 12), dtype=torch.float64, device=dev)
+    check(lib().lmc_vacancy_scan(model.h, dptr(occ), int(n), int(vacancy), int(solvent), dptr(s), K,
+                                 dptr(nb), dptr(ea), dptr(de), capi.stream_ptr(stream)))
+    return nb, ea, de
+
+
 def swap_de(model, batch, pairs, stream=None):
     torch = _torch()
     p = torch.as_tensor(np.asarray(pairs, np.int64).reshape(-1)).to(batch.occ.device)

@@ -68,6 +68,7 @@ SIGNATURES = {
     "lmc_jump_symmetries": (C.c_int, [_i32p]),
     "lmc_jump_orbit_clusters": (_i32, [_i32, _i64, _i64, _i32, _i64p, _i32]),
     "lmc_eval_jumps": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "lmc_vacancy_scan": (C.c_int, [_vp, _vp, _i32, _i64, _i32, _vp, _i32, _vp, _vp, _vp, _vp]),
     "lmc_eval_pair": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _f64p, _f64p]),
     "lmc_swap_de": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lmc_site_de": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),

@@ -825,6 +825,29 @@ int lmc_eval_jumps(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_of
   });
 }
 
+int lmc_vacancy_scan(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64_t vacancy,
+                     int32_t solvent, const int64_t *d_sites, int32_t n_sites, int64_t *d_nb,
+                     double *d_ea, double *d_de, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    if (m->m->kind != LMC_MODEL_E0 && m->m->kind != LMC_MODEL_QUARTIC)
+      throw std::invalid_argument("lmc_vacancy_scan needs an E0 or Quartic model");
+    if (n_sites < 0) throw std::invalid_argument("n_sites < 0");
+    if (ncell < 4) throw std::invalid_argument("ncell must be >= 4");
+    const int64_t N = 4ll * ncell * ncell * ncell;
+    if (vacancy < -1 || vacancy >= N) throw std::invalid_argument("vacancy site out of range");
+    if (solvent < 0 || solvent >= m->m->S || solvent == m->m->x_code)
+      throw std::invalid_argument("solvent must be an element code");
+    if (n_sites > 0 && (!d_occ || !d_sites || !d_nb || !d_ea || !d_de))
+      throw std::invalid_argument("null buffer");
+    const auto &t = lmc::Upload(*m->m);
+    Check(lmc::LaunchVacancyScan(t, d_occ, ncell, vacancy, solvent, d_sites, n_sites, d_nb, d_ea,
+                                 d_de, m->m->dev_err, S(stream)),
+          "vacancy scan kernel");
+    return LMC_OK;
+  });
+}
+
 int lmc_check_errors(const lmc_model *m) {
   return Guard([&] {
     RequireModel(m);

@@ -124,6 +124,12 @@ hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *
                            const int32_t *ncell, const int64_t *vac, int32_t R, int64_t *nb,
                            double *ea, double *de, int32_t *err, hipStream_t s, bool solvent,
                            uint8_t *env_out = nullptr, double *ekra_out = nullptr);
+// The 12 jumps of a vacancy placed at each of sites[K] in the single occupancy occ (n^3 cells),
+// with site sub_site holding species sub_code instead (the ExitTime analysis' configuration);
+// outputs [K][12] in neighbour-id order.
+hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, int64_t sub_site,
+                             int sub_code, const int64_t *sites, int32_t K, int64_t *nb,
+                             double *ea, double *de, int32_t *err, hipStream_t s);
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
                      hipStream_t s, bool solvent);
 // Everything LaunchKmc sets up outside the stream (LDS plan, kernel attributes), so that its

@@ -178,9 +178,16 @@ __device__ __forceinline__ void GatherIssue(const JumpLds &L, const uint8_t *occ
   }
 }
 
-template <int kBT = kThreads>
+// Vacancy scan (ExitTime analysis): the stored configuration with site `site` holding species
+// `code` (the real vacancy turned into solvent), and the vacancy placed at the jump origin.
+struct GatherSub {
+  int64_t site;
+  int code;
+};
+
+template <int kBT = kThreads, bool kScan = false>
 __device__ __forceinline__ void GatherStore(const KTables &t, JumpLds &L, int32_t *err, int tid,
-                                            const Gathered<kBT> &G) {
+                                            const Gathered<kBT> &G, const GatherSub &sub = {-1, 0}) {
   constexpr int kTotal = 12 * kEnvSlots;
 #pragma unroll
   for (int u = 0; u < Gathered<kBT>::kU; ++u) {
@@ -188,6 +195,10 @@ __device__ __forceinline__ void GatherStore(const KTables &t, JumpLds &L, int32_
     if (i < kTotal) {
       const int k = i / kEnvSlots, j = i - k * kEnvSlots;
       uint8_t x = G.v[u];
+      if (kScan) {
+        if (G.s[u] == sub.site) x = static_cast<uint8_t>(sub.code);
+        if (j == t.v_slot) x = static_cast<uint8_t>(t.x_code);
+      }
       if (x >= t.S) {  // corrupt occupancy: flag it and keep every table index in range
         atomicOr(err, 4);
         x = 0;
@@ -537,16 +548,17 @@ __device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
 }
 
 // `draw`: the replica whose next random numbers the last thread draws while the environment loads
-// are in flight (null: none).
-template <bool kSolvent>
+// are in flight (null: none). kScan: the gather applies `sub` (see GatherSub).
+template <bool kSolvent, bool kScan = false>
 __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, const Tabs &T,
                                               const uint8_t *occ, int n, const XYZ &vac, int sv,
                                               double beta, int32_t *err, int tid,
-                                              const ReplicaLds *draw = nullptr) {
+                                              const ReplicaLds *draw = nullptr,
+                                              const GatherSub &sub = {-1, 0}) {
   Gathered<> G;
   GatherIssue(L, occ, n, vac, tid, G);
   if (draw && tid == kThreads - 1) DrawStep(L, *draw);
-  GatherStore(t, L, err, tid, G);
+  GatherStore<kThreads, kScan>(t, L, err, tid, G, sub);
   __syncthreads();
   if (kSolvent) SolventClusters(t, L, T, sv, tid);
   else DenseClusters(t, L, T, tid);

@@ -28,26 +28,31 @@
 namespace lmc {
 namespace {
 
-template <bool kSolvent, bool kBL>
+// One workgroup per replica r (vacancy vac[r] in occupancy occ + off[r] of ncell[r]^3 cells).
+// kScan: one workgroup per scanned site vac[r] of the single occupancy occ (scan_n^3 cells), with
+// the gather substitution `sub`.
+template <bool kSolvent, bool kBL, bool kScan>
 __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout lay,
                                                             const uint8_t *occ,
                                                             const int64_t *off,
                                                             const int32_t *ncell,
                                                             const int64_t *vac, int64_t *nb,
                                                             double *ea, double *de, int32_t *err,
-                                                            uint8_t *env_out, double *ekra_out) {
+                                                            uint8_t *env_out, double *ekra_out,
+                                                            int scan_n, GatherSub sub) {
   __shared__ JumpLds L;
   extern __shared__ char dyn[];
   const int r = blockIdx.x;
-  if (!ValidSite(ncell[r], vac[r])) {
+  const int n = kScan ? scan_n : ncell[r];
+  if (!ValidSite(n, vac[r])) {
     if (threadIdx.x == 0) atomicOr(err, 8);
     return;
   }
   const Tabs T =
       LoadJumpTables<kSolvent, kBL>(t, L, dyn, lay, threadIdx.x, kThreads, threadIdx.x, kThreads);
   __syncthreads();
-  EvaluateJumps<kSolvent>(t, L, T, occ + off[r], ncell[r], ToXYZ(vac[r], ncell[r]),
-                          t.solvent_default, 0.0, err, threadIdx.x);
+  EvaluateJumps<kSolvent, kScan>(t, L, T, kScan ? occ : occ + off[r], n, ToXYZ(vac[r], n),
+                                 t.solvent_default, 0.0, err, threadIdx.x, nullptr, sub);
   const int k = threadIdx.x;
   if (k < 12) {
     const int64_t o = static_cast<int64_t>(r) * 12 + k;
@@ -207,11 +212,28 @@ hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *
     hipError_t e = AllowLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kernel, dim3(R), dim3(kThreads), lay.bytes, s, t, lay, occ, off, ncell,
-                       vac, nb, ea, de, err, env_out, ekra_out);
+                       vac, nb, ea, de, err, env_out, ekra_out, 0, GatherSub{-1, 0});
     return hipGetLastError();
   };
-  if (lay.b1_lds) return solvent ? go(EvalJumpsKernel<true, true>) : go(EvalJumpsKernel<false, true>);
-  return solvent ? go(EvalJumpsKernel<true, false>) : go(EvalJumpsKernel<false, false>);
+  if (lay.b1_lds)
+    return solvent ? go(EvalJumpsKernel<true, true, false>) : go(EvalJumpsKernel<false, true, false>);
+  return solvent ? go(EvalJumpsKernel<true, false, false>) : go(EvalJumpsKernel<false, false, false>);
+}
+
+hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, int64_t sub_site,
+                             int sub_code, const int64_t *sites, int32_t K, int64_t *nb,
+                             double *ea, double *de, int32_t *err, hipStream_t s) {
+  if (K <= 0) return hipSuccess;
+  const DynLayout lay = PlanLds(t, K, false);
+  auto go = [&](auto kernel) {
+    hipError_t e = AllowLds(kernel, lay.bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kernel, dim3(K), dim3(kThreads), lay.bytes, s, t, lay, occ, nullptr,
+                       nullptr, sites, nb, ea, de, err, nullptr, nullptr, n,
+                       GatherSub{sub_site, sub_code});
+    return hipGetLastError();
+  };
+  return lay.b1_lds ? go(EvalJumpsKernel<false, true, true>) : go(EvalJumpsKernel<false, false, true>);
 }
 
 hipError_t PrepareKmc(const KTables &t, int32_t R, bool solvent) {

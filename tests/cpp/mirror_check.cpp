@@ -102,6 +102,15 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
               "\"sur_fallbacks\": %llu, \"sur_mse_size\": %zu}\n",
               static_cast<unsigned long long>(su.steps[0]), su.time[0], static_cast<long long>(su.vacancy[0]),
               static_cast<unsigned long long>(events), static_cast<unsigned long long>(fallbacks), mse.size());
+  // exit-time analysis of the starting configuration: three sites' barrier lists
+  const ansys::ExitTime et(c, "Al", 500.0, barrier);
+  const auto [lists, mean, exit_time] = et.GetBarrierListAndExitTime();
+  for (size_t site : {v, (v + 7) % c.GetNumAtoms(), (v + 301) % c.GetNumAtoms()}) {
+    std::printf("{\"exit_site\": %zu, \"mean\": %.17g, \"exit_time\": %.17g, \"barriers\": [", site,
+                mean[site], exit_time[site]);
+    for (size_t j = 0; j < 12; ++j) std::printf("%s%.17g", j ? ", " : "", lists[site][j]);
+    std::printf("]}\n");
+  }
   return 0;
 }
 

@@ -73,3 +73,15 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
     assert (u["sur_steps"], u["sur_vacancy"]) == (k["kmc_steps"], k["kmc_vacancy"])
     assert abs(u["sur_time"] - k["kmc_time"]) <= 1e-12 * abs(k["kmc_time"])
     assert u["sur_events"] == u["sur_fallbacks"] == 12 * 201 and u["sur_mse_size"] == 8
+    vac = int(np.flatnonzero(occ == species.index("X"))[0])
+    for x in (x for x in rows if "exit_site" in x):  # ExitTime barrier lists
+        o = occ.copy()
+        o[vac] = species.index("Al")
+        o[x["exit_site"]] = species.index("X")
+        ref = sorted(ob.barrier_de(o, x["exit_site"], int(nb))[0]
+                     for nb in lat.neighbors(x["exit_site"], 1))
+        np.testing.assert_allclose(x["barriers"], ref, rtol=0, atol=1e-9)
+        assert abs(x["mean"] - np.mean(ref)) <= 1e-9
+        kT = 8.617333262145e-5 * 500.0
+        t = 1 / (1e13 * np.exp(-np.array(ref) / kT).sum())
+        assert abs(x["exit_time"] - t) <= 1e-8 * t

@@ -1,0 +1,186 @@
+"""Exit-time analysis (ansys::ExitTime, lmc/ansys/src/ExitTime.cpp): jump classes and the
+absorbing Markov chain on the CPU, the GPU vacancy scan and the per-site barrier lists against
+the oracle."""
+import numpy as np
+import pytest
+
+from conftest import random_alloc
+
+from latticemontecarlo_amd import ansys as A
+from latticemontecarlo_amd import fcc_neighbors
+
+KB = 8.617333262145e-5
+
+
+def test_jump_classes_of_one_site():
+    c = A.jump_classes(6, [0])
+    # the 12 jumps out; each first neighbour has 4 neighbours in common with the site (on) and 7
+    # farther out (off); its jump back into the set is in no class
+    assert {k: len(v) for k, v in c.items()} == {"in": 0, "to": 12, "on": 48, "off": 84}
+
+
+def test_jump_classes_of_a_pair():
+    a = 0
+    b = int(sorted(fcc_neighbors(6, a, 1))[0])
+    c = A.jump_classes(6, [a, b])
+    shell = set(A.first_shell(6, [a, b])) - {a, b}
+    assert len(shell) == 12 + 12 - 4 - 2  # 4 common first neighbours
+    assert sorted(c["in"]) == [(a, b), (b, a)]
+    assert len(c["to"]) == 22
+    into = sum(len({a, b} & set(int(x) for x in fcc_neighbors(6, s, 1))) for s in shell)
+    assert len(c["on"]) + len(c["off"]) == 12 * len(shell) - into
+    on = set(c["on"])
+    assert all((t, s) in on for s, t in on)  # shell -> shell jumps come in both directions
+
+
+def _chain(n, sites, rng):
+    nbl, bl = {}, {}
+    for s in set(A.first_shell(n, sites)):
+        nbl[int(s)] = sorted(int(x) for x in fcc_neighbors(n, int(s), 1))
+        bl[int(s)] = rng.uniform(0.4, 0.8, 12)
+    return nbl, bl
+
+
+def test_markov_single_site_is_the_residence_time():
+    rng = np.random.default_rng(3)
+    nbl, bl = _chain(6, [5], rng)
+    beta = 1 / KB / 500.0
+    t = A.ExitTime.markov_escape_time([5], nbl, bl, {5: 0.0}, beta)
+    assert t == pytest.approx(1.0 / (1e13 * np.exp(-bl[5] * beta)).sum(), rel=1e-13)
+
+
+def test_markov_pair_closed_form():
+    """Two sites that are first neighbours, every other jump absorbing, one barrier inside and
+    one outside: the escape time is tau / (1 - a), a the probability of the internal jump."""
+    a_, b_ = 0, int(sorted(fcc_neighbors(6, 0, 1))[0])
+    nbl = {s: sorted(int(x) for x in fcc_neighbors(6, s, 1)) for s in (a_, b_)}
+    beta = 1 / KB / 600.0
+    e_in, e_out = 0.5, 0.7
+    bl = {s: np.where(np.array(nbl[s]) == (b_ if s == a_ else a_), e_in, e_out) for s in (a_, b_)}
+    r_in, r_out = 1e13 * np.exp(-e_in * beta), 1e13 * np.exp(-e_out * beta)
+    tau = 1.0 / (r_in + 11 * r_out)
+    a = r_in * tau
+    t = A.ExitTime.markov_escape_time([a_, b_], nbl, bl, {a_: 0.1, b_: 0.1}, beta)
+    assert t == pytest.approx(tau / (1 - a), rel=1e-12)
+
+
+def test_markov_matches_the_neumann_series():
+    """p^T (I - T)^-1 tau against the sum over path lengths p^T sum_k T^k tau."""
+    rng = np.random.default_rng(8)
+    a = 0
+    sites = [a] + [int(x) for x in sorted(fcc_neighbors(6, a, 1))[:4]]
+    nbl, bl = _chain(6, sites, rng)
+    base = {s: rng.uniform(-0.1, 0.1) for s in sites}
+    beta = 1 / KB / 700.0
+    t = A.ExitTime.markov_escape_time(sites, nbl, bl, base, beta)
+    idx = {s: i for i, s in enumerate(sites)}
+    T = np.zeros((5, 5))
+    tau = np.zeros(5)
+    for s in sites:
+        rates = np.exp(-bl[s] * beta)
+        tau[idx[s]] = 1 / (1e13 * rates.sum())
+        for j, nb in enumerate(nbl[s]):
+            if nb in idx:
+                T[idx[s], idx[nb]] += rates[j] / rates.sum()
+    p = np.array([np.exp(-base[s] * beta) for s in sites])
+    p /= p.sum()
+    acc, v = 0.0, tau.copy()
+    for _ in range(2000):
+        acc += p @ v
+        v = T @ v
+    assert t == pytest.approx(acc, rel=1e-10)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _oracle_scan(om, occ, vac, solvent, x, sites, n):
+    base = occ.copy()
+    if vac >= 0:
+        base[vac] = solvent
+    out = []
+    for s in sites:
+        o = base.copy()
+        o[s] = x
+        out.append([(nb, *om.barrier_de(o, s, nb)) for nb in sorted(int(y) for y in fcc_neighbors(n, s, 1))])
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,kind", [("e0_3n", 1), ("q_4", 2)])
+def test_vacancy_scan_matches_oracle(torch, coeffs, name, kind):
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs[name]
+    n = 6
+    gm = L.Model(path, els, kind)
+    sp = gm.species
+    occ = random_alloc(n, sp, 0.08, seed=71)
+    vac = int(np.flatnonzero(occ == sp.index("X"))[0])
+    rng = np.random.default_rng(5)
+    solutes = np.flatnonzero((occ != sp.index("Al")) & (occ != sp.index("X")))
+    sites = ([vac] + [int(x) for x in sorted(fcc_neighbors(n, vac, 1))[:3]]
+             + [int(x) for x in rng.choice(solutes, 4, replace=False)]
+             + [int(x) for x in rng.choice(4 * n ** 3, 12, replace=False)])
+    print(f"vacancy scan {name} n={n} sites={len(sites)}", flush=True)
+    nb, ea, de = L.api.vacancy_scan(gm, torch.from_numpy(occ).cuda(), n, vac, sp.index("Al"), sites)
+    gm.check_errors()
+    nb, ea, de = nb.cpu().numpy(), ea.cpu().numpy(), de.cpu().numpy()
+    om = O.Model(O.Lattice(n), path, els, kind)
+    ref = _oracle_scan(om, occ, vac, sp.index("Al"), sp.index("X"), sites, n)
+    for i, rows in enumerate(ref):
+        for j, (rnb, rea, rde) in enumerate(rows):
+            assert nb[i, j] == rnb
+            assert abs(ea[i, j] - rea) <= 1e-9 and abs(de[i, j] - rde) <= 1e-9, (i, j)
+
+
+@pytest.mark.gpu
+def test_exit_time_analysis_matches_oracle(torch, coeffs):
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs["e0_3"]
+    n = 4
+    gm = L.Model(path, els, L.E0)
+    sp = gm.species
+    occ = random_alloc(n, sp, 0.1, seed=13)
+    vac = int(np.flatnonzero(occ == sp.index("X"))[0])
+    T = 500.0
+    et = A.ExitTime(gm, occ, n, sp.index("Al"), T)
+    print(f"exit-time analysis n={n} over {et.N} sites", flush=True)
+    lists, means, exits = et.barrier_list_and_exit_time()
+    om = O.Model(O.Lattice(n), path, els, O.E0)
+    ref = _oracle_scan(om, occ, vac, sp.index("Al"), sp.index("X"), range(et.N), n)
+    ref_lists = np.sort(np.array([[r[1] for r in rows] for rows in ref]), axis=1)
+    np.testing.assert_allclose(lists, ref_lists, rtol=0, atol=1e-9)
+    beta = 1 / KB / T
+    np.testing.assert_allclose(means, ref_lists.mean(1), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(exits, 1 / (1e13 * np.exp(-ref_lists * beta).sum(1)), rtol=1e-8)
+    # a cluster of solutes around the vacancy: energetics, barrier classes, escape time
+    cluster = [vac] + [int(x) for x in sorted(fcc_neighbors(n, vac, 1))[:2]]
+    pair, nbl, bl, dl = et.jump_energetics(cluster)
+    plus = A.first_shell(n, cluster)
+    assert np.isnan(bl[np.setdiff1d(np.arange(et.N), plus)]).all()
+    for s in plus:
+        for j in range(12):
+            rea, rde = om.barrier_de(_modified(occ, vac, sp, s), int(s), int(nbl[s, j]))
+            assert abs(bl[s, j] - rea) <= 1e-9 and abs(dl[s, j] - rde) <= 1e-9
+    avg = et.average_barriers(cluster, pair)
+    classes = A.jump_classes(n, cluster)
+    for c, key in enumerate(("in", "to", "on", "off")):
+        v = [om.barrier_de(_modified(occ, vac, sp, s), s, t) for s, t in classes[key]]
+        k = np.array([a - d / 2 for a, d in v])
+        assert abs(avg[c] - k.mean()) <= 1e-9 and abs(avg[4 + c] - k.std()) <= 1e-9
+    base = {int(s): 0.0 for s in plus}
+    t = et.markov_escape_time(plus, nbl, bl, base, et.beta_escape)
+    assert np.isfinite(t) and t > 0
+
+
+def _modified(occ, vac, sp, s):
+    o = occ.copy()
+    o[vac] = sp.index("Al")
+    o[s] = sp.index("X")
+    return o
