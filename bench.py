@@ -46,6 +46,24 @@ def measured_traffic(R, n, chunk):
     return None
 
 
+def measured_issue(R, n):
+    """VALU issue utilisation and instructions per replica-step of the KMC kernel from the
+    committed rocprofv3 SQ-counter summary of this workload; None when none exists."""
+    path = os.path.join(ROOT, "profiles", f"kmc_e0_{R}x{4 * n ** 3}_sq_counters.txt")
+    if not os.path.exists(path):
+        return None
+    out = {"source": os.path.relpath(path, ROOT)}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("per replica-step:"):
+                v = line.split(":")[1].replace(",", "").split()
+                out["valu_insts_per_replica_step"] = int(v[0])
+                out["lds_insts_per_replica_step"] = int(v[2])
+            elif line.startswith("VALU issue:"):
+                out["valu_issue_frac"] = float(line.split("=")[-1].split()[0])
+    return out
+
+
 def make_workload(R, n, seed):
     from latticemontecarlo_amd import synth
     rng = np.random.default_rng(seed)
@@ -441,7 +459,10 @@ def main():
                    "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "note": "the kernel does not stream: its working set is cache/LDS resident "
+                             "and the step is latency- and VALU/LDS-issue bound (issue)",
+                     "issue": measured_issue(R, n)},
     }
     log(f"GPU: {value:.4g} KMC steps/s ({12 * value:.4g} event evaluations/s), "
         f"{elapsed:.3f} s for {replica_steps} replica-steps")
