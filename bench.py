@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 ALGO_BYTES_PER_STEP = 677  # 135 unique sites x (1 B species + 4 B index) + 2 B written
+FORMS = ["dense", "solvent-baseline", "dense-wide"]  # kernel shapes of the autotuner
 METRIC = "KMC steps/s over all replicas, surrogate vs plain GPU at \u22645 meV barrier MAE"
 HBM_PEAK_GBS = 8000.0
 
@@ -267,7 +268,7 @@ def large_workload(args):
                                    "quaternary E0 barrier model",
                        "value_path": "plain GPU path (exact); the surrogate path is reported "
                                      "under surrogate",
-                       "formulation": ["dense", "solvent-baseline"][tune["formulation"]],
+                       "formulation": FORMS[tune["formulation"]],
                        "event_evaluations_per_s": round(12 * value, 1)},
             "surrogate": s}
     print(json.dumps(line), flush=True)
@@ -337,7 +338,7 @@ def uneven_workload(args):
             "data": "synthetic",
             "config": {"workload": "uneven-size batched KMC alongside CanonicalMc", "kmc_replicas": Rk,
                        "kmc_cells": [int(sizes.min()), int(sizes.max())],
-                       "kmc_formulation": ["dense", "solvent-baseline"][tune["formulation"]],
+                       "kmc_formulation": FORMS[tune["formulation"]],
                        "cmc_replicas": Rc, "cmc_sites": Nc, "cmc_trials_per_replica": trials,
                        "cmc_trials_per_s": round(cval, 1),
                        "cmc_acceptance": round(float(cst["accepted"].sum()) / max(1, cdone), 4)}}
@@ -390,8 +391,9 @@ def main():
                            maximum_steps=total_steps + 10)
     stream = torch.cuda.current_stream()
     tune = drv.autotune(steps=300)
-    log(f"autotune: dense {tune['ms_dense']:.3f} ms, solvent baseline {tune['ms_solvent']:.3f} ms "
-        f"per KMC step of the batch -> formulation {tune['formulation']}")
+    log(f"autotune: dense {tune['ms_dense']:.4f} ms, solvent baseline {tune['ms_solvent']:.4f} ms, "
+        f"dense wide {tune['ms_wide']:.4f} ms per KMC step of the batch -> "
+        f"{FORMS[tune['formulation']]}")
     t0 = time.time()
     left = args.warmup
     while left > 0:
@@ -454,7 +456,7 @@ def main():
                    "replicas": R, "sites_per_replica": 4 * n ** 3,
                    "steps_per_replica": args.steps, "steps_per_launch": args.chunk,
                    "launch": "eager" if args.eager else "hip graph",
-                   "formulation": ["dense", "solvent-baseline"][tune["formulation"]],
+                   "formulation": FORMS[tune["formulation"]],
                    "event_evaluations_per_s": round(12 * value, 1),
                    "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,

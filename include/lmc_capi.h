@@ -182,10 +182,16 @@ void lmc_kmc_free(lmc_kmc *k);
 int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const double *temp);
 /* Per-replica trace of the next `capacity` steps (selected neighbour, Ea, dE, time). 0 = off. */
 int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity);
-/* Formulation of this driver's kernel (see lmc_model_set_formulation). */
+/* Kernel of this driver: 0 dense (256 threads per replica, up to four replicas per CU),
+ * 1 solvent baseline (see lmc_model_set_formulation), 2 dense wide shape (768 threads per
+ * replica, one per CU: for batches of at most one replica per CU). No reference counterpart. */
 int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation);
-/* Autotuner: runs `steps` KMC steps with each formulation on a scratch copy of the batch state
- * (HIP events), keeps the faster and leaves the replicas untouched; ms_* are per step. */
+/* Autotuner (no reference counterpart): runs `steps` KMC steps with each kernel shape on a
+ * scratch copy of the batch state (HIP events), keeps the fastest and leaves the replicas
+ * untouched. ms_per_step[3]: per-step times of formulations 0, 1, 2 (NaN when not tried: the
+ * wide shape is tried only when every replica has a CU of its own). */
+int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_per_step);
+/* lmc_kmc_autotune_shapes reporting formulations 0 and 1 only. */
 int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dense,
                      double *ms_solvent);
 /* Advance every replica by `steps` steps (fewer if it reaches maximum_steps or escapes).

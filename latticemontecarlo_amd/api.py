@@ -261,9 +261,12 @@ class KineticMcFirst:
         check(lib().lmc_kmc_set_formulation(self.h, formulation))
 
     def autotune(self, steps=200):
-        ch, md, ms = C.c_int32(), C.c_double(), C.c_double()
-        check(lib().lmc_kmc_autotune(self.h, steps, C.byref(ch), C.byref(md), C.byref(ms)))
-        return dict(formulation=ch.value, ms_dense=md.value, ms_solvent=ms.value)
+        """Times each kernel shape for `steps` steps on a scratch copy and keeps the fastest:
+        formulation 0 dense, 1 solvent baseline, 2 dense wide (tried only with at most one
+        replica per CU); ms_* per batch step (NaN when not tried)."""
+        ch, ms = C.c_int32(), (C.c_double * 3)()
+        check(lib().lmc_kmc_autotune_shapes(self.h, steps, C.byref(ch), ms))
+        return dict(formulation=ch.value, ms_dense=ms[0], ms_solvent=ms[1], ms_wide=ms[2])
 
     def run(self, steps, stream=None):
         s = capi.stream_ptr(stream) if stream is not None else None

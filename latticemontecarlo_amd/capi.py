@@ -81,6 +81,7 @@ SIGNATURES = {
     "lmc_kmc_set_trace": (C.c_int, [_vp, _u64]),
     "lmc_kmc_set_formulation": (C.c_int, [_vp, _i32]),
     "lmc_kmc_autotune": (C.c_int, [_vp, _i32, _i32p, _f64p, _f64p]),
+    "lmc_kmc_autotune_shapes": (C.c_int, [_vp, _i32, _vp, _vp]),
     "lmc_kmc_run": (C.c_int, [_vp, _u64, _vp]),
     "lmc_kmc_state": (C.c_int, [_vp, _u64p, _f64p, _f64p, _f64p, _f64p, _i64p, _i64p, _f64p,
                                 _i32p]),

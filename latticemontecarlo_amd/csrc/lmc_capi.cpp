@@ -343,17 +343,25 @@ const KTables &Upload(Model &m) {
   // read the zero at the end of every weight row. Their sites are slot 63 (rank entries 0), in
   // each list's own byte roles: a state entry's byte 0 indexes the rank*8 bytes and byte 1 the
   // 16-bit rank*8*ne words (offset 2*63); an orbit entry the other way round.
-  auto pad = [](std::vector<uint32_t> &v, uint32_t roles, uint32_t zero_at) {
+  // The wide kernel shape (64 lanes per event, 4 entries each) reads copies padded to multiples
+  // of kClusterPadWide.
+  auto pad = [](std::vector<uint32_t> &v, int to, uint32_t roles, uint32_t zero_at) {
     if (zero_at > 65535) throw std::invalid_argument("weight table too large for the packed tables");
-    while (v.size() % kClusterPad) v.push_back(roles | (zero_at << 16));
+    while (v.size() % static_cast<size_t>(to)) v.push_back(roles | (zero_at << 16));
   };
-  pad(state, 63u | (126u << 8), static_cast<uint32_t>(nr * 8));
-  pad(mmm, 126u | (63u << 8), static_cast<uint32_t>(F1e * 8));
-  pad(mm2, 126u | (63u << 8), static_cast<uint32_t>(F2e * 8));
+  const uint32_t state_roles = 63u | (126u << 8), orbit_roles = 126u | (63u << 8);
+  std::vector<uint32_t> state_w = state, mmm_w = mmm, mm2_w = mm2;
+  pad(state, kClusterPad, state_roles, static_cast<uint32_t>(nr * 8));
+  pad(mmm, kClusterPad, orbit_roles, static_cast<uint32_t>(F1e * 8));
+  pad(mm2, kClusterPad, orbit_roles, static_cast<uint32_t>(F2e * 8));
+  pad(state_w, kClusterPadWide, state_roles, static_cast<uint32_t>(nr * 8));
+  pad(mmm_w, kClusterPadWide, orbit_roles, static_cast<uint32_t>(F1e * 8));
+  pad(mm2_w, kClusterPadWide, orbit_roles, static_cast<uint32_t>(F2e * 8));
   // the 16-bit rank words are read at even byte offsets only
-  for (uint32_t e : state)
-    if ((e >> 8) & 1) throw std::logic_error("state entry with an odd rank-word offset");
-  for (const auto *v : {&mmm, &mm2})
+  for (const auto *v : {&state, &state_w})
+    for (uint32_t e : *v)
+      if ((e >> 8) & 1) throw std::logic_error("state entry with an odd rank-word offset");
+  for (const auto *v : {&mmm, &mm2, &mmm_w, &mm2_w})
     for (uint32_t e : *v)
       if (e & 1) throw std::logic_error("orbit entry with an odd rank-word offset");
   std::vector<uint32_t> site_cl;
@@ -384,6 +392,9 @@ const KTables &Upload(Model &m) {
   b.Add(state, &k.state);
   b.Add(mmm, &k.mmm);
   b.Add(mm2, &k.mm2);
+  b.Add(state_w, &k.state_w);
+  b.Add(mmm_w, &k.mmm_w);
+  b.Add(mm2_w, &k.mm2_w);
   b.Add(site_cl, &k.site_cl);
   b.Add(site_off, &k.site_off);
   b.Add(m.w, &k.w);
@@ -420,6 +431,9 @@ const KTables &Upload(Model &m) {
   k.n_state = static_cast<int>(state.size());
   k.n_mmm = static_cast<int>(mmm.size());
   k.n_mm2 = static_cast<int>(mm2.size());
+  k.n_state_w = static_cast<int>(state_w.size());
+  k.n_mmm_w = static_cast<int>(mmm_w.size());
+  k.n_mm2_w = static_cast<int>(mm2_w.size());
   k.n_site = static_cast<int>(site_cl.size());
   k.n_eci = static_cast<int>(m.theta.size());
   k.v_slot = jt.v_slot;
@@ -1039,13 +1053,31 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
 void lmc_kmc_free(lmc_kmc *k) { delete k; }
 
 int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation) {
-  if (!k || formulation < 0 || formulation > 1) return Fail(LMC_ERR_ARG, "bad formulation");
+  if (!k || formulation < 0 || formulation > 2) return Fail(LMC_ERR_ARG, "bad formulation");
   k->formulation = formulation;
   return LMC_OK;
 }
 
+static int DeviceCUs() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  return cus;
+}
+
 int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dense,
                      double *ms_solvent) {
+  double ms[3];
+  const int rc = lmc_kmc_autotune_shapes(k, steps, chosen, ms);
+  if (rc == LMC_OK) {
+    if (ms_dense) *ms_dense = ms[0];
+    if (ms_solvent) *ms_solvent = ms[1];
+  }
+  return rc;
+}
+
+int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_per_step) {
   return Guard([&] {
     if (!k || steps <= 0) throw std::invalid_argument("bad argument");
     const auto &t = lmc::Upload(*k->bm->m);
@@ -1062,12 +1094,15 @@ int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dens
       Check(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice), "hipMemcpy D2D");
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    float ms[2] = {0, 0};
+    float ms[3] = {0, 0, 0};
+    // the wide shape only when every replica has a CU of its own
+    const int cus = DeviceCUs();
+    const int forms = (cus > 0 && k->d.R <= cus) ? 3 : 2;
     try {
       Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       Check(hipEventCreate(&e0), "hipEventCreate");
       Check(hipEventCreate(&e1), "hipEventCreate");
-      for (int f = 0; f < 2; ++f) {
+      for (int f = 0; f < forms; ++f) {
         for (void *p : owned) (void)hipFree(p);
         owned.clear();
         clone(s.occ, k->d.occ, static_cast<size_t>(k->total_sites));
@@ -1082,7 +1117,7 @@ int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dens
         clone(s.com, k->d.com, 3 * R);
         clone(s.error, k->d.error, 1);
         Check(hipEventRecord(e0, nullptr), "hipEventRecord");
-        Check(lmc::LaunchKmc(t, s, static_cast<uint64_t>(steps), 0, nullptr, f == 1),
+        Check(lmc::LaunchKmc(t, s, static_cast<uint64_t>(steps), 0, nullptr, f),
               "autotune KMC launch");
         Check(hipEventRecord(e1, nullptr), "hipEventRecord");
         Check(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -1097,10 +1132,14 @@ int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dens
     for (void *p : owned) (void)hipFree(p);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    k->formulation = ms[1] <= ms[0] ? 1 : 0;
+    int best = 0;
+    for (int f = 1; f < forms; ++f)
+      if (ms[f] < ms[best]) best = f;
+    k->formulation = best;
     if (chosen) *chosen = k->formulation;
-    if (ms_dense) *ms_dense = ms[0] / steps;
-    if (ms_solvent) *ms_solvent = ms[1] / steps;
+    if (ms_per_step)
+      for (int f = 0; f < 3; ++f)
+        ms_per_step[f] = f < forms ? ms[f] / steps : std::numeric_limits<double>::quiet_NaN();
     return LMC_OK;
   });
 }
@@ -1185,12 +1224,12 @@ static void BuildGraph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches) 
       if (k->sur)
         Check(PrepareSurrogate(k), "surrogate KMC set-up");
       else
-        Check(lmc::PrepareKmc(t, k->d.R, k->formulation == 1), "KMC set-up");
+        Check(lmc::PrepareKmc(t, k->d.R, k->formulation), "KMC set-up");
       Check(hipStreamBeginCapture(k->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       hipError_t e = hipSuccess;
       for (int32_t i = 0; i < launches && e == hipSuccess; ++i)
         e = k->sur ? RunSurrogateKmc(k, steps_per_launch, k->capture)
-                   : lmc::LaunchKmc(t, k->d, steps_per_launch, 0, k->capture, k->formulation == 1);
+                   : lmc::LaunchKmc(t, k->d, steps_per_launch, 0, k->capture, k->formulation);
       hipGraph_t g = nullptr;
       const hipError_t e2 = hipStreamEndCapture(k->capture, &g);
       Check(e, "KMC launch under capture");
@@ -1213,7 +1252,7 @@ int lmc_kmc_run(lmc_kmc *k, uint64_t steps, void *stream) {
     if (k->sur)
       Check(RunSurrogateKmc(k, steps, S(stream)), "surrogate KMC kernel");
     else
-      Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream), k->formulation == 1),
+      Check(lmc::LaunchKmc(t, k->d, steps, k->trace_pos, S(stream), k->formulation),
             "KMC kernel");
     k->trace_pos += steps;
     return LMC_OK;

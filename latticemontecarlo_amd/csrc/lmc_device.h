@@ -19,6 +19,8 @@ constexpr int kMaxStateClusters = 512;
 constexpr int kMaxOrbitClusters = 768;
 // Cluster lists are padded to multiples of this (4 entries per lane of an event's 21 lanes).
 constexpr int kClusterPad = 84;
+// Copies for the wide kernel shape: 4 entries per lane of an event's 64 lanes.
+constexpr int kClusterPadWide = 256;
 constexpr int kMaxSiteClusters = 256;
 constexpr int kMaxW = 1024;  // w1 | w2 | w3 for S <= 6
 
@@ -29,6 +31,7 @@ struct KTables {
   const uint32_t *state;    // one-endpoint state clusters
   const uint32_t *mmm;
   const uint32_t *mm2;
+  const uint32_t *state_w, *mmm_w, *mm2_w;  // the same lists padded to kClusterPadWide
   const uint32_t *site_cl;
   const int32_t *site_off;  // [43]
   const double *w;          // theta/N by (label, codes): w1[S] | w2[3][S][S] | w3[4][S][S][S]
@@ -59,6 +62,7 @@ struct KTables {
   uint64_t vac_shell_mask;  // env slots within 3NN of the vacancy
   int n_state, n_mmm, n_mm2, n_site, n_eci;  // n_mm2 counts forward + backward entries
   int n_state_v;            // state entries of the vacancy endpoint (the rest: neighbour's)
+  int n_state_w, n_mmm_w, n_mm2_w;
   int v_slot, n_slot;
   int kind, S, ne, x_code, F1, F2, nw;
   int nwd, wrow;
@@ -130,11 +134,13 @@ hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *
 hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, int64_t sub_site,
                              int sub_code, const int64_t *sites, int32_t K, int64_t *nb,
                              double *ea, double *de, int32_t *err, hipStream_t s);
+// form: 0 dense (256 threads per replica), 1 solvent baseline, 2 dense wide shape (768 threads
+// per replica, one workgroup per CU).
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
-                     hipStream_t s, bool solvent);
+                     hipStream_t s, int form);
 // Everything LaunchKmc sets up outside the stream (LDS plan, kernel attributes), so that its
 // launches can then be captured in a graph.
-hipError_t PrepareKmc(const KTables &t, int32_t R, bool solvent);
+hipError_t PrepareKmc(const KTables &t, int32_t R, int form);
 hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off,
                         const int32_t *ncell, const int64_t *pairs, int32_t R, double *de,
                         hipStream_t s);

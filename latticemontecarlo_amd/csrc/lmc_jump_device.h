@@ -71,7 +71,7 @@ struct Tabs {  // resolved table pointers (LDS or global)
 // dynamic tables (threads gtid of gsize); returns the resolved table pointers.
 // kBL: the barrier weight tables live in LDS (lay.b1_lds, and lay.b2_lds for the quartic model),
 // known at compile time so that their reads are LDS instructions.
-template <bool kSolvent, bool kBL>
+template <bool kSolvent, bool kBL, bool kWide = false>
 __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const DynLayout &lay,
                                int tid, int lsize, int gtid, int gsize) {
   for (int i = tid; i < 12 * kEnvSlots; i += lsize) {
@@ -114,15 +114,20 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     for (int i = gtid; i < 3 * (kEnvSlots + 1); i += gsize) io[i] = t.inc_off[i];
     T.inc_off = io;
   } else {
+    // the wide shape reads the copies padded to its batch size
+    const int ns = kWide ? t.n_state_w : t.n_state, no = kWide ? t.n_mmm_w : t.n_mmm;
+    const int nq = kWide ? t.n_mm2_w : t.n_mm2;
+    const uint32_t *gs = kWide ? t.state_w : t.state, *go = kWide ? t.mmm_w : t.mmm;
+    const uint32_t *gq = kWide ? t.mm2_w : t.mm2;
     uint32_t *p = reinterpret_cast<uint32_t *>(dyn + lay.lists);
-    for (int i = gtid; i < t.n_state; i += gsize) p[i] = t.state[i];
+    for (int i = gtid; i < ns; i += gsize) p[i] = gs[i];
     T.state = p;
-    p += t.n_state;
-    for (int i = gtid; i < t.n_mmm; i += gsize) p[i] = t.mmm[i];
+    p += ns;
+    for (int i = gtid; i < no; i += gsize) p[i] = go[i];
     T.mmm = p;
-    p += t.n_mmm;
+    p += no;
     if (t.kind == kQuartic)
-      for (int i = gtid; i < t.n_mm2; i += gsize) p[i] = t.mm2[i];
+      for (int i = gtid; i < nq; i += gsize) p[i] = gq[i];
     T.mm2 = p;
   }
   return T;
@@ -278,16 +283,17 @@ __device__ __forceinline__ void OrbitBatch(const char *bm, const EventRanks &er,
     acc[u] += *reinterpret_cast<const double *>(bm + (x0[u] + x1[u] + (d[u] >> 16)));
 }
 
-// Four state and four orbit clusters per lane at once (entries c + u kLanes, u < 4): eight
+// Four state and four orbit clusters per lane at once (entries c + u kStride, u < 4): eight
 // independent load chains, issued level by level.
+template <int kStride>
 __device__ __forceinline__ void StateOrbitBatch(const char *wm, const char *bm, const EventRanks &er,
                                                 const uint32_t *sl, const uint32_t *ol, int c,
                                                 double *a, double *b) {
   uint32_t ds[4], dq[4], x1[4], x2[4], y0[4], y1[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    ds[u] = sl[c + u * kLanes];
-    dq[u] = ol[c + u * kLanes];
+    ds[u] = sl[c + u * kStride];
+    dq[u] = ol[c + u * kStride];
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -313,17 +319,18 @@ __device__ __forceinline__ void StateOrbitBatch(const char *wm, const char *bm, 
 }
 
 // Dense sums of an event's state list (dE, row wm) and orbit list (logit, row bm), lanes lane,
-// lane + kLanes, ...: both lists are padded to whole batches (multiples of 4 kLanes entries)
+// lane + kStride, ...: both lists are padded to whole batches (multiples of 4 kStride entries)
 // and advance together.
+template <int kStride = kLanes>
 __device__ __forceinline__ void StateOrbitRange(const char *wm, const char *bm, const EventRanks &er,
                                                 const uint32_t *sl, int ns, const uint32_t *ol,
                                                 int no, int lane, double &de, double &la) {
   double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
   const int both = ns < no ? ns : no;
   int c = lane;
-  for (; c < both; c += 4 * kLanes) StateOrbitBatch(wm, bm, er, sl, ol, c, a, b);
-  for (; c < ns; c += 4 * kLanes) StateBatch<4>(wm, er, sl, c, kLanes, a);
-  for (; c < no; c += 4 * kLanes) OrbitBatch<4>(bm, er, ol, c, kLanes, b);
+  for (; c < both; c += 4 * kStride) StateOrbitBatch<kStride>(wm, bm, er, sl, ol, c, a, b);
+  for (; c < ns; c += 4 * kStride) StateBatch<4>(wm, er, sl, c, kStride, a);
+  for (; c < no; c += 4 * kStride) OrbitBatch<4>(bm, er, ol, c, kStride, b);
   de = (a[0] + a[1]) + (a[2] + a[3]);
   la = (b[0] + b[1]) + (b[2] + b[3]);
 }
@@ -529,6 +536,60 @@ __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const
     la += b[1];
     lb += b[2];
   }
+  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
+    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
+  double ea, ekra;
+  Barrier(t, L, k, de, la, lb, ea, ekra);
+  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);  // unknown cluster type
+  StoreEvent(L, k, ea, de, ekra, beta);
+}
+
+// Wide shape (kWideThreads threads per replica): wave k evaluates event k with its 64 lanes (the
+// lists padded to kClusterPadWide, 4 entries of each per lane and batch), sums the lane partials
+// by a butterfly of shuffles, and lane 0 finishes the event. No workgroup barrier separates the
+// cluster sums from the finish.
+constexpr int kWideThreads = 12 * 64;
+static_assert(kClusterPadWide == 4 * 64, "wide lists are padded to whole batches of 64 lanes");
+
+// v of lane l - kShift within the lane's row of 16 (0 past the row start): DPP row_shr on both
+// halves of the double.
+template <int kShift>
+__device__ __forceinline__ double RowShr(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + kShift, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + kShift, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double ReadLane(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a full wave, returned to every lane: shifted adds within each row of
+// 16 leave the row sum in its last lane; the four row sums are then added in order.
+__device__ __forceinline__ double WaveSum(double v) {
+  v += RowShr<1>(v);
+  v += RowShr<2>(v);
+  v += RowShr<4>(v);
+  v += RowShr<8>(v);
+  return (ReadLane(v, 15) + ReadLane(v, 31)) + (ReadLane(v, 47) + ReadLane(v, 63));
+}
+
+__device__ void WideEvent(const KTables &t, JumpLds &L, const Tabs &T, double beta, int32_t *err,
+                          int tid) {
+  const int k = tid >> 6, lane = tid & 63;
+  const EventRanks &er = L.er[k];
+  const int M = L.env[k][t.n_slot];
+  double de, la = 0.0, lb = 0.0;
+  if (t.kind == kE0 || t.kind == kQuartic)
+    StateOrbitRange<64>(WRow(t, T, M), BRow(T.b1, t.F1, t, M), er, T.state, t.n_state_w, T.mmm,
+                        t.n_mmm_w, lane, de, la);
+  else
+    de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state_w, lane, 64);
+  if (t.kind == kQuartic) lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2_w, lane, 64);
+  de = WaveSum(de);
+  la = WaveSum(la);
+  lb = WaveSum(lb);
+  if (lane != 0) return;
   if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
     la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
   double ea, ekra;

@@ -69,6 +69,59 @@ __global__ __launch_bounds__(kThreads) void EvalJumpsKernel(KTables t, DynLayout
   }
 }
 
+// Replica r's state into LDS (one thread).
+__device__ void LoadReplica(const KmcDev &k, int r, int n, ReplicaLds &Q) {
+  Q.steps = k.steps[r];
+  Q.max_steps = k.max_steps[r];
+  Q.seed = k.seed[r];
+  Q.rid = k.rid[r];
+  Q.time = k.time[r];
+  Q.energy = k.energy[r];
+  Q.abse = k.abs_energy[r];
+  Q.T = k.temp[r];
+  Q.cv = k.cv[r];
+  Q.cs = k.cs[r];
+  Q.smass = k.solute_mass[r];
+  Q.vac = k.vac[r];
+  Q.flags = k.flags[r];
+  Q.esc = k.escaped[r];
+  Q.solvent = k.solvent[r];
+  for (int a = 0; a < 3; ++a) {
+    Q.disp[a] = k.disp[3 * r + a];
+    Q.com[a] = k.com[3 * r + a];
+  }
+  Q.done = (Q.steps > Q.max_steps || Q.esc || !ValidSite(n, Q.vac)) ? 1 : 0;
+  Q.beta = 1.0 / kBoltzmann / Q.T;
+  SetVacancyXYZ(Q, n);
+}
+
+// Replica r's state back to HBM (one thread).
+__device__ void StoreReplica(const KmcDev &k, int r, const ReplicaLds &Q) {
+  k.steps[r] = Q.steps;
+  k.time[r] = Q.time;
+  k.energy[r] = Q.energy;
+  k.abs_energy[r] = Q.abse;
+  k.temp[r] = Q.T;
+  k.vac[r] = Q.vac;
+  k.escaped[r] = Q.esc;
+  for (int a = 0; a < 3; ++a) {
+    k.disp[3 * r + a] = Q.disp[a];
+    k.com[3 * r + a] = Q.com[a];
+  }
+}
+
+// UpdateTemperature (KineticMcAbstract.cpp:45-50), before any lane reads beta.
+__device__ __forceinline__ void UpdateTemperature(const KmcDev &k, ReplicaLds &Q, int tid) {
+  if (k.n_tt > 0) {
+    if (tid == 0) {
+      Q.T = ScheduleTemperature(k.tt_time, k.tt_temp, k.n_tt, Q.time);
+      Q.beta = 1.0 / kBoltzmann / Q.T;
+    }
+    __syncthreads();
+  }
+}
+
+// Persistent KMC: one 256-thread workgroup per replica, up to four per CU.
 template <bool kSolvent, bool kBL>
 __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout lay, KmcDev k,
                                                       uint64_t nsteps, uint64_t trace_base) {
@@ -80,63 +133,59 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
   const Tabs T = LoadJumpTables<kSolvent, kBL>(t, L, dyn, lay, tid, kThreads, tid, kThreads);
   uint8_t *occ = k.occ + k.off[r];
   const int n = k.ncell[r];
-  if (tid == 0) {
-    Q.steps = k.steps[r];
-    Q.max_steps = k.max_steps[r];
-    Q.seed = k.seed[r];
-    Q.rid = k.rid[r];
-    Q.time = k.time[r];
-    Q.energy = k.energy[r];
-    Q.abse = k.abs_energy[r];
-    Q.T = k.temp[r];
-    Q.cv = k.cv[r];
-    Q.cs = k.cs[r];
-    Q.smass = k.solute_mass[r];
-    Q.vac = k.vac[r];
-    Q.flags = k.flags[r];
-    Q.esc = k.escaped[r];
-    Q.solvent = k.solvent[r];
-    for (int a = 0; a < 3; ++a) {
-      Q.disp[a] = k.disp[3 * r + a];
-      Q.com[a] = k.com[3 * r + a];
-    }
-    Q.done = (Q.steps > Q.max_steps || Q.esc || !ValidSite(n, Q.vac)) ? 1 : 0;
-    Q.beta = 1.0 / kBoltzmann / Q.T;
-    SetVacancyXYZ(Q, n);
-  }
+  if (tid == 0) LoadReplica(k, r, n, Q);
   __syncthreads();
   const int sv = Q.solvent;
   for (uint64_t it = 0; it < nsteps; ++it) {
     if (Q.done) break;
-    if (k.n_tt > 0) {  // UpdateTemperature (KineticMcAbstract.cpp:45-50), before any lane reads beta
-      if (tid == 0) {
-        Q.T = ScheduleTemperature(k.tt_time, k.tt_temp, k.n_tt, Q.time);
-        Q.beta = 1.0 / kBoltzmann / Q.T;
-      }
-      __syncthreads();
-    }
+    UpdateTemperature(k, Q, tid);
     EvaluateJumps<kSolvent>(t, L, T, occ, n, VacancyXYZ(Q), sv, Q.beta, k.error, tid, &Q);
     SelectAndApply(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }
-  if (tid == 0) {
-    k.steps[r] = Q.steps;
-    k.time[r] = Q.time;
-    k.energy[r] = Q.energy;
-    k.abs_energy[r] = Q.abse;
-    k.temp[r] = Q.T;
-    k.vac[r] = Q.vac;
-    k.escaped[r] = Q.esc;
-    for (int a = 0; a < 3; ++a) {
-      k.disp[3 * r + a] = Q.disp[a];
-      k.com[3 * r + a] = Q.com[a];
+  if (tid == 0) StoreReplica(k, r, Q);
+}
+
+// Persistent KMC, wide shape: one 768-thread workgroup per replica (wave k evaluates event k),
+// for batches of at most one replica per CU, where a step's latency sets the rate.
+template <bool kBL>
+__global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynLayout lay,
+                                                                 KmcDev k, uint64_t nsteps,
+                                                                 uint64_t trace_base) {
+  __shared__ JumpLds L;
+  __shared__ ReplicaLds Q;
+  extern __shared__ char dyn[];
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const Tabs T =
+      LoadJumpTables<false, kBL, true>(t, L, dyn, lay, tid, kWideThreads, tid, kWideThreads);
+  uint8_t *occ = k.occ + k.off[r];
+  const int n = k.ncell[r];
+  if (tid == 0) LoadReplica(k, r, n, Q);
+  __syncthreads();
+  for (uint64_t it = 0; it < nsteps; ++it) {
+    if (Q.done) break;
+    UpdateTemperature(k, Q, tid);
+    {
+      Gathered<kWideThreads> G;
+      GatherIssue<kWideThreads>(L, occ, n, VacancyXYZ(Q), tid, G);
+      if (tid == kWideThreads - 1) DrawStep(L, Q);
+      GatherStore<kWideThreads>(t, L, k.error, tid, G);
     }
+    __syncthreads();
+    WideEvent(t, L, T, Q.beta, k.error, tid);
+    __syncthreads();
+    SelectAndApply(t, k, L, Q, occ, r, trace_base + it, tid);
+    __syncthreads();
   }
+  if (tid == 0) StoreReplica(k, r, Q);
 }
 
 // Dynamic LDS plan: weight tables go to LDS while the workgroup stays within its share of the
 // CU's LDS (up to 4 resident workgroups per CU when the batch fills the chip).
-DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
+// form: 0 dense, 1 solvent baseline, 2 dense wide shape (one workgroup per CU).
+DynLayout PlanLds(const KTables &t, int32_t R, int form) {
+  const bool solvent = form == 1;
   static int cus = 0;
   static size_t lds_cu = 0, lds_block = 0;
   if (!cus) {
@@ -152,7 +201,7 @@ DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
       lds_cu = lds_block = 65536;
     }
   }
-  const int per_cu = std::min(4, std::max(1, (R + cus - 1) / cus));
+  const int per_cu = form == 2 ? 1 : std::min(4, std::max(1, (R + cus - 1) / cus));
   const size_t stat = sizeof(JumpLds) + sizeof(ReplicaLds) + 256;
   const size_t share = std::min(lds_block, lds_cu / per_cu);
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
@@ -169,7 +218,10 @@ DynLayout PlanLds(const KTables &t, int32_t R, bool solvent) {
     at = al(at + sizeof(uint16_t) * 3 * (kEnvSlots + 1));
   } else {
     lay.lists = static_cast<int>(at);
-    at = al(at + sizeof(uint32_t) * (t.n_state + t.n_mmm + (t.kind == kQuartic ? t.n_mm2 : 0)));
+    if (form == 2)
+      at = al(at + sizeof(uint32_t) * (t.n_state_w + t.n_mmm_w + (t.kind == kQuartic ? t.n_mm2_w : 0)));
+    else
+      at = al(at + sizeof(uint32_t) * (t.n_state + t.n_mmm + (t.kind == kQuartic ? t.n_mm2 : 0)));
     // (n_mm2 counts the forward and the backward entries)
   }
   const size_t b1 = sizeof(double) * static_cast<size_t>(t.ne) * t.F1;
@@ -207,7 +259,7 @@ hipError_t LaunchEvalJumps(const KTables &t, const uint8_t *occ, const int64_t *
                            double *ea, double *de, int32_t *err, hipStream_t s, bool solvent,
                            uint8_t *env_out, double *ekra_out) {
   if (R <= 0) return hipSuccess;
-  const DynLayout lay = PlanLds(t, R, solvent);
+  const DynLayout lay = PlanLds(t, R, solvent ? 1 : 0);
   auto go = [&](auto kernel) {
     hipError_t e = AllowLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
@@ -224,7 +276,7 @@ hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, in
                              int sub_code, const int64_t *sites, int32_t K, int64_t *nb,
                              double *ea, double *de, int32_t *err, hipStream_t s) {
   if (K <= 0) return hipSuccess;
-  const DynLayout lay = PlanLds(t, K, false);
+  const DynLayout lay = PlanLds(t, K, 0);
   auto go = [&](auto kernel) {
     hipError_t e = AllowLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
@@ -236,27 +288,33 @@ hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, in
   return lay.b1_lds ? go(EvalJumpsKernel<false, true, true>) : go(EvalJumpsKernel<false, false, true>);
 }
 
-hipError_t PrepareKmc(const KTables &t, int32_t R, bool solvent) {
+// The instantiation of form (0 dense, 1 solvent baseline, 2 dense wide) and LDS placement, with
+// its block size; f(kernel, threads).
+template <typename F>
+hipError_t WithKmcKernel(int form, bool b1_lds, F &&f) {
+  if (form == 2) return b1_lds ? f(KmcWideKernel<true>, kWideThreads) : f(KmcWideKernel<false>, kWideThreads);
+  if (form == 1) return b1_lds ? f(KmcKernel<true, true>, kThreads) : f(KmcKernel<true, false>, kThreads);
+  return b1_lds ? f(KmcKernel<false, true>, kThreads) : f(KmcKernel<false, false>, kThreads);
+}
+
+hipError_t PrepareKmc(const KTables &t, int32_t R, int form) {
   if (R <= 0) return hipSuccess;
-  const DynLayout lay = PlanLds(t, R, solvent);
-  if (lay.b1_lds)
-    return solvent ? AllowLds(KmcKernel<true, true>, lay.bytes) : AllowLds(KmcKernel<false, true>, lay.bytes);
-  return solvent ? AllowLds(KmcKernel<true, false>, lay.bytes) : AllowLds(KmcKernel<false, false>, lay.bytes);
+  const DynLayout lay = PlanLds(t, R, form);
+  return WithKmcKernel(form, lay.b1_lds != 0,
+                       [&](auto kernel, int) { return AllowLds(kernel, lay.bytes); });
 }
 
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
-                     hipStream_t s, bool solvent) {
+                     hipStream_t s, int form) {
   if (k.R <= 0 || steps == 0) return hipSuccess;
-  const DynLayout lay = PlanLds(t, k.R, solvent);
-  auto go = [&](auto kernel) {
+  const DynLayout lay = PlanLds(t, k.R, form);
+  return WithKmcKernel(form, lay.b1_lds != 0, [&](auto kernel, int threads) {
     hipError_t e = AllowLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kernel, dim3(k.R), dim3(kThreads), lay.bytes, s, t, lay, k, steps,
+    hipLaunchKernelGGL(kernel, dim3(k.R), dim3(threads), lay.bytes, s, t, lay, k, steps,
                        trace_base);
     return hipGetLastError();
-  };
-  if (lay.b1_lds) return solvent ? go(KmcKernel<true, true>) : go(KmcKernel<false, true>);
-  return solvent ? go(KmcKernel<true, false>) : go(KmcKernel<false, false>);
+  });
 }
 
 }  // namespace lmc

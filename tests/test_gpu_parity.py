@@ -156,9 +156,10 @@ def _compare_kmc(gst, gtr, gocc, refs, steps):
         np.testing.assert_allclose(gst["time"][r], st["time"], rtol=1e-12)
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 @pytest.mark.parametrize("name,kind", [("e0_3n", 1), ("q_3n", 2)])
 def test_kmc_trajectory_matches_oracle(torch, coeffs, name, kind, form):
+    """Every kernel shape (0 dense, 1 solvent baseline, 2 dense wide) against the oracle."""
     steps = 400
     gst, gtr, gocc, refs = _run_kmc_pair(coeffs, name, kind, n=6, R=4, steps=steps,
                                          temps=np.array([450.0, 500.0, 600.0, 700.0]), form=form)
@@ -166,7 +167,7 @@ def test_kmc_trajectory_matches_oracle(torch, coeffs, name, kind, form):
 
 
 def test_kmc_autotune_keeps_results(torch, coeffs):
-    """The autotuner times both formulations without changing the replicas' state."""
+    """The autotuner times every kernel shape without changing the replicas' state."""
     import latticemontecarlo_amd as L
     path, els = coeffs["e0_3"]
     gb = L.Model(path, els, L.E0)
@@ -175,7 +176,8 @@ def test_kmc_autotune_keeps_results(torch, coeffs):
     a = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
     b = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
     info = a.autotune(steps=50)
-    assert info["formulation"] in (0, 1) and info["ms_dense"] > 0 and info["ms_solvent"] > 0
+    assert info["formulation"] in (0, 1, 2) and info["ms_dense"] > 0 and info["ms_solvent"] > 0
+    assert info["ms_wide"] > 0  # 8 replicas: every one has a CU of its own
     a.run(200)
     b.run(200)
     sa, sb = a.state(), b.state()
@@ -224,7 +226,7 @@ def test_cmc_trajectory_matches_oracle(torch, coeffs):
         assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * trials
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 def test_kmc_graph_replay_matches_eager(torch, coeffs, form):
     """Launches replayed from a captured HIP graph do exactly what the same launches do eagerly."""
     import latticemontecarlo_amd as L
