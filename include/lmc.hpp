@@ -9,6 +9,7 @@
 //   pred::EnergyChangePredictorPairSite            EnergyChangePredictorPairSite.h:16-25
 //   mc::KineticMcFirstBatch, mc::CanonicalMcBatch  KineticMcFirstOmp.h:11-25,
 //                                                  CanonicalMcSerial.h:11-20 (R replicas each)
+//   mc::KineticMcChainBatch                        KineticMcChainOmpi.h (second-order steps)
 //   pred::Surrogate                                no counterpart (learned jump evaluation)
 //   ansys::ExitTime                                lmc/ansys/include/ExitTime.h:12-63 (barrier
 //                                                  lists and exit times)
@@ -365,6 +366,8 @@ class KineticMcFirstBatch {
     return chosen;
   }
   void Run(uint64_t steps, void *stream = nullptr) { lmc::Check(lmc_kmc_run(k_.get(), steps, stream)); }
+  // Kernel shape (see lmc_kmc_set_formulation).
+  void SetFormulation(int32_t formulation) { lmc::Check(lmc_kmc_set_formulation(k_.get(), formulation)); }
   // `launches` launches of `steps_per_launch` steps replayed from one captured HIP graph.
   void RunGraph(uint64_t steps_per_launch, int32_t launches, void *stream = nullptr) {
     lmc::Check(lmc_kmc_run_graph(k_.get(), steps_per_launch, launches, stream));
@@ -422,6 +425,16 @@ class KineticMcFirstBatch {
   std::unique_ptr<lmc_kmc, Free> k_;
   size_t R_ = 0;
   uint64_t max_steps_ = 0;
+};
+
+// KineticMcChainOmpi for R independent replicas: second-order residence-time steps.
+class KineticMcChainBatch : public KineticMcFirstBatch {
+ public:
+  KineticMcChainBatch(const std::vector<cfg::Config> &configs, const std::vector<lmc_kmc_replica> &params,
+                      const pred::VacancyMigrationPredictor &barrier, const pred::EnergyPredictor *energy)
+      : KineticMcFirstBatch(configs, params, barrier, energy) {
+    SetFormulation(3);
+  }
 };
 
 // CanonicalMcSerial for R independent replicas (swap trials, Metropolis acceptance).

@@ -184,7 +184,12 @@ int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const
 int lmc_kmc_set_trace(lmc_kmc *k, uint64_t capacity);
 /* Kernel of this driver: 0 dense (256 threads per replica, up to four replicas per CU),
  * 1 solvent baseline (see lmc_model_set_formulation), 2 dense wide shape (768 threads per
- * replica, one per CU: for batches of at most one replica per CU). No reference counterpart. */
+ * replica, one per CU: for batches of at most one replica per CU); these are first-order steps
+ * and have no reference counterpart. 3: second-order residence-time steps, replacing
+ * KineticMcChainOmpi (lmc/mc/include/KineticMcChainOmpi.h, KineticMcChainOmpi.cpp:56-151):
+ * the 12 x 12 jumps out of the vacancy's first neighbours after each virtual jump, the
+ * second-order probabilities and the deterministic time t_2; the previous site starts as the
+ * vacancy's first neighbour in id order (KineticMcAbstract.cpp:252). Takes no surrogate. */
 int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation);
 /* Autotuner (no reference counterpart): runs `steps` KMC steps with each kernel shape on a
  * scratch copy of the batch state (HIP events), keeps the fastest and leaves the replicas

@@ -337,6 +337,21 @@ class KineticMcFirst:
                     time=t.reshape(shape))
 
 
+class KineticMcChain(KineticMcFirst):
+    """Batched KineticMcChainOmpi: second-order residence-time steps (formulation 3), the 144
+    jumps out of the vacancy's first neighbours evaluated on the GPU each step. Same arguments
+    as KineticMcFirst."""
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.set_formulation(3)
+
+    def autotune(self, steps=200):
+        """Chain steps have one kernel shape; nothing to choose."""
+        return dict(formulation=3, ms_dense=float("nan"), ms_solvent=float("nan"),
+                    ms_wide=float("nan"))
+
+
 class CanonicalMc:
     """Batched CanonicalMcSerial driver (optional linear temperature ramp)."""
 

@@ -578,6 +578,7 @@ struct lmc_kmc {
     DevFree(d.disp);
     DevFree(d.com);
     DevFree(d.error);
+    DevFree(d.prev_j);
     DevFree(d.tt_time);
     DevFree(d.tt_temp);
     DevFree(d.tr_sel);
@@ -1038,6 +1039,15 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
     d.disp = DevCopy(std::vector<int64_t>(3 * R, 0));
     d.com = DevCopy(std::vector<double>(3 * R, 0.0));
     d.error = DevCopy(std::vector<int32_t>(1, 0));
+    {  // chain steps start with the vacancy's first neighbour in id order as the previous site
+      std::vector<int64_t> prev(R);
+      for (int r = 0; r < R; ++r) {
+        int64_t nb[12];
+        (void)lmc_fcc_neighbors(ncell[r], vac[r], 1, nb);
+        prev[r] = nb[0];
+      }
+      d.prev_j = DevCopy(prev);
+    }
     lmc::Upload(*barrier->m);
     if (energy) {
       RequireModel(energy);
@@ -1053,7 +1063,8 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
 void lmc_kmc_free(lmc_kmc *k) { delete k; }
 
 int lmc_kmc_set_formulation(lmc_kmc *k, int32_t formulation) {
-  if (!k || formulation < 0 || formulation > 2) return Fail(LMC_ERR_ARG, "bad formulation");
+  if (!k || formulation < 0 || formulation > 3) return Fail(LMC_ERR_ARG, "bad formulation");
+  if (formulation == 3 && k->sur) return Fail(LMC_ERR_ARG, "chain steps take no surrogate");
   k->formulation = formulation;
   return LMC_OK;
 }
@@ -1080,6 +1091,12 @@ int lmc_kmc_autotune(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_dens
 int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *ms_per_step) {
   return Guard([&] {
     if (!k || steps <= 0) throw std::invalid_argument("bad argument");
+    if (k->formulation == 3) {  // chain steps have one kernel shape
+      if (chosen) *chosen = 3;
+      if (ms_per_step)
+        for (int f = 0; f < 3; ++f) ms_per_step[f] = std::numeric_limits<double>::quiet_NaN();
+      return LMC_OK;
+    }
     const auto &t = lmc::Upload(*k->bm->m);
     const size_t R = static_cast<size_t>(k->d.R);
     // Scratch copy of every array the KMC kernel writes; read-only arrays are shared.
@@ -1836,6 +1853,7 @@ int lmc_kmc_set_surrogate(lmc_kmc *k, const lmc_surrogate *s, double threshold, 
     if (!k) throw std::invalid_argument("null driver");
     if (s && s->M != 4) throw std::invalid_argument("the KMC surrogate path needs 4 members");
     if (s && s->S != k->bm->m->S) throw std::invalid_argument("surrogate built for another model");
+    if (s && k->formulation == 3) throw std::invalid_argument("chain steps take no surrogate");
     k->sur = s;
     k->sur_threshold = threshold;
     k->sur_window = window;

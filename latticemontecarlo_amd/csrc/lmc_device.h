@@ -97,6 +97,7 @@ struct KmcDev {
   int64_t *disp = nullptr;      // [R][3] vacancy displacement, a/2 units
   double *com = nullptr;        // [R][3] solute centre-of-mass trajectory
   int32_t *error = nullptr;     // [1] first error code
+  int64_t *prev_j = nullptr;    // [R] chain steps: the site the vacancy came from
   // optional temperature schedule (shared)
   int32_t n_tt = 0;
   double *tt_time = nullptr, *tt_temp = nullptr;
@@ -135,7 +136,7 @@ hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, in
                              int sub_code, const int64_t *sites, int32_t K, int64_t *nb,
                              double *ea, double *de, int32_t *err, hipStream_t s);
 // form: 0 dense (256 threads per replica), 1 solvent baseline, 2 dense wide shape (768 threads
-// per replica, one workgroup per CU).
+// per replica, one workgroup per CU), 3 second-order chain steps (KineticMcChainOmpi).
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
                      hipStream_t s, int form);
 // Everything LaunchKmc sets up outside the stream (LDS plan, kernel attributes), so that its

@@ -664,42 +664,26 @@ __device__ __forceinline__ bool ChooseEvent(JumpLds &L, const ReplicaLds &Q, int
   return true;
 }
 
-// One lane: advances the clock and applies the chosen jump (KineticMcAbstract.cpp:147-181)
-// with energy change de.
-__device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, ReplicaLds &Q,
-                           uint8_t *occ, int r, uint64_t trace_index, int sel, double dt,
-                           double de) {
-  if (Q.flags & 2) {  // IsEscaped (KineticMcAbstract.cpp:192-223)
-    // walk the set bits of the shell mask without unrolling: an unrolled loop keeps one scalar
-    // predicate per slot live through the whole kernel
-    bool all = true;
-    uint64_t m = t.vac_shell_mask;
-#pragma unroll 1
-    while (m && all) {
-      const int j = __ffsll(static_cast<unsigned long long>(m)) - 1;
-      m &= m - 1;
-      all = L.env[0][j] == Q.solvent;
-    }
-    if (all) {
-      Q.esc = 1;
-      Q.steps = Q.max_steps + 1;
-    }
+// One lane: advances the clock and applies a jump of the vacancy to site `jump` (offset
+// dx, dy, dz; migrating species mig) with barrier ea and energy change de
+// (KineticMcAbstract.cpp:147-181). `escaped`: IsEscaped of the step.
+__device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint8_t *occ, int r,
+                          uint64_t trace_index, int64_t jump, int dx, int dy, int dz, int mig,
+                          double ea, double de, double dt, bool escaped) {
+  if (escaped) {
+    Q.esc = 1;
+    Q.steps = Q.max_steps + 1;
   }
-  const int dir = L.dir[sel];
-  const int64_t jump = L.snb[sel];
   if (trace_index < k.trace_cap) {
     const uint64_t o = static_cast<uint64_t>(r) * k.trace_cap + trace_index;
     k.tr_sel[o] = jump;
-    k.tr_ea[o] = L.ea[sel];
+    k.tr_ea[o] = ea;
     k.tr_de[o] = de;
     k.tr_time[o] = Q.time + dt;
   }
   Q.time += dt;
   Q.energy += de;
   Q.abse += de;
-  int dx, dy, dz;
-  Unpack16(L.off[dir * kEnvSlots + t.n_slot], dx, dy, dz);
-  const int mig = L.env[dir][t.n_slot];
   if ((Q.flags & 4) && mig != Q.solvent) {  // solute COM (KineticMcAbstract.cpp:163-177)
     const double wgt = k.mass[mig] / Q.smass;
     Q.com[0] -= dx * kHalfLattice * wgt;
@@ -719,6 +703,30 @@ __device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, 
   if (Q.steps > Q.max_steps || Q.esc) Q.done = 1;
 }
 
+// One lane: applies the chosen event `sel` of the first-order step with energy change de.
+__device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, ReplicaLds &Q,
+                           uint8_t *occ, int r, uint64_t trace_index, int sel, double dt,
+                           double de) {
+  bool all = false;
+  if (Q.flags & 2) {  // IsEscaped (KineticMcAbstract.cpp:192-223)
+    // walk the set bits of the shell mask without unrolling: an unrolled loop keeps one scalar
+    // predicate per slot live through the whole kernel
+    all = true;
+    uint64_t m = t.vac_shell_mask;
+#pragma unroll 1
+    while (m && all) {
+      const int j = __ffsll(static_cast<unsigned long long>(m)) - 1;
+      m &= m - 1;
+      all = L.env[0][j] == Q.solvent;
+    }
+  }
+  const int dir = L.dir[sel];
+  int dx, dy, dz;
+  Unpack16(L.off[dir * kEnvSlots + t.n_slot], dx, dy, dz);
+  ApplyJump(t, k, Q, occ, r, trace_index, L.snb[sel], dx, dy, dz, L.env[dir][t.n_slot], L.ea[sel],
+            de, dt, all);
+}
+
 __device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, ReplicaLds &Q,
                                uint8_t *occ, int r, uint64_t trace_index, int tid) {
   const int i = tid;
@@ -728,6 +736,128 @@ __device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, Re
   double dt;
   if (!ChooseEvent(L, Q, i, sel, dt)) return;
   ApplyEvent(t, k, L, Q, occ, r, trace_index, sel, dt, L.de[sel]);
+}
+
+// ---- Second-order (chain) steps: KineticMcChainOmpi (lmc/mc/src/KineticMcChainOmpi.cpp) ----
+
+// Per first neighbour i of the vacancy k (in id order): its site, offset and species, and the
+// event k -> i taken as the reverse of i -> k after the jump, with the total rate out of i.
+struct ChainLds {
+  int64_t nid[12];  // by direction
+  int64_t sid[12];  // by rank
+  uint16_t soff[12];
+  uint8_t sm[12];
+  double barrier[12], de[12], rate[12], back[12], total_i[12];
+  int64_t prev;     // the site the vacancy came from
+};
+
+// Wave 0: the vacancy's 12 first neighbours ranked by id, with their offsets and species.
+__device__ void ChainNeighbours(const KTables &t, const JumpLds &L, ChainLds &C,
+                                const ReplicaLds &Q, const uint8_t *occ, int n, int tid) {
+  if (tid >= 64) return;
+  const bool on = tid < 12;
+  int64_t id = 0;
+  uint16_t o = 0;
+  if (on) {
+    o = L.off[tid * kEnvSlots + t.n_slot];
+    int dx, dy, dz;
+    Unpack16(o, dx, dy, dz);
+    id = SiteAt(VacancyXYZ(Q), n, dx, dy, dz);
+    C.nid[tid] = id;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (!on) return;
+  int rk = 0;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) rk += C.nid[j] < id;
+  C.sid[rk] = id;
+  C.soff[rk] = o;
+  C.sm[rk] = occ[id];
+}
+
+// Lane 0 after the pass with the vacancy at neighbour i (rank order): the total rate out of i
+// and the event k -> i as the reverse of i -> k (KineticMcChainOmpi.cpp:56-91, JumpEvent.cpp:6-13,
+// 25-30,51-54).
+__device__ void ChainPass(JumpLds &L, ChainLds &C, const ReplicaLds &Q, int i) {
+  double tot = 0.0, ea = 0.0, de = 0.0;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    tot += L.rate[j];
+    if (L.snb[j] == Q.vac) {
+      ea = L.ea[j];
+      de = L.de[j];
+    }
+  }
+  C.total_i[i] = tot;
+  C.barrier[i] = ea - de;
+  C.de[i] = -de;
+  C.rate[i] = exp(-C.barrier[i] * Q.beta);
+  C.back[i] = exp((C.de[i] - C.barrier[i]) * Q.beta);
+  L.bad = 0;  // every event lane has read it
+}
+
+// Lane 0: second-order probabilities and time (KineticMcChainOmpi.cpp:93-151; the rank sums in
+// neighbour order), the selection with u2 (KineticMcAbstract.cpp:106-123) and the jump.
+__device__ void ChainSelectAndApply(const KTables &t, const KmcDev &k, const JumpLds &L,
+                                    ChainLds &C, ReplicaLds &Q, uint8_t *occ, int n, int r,
+                                    uint64_t trace_index) {
+  double total = 0.0;
+  for (int i = 0; i < 12; ++i) total += C.rate[i];
+  const double t1 = 1 / total / kPrefactor;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, b[12];
+  int jprev = -1;
+  for (int i = 0; i < 12; ++i) {
+    const double p_ki = C.rate[i] / total;
+    const double p_ik = C.back[i] / C.total_i[i];
+    const double bb = p_ki * p_ik;
+    b[i] = p_ki * (1 - p_ik);
+    const bool prev = C.sid[i] == C.prev;
+    if (prev) jprev = i;
+    const double ti = 1 / C.total_i[i] / kPrefactor;
+    s[0] += bb;
+    s[1] += b[i];
+    s[2] += prev ? 0.0 : bb;
+    s[3] += prev ? 0.0 : b[i];
+    s[4] += prev ? b[i] : 0.0;
+    s[5] += prev ? p_ki : 0.0;
+    s[6] += (t1 + ti) * bb;
+    s[7] += prev ? 0.0 : (t1 + ti) * bb;
+  }
+  const double ts = s[6] / s[0], ts_j = s[7] / s[2];
+  const double inv = 1 / (1 - s[5]);
+  double dt = inv * (s[3] * t1 + s[2] * (ts_j + t1 + s[0] / s[1] * ts));
+  if (Q.flags & 1) dt *= L.corr;
+  const double u2 = L.u2;
+  double cum = 0.0;
+  int sel = 11;
+  for (int i = 0; i < 11; ++i) {
+    cum += i == jprev ? inv * (s[2] / s[1]) * s[4] : inv * (1 + s[2] / s[1]) * b[i];
+    if (!(cum < u2)) {
+      sel = i;
+      break;
+    }
+  }
+  bool all = false;
+  if (Q.flags & 2) {  // IsEscaped (KineticMcAbstract.cpp:192-223), read from the occupancy
+    all = true;
+    uint64_t m = t.vac_shell_mask;
+#pragma unroll 1
+    while (m && all) {
+      const int j = __ffsll(static_cast<unsigned long long>(m)) - 1;
+      m &= m - 1;
+      int dx, dy, dz;
+      Unpack16(L.off[j], dx, dy, dz);
+      all = occ[SiteAt(VacancyXYZ(Q), n, dx, dy, dz)] == Q.solvent;
+    }
+  }
+  int dx, dy, dz;
+  Unpack16(C.soff[sel], dx, dy, dz);
+  const int64_t from = Q.vac;
+  ApplyJump(t, k, Q, occ, r, trace_index, C.sid[sel], dx, dy, dz, C.sm[sel], C.barrier[sel],
+            C.de[sel], dt, all);
+  C.prev = from;  // KineticMcChainAbstract::OneStepSimulation (KineticMcAbstract.cpp:260-263)
 }
 
 }  // namespace

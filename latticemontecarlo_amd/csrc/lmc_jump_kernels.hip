@@ -181,9 +181,56 @@ __global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynL
   if (tid == 0) StoreReplica(k, r, Q);
 }
 
+// Persistent second-order (chain) KMC, KineticMcChainOmpi: one 256-thread workgroup per replica.
+// Each step evaluates the 12 jumps out of every first neighbour i of the vacancy k in the
+// configuration after the jump k -> i (12 passes of the dense evaluation: vacancy at i, site k
+// holding i's species), then lane 0 forms the second-order probabilities and time and jumps.
+template <bool kBL>
+__global__ __launch_bounds__(kThreads, 4) void KmcChainKernel(KTables t, DynLayout lay, KmcDev k,
+                                                           uint64_t nsteps, uint64_t trace_base) {
+  __shared__ JumpLds L;
+  __shared__ ReplicaLds Q;
+  __shared__ ChainLds C;
+  extern __shared__ char dyn[];
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  const Tabs T = LoadJumpTables<false, kBL>(t, L, dyn, lay, tid, kThreads, tid, kThreads);
+  uint8_t *occ = k.occ + k.off[r];
+  const int n = k.ncell[r];
+  if (tid == 0) {
+    LoadReplica(k, r, n, Q);
+    C.prev = k.prev_j[r];
+    L.bad = 0;
+  }
+  __syncthreads();
+  const int sv = Q.solvent;
+  for (uint64_t it = 0; it < nsteps; ++it) {
+    if (Q.done) break;
+    UpdateTemperature(k, Q, tid);
+    ChainNeighbours(t, L, C, Q, occ, n, tid);
+    if (tid == kThreads - 1) DrawStep(L, Q);
+    __syncthreads();
+    for (int i = 0; i < 12; ++i) {
+      int dx, dy, dz;
+      Unpack16(C.soff[i], dx, dy, dz);
+      const XYZ vi{Wrap(Q.vx + dx, Q.m2), Wrap(Q.vy + dy, Q.m2), Wrap(Q.vz + dz, Q.m2)};
+      EvaluateJumps<false, true>(t, L, T, occ, n, vi, sv, Q.beta, k.error, tid, nullptr,
+                                 GatherSub{Q.vac, C.sm[i]});
+      if (tid == 0) ChainPass(L, C, Q, i);
+      __syncthreads();
+    }
+    if (tid == 0) ChainSelectAndApply(t, k, L, C, Q, occ, n, r, trace_base + it);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    StoreReplica(k, r, Q);
+    k.prev_j[r] = C.prev;
+  }
+}
+
 // Dynamic LDS plan: weight tables go to LDS while the workgroup stays within its share of the
 // CU's LDS (up to 4 resident workgroups per CU when the batch fills the chip).
-// form: 0 dense, 1 solvent baseline, 2 dense wide shape (one workgroup per CU).
+// form: 0 dense, 1 solvent baseline, 2 dense wide shape (one workgroup per CU), 3 chain steps.
 DynLayout PlanLds(const KTables &t, int32_t R, int form) {
   const bool solvent = form == 1;
   static int cus = 0;
@@ -202,7 +249,7 @@ DynLayout PlanLds(const KTables &t, int32_t R, int form) {
     }
   }
   const int per_cu = form == 2 ? 1 : std::min(4, std::max(1, (R + cus - 1) / cus));
-  const size_t stat = sizeof(JumpLds) + sizeof(ReplicaLds) + 256;
+  const size_t stat = sizeof(JumpLds) + sizeof(ReplicaLds) + (form == 3 ? sizeof(ChainLds) : 0) + 256;
   const size_t share = std::min(lds_block, lds_cu / per_cu);
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   DynLayout lay{};
@@ -288,10 +335,11 @@ hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, in
   return lay.b1_lds ? go(EvalJumpsKernel<false, true, true>) : go(EvalJumpsKernel<false, false, true>);
 }
 
-// The instantiation of form (0 dense, 1 solvent baseline, 2 dense wide) and LDS placement, with
-// its block size; f(kernel, threads).
+// The instantiation of form (0 dense, 1 solvent baseline, 2 dense wide, 3 second-order chain
+// steps) and LDS placement, with its block size; f(kernel, threads).
 template <typename F>
 hipError_t WithKmcKernel(int form, bool b1_lds, F &&f) {
+  if (form == 3) return b1_lds ? f(KmcChainKernel<true>, kThreads) : f(KmcChainKernel<false>, kThreads);
   if (form == 2) return b1_lds ? f(KmcWideKernel<true>, kWideThreads) : f(KmcWideKernel<false>, kWideThreads);
   if (form == 1) return b1_lds ? f(KmcKernel<true, true>, kThreads) : f(KmcKernel<true, false>, kThreads);
   return b1_lds ? f(KmcKernel<false, true>, kThreads) : f(KmcKernel<false, false>, kThreads);

@@ -1044,6 +1044,11 @@ struct Kmc {
   int32_t ev_nb[12];
   double ev_ea[12], ev_de[12], ev_rate[12], ev_cum[12];
   double total_rate = 0;
+  // second-order (chain) steps: per first neighbour i of the vacancy k, the event k -> i as the
+  // reverse of i -> k after the jump, and the total rate out of i
+  bool chain = false;
+  size_t prev_j = 0;  // the site the vacancy came from (KineticMcChainAbstract, :252)
+  double ch_barrier[12], ch_de[12], ch_rate[12], ch_back[12], ch_total_i[12];
 
   // TimeTemperatureInterpolator::GetTemperature (TimeTemperatureInterpolator.cpp:44-65)
   double TAt(double t) const {
@@ -1091,6 +1096,70 @@ struct Kmc {
       ev_cum[i] = cum;
     }
   }
+  // KineticMcChainOmpi::BuildEventList (KineticMcChainOmpi.cpp:56-91), one pass per first
+  // neighbour i of the vacancy k (the reference runs one MPI rank per i); the rank sums are
+  // taken in neighbour order.
+  void BuildChainEvents() {
+    const size_t k = vac;
+    const auto &nbk = lat->nn[0][k];
+    for (int i = 0; i < 12; ++i) {
+      const size_t is = nbk[i];
+      std::swap(occ[k], occ[is]);  // LatticeJump({k, i})
+      const auto &nbi = lat->nn[0][is];
+      double tot = 0.0;
+      for (int l = 0; l < 12; ++l) {
+        const auto r = Predict(is, nbi[l]);
+        if (nbi[l] == k) {  // GetReverseJumpEvent (JumpEvent.cpp:51-54)
+          ch_barrier[i] = r.first - r.second;
+          ch_de[i] = -r.second;
+        }
+        tot += std::exp(-r.first * beta);
+      }
+      std::swap(occ[k], occ[is]);  // LatticeJump({i, k})
+      ch_total_i[i] = tot;
+      ch_rate[i] = std::exp(-ch_barrier[i] * beta);
+      ch_back[i] = std::exp((ch_de[i] - ch_barrier[i]) * beta);  // GetBackwardRate
+    }
+    total_rate = 0.0;
+    for (int i = 0; i < 12; ++i) total_rate += ch_rate[i];  // MPI_Allreduce
+  }
+  // KineticMcChainOmpi::CalculateTime (KineticMcChainOmpi.cpp:93-151): second-order time and
+  // event probabilities (cumulative in ev_cum).
+  double ChainTime() {
+    const auto &nbk = lat->nn[0][vac];
+    const double t1 = 1 / total_rate / kPrefactor;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, b[12];
+    bool prev[12];
+    for (int i = 0; i < 12; ++i) {
+      const double p_ki = ch_rate[i] / total_rate;
+      const double p_ik = ch_back[i] / ch_total_i[i];
+      const double bb = p_ki * p_ik;
+      b[i] = p_ki * (1 - p_ik);
+      prev[i] = nbk[i] == prev_j;
+      const double ti = 1 / ch_total_i[i] / kPrefactor;
+      s[0] += bb;
+      s[1] += b[i];
+      s[2] += prev[i] ? 0.0 : bb;
+      s[3] += prev[i] ? 0.0 : b[i];
+      s[4] += prev[i] ? b[i] : 0.0;
+      s[5] += prev[i] ? p_ki : 0.0;
+      s[6] += (t1 + ti) * bb;
+      s[7] += prev[i] ? 0.0 : (t1 + ti) * bb;
+    }
+    const double ts = s[6] / s[0], ts_j = s[7] / s[2];
+    const double inv = 1 / (1 - s[5]);
+    const double t2 = inv * (s[3] * t1 + s[2] * (ts_j + t1 + s[0] / s[1] * ts));
+    double cum = 0.0;
+    for (int i = 0; i < 12; ++i) {
+      const double p = prev[i] ? inv * (s[2] / s[1]) * s[4] : inv * (1 + s[2] / s[1]) * b[i];
+      cum += p;
+      ev_cum[i] = cum;
+      ev_nb[i] = static_cast<int32_t>(nbk[i]);
+      ev_ea[i] = ch_barrier[i];
+      ev_de[i] = ch_de[i];
+    }
+    return t2;
+  }
   bool IsSolventAll() const {  // IsEscaped (KineticMcAbstract.cpp:192-223)
     for (int s = 0; s < 3; ++s)
       for (size_t id : lat->nn[s][vac])
@@ -1103,12 +1172,18 @@ struct Kmc {
       T = TAt(time);
       beta = 1.0 / kBoltzmann / T;
     }
-    BuildEvents();
     uint32_t r[4];
     Draw(seed, steps, replica, 0u, r);
     const double u1 = UnitOpenLeft(Bits64(r[0], r[1]));
     const double u2 = UnitOpenRight(Bits64(r[2], r[3]));
-    double dt = -std::log(u1) / total_rate / kPrefactor;  // CalculateTime (:79-82)
+    double dt;
+    if (chain) {
+      BuildChainEvents();
+      dt = ChainTime();
+    } else {
+      BuildEvents();
+      dt = -std::log(u1) / total_rate / kPrefactor;  // CalculateTime (:79-82)
+    }
     if (rate_corrector) {  // RateCorrector::GetTimeCorrectionFactor (RateCorrector.hpp:17-24)
       dt *= cv / (1.64 * std::exp(-(0.66 / kBoltzmann / T - 0.7))) / (1 - 13 * cs);
     }
@@ -1141,6 +1216,7 @@ struct Kmc {
     for (int k = 0; k < 3; ++k) disp[k] += static_cast<int64_t>(std::llround(rd[k] * 2 * lat->n));
     std::swap(occ[vac], occ[jump]);  // Config::LatticeJump (Config.cpp:431-456)
     ++steps;
+    prev_j = vac;  // KineticMcChainAbstract::OneStepSimulation (KineticMcAbstract.cpp:260-263)
     vac = jump;
   }
 };
@@ -1193,6 +1269,8 @@ static Kmc *NewKmc(const Model *bm, const Model *em, const Lattice *lat, const u
   std::sort(k->tt.begin(), k->tt.end());
   if (p->lru_capacity > 0) k->lru = std::make_unique<Lru>(static_cast<size_t>(p->lru_capacity));
   k->threads = std::max(1, p->omp_threads);
+  k->chain = p->chain != 0;
+  k->prev_j = lat->nn[0][vac][0];  // first neighbour in id order (KineticMcAbstract.cpp:252)
   return k.release();
 }
 
@@ -1425,7 +1503,13 @@ void orc_kmc_occupancy(const void *kv, uint8_t *occ) {
 int orc_kmc_events(void *kv, int32_t *nb, double *ea, double *de, double *rate, double *cum) {
   Kmc &k = *static_cast<Kmc *>(kv);
   try {
-    k.BuildEvents();
+    if (k.chain) {
+      k.BuildChainEvents();
+      (void)k.ChainTime();
+      for (int i = 0; i < 12; ++i) k.ev_rate[i] = k.ch_rate[i];
+    } else {
+      k.BuildEvents();
+    }
   } catch (const std::exception &) {
     return -1;
   }
@@ -1437,6 +1521,19 @@ int orc_kmc_events(void *kv, int32_t *nb, double *ea, double *de, double *rate, 
     cum[i] = k.ev_cum[i];
   }
   return 0;
+}
+double orc_kmc_step_time(void *kv) {
+  Kmc &k = *static_cast<Kmc *>(kv);
+  try {
+    if (k.chain) {
+      k.BuildChainEvents();
+      return k.ChainTime();
+    }
+    k.BuildEvents();
+    return 1 / k.total_rate / kPrefactor;
+  } catch (const std::exception &) {
+    return -1.0;
+  }
 }
 uint64_t orc_kmc_lru_hits(const void *kv) {
   const Kmc &k = *static_cast<const Kmc *>(kv);

@@ -80,6 +80,7 @@ typedef struct {
   const double *tt_temp;
   int lru_capacity;        // 0 = no memo (VacancyMigrationPredictor*Lru)
   int omp_threads;         // threads for the 12-event loop (1 = serial)
+  int chain;               // 1: second-order residence-time steps (KineticMcChainOmpi.cpp:56-151)
 } orc_kmc_params;
 
 void *orc_kmc_new(const void *barrier_model, const void *energy_model, const void *lat,
@@ -97,7 +98,12 @@ void orc_kmc_state(const void *kmc, uint64_t *steps, double *time, double *energ
                    double *com3, int *escaped);
 void orc_kmc_occupancy(const void *kmc, uint8_t *occ);
 // Events of the current step (12 x {neighbour id, Ea, dE, rate, cumulative probability}).
+// (chain mode: barriers, energy changes and forward rates of the events k -> i, second-order
+// cumulative probabilities)
 int orc_kmc_events(void *kmc, int32_t *nb, double *ea, double *de, double *rate, double *cum);
+// Residence time of the current step: the second-order time t_2 in chain mode, otherwise the
+// mean 1 / (nu total rate). -1 on error.
+double orc_kmc_step_time(void *kmc);
 uint64_t orc_kmc_lru_hits(const void *kmc);
 
 // CanonicalMcSerial driver (CanonicalMcAbstract.cpp:43-101, CanonicalMcSerial.cpp:40-51) with an

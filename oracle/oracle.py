@@ -26,7 +26,7 @@ class KmcParams(C.Structure):
                 ("restart_time", C.c_double), ("rate_corrector", C.c_int),
                 ("early_stop", C.c_int), ("solute_disp", C.c_int), ("n_tt", C.c_int),
                 ("tt_time", _f64p), ("tt_temp", _f64p), ("lru_capacity", C.c_int),
-                ("omp_threads", C.c_int)]
+                ("omp_threads", C.c_int), ("chain", C.c_int)]
 
 
 class CmcParams(C.Structure):
@@ -78,6 +78,7 @@ def lib():
                                      _i32p, _i64p, _f64p, C.POINTER(C.c_int)]),
             "orc_kmc_occupancy": (None, [vp, _u8p]),
             "orc_kmc_events": (C.c_int, [vp, _i32p, _f64p, _f64p, _f64p, _f64p]),
+            "orc_kmc_step_time": (C.c_double, [vp]),
             "orc_kmc_lru_hits": (C.c_uint64, [vp]),
             "orc_cmc_new": (vp, [vp, vp, vp, _u8p, C.POINTER(CmcParams), C.c_char_p, C.c_int]),
             "orc_cmc_free": (None, [vp]),
@@ -217,11 +218,13 @@ def philox(key, ctr):
 
 
 class Kmc:
-    """KineticMcFirst trajectory of one replica."""
+    """KineticMcFirst trajectory of one replica (chain=True: KineticMcChainOmpi's second-order
+    steps)."""
 
     def __init__(self, barrier_model, energy_model, lattice, occ, temperature, seed, replica=0,
                  restart_steps=0, restart_energy=0.0, restart_time=0.0, rate_corrector=False,
-                 early_stop=False, solute_disp=False, tt=None, lru_capacity=0, omp_threads=1):
+                 early_stop=False, solute_disp=False, tt=None, lru_capacity=0, omp_threads=1,
+                 chain=False):
         p = KmcParams()
         p.temperature = temperature
         p.seed = seed
@@ -241,6 +244,7 @@ class Kmc:
             p.tt_temp = _ptr(self._tt[1], _f64p)
         p.lru_capacity = lru_capacity
         p.omp_threads = omp_threads
+        p.chain = int(chain)
         err = C.create_string_buffer(512)
         occ = np.ascontiguousarray(occ, np.uint8)
         self.h = lib().orc_kmc_new(barrier_model.h, energy_model.h if energy_model else None,
@@ -288,6 +292,13 @@ class Kmc:
         if lib().orc_kmc_events(self.h, _ptr(nb, _i32p), *[_ptr(x, _f64p) for x in a]):
             raise RuntimeError("events failed")
         return dict(nb=nb, ea=a[0], de=a[1], rate=a[2], cum=a[3])
+
+    def step_time(self):
+        """Residence time of the current step (chain: the second-order t_2)."""
+        t = lib().orc_kmc_step_time(self.h)
+        if t < 0:
+            raise RuntimeError("step time failed")
+        return t
 
     def lru_hits(self):
         return lib().orc_kmc_lru_hits(self.h)

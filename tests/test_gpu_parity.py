@@ -135,7 +135,7 @@ def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, **opts):
     refs = []
     for r in range(R):
         k = O.Kmc(ob, oe, lat, occs[r], float(np.broadcast_to(temps, (R,))[r]), seed, replica=r,
-                  tt=tt, **opts)
+                  tt=tt, chain=form == 3, **opts)
         tr = k.run(steps, trace=True)
         refs.append((k.state(), tr, k.occupancy()))
     return gst, gtr, gocc, refs
@@ -163,6 +163,17 @@ def test_kmc_trajectory_matches_oracle(torch, coeffs, name, kind, form):
     steps = 400
     gst, gtr, gocc, refs = _run_kmc_pair(coeffs, name, kind, n=6, R=4, steps=steps,
                                          temps=np.array([450.0, 500.0, 600.0, 700.0]), form=form)
+    _compare_kmc(gst, gtr, gocc, refs, steps)
+
+
+@pytest.mark.parametrize("name,kind,opts", [("e0_3n", 1, {}),
+                                            ("q_3n", 2, {"rate_corrector": True, "solute_disp": True})])
+def test_chain_kmc_trajectory_matches_oracle(torch, coeffs, name, kind, opts):
+    """Second-order (chain) steps (formulation 3) against the oracle's KineticMcChainOmpi."""
+    steps = 150
+    print(f"chain KMC {name} {opts}", flush=True)
+    gst, gtr, gocc, refs = _run_kmc_pair(coeffs, name, kind, n=6, R=3, steps=steps,
+                                         temps=np.array([450.0, 550.0, 700.0]), form=3, **opts)
     _compare_kmc(gst, gtr, gocc, refs, steps)
 
 
@@ -226,7 +237,7 @@ def test_cmc_trajectory_matches_oracle(torch, coeffs):
         assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * trials
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("form", [0, 1, 2, 3])
 def test_kmc_graph_replay_matches_eager(torch, coeffs, form):
     """Launches replayed from a captured HIP graph do exactly what the same launches do eagerly."""
     import latticemontecarlo_amd as L

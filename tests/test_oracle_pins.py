@@ -331,3 +331,59 @@ def test_oracle_event_choice_follows_rates(coeffs):
     expect = probs * 3000
     chi2 = ((counts - expect) ** 2 / np.maximum(expect, 1e-9)).sum()
     assert chi2 < 45  # 11 dof, p ~ 1e-6
+
+
+def test_oracle_chain_step_pins(coeffs):
+    """Second-order (chain) steps, KineticMcChainOmpi.cpp:56-151: the probabilities of the 12
+    jumps sum to one; each jump's barrier, taken as the reverse of i -> k after the jump, equals
+    the direct first-order barrier by the KRA identity; in pure solvent every barrier is equal
+    and t_2 = 13/11 * 1 / (12 nu r), with 1/144 for the jump back to the previous site and
+    13/144 for the others."""
+    path, els = coeffs["e0_3n"]
+    n = 5
+    lat = O.Lattice(n)
+    bm = O.Model(lat, path, els, O.E0)
+    species = sorted(set(els) | {"X"})
+    occ = random_alloc(n, species, 0.1, seed=21)
+    T = 600.0
+    kc = O.Kmc(bm, None, lat, occ, T, seed=3, chain=True)
+    k1 = O.Kmc(bm, None, lat, occ, T, seed=3)
+    ev, e1 = kc.events(), k1.events()
+    assert np.array_equal(ev["nb"], e1["nb"])
+    assert abs(ev["cum"][-1] - 1.0) < 1e-12 and np.all(np.diff(ev["cum"]) >= 0)
+    # Ea = max(0, ...) inactive both ways (a clamped reverse barrier gives the chain dE instead)
+    clamp = (e1["ea"] > 1e-6) & (e1["ea"] - e1["de"] > 1e-6)
+    assert clamp.sum() >= 6
+    np.testing.assert_allclose(ev["ea"][clamp], e1["ea"][clamp], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(ev["de"], e1["de"], rtol=0, atol=1e-12)
+    # pure solvent
+    pure = np.full_like(occ, species.index("Al"))
+    v = 17
+    pure[v] = species.index("X")
+    kp = O.Kmc(bm, None, lat, pure, T, seed=3, chain=True)
+    evp = kp.events()
+    assert np.ptp(evp["ea"]) < 1e-14 and np.abs(evp["de"]).max() < 1e-14
+    r = np.exp(-evp["ea"][0] / (8.617333262145e-5 * T))
+    assert kp.step_time() == pytest.approx(13 / 11 / (12 * 1e13 * r), rel=1e-12)
+    p = np.diff(np.concatenate([[0.0], evp["cum"]]))
+    prev = list(evp["nb"]).index(int(lat.neighbors(v, 1)[0]))  # initial previous site
+    assert p[prev] == pytest.approx(1 / 144, rel=1e-12)
+    assert np.allclose(np.delete(p, prev), 13 / 144, rtol=1e-12)
+
+
+def test_oracle_chain_kmc_invariants(coeffs):
+    path, els = coeffs["e0_3n"]
+    n = 5
+    lat = O.Lattice(n)
+    bm = O.Model(lat, path, els, O.E0)
+    em = O.Model(lat, path, els, O.ENERGY)
+    species = sorted(set(els) | {"X"})
+    occ = random_alloc(n, species, 0.1, seed=23)
+    k = O.Kmc(bm, em, lat, occ, 550.0, seed=4, replica=1, chain=True)
+    tr = k.run(200, trace=True)
+    st = k.state()
+    occ2 = k.occupancy()
+    assert np.array_equal(np.bincount(occ, minlength=4), np.bincount(occ2, minlength=4))
+    assert np.all(np.diff(np.concatenate([[0.0], tr["time"]])) > 0)
+    assert abs(st["energy"] - tr["de"].sum()) < 1e-9
+    assert abs(st["abs_energy"] - em.total_energy(occ2)) < 1e-9
