@@ -274,6 +274,53 @@ def large_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def chain_workload(args):
+    """Second-order (chain) KMC, KineticMcChainOmpi, on the batched workload (1024 replicas of
+    20^3 cells, Mg and Zn 1-5 at.% each, T 400-700 K, E0): every step evaluates the 144 jumps
+    out of the vacancy's first neighbours. CPU baseline: the oracle's chain steps, independent
+    single-thread replicas of 10^3 cells on the host cores. Prints one JSON line."""
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import synth
+    R, n = args.replicas, args.ncell
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0.json")
+    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "E0", seed=args.seed)
+    bm = L.Model(coeff, ["Al", "Mg", "Zn"], L.E0)
+    occs, temps = make_workload(R, n, args.seed)
+    drv = L.KineticMcChain(bm, None, [n] * R, occs, temps, seed=args.seed, maximum_steps=10 ** 9)
+    steps, chunk = args.steps, min(args.chunk, args.steps)
+    elapsed = timed_plain(L, drv, R, steps, args.warmup, chunk)
+    bm.check_errors()
+    value = R * steps / elapsed
+    log(f"chain: {value:.4g} KMC steps/s over {R} replicas ({elapsed:.2f} s, "
+        f"{144 * value:.4g} jump evaluations/s)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        lat = O.Lattice(10)
+        om = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.E0)
+        occ = synth.random_occupancy(10, ["Al", "Mg", "X", "Zn"], {"Mg": 80, "Zn": 80}, seed=1)
+        O.kmc_baseline(om, None, lat, occ, 500.0, 7, 1, 1, 100000, 20, chain=True)  # tables
+        secs, _ = O.kmc_baseline(om, None, lat, occ, 500.0, 7, cores, 1, 100000, 200, chain=True)
+        thru = cores * 200 / secs
+        log(f"cpu baseline (chain): {thru:.1f} steps/s ({cores} single-thread replicas)")
+        cpu = {"value": round(thru, 2), "unit": "KMC steps/s", "cores": cores, "kind": "port",
+               "sample": f"oracle KineticMcChainOmpi steps (E0, LRU 100000), 10^3 cells x 4000 "
+                         f"sites, T=500 K: {cores} single-thread replicas x 200 steps"}
+    line = {"metric": "KMC steps/s over all replicas (second-order chain steps)",
+            "value": round(value, 1), "unit": "KMC steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": round(elapsed / steps * 1e3, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"second-order chain KMC, {R} replicas x {n}^3 FCC cells, Al with "
+                                   "1-5 at.% Mg and Zn each, T 400-700 K, E0 barrier model",
+                       "jump_evaluations_per_s": round(144 * value, 1)},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def uneven_workload(args):
     """Uneven-size batched KMC alongside canonical MC: 256 KMC replicas of n^3 cells with n drawn
     from 8..40 (2048-256000 sites; Mg, Zn 1-5 at.% each, T 400-700 K) on one stream and 64
@@ -357,7 +404,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch eagerly instead of a HIP graph")
-    ap.add_argument("--workload", choices=["batched", "uneven", "large"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain"], default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
     args = ap.parse_args()
@@ -377,6 +424,9 @@ def main():
         return
     if args.workload == "large":
         large_workload(args)
+        return
+    if args.workload == "chain":
+        chain_workload(args)
         return
     R, n = args.replicas, args.ncell
     tmp = tempfile.mkdtemp()

@@ -1626,7 +1626,7 @@ void orc_cmc_occupancy(const void *cv, uint8_t *occ) {
 
 double orc_kmc_baseline(const void *bm, const void *em, const void *lat, const uint8_t *occ,
                         double temperature, uint64_t seed, int replicas, int threads_per_replica,
-                        int lru_capacity, uint64_t steps, uint64_t *lru_hits) {
+                        int lru_capacity, uint64_t steps, uint64_t *lru_hits, int chain) {
   const Model &m = *static_cast<const Model *>(bm);
   const Lattice &L = *static_cast<const Lattice *>(lat);
   // Build every pair table up front, as the predictor constructor does (...Quartic.cpp:64-100).
@@ -1641,6 +1641,7 @@ double orc_kmc_baseline(const void *bm, const void *em, const void *lat, const u
     p.replica = static_cast<uint32_t>(r);
     p.lru_capacity = lru_capacity;
     p.omp_threads = threads_per_replica;
+    p.chain = chain;
     ks.emplace_back(NewKmc(&m, static_cast<const Model *>(em), &L, occ, &p));
   }
 #ifdef _OPENMP

@@ -132,7 +132,7 @@ void orc_cmc_occupancy(const void *cmc, uint8_t *occ);
 double orc_kmc_baseline(const void *barrier_model, const void *energy_model, const void *lat,
                         const uint8_t *occ, double temperature, uint64_t seed, int replicas,
                         int threads_per_replica, int lru_capacity, uint64_t steps,
-                        uint64_t *lru_hits);
+                        uint64_t *lru_hits, int chain);
 
 #ifdef __cplusplus
 }

@@ -89,7 +89,7 @@ def lib():
             "orc_cmc_occupancy": (None, [vp, _u8p]),
             "orc_kmc_baseline": (C.c_double, [vp, vp, vp, _u8p, C.c_double, C.c_uint64,
                                               C.c_int, C.c_int, C.c_int, C.c_uint64,
-                                              C.POINTER(C.c_uint64)]),
+                                              C.POINTER(C.c_uint64), C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -347,10 +347,11 @@ class Cmc:
 
 
 def kmc_baseline(barrier_model, energy_model, lattice, occ, temperature, seed, replicas,
-                 threads_per_replica, lru_capacity, steps):
+                 threads_per_replica, lru_capacity, steps, chain=False):
     hits = C.c_uint64()
     occ = np.ascontiguousarray(occ, np.uint8)
     secs = lib().orc_kmc_baseline(barrier_model.h, energy_model.h if energy_model else None,
                                   lattice.h, _ptr(occ, _u8p), temperature, seed, replicas,
-                                  threads_per_replica, lru_capacity, steps, C.byref(hits))
+                                  threads_per_replica, lru_capacity, steps, C.byref(hits),
+                                  int(chain))
     return secs, hits.value
