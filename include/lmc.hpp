@@ -10,6 +10,7 @@
 //   mc::KineticMcFirstBatch, mc::CanonicalMcBatch  KineticMcFirstOmp.h:11-25,
 //                                                  CanonicalMcSerial.h:11-20 (R replicas each)
 //   mc::KineticMcChainBatch                        KineticMcChainOmpi.h (second-order steps)
+//   mc::SimulatedAnnealingBatch                    SimulatedAnnealing.h (adaptive schedule)
 //   pred::Surrogate                                no counterpart (learned jump evaluation)
 //   ansys::ExitTime                                lmc/ansys/include/ExitTime.h:12-63 (barrier
 //                                                  lists and exit times)
@@ -459,6 +460,13 @@ class CanonicalMcBatch {
     R_ = configs.size();
   }
   void Run(uint64_t trials, void *stream = nullptr) { lmc::Check(lmc_cmc_run(c_.get(), trials, stream)); }
+  // SimulatedAnnealing's adaptive schedule from t_start (see lmc_cmc_set_annealing).
+  void SetAnnealing(bool on) { lmc::Check(lmc_cmc_set_annealing(c_.get(), on ? 1 : 0)); }
+  [[nodiscard]] std::vector<double> GetAnnealingTemperature() const {
+    std::vector<double> T(R_);
+    lmc::Check(lmc_cmc_annealing_state(c_.get(), T.data(), nullptr, nullptr));
+    return T;
+  }
   [[nodiscard]] std::vector<double> GetEnergy() const {
     std::vector<double> e(R_);
     lmc::Check(lmc_cmc_state(c_.get(), nullptr, e.data(), nullptr, nullptr));
@@ -476,6 +484,19 @@ class CanonicalMcBatch {
   };
   std::unique_ptr<lmc_cmc, Free> c_;
   size_t R_ = 0;
+};
+
+// SimulatedAnnealing (SimulatedAnnealing.h) for R independent replicas: swap trials with the
+// reference's adaptive schedule from params[r].t_start over params[r].maximum_steps trials.
+class SimulatedAnnealingBatch : public CanonicalMcBatch {
+ public:
+  SimulatedAnnealingBatch(const std::vector<cfg::Config> &configs,
+                          const std::vector<lmc_cmc_replica> &params,
+                          const pred::EnergyChangePredictorPairSite &pairsite,
+                          const pred::EnergyPredictor *energy)
+      : CanonicalMcBatch(configs, params, pairsite, energy) {
+    SetAnnealing(true);
+  }
 };
 
 }  // namespace mc

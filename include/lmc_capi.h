@@ -240,6 +240,16 @@ int lmc_cmc_create(const lmc_model *pairsite, const lmc_model *energy, int32_t n
                    lmc_cmc **out);
 void lmc_cmc_free(lmc_cmc *c);
 int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity);
+/* Simulated annealing, replacing SimulatedAnnealing (lmc/mc/include/SimulatedAnnealing.h,
+ * SimulatedAnnealing.cpp:99-185): the same swap trials with the reference's adaptive schedule
+ * instead of the ramp, from T = t_start: cooling by 0.99 after each window of N/1000 trials
+ * whose acceptance exceeds 0.5, up to 5 reheats by 1.10 after N/20 trials without improvement
+ * when the acceptance falls below 0.05, and the baseline cooling exp(-3/N) per trial
+ * (N = maximum_steps). Energies are relative to the starting configuration (the reference
+ * subtracts chemical potentials). enable = 0 returns to the ramp. Synchronous. */
+int lmc_cmc_set_annealing(lmc_cmc *c, int32_t enable);
+/* Per replica: current temperature, lowest energy at a trial start, reheats done. */
+int lmc_cmc_annealing_state(lmc_cmc *c, double *temperature, double *lowest, uint32_t *reheats);
 int lmc_cmc_run(lmc_cmc *c, uint64_t trials, void *stream);
 int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energy,
                   uint64_t *accepted);

@@ -5,7 +5,7 @@ include/lmc_capi.h); this package is the Python mirror of the reference's predic
 interface over that ABI.
 """
 from .api import (E0, ENERGY, PAIRSITE, QUARTIC, CanonicalMc, KineticMcChain, KineticMcFirst, Model,  # noqa: F401,E501
-                  ReplicaBatch, Surrogate, augment, eval_jumps, fcc_neighbors, fcc_num_sites,
+                  ReplicaBatch, SimulatedAnnealing, Surrogate, augment, eval_jumps, fcc_neighbors, fcc_num_sites,
                   jump_environment, jump_orbit_clusters, jump_samples, jump_state_clusters, jump_symmetries,
                   site_de, swap_de, total_energy, vacancy_scan)
 from . import ansys  # noqa: F401,E402

@@ -424,6 +424,25 @@ class CanonicalMc:
                     acc=acc.reshape(shape))
 
 
+class SimulatedAnnealing(CanonicalMc):
+    """Batched SimulatedAnnealing (SimulatedAnnealing.cpp:99-185): swap trials with the
+    reference's adaptive schedule from T = t_start over maximum_steps trials; energies relative to
+    the starting configuration. Arguments as CanonicalMc without t_end."""
+
+    def __init__(self, pairsite_model, energy_model, ncell, occ_list, t_start, seed, maximum_steps,
+                 **kwargs):
+        super().__init__(pairsite_model, energy_model, ncell, occ_list, t_start, t_start, seed,
+                         maximum_steps, **kwargs)
+        check(lib().lmc_cmc_set_annealing(self.h, 1))
+
+    def annealing_state(self):
+        R = self.R
+        T, low, rh = np.zeros(R), np.zeros(R), np.zeros(R, np.uint32)
+        check(lib().lmc_cmc_annealing_state(self.h, ptr(T, capi._f64p), ptr(low, capi._f64p),
+                                            ptr(rh, C.POINTER(C.c_uint32))))
+        return dict(temperature=T, lowest=low, reheats=rh)
+
+
 def jump_samples(model, batch, vac, stream=None):
     """Exact training samples of the 12 jumps per replica (event order): environment rows
     (uint8 [12R, 64]), E_kra and dE (float64 [12R])."""

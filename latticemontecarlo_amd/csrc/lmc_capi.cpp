@@ -593,6 +593,7 @@ struct lmc_cmc {
   lmc::CmcDev d;
   uint64_t trace_pos = 0;
   ~lmc_cmc() {
+    DevFree(d.anneal);
     DevFree(d.occ);
     DevFree(d.off);
     DevFree(d.ncell);
@@ -1416,6 +1417,47 @@ int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity) {
       c->d.tr_de = DevAlloc<double>(n);
       c->d.tr_acc = DevAlloc<int8_t>(n);
       Check(hipMemset(c->d.tr_p, 0xff, n * sizeof(int64_t)), "hipMemset");
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_set_annealing(lmc_cmc *c, int32_t enable) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    DevFree(c->d.anneal);
+    c->d.anneal = nullptr;
+    if (!enable) return LMC_OK;
+    const size_t R = static_cast<size_t>(c->d.R);
+    std::vector<double> t0(R), en(R);
+    std::vector<uint64_t> st(R);
+    Check(hipMemcpy(t0.data(), c->d.t0, R * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    Check(hipMemcpy(en.data(), c->d.energy, R * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    Check(hipMemcpy(st.data(), c->d.steps, R * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    std::vector<lmc::CmcAnneal> a(R);
+    for (size_t r = 0; r < R; ++r) {  // SimulatedAnnealing constructor (SimulatedAnnealing.cpp:66-73)
+      if (!(t0[r] > 0)) throw std::invalid_argument("annealing needs a positive t_start");
+      a[r] = lmc::CmcAnneal{t0[r], en[r], en[r], st[r], 0, 0, 0, 0, 0};
+    }
+    c->d.anneal = DevCopy(a);
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_annealing_state(lmc_cmc *c, double *temperature, double *lowest, uint32_t *reheats) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    if (!c->d.anneal) throw std::invalid_argument("annealing is off");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t R = static_cast<size_t>(c->d.R);
+    std::vector<lmc::CmcAnneal> a(R);
+    Check(hipMemcpy(a.data(), c->d.anneal, R * sizeof(lmc::CmcAnneal), hipMemcpyDeviceToHost),
+          "hipMemcpy");
+    for (size_t r = 0; r < R; ++r) {
+      if (temperature) temperature[r] = a[r].T;
+      if (lowest) lowest[r] = a[r].lowest;
+      if (reheats) reheats[r] = a[r].reheats;
     }
     return LMC_OK;
   });

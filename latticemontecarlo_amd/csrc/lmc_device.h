@@ -108,8 +108,17 @@ struct KmcDev {
   double mass[kMaxSpecies] = {};  // element masses by species code
 };
 
+// SimulatedAnnealing's adaptive schedule state of one replica
+// (lmc/mc/include/SimulatedAnnealing.h:31-71).
+struct CmcAnneal {
+  double T, lowest, recent_best;
+  uint64_t last_improvement, last_reheat;
+  uint32_t window_trials, window_accepts, reheats, pad;
+};
+
 struct CmcDev {
   int32_t R = 0;
+  CmcAnneal *anneal = nullptr;  // [R]; null: the linear ramp t0 -> t1
   uint8_t *occ = nullptr;
   int64_t *off = nullptr;
   int32_t *ncell = nullptr;

@@ -151,6 +151,41 @@ __global__ __launch_bounds__(kThreads) void SiteDeKernel(KTables t, const uint8_
 
 // CanonicalMcSerial (CanonicalMcSerial.cpp:40-51, CanonicalMcAbstract.cpp:43-101) with an
 // optional linear temperature ramp t0 -> t1 over maximum_steps.
+// SimulatedAnnealing::UpdateTemperature (SimulatedAnnealing.cpp:99-133) after the trial of step
+// `steps`, with maximum_steps N and the energy after the trial.
+__device__ void AnnealUpdate(CmcAnneal &A, bool accepted, uint64_t steps, uint64_t N,
+                             double energy) {
+  const uint64_t window = max(1ull, static_cast<unsigned long long>(N * 0.001));
+  const uint64_t trigger = max(1ull, static_cast<unsigned long long>(N * 0.05));
+  const uint64_t cooldown = max(1ull, static_cast<unsigned long long>(N * 0.10));
+  ++A.window_trials;
+  if (accepted) {
+    ++A.window_accepts;
+    if (energy < A.recent_best - 1e-4) {
+      A.recent_best = energy;
+      A.last_improvement = steps;
+    }
+  }
+  if (A.window_trials >= window) {
+    const double acc = static_cast<double>(A.window_accepts) / static_cast<double>(A.window_trials);
+    if (acc > 0.50) A.T *= 0.99;
+    A.window_trials = 0;
+    A.window_accepts = 0;
+  }
+  const double acc_est = A.window_trials > 0u ? static_cast<double>(A.window_accepts) /
+                                                     static_cast<double>(A.window_trials)
+                                               : 1.0;
+  if (A.reheats < 5 && steps - A.last_improvement >= trigger &&
+      steps - A.last_reheat >= cooldown && acc_est < 0.05) {
+    A.T *= 1.10;
+    A.last_improvement = steps;
+    A.last_reheat = steps;
+    A.recent_best = energy;
+    ++A.reheats;
+  }
+  A.T *= exp(-3.0 / static_cast<double>(max(1ull, static_cast<unsigned long long>(N))));
+}
+
 __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint64_t ntrials,
                                                       uint64_t trace_base) {
   __shared__ SiteLds L;
@@ -164,6 +199,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
   const uint64_t N = 4ull * n * n * n;
   uint64_t steps = 0, maxs = 0, acc = 0;
   double energy = 0, abse = 0;
+  CmcAnneal A{};
   if (tid == 0) {
     steps = c.steps[r];
     maxs = c.max_steps[r];
@@ -171,6 +207,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     energy = c.energy[r];
     abse = c.abs_energy[r];
     s_done = steps > maxs ? 1 : 0;
+    if (c.anneal) A = c.anneal[r];
   }
   __syncthreads();
   for (uint64_t it = 0; it < ntrials; ++it) {
@@ -197,9 +234,15 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     const int64_t p = s_p, q = s_q;
     const double de = SwapDeBlock(t, L, occ, n, p, q);
     if (tid == 0) {
-      const double T = c.t0[r] + (c.t1[r] - c.t0[r]) *
-                                     (static_cast<double>(steps) / static_cast<double>(maxs));
-      const double beta = 1.0 / kBoltzmann / T;
+      double beta;
+      if (c.anneal) {
+        if (energy < A.lowest - 1e-4) A.lowest = energy;  // Dump (SimulatedAnnealing.cpp:80-97)
+        beta = 1.0 / kBoltzmann / fmax(A.T, 1e-12);
+      } else {
+        const double T = c.t0[r] + (c.t1[r] - c.t0[r]) *
+                                       (static_cast<double>(steps) / static_cast<double>(maxs));
+        beta = 1.0 / kBoltzmann / T;
+      }
       bool accept = de < 0;
       if (!accept) {
         const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 0xFFFFFFFEu);
@@ -221,6 +264,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
         abse += de;
         ++acc;
       }
+      if (c.anneal) AnnealUpdate(A, accept, steps, maxs, energy);
       ++steps;
       if (steps > maxs) s_done = 1;
     }
@@ -231,6 +275,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     c.accepted[r] = acc;
     c.energy[r] = energy;
     c.abs_energy[r] = abse;
+    if (c.anneal) c.anneal[r] = A;
   }
 }
 

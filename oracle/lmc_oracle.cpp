@@ -1283,6 +1283,44 @@ struct Cmc {
   double t0, t1, energy, abs_energy;
   uint64_t steps, seed, accepted = 0;
   uint32_t replica;
+  // SimulatedAnnealing schedule (lmc/mc/include/SimulatedAnnealing.h:31-71)
+  bool anneal = false;
+  double T = 0, lowest = 0, recent_best = 0;
+  uint64_t last_improvement = 0, last_reheat = 0;
+  uint32_t window_trials = 0, window_accepts = 0, reheats = 0;
+  // SimulatedAnnealing::UpdateTemperature (SimulatedAnnealing.cpp:99-133), after the trial of
+  // step `steps` with maximum_steps N
+  void UpdateTemperature(bool accepted_trial, uint64_t N) {
+    const uint64_t window = std::max<uint64_t>(1, static_cast<uint64_t>(N * 0.001));
+    const uint64_t trigger = std::max<uint64_t>(1, static_cast<uint64_t>(N * 0.05));
+    const uint64_t cooldown = std::max<uint64_t>(1, static_cast<uint64_t>(N * 0.10));
+    ++window_trials;
+    if (accepted_trial) {
+      ++window_accepts;
+      if (energy < recent_best - 1e-4) {
+        recent_best = energy;
+        last_improvement = steps;
+      }
+    }
+    if (window_trials >= window) {
+      const double acc = static_cast<double>(window_accepts) / static_cast<double>(window_trials);
+      if (acc > 0.50) T *= 0.99;
+      window_trials = 0;
+      window_accepts = 0;
+    }
+    const double acc_est = window_trials > 0u
+                               ? static_cast<double>(window_accepts) / static_cast<double>(window_trials)
+                               : 1.0;
+    if (reheats < 5 && steps - last_improvement >= trigger && steps - last_reheat >= cooldown &&
+        acc_est < 0.05) {
+      T *= 1.10;
+      last_improvement = steps;
+      last_reheat = steps;
+      recent_best = energy;
+      ++reheats;
+    }
+    T *= std::exp(-3.0 / static_cast<double>(std::max<uint64_t>(1, N)));
+  }
 };
 
 }  // namespace orc
@@ -1554,6 +1592,10 @@ void *orc_cmc_new(const void *pm, const void *em, const void *lat, const uint8_t
     c->steps = p->restart_steps;
     c->energy = p->restart_energy;
     c->abs_energy = 0;
+    c->anneal = p->anneal != 0;
+    c->T = p->t_start;
+    c->lowest = c->recent_best = c->energy;
+    c->last_improvement = c->steps;
     if (em) {
       const Model &m = *static_cast<const Model *>(em);
       c->abs_energy = DotSeq(m.theta, TotalEncode(m, occ));
@@ -1565,6 +1607,12 @@ void *orc_cmc_new(const void *pm, const void *em, const void *lat, const uint8_t
   }
 }
 void orc_cmc_free(void *c) { delete static_cast<Cmc *>(c); }
+void orc_cmc_anneal_state(const void *cv, double *temperature, double *lowest, uint32_t *reheats) {
+  const Cmc &c = *static_cast<const Cmc *>(cv);
+  *temperature = c.T;
+  *lowest = c.lowest;
+  *reheats = c.reheats;
+}
 int orc_cmc_run(void *cv, uint64_t trials, uint64_t maximum_steps, int32_t *tr_p, int32_t *tr_q,
                 double *tr_de, int8_t *tr_acc) {
   Cmc &c = *static_cast<Cmc *>(cv);
@@ -1583,9 +1631,15 @@ int orc_cmc_run(void *cv, uint64_t trials, uint64_t maximum_steps, int32_t *tr_p
         if (c.occ[p] != c.occ[q]) break;
       }
       const double de = SwapDe(m, c.occ.data(), p, q);
-      const double T = c.t0 + (c.t1 - c.t0) * (static_cast<double>(c.steps) /
-                                               static_cast<double>(maximum_steps));
-      const double beta = 1.0 / kBoltzmann / T;
+      double beta;
+      if (c.anneal) {
+        if (c.energy < c.lowest - 1e-4) c.lowest = c.energy;  // Dump (SimulatedAnnealing.cpp:80-97)
+        beta = 1.0 / kBoltzmann / std::max(c.T, 1e-12);
+      } else {
+        const double T = c.t0 + (c.t1 - c.t0) * (static_cast<double>(c.steps) /
+                                                 static_cast<double>(maximum_steps));
+        beta = 1.0 / kBoltzmann / T;
+      }
       bool acc = false;  // SelectEvent (CanonicalMcAbstract.cpp:86-101)
       if (de < 0) {
         acc = true;
@@ -1604,6 +1658,7 @@ int orc_cmc_run(void *cv, uint64_t trials, uint64_t maximum_steps, int32_t *tr_p
       if (tr_q) tr_q[i] = static_cast<int32_t>(q);
       if (tr_de) tr_de[i] = de;
       if (tr_acc) tr_acc[i] = acc ? 1 : 0;
+      if (c.anneal) c.UpdateTemperature(acc, maximum_steps);
       ++c.steps;
     }
     return 0;

@@ -115,9 +115,12 @@ typedef struct {
   uint32_t replica;
   uint64_t restart_steps;
   double restart_energy;
+  int anneal;  // 1: SimulatedAnnealing's adaptive schedule from T = t_start (t_end unused)
 } orc_cmc_params;
 void *orc_cmc_new(const void *pairsite_model, const void *energy_model, const void *lat,
                   const uint8_t *occ, const orc_cmc_params *p, char *err, int errlen);
+// Annealing state: current temperature, lowest energy seen at a step start, reheats done.
+void orc_cmc_anneal_state(const void *cmc, double *temperature, double *lowest, uint32_t *reheats);
 void orc_cmc_free(void *cmc);
 // Optional per-trial trace: site pair, dE, accepted flag.
 int orc_cmc_run(void *cmc, uint64_t trials, uint64_t maximum_steps, int32_t *tr_p, int32_t *tr_q,

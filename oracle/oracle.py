@@ -32,7 +32,7 @@ class KmcParams(C.Structure):
 class CmcParams(C.Structure):
     _fields_ = [("t_start", C.c_double), ("t_end", C.c_double), ("seed", C.c_uint64),
                 ("replica", C.c_uint32), ("restart_steps", C.c_uint64),
-                ("restart_energy", C.c_double)]
+                ("restart_energy", C.c_double), ("anneal", C.c_int)]
 
 
 def build():
@@ -81,6 +81,7 @@ def lib():
             "orc_kmc_step_time": (C.c_double, [vp]),
             "orc_kmc_lru_hits": (C.c_uint64, [vp]),
             "orc_cmc_new": (vp, [vp, vp, vp, _u8p, C.POINTER(CmcParams), C.c_char_p, C.c_int]),
+            "orc_cmc_anneal_state": (None, [vp, _f64p, _f64p, C.POINTER(C.c_uint32)]),
             "orc_cmc_free": (None, [vp]),
             "orc_cmc_run": (C.c_int, [vp, C.c_uint64, C.c_uint64, _i32p, _i32p, _f64p,
                                       C.POINTER(C.c_int8)]),
@@ -305,11 +306,12 @@ class Kmc:
 
 
 class Cmc:
-    """CanonicalMcSerial trajectory of one replica."""
+    """CanonicalMcSerial trajectory of one replica (anneal=True: SimulatedAnnealing's adaptive
+    schedule from T = t_start)."""
 
     def __init__(self, pairsite_model, energy_model, lattice, occ, t_start, t_end, seed,
-                 replica=0, restart_steps=0, restart_energy=0.0):
-        p = CmcParams(t_start, t_end, seed, replica, restart_steps, restart_energy)
+                 replica=0, restart_steps=0, restart_energy=0.0, anneal=False):
+        p = CmcParams(t_start, t_end, seed, replica, restart_steps, restart_energy, int(anneal))
         err = C.create_string_buffer(512)
         occ = np.ascontiguousarray(occ, np.uint8)
         self.h = lib().orc_cmc_new(pairsite_model.h, energy_model.h if energy_model else None,
@@ -334,6 +336,11 @@ class Cmc:
         if rc:
             raise RuntimeError("oracle CMC failed")
         return tr
+
+    def anneal_state(self):
+        T, low, rh = C.c_double(), C.c_double(), C.c_uint32()
+        lib().orc_cmc_anneal_state(self.h, C.byref(T), C.byref(low), C.byref(rh))
+        return dict(temperature=T.value, lowest=low.value, reheats=rh.value)
 
     def state(self):
         st, e, ae, acc = C.c_uint64(), C.c_double(), C.c_double(), C.c_uint64()

@@ -237,6 +237,40 @@ def test_cmc_trajectory_matches_oracle(torch, coeffs):
         assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * trials
 
 
+def test_annealing_matches_oracle(torch, coeffs):
+    """SimulatedAnnealing's adaptive schedule (SimulatedAnnealing.cpp:99-185): trials, acceptance,
+    temperature, lowest energy and reheats against the oracle, over two launches."""
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs["e0_3n"]
+    n, R, N = 5, 3, 4000
+    gp = L.Model(path, els, L.PAIRSITE)
+    occs = [random_alloc(n, gp.species, 0.08, seed=950 + r, vacancy=False) for r in range(R)]
+    seed, T0 = 77, 2500.0
+    print(f"annealing n={n} R={R} N={N}", flush=True)
+    drv = L.SimulatedAnnealing(gp, None, [n] * R, occs, T0, seed, maximum_steps=N)
+    drv.set_trace(N)
+    drv.run(N // 2)
+    drv.run(N // 2)
+    gst, gtr, gocc, ga = drv.state(), drv.trace(), drv.occupancy(), drv.annealing_state()
+    lat = O.Lattice(n)
+    op = O.Model(lat, path, els, O.PAIRSITE)
+    for r in range(R):
+        c = O.Cmc(op, None, lat, occs[r], T0, T0, seed, replica=r, anneal=True)
+        tr = c.run(N, N, trace=True)
+        st, oa = c.state(), c.anneal_state()
+        assert np.array_equal(gtr["p"][r], tr["p"]) and np.array_equal(gtr["q"][r], tr["q"])
+        assert np.array_equal(gtr["acc"][r], tr["acc"])
+        np.testing.assert_allclose(gtr["de"][r], tr["de"], rtol=0, atol=TOL_E)
+        assert np.array_equal(gocc[r], c.occupancy())
+        assert int(gst["accepted"][r]) == st["accepted"]
+        assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * N
+        assert ga["temperature"][r] == pytest.approx(oa["temperature"], rel=1e-12)
+        assert abs(ga["lowest"][r] - oa["lowest"]) <= TOL_E * N
+        assert int(ga["reheats"][r]) == oa["reheats"]
+    assert ga["temperature"].max() < T0
+
+
 @pytest.mark.parametrize("form", [0, 1, 2, 3])
 def test_kmc_graph_replay_matches_eager(torch, coeffs, form):
     """Launches replayed from a captured HIP graph do exactly what the same launches do eagerly."""

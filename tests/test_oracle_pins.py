@@ -387,3 +387,37 @@ def test_oracle_chain_kmc_invariants(coeffs):
     assert np.all(np.diff(np.concatenate([[0.0], tr["time"]])) > 0)
     assert abs(st["energy"] - tr["de"].sum()) < 1e-9
     assert abs(st["abs_energy"] - em.total_energy(occ2)) < 1e-9
+
+
+def _anneal_run(coeffs, T0, max_steps, trials, seed):
+    path, els = coeffs["e0_3"]
+    n = 5
+    lat = O.Lattice(n)
+    pm = O.Model(lat, path, els, O.PAIRSITE)
+    species = sorted(set(els) | {"X"})
+    occ = random_alloc(n, species, 0.1, seed=seed, vacancy=False)
+    c = O.Cmc(pm, None, lat, occ, T0, T0, seed=seed, anneal=True)
+    tr = c.run(trials, maximum_steps=max_steps, trace=True)
+    return c.anneal_state(), tr
+
+
+def test_oracle_anneal_baseline_cooling(coeffs):
+    """Windows and reheat triggers out of reach: T = T0 exp(-3 k / N) after k trials
+    (SimulatedAnnealing.cpp:126-128)."""
+    N, k, T0 = 10 ** 9, 300, 900.0
+    st, _ = _anneal_run(coeffs, T0, N, k, seed=3)
+    assert st["temperature"] == pytest.approx(T0 * np.exp(-3 * k / N), rel=1e-12)
+    assert st["reheats"] == 0
+
+
+def test_oracle_anneal_window_cooling(coeffs):
+    """Windows of N/1000 trials: each window with an acceptance ratio above 0.5 cools T by 0.99
+    (SimulatedAnnealing.cpp:110-120), counted here from the acceptance trace; fewer trials than
+    the reheat trigger (N/20)."""
+    N, k, T0 = 10000, 400, 5000.0
+    st, tr = _anneal_run(coeffs, T0, N, k, seed=4)
+    acc = tr["acc"].astype(int).reshape(-1, 10)
+    m = int((acc.sum(1) / 10 > 0.5).sum())
+    assert 0 < m < acc.shape[0]
+    assert st["temperature"] == pytest.approx(T0 * np.exp(-3 * k / N) * 0.99 ** m, rel=1e-12)
+    assert st["reheats"] == 0
