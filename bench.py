@@ -321,6 +321,70 @@ def chain_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def anneal_workload(args):
+    """Simulated annealing (SimulatedAnnealing.cpp:99-185): 64 replicas of 30^3 cells (108000
+    sites, Al/Mg/Zn 90/5/5) from T = 800 K over --cmc-trials trials each, pair-site swap dE.
+    CPU baseline: the oracle's annealing trials on the host cores. Prints one JSON line."""
+    import torch
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import synth
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0.json")
+    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "E0", seed=args.seed)
+    ps = L.Model(coeff, ["Al", "Mg", "Zn"], L.PAIRSITE)
+    species = ["Al", "Mg", "X", "Zn"]
+    R, n, trials = 64, 30, args.cmc_trials
+    N = 4 * n ** 3
+    occs = [synth.random_occupancy(n, species, {"Mg": round(0.05 * N), "Zn": round(0.05 * N)},
+                                   seed=args.seed * 13 + r, vacancy=False) for r in range(R)]
+    sa = L.SimulatedAnnealing(ps, None, [n] * R, occs, 800.0, args.seed, maximum_steps=trials)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    chunk = max(1, trials // 20)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(max(1, trials // chunk)):
+        sa.run(chunk, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3
+    st, an = sa.state(), sa.annealing_state()
+    ps.check_errors()
+    done = int(st["steps"].sum())
+    value = done / secs
+    log(f"annealing: {value:.4g} trials/s over {R} replicas ({secs:.2f} s), final T "
+        f"{an['temperature'].min():.1f}-{an['temperature'].max():.1f} K, reheats {an['reheats'].sum()}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        lat = O.Lattice(n)
+        om = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.PAIRSITE)
+        c = O.Cmc(om, None, lat, occs[0], 800.0, 800.0, args.seed, anneal=True)
+        k = 20000
+        t0 = time.perf_counter()
+        c.run(k, trials)
+        one = k / (time.perf_counter() - t0)
+        cores = min(16, os.cpu_count() or 1)
+        cpu = {"value": round(one * cores, 1), "unit": "trials/s", "cores": cores, "kind": "port",
+               "sample": f"oracle annealing trials, one replica of 30^3 cells, {k} trials "
+                         f"single-threaded ({one:.1f} trials/s) times {cores} cores"}
+        log(f"cpu baseline (annealing): {one:.1f} trials/s per core")
+    line = {"metric": "simulated-annealing trials/s over all replicas", "value": round(value, 1),
+            "unit": "trials/s", "n_gpus": 1, "steps": trials, "warmup": 0,
+            "ms_per_step": round(secs / trials * 1e3, 6), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"simulated annealing, {R} replicas x {n}^3 FCC cells "
+                                   f"({N} sites), Al/Mg/Zn 90/5/5, T0 800 K, pair-site swap dE",
+                       "final_temperature_K": [round(float(an["temperature"].min()), 2),
+                                               round(float(an["temperature"].max()), 2)],
+                       "reheats": int(an["reheats"].sum()),
+                       "acceptance": round(float(st["accepted"].sum()) / max(1, done), 4)},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def uneven_workload(args):
     """Uneven-size batched KMC alongside canonical MC: 256 KMC replicas of n^3 cells with n drawn
     from 8..40 (2048-256000 sites; Mg, Zn 1-5 at.% each, T 400-700 K) on one stream and 64
@@ -404,7 +468,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch eagerly instead of a HIP graph")
-    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain", "anneal"], default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
     args = ap.parse_args()
@@ -427,6 +491,9 @@ def main():
         return
     if args.workload == "chain":
         chain_workload(args)
+        return
+    if args.workload == "anneal":
+        anneal_workload(args)
         return
     R, n = args.replicas, args.ncell
     tmp = tempfile.mkdtemp()
