@@ -460,6 +460,8 @@ class CanonicalMcBatch {
     R_ = configs.size();
   }
   void Run(uint64_t trials, void *stream = nullptr) { lmc::Check(lmc_cmc_run(c_.get(), trials, stream)); }
+  // Occupancy in LDS during runs where it fits (default), or in HBM; results are identical.
+  void SetLdsOccupancy(bool on) { lmc::Check(lmc_cmc_set_lds_occupancy(c_.get(), on ? 1 : 0)); }
   // SimulatedAnnealing's adaptive schedule from t_start (see lmc_cmc_set_annealing).
   void SetAnnealing(bool on) { lmc::Check(lmc_cmc_set_annealing(c_.get(), on ? 1 : 0)); }
   [[nodiscard]] std::vector<double> GetAnnealingTemperature() const {

@@ -250,6 +250,11 @@ int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity);
 int lmc_cmc_set_annealing(lmc_cmc *c, int32_t enable);
 /* Per replica: current temperature, lowest energy at a trial start, reheats done. */
 int lmc_cmc_annealing_state(lmc_cmc *c, double *temperature, double *lowest, uint32_t *reheats);
+/* No reference counterpart (a placement choice): enable != 0, the default, holds each
+ * replica's occupancy in LDS while a run works on it, when the largest replica fits beside the
+ * kernel's tables (about 150 000 sites in a 160 KB workgroup); 0 keeps it in HBM and leaves the
+ * CU's LDS to kernels that run beside it. Results are the same either way. */
+int lmc_cmc_set_lds_occupancy(lmc_cmc *c, int32_t enable);
 int lmc_cmc_run(lmc_cmc *c, uint64_t trials, void *stream);
 int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energy,
                   uint64_t *accepted);

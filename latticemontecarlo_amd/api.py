@@ -393,6 +393,11 @@ class CanonicalMc:
         check(lib().lmc_cmc_set_trace(self.h, capacity))
         self.trace_cap = capacity
 
+    def set_lds_occupancy(self, enable):
+        """Hold each replica's occupancy in LDS during runs where it fits (default True), or in
+        HBM (leaves the CU's LDS to kernels running beside it). Results are identical."""
+        check(lib().lmc_cmc_set_lds_occupancy(self.h, 1 if enable else 0))
+
     def run(self, trials, stream=None):
         s = capi.stream_ptr(stream) if stream is not None else None
         check(lib().lmc_cmc_run(self.h, trials, s))

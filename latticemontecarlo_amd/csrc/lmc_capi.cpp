@@ -1352,6 +1352,8 @@ int lmc_cmc_create(const lmc_model *pairsite, const lmc_model *energy, int32_t R
     auto c = std::make_unique<lmc_cmc>();
     c->pm = pairsite;
     c->d.R = R;
+    for (int r = 0; r < R; ++r)
+      c->d.max_sites = std::max<int64_t>(c->d.max_sites, 4ll * ncell[r] * ncell[r] * ncell[r]);
     std::vector<uint64_t> steps(R), maxs(R), seed(R);
     std::vector<uint32_t> rid(R);
     std::vector<double> t0(R), t1(R), en(R);
@@ -1418,6 +1420,14 @@ int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity) {
       c->d.tr_acc = DevAlloc<int8_t>(n);
       Check(hipMemset(c->d.tr_p, 0xff, n * sizeof(int64_t)), "hipMemset");
     }
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_set_lds_occupancy(lmc_cmc *c, int32_t enable) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    c->d.lds_occupancy = enable ? 1 : 0;
     return LMC_OK;
   });
 }

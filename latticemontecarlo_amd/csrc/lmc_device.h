@@ -118,6 +118,8 @@ struct CmcAnneal {
 
 struct CmcDev {
   int32_t R = 0;
+  int64_t max_sites = 0;   // largest replica, in sites
+  int lds_occupancy = 1;   // hold each replica's occupancy in LDS while it runs, where it fits
   CmcAnneal *anneal = nullptr;  // [R]; null: the linear ramp t0 -> t1
   uint8_t *occ = nullptr;
   int64_t *off = nullptr;

@@ -551,29 +551,6 @@ __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const
 constexpr int kWideThreads = 12 * 64;
 static_assert(kClusterPadWide == 4 * 64, "wide lists are padded to whole batches of 64 lanes");
 
-// v of lane l - kShift within the lane's row of 16 (0 past the row start): DPP row_shr on both
-// halves of the double.
-template <int kShift>
-__device__ __forceinline__ double RowShr(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + kShift, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + kShift, 0xF, 0xF, true);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double ReadLane(double v, int lane) {
-  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                          __builtin_amdgcn_readlane(__double2loint(v), lane));
-}
-
-// Sum over the 64 lanes of a full wave, returned to every lane: shifted adds within each row of
-// 16 leave the row sum in its last lane; the four row sums are then added in order.
-__device__ __forceinline__ double WaveSum(double v) {
-  v += RowShr<1>(v);
-  v += RowShr<2>(v);
-  v += RowShr<4>(v);
-  v += RowShr<8>(v);
-  return (ReadLane(v, 15) + ReadLane(v, 31)) + (ReadLane(v, 47) + ReadLane(v, 63));
-}
-
 __device__ void WideEvent(const KTables &t, JumpLds &L, const Tabs &T, double beta, int32_t *err,
                           int tid) {
   const int k = tid >> 6, lane = tid & 63;

@@ -15,7 +15,8 @@ struct SiteLds {
   int32_t off[kSiteSlots];
   double w[kMaxW];
   uint8_t env[2][48];
-  double red[kThreads];
+  int slot[2];                 // partner slots of a swap (q around p, p around q)
+  double part[kThreads / 64];  // per-wave partial sums
 };
 
 __device__ void LoadSiteTables(const KTables &t, SiteLds &L) {
@@ -33,7 +34,7 @@ __device__ __forceinline__ double SiteWeight(const KTables &t, const SiteLds &L,
 }
 
 // Sum over clusters containing site x of w(x -> en) - w(x -> eo), skipping clusters that
-// contain env slot `skip` (the partner of a coupled swap). Result in lane 0.
+// contain env slot `skip` (the partner of a coupled swap): this thread's share.
 __device__ double SitePartial(const KTables &t, const SiteLds &L, int which, int eo, int en,
                               int skip) {
   const uint8_t *e = L.env[which];
@@ -47,63 +48,68 @@ __device__ double SitePartial(const KTables &t, const SiteLds &L, int which, int
   return acc;
 }
 
-__device__ double BlockSum(SiteLds &L, double v) {  // fixed-order tree reduction
-  L.red[threadIdx.x] = v;
+// Fixed-order block sum, valid in thread 0: DPP sums within each wave, then the wave sums in
+// wave order. One barrier; L.part is rewritten only after the caller's next barrier.
+__device__ double BlockSum(SiteLds &L, double v) {
+  v = WaveSum(v);
+  if ((threadIdx.x & 63) == 0) L.part[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (static_cast<int>(threadIdx.x) < s) L.red[threadIdx.x] += L.red[threadIdx.x + s];
-    __syncthreads();
-  }
-  const double r = L.red[0];
-  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; ++w) r += L.part[w];
   return r;
 }
 
-__device__ void GatherSite(const SiteLds &L, uint8_t *dst, const uint8_t *occ, int n, int64_t x,
-                           int S) {
-  const XYZ c = ToXYZ(x, n);
-  for (int i = threadIdx.x; i < kSiteSlots; i += blockDim.x) {
+// Species in the 43-site environment of x (slot i read by lane i of the calling wave).
+__device__ void GatherSite(const SiteLds &L, uint8_t *dst, const uint8_t *occ, int n,
+                           const XYZ &c, int S, int lane) {
+  if (lane < kSiteSlots) {
     int dx, dy, dz;
-    Unpack(L.off[i], dx, dy, dz);
+    Unpack(L.off[lane], dx, dy, dz);
     const uint8_t v = occ[SiteAt(c, n, dx, dy, dz)];
-    dst[i] = v < S ? v : 0;  // codes are validated by the callers; keep LDS indices in range
+    dst[lane] = v < S ? v : 0;  // codes are validated by the callers; keep LDS indices in range
   }
 }
 
-// Slot of q in the 43-site environment of p, or -1 when q is beyond the third shell.
-__device__ int PartnerSlot(const SiteLds &L, int n, int64_t p, int64_t q) {
-  const XYZ a = ToXYZ(p, n), b = ToXYZ(q, n);
+// Periodic minimum image of a coordinate difference on the 2n axis, in [-n, n).
+__device__ __forceinline__ int MinImage(int v, int n) {  // |v| < 2n
   const int m = 2 * n;
-  auto mi = [m, n](int v) {
-    v = ((v % m) + m) % m;
-    return v >= n ? v - m : v;
-  };
-  const int dx = mi(b.X - a.X), dy = mi(b.Y - a.Y), dz = mi(b.Z - a.Z);
-  for (int i = 1; i < kSiteSlots; ++i) {
-    int x, y, z;
-    Unpack(L.off[i], x, y, z);
-    if (x == dx && y == dy && z == dz) return i;
-  }
-  return -1;
+  v += v < 0 ? m : 0;
+  return v >= n ? v - m : v;
 }
 
-// dE of swapping the species at p and q. Far pairs: dE(p) + dE(q) on the unswapped config
-// (...WithoutCoupling :136-146). Near pairs: clusters containing both keep their multiset and
-// are skipped (...WithCoupling :96-134). All threads call; result valid in every thread.
-__device__ double SwapDeBlock(const KTables &t, SiteLds &L, const uint8_t *occ, int n, int64_t p,
-                              int64_t q) {
-  __shared__ int s_qs, s_ps;
-  const int ep = occ[p], eq = occ[q];
-  if (ep == eq) return 0.0;
-  GatherSite(L, L.env[0], occ, n, p, t.S);
-  GatherSite(L, L.env[1], occ, n, q, t.S);
-  if (threadIdx.x == 0) {
-    s_qs = PartnerSlot(L, n, p, q);
-    s_ps = PartnerSlot(L, n, q, p);
+// Slot of b in the environment of a: the first slot whose offset is the minimum-image
+// difference b - a, or -1 beyond the third shell. Lane i of the calling wave tests slot i.
+__device__ int PartnerSlot(const SiteLds &L, int n, const XYZ &a, const XYZ &b, int lane) {
+  const int dx = MinImage(b.X - a.X, n), dy = MinImage(b.Y - a.Y, n), dz = MinImage(b.Z - a.Z, n);
+  bool hit = false;
+  if (lane >= 1 && lane < kSiteSlots) {
+    int x, y, z;
+    Unpack(L.off[lane], x, y, z);
+    hit = x == dx && y == dy && z == dz;
   }
-  __syncthreads();
-  const int qs = s_qs, ps = s_ps;
-  double v = SitePartial(t, L, 0, ep, eq, qs) + SitePartial(t, L, 1, eq, ep, ps);
+  const uint64_t m = __ballot(hit);
+  return m ? __builtin_ctzll(m) : -1;
+}
+
+// Environments and partner slots of a swap p <-> q: wave 0 gathers around p, wave 1 around q.
+// Results in L.env and L.slot after the caller's barrier.
+__device__ void GatherPair(SiteLds &L, const uint8_t *occ, int n, int64_t p, int64_t q, int S) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave > 1) return;
+  const XYZ a = ToXYZ(wave ? q : p, n), b = ToXYZ(wave ? p : q, n);
+  GatherSite(L, L.env[wave], occ, n, a, S, lane);
+  const int s = PartnerSlot(L, n, a, b, lane);
+  if (lane == 0) L.slot[wave] = s;
+}
+
+// dE of swapping species ep at p and eq at q, from the environments of GatherPair. Far pairs:
+// dE(p) + dE(q) on the unswapped config (...WithoutCoupling :136-146). Near pairs: clusters
+// containing both keep their multiset and are skipped (...WithCoupling :96-134). All threads
+// call; the result is valid in thread 0.
+__device__ double SwapDe(const KTables &t, SiteLds &L, int ep, int eq) {
+  const double v =
+      SitePartial(t, L, 0, ep, eq, L.slot[0]) + SitePartial(t, L, 1, eq, ep, L.slot[1]);
   return BlockSum(L, v);
 }
 
@@ -118,9 +124,16 @@ __global__ __launch_bounds__(kThreads) void SwapDeKernel(KTables t, const uint8_
     if (threadIdx.x == 0) de[r] = __builtin_nan("");
     return;
   }
+  const int ep = o[p], eq = o[q];
+  if (ep == eq) {
+    if (threadIdx.x == 0) de[r] = 0.0;
+    return;
+  }
   LoadSiteTables(t, L);
   __syncthreads();
-  const double v = SwapDeBlock(t, L, o, ncell[r], p, q);
+  GatherPair(L, o, ncell[r], p, q, t.S);
+  __syncthreads();
+  const double v = SwapDe(t, L, ep, eq);
   if (threadIdx.x == 0) de[r] = v;
 }
 
@@ -143,21 +156,31 @@ __global__ __launch_bounds__(kThreads) void SiteDeKernel(KTables t, const uint8_
     if (threadIdx.x == 0) de[r] = 0.0;
     return;
   }
-  GatherSite(L, L.env[0], o, ncell[r], sites[r], t.S);
+  if (threadIdx.x < 64)
+    GatherSite(L, L.env[0], o, ncell[r], ToXYZ(sites[r], ncell[r]), t.S, threadIdx.x);
   __syncthreads();
   const double v = BlockSum(L, SitePartial(t, L, 0, eo, en, -1));
   if (threadIdx.x == 0) de[r] = v;
 }
 
-// CanonicalMcSerial (CanonicalMcSerial.cpp:40-51, CanonicalMcAbstract.cpp:43-101) with an
-// optional linear temperature ramp t0 -> t1 over maximum_steps.
-// SimulatedAnnealing::UpdateTemperature (SimulatedAnnealing.cpp:99-133) after the trial of step
-// `steps`, with maximum_steps N and the energy after the trial.
-__device__ void AnnealUpdate(CmcAnneal &A, bool accepted, uint64_t steps, uint64_t N,
-                             double energy) {
-  const uint64_t window = max(1ull, static_cast<unsigned long long>(N * 0.001));
-  const uint64_t trigger = max(1ull, static_cast<unsigned long long>(N * 0.05));
-  const uint64_t cooldown = max(1ull, static_cast<unsigned long long>(N * 0.10));
+// SimulatedAnnealing::UpdateTemperature (SimulatedAnnealing.cpp:99-133) with maximum_steps N:
+// the window, trigger and cooldown lengths and the per-step decay factor exp(-3/N).
+struct AnnealSchedule {
+  uint64_t window, trigger, cooldown;
+  double decay;
+};
+__device__ AnnealSchedule MakeAnnealSchedule(uint64_t N) {
+  AnnealSchedule s;
+  s.window = max(1ull, static_cast<unsigned long long>(N * 0.001));
+  s.trigger = max(1ull, static_cast<unsigned long long>(N * 0.05));
+  s.cooldown = max(1ull, static_cast<unsigned long long>(N * 0.10));
+  s.decay = exp(-3.0 / static_cast<double>(max(1ull, static_cast<unsigned long long>(N))));
+  return s;
+}
+
+// The temperature update after the trial of step `steps` with the energy after the trial.
+__device__ void AnnealUpdate(CmcAnneal &A, const AnnealSchedule &s, bool accepted,
+                             uint64_t steps, double energy) {
   ++A.window_trials;
   if (accepted) {
     ++A.window_accepts;
@@ -166,7 +189,7 @@ __device__ void AnnealUpdate(CmcAnneal &A, bool accepted, uint64_t steps, uint64
       A.last_improvement = steps;
     }
   }
-  if (A.window_trials >= window) {
+  if (A.window_trials >= s.window) {
     const double acc = static_cast<double>(A.window_accepts) / static_cast<double>(A.window_trials);
     if (acc > 0.50) A.T *= 0.99;
     A.window_trials = 0;
@@ -175,64 +198,115 @@ __device__ void AnnealUpdate(CmcAnneal &A, bool accepted, uint64_t steps, uint64
   const double acc_est = A.window_trials > 0u ? static_cast<double>(A.window_accepts) /
                                                      static_cast<double>(A.window_trials)
                                                : 1.0;
-  if (A.reheats < 5 && steps - A.last_improvement >= trigger &&
-      steps - A.last_reheat >= cooldown && acc_est < 0.05) {
+  if (A.reheats < 5 && steps - A.last_improvement >= s.trigger &&
+      steps - A.last_reheat >= s.cooldown && acc_est < 0.05) {
     A.T *= 1.10;
     A.last_improvement = steps;
     A.last_reheat = steps;
     A.recent_best = energy;
     ++A.reheats;
   }
-  A.T *= exp(-3.0 / static_cast<double>(max(1ull, static_cast<unsigned long long>(N))));
+  A.T *= s.decay;
 }
 
+constexpr uint32_t kMaxPairAttempts = 1u << 20;
+
+__device__ __forceinline__ int64_t ReadLane64(int64_t v, int lane) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+  const uint32_t hi =
+      static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// The pair of step `steps`: the first attempt att = 0, 1, ... whose Philox draw (counter 2 att +
+// 1) puts different species on p and q (CanonicalMcAbstract.cpp:43-101 redraws until they
+// differ). Lane l of wave 0 tests attempt 64k + l in round k; the lowest hit of the first round
+// with one is the sequential answer. Returns false after kMaxPairAttempts attempts.
+__device__ bool DrawPair(const CmcDev &c, int r, uint64_t steps, const uint8_t *occ, uint64_t N,
+                         int lane, int64_t &p, int64_t &q) {
+  for (uint32_t base = 0; base < kMaxPairAttempts; base += 64) {
+    const uint32_t att = base + static_cast<uint32_t>(lane);
+    const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 2u * att + 1u);
+    const int64_t a = static_cast<int64_t>(MulHi64(Join64(u.x, u.y), N));
+    const int64_t b = static_cast<int64_t>(MulHi64(Join64(u.z, u.w), N));
+    const uint64_t m = __ballot(occ[a] != occ[b]);
+    if (m) {
+      const int w = __builtin_ctzll(m);
+      p = ReadLane64(a, w);
+      q = ReadLane64(b, w);
+      return true;
+    }
+  }
+  return false;
+}
+
+// CanonicalMcSerial (CanonicalMcSerial.cpp:40-51, CanonicalMcAbstract.cpp:43-101) with an
+// optional linear temperature ramp t0 -> t1 over maximum_steps, or the adaptive annealing
+// schedule. One workgroup per replica. kLdsOcc: the replica's occupancy (N bytes, N a multiple
+// of 4) lives in dynamic LDS for the launch and is written back at its end.
+// Per trial: wave 0 draws the pair while lane 0 of wave 1 draws the Metropolis uniform; waves 0
+// and 1 gather the two environments; all waves sum the clusters; thread 0 accepts and swaps.
+template <bool kLdsOcc>
 __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint64_t ntrials,
                                                       uint64_t trace_base) {
   __shared__ SiteLds L;
   __shared__ int64_t s_p, s_q;
   __shared__ int s_done;
+  __shared__ double s_u;
+  extern __shared__ uint32_t occ_words[];
   const int r = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   LoadSiteTables(t, L);
-  uint8_t *occ = c.occ + c.off[r];
   const int n = c.ncell[r];
   const uint64_t N = 4ull * n * n * n;
-  uint64_t steps = 0, maxs = 0, acc = 0;
+  uint8_t *occ = c.occ + c.off[r];
+  if constexpr (kLdsOcc) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(occ);
+    for (uint64_t i = tid; i < N / 4; i += kThreads) occ_words[i] = src[i];
+    occ = reinterpret_cast<uint8_t *>(occ_words);
+  }
+  uint64_t steps = c.steps[r];
+  const uint64_t maxs = c.max_steps[r];
+  uint64_t acc = 0;
   double energy = 0, abse = 0;
   CmcAnneal A{};
+  AnnealSchedule sched{};
   if (tid == 0) {
-    steps = c.steps[r];
-    maxs = c.max_steps[r];
     acc = c.accepted[r];
     energy = c.energy[r];
     abse = c.abs_energy[r];
     s_done = steps > maxs ? 1 : 0;
-    if (c.anneal) A = c.anneal[r];
+    if (c.anneal) {
+      A = c.anneal[r];
+      sched = MakeAnnealSchedule(maxs);
+    }
   }
   __syncthreads();
   for (uint64_t it = 0; it < ntrials; ++it) {
     if (s_done) break;
-    if (tid == 0) {
+    if (wave == 0) {
       int64_t p = 0, q = 0;
-      uint32_t att = 0;
-      for (;; ++att) {
-        if (att >= (1u << 20)) {
+      const bool ok = DrawPair(c, r, steps, occ, N, lane, p, q);
+      if (lane == 0) {
+        if (ok) {
+          s_p = p;
+          s_q = q;
+        } else {
           atomicOr(c.error, 2);
           s_done = 1;
-          break;
         }
-        const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 2u * att + 1u);
-        p = static_cast<int64_t>(MulHi64(Join64(u.x, u.y), N));
-        q = static_cast<int64_t>(MulHi64(Join64(u.z, u.w), N));
-        if (occ[p] != occ[q]) break;
       }
-      s_p = p;
-      s_q = q;
+    } else if (tid == 64) {
+      const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 0xFFFFFFFEu);
+      s_u = UnitOpenRight(Join64(u.x, u.y));
     }
     __syncthreads();
     if (s_done) break;
     const int64_t p = s_p, q = s_q;
-    const double de = SwapDeBlock(t, L, occ, n, p, q);
+    const int ep = occ[p], eq = occ[q];
+    GatherPair(L, occ, n, p, q, t.S);
+    __syncthreads();
+    const double de = SwapDe(t, L, ep, eq);
     if (tid == 0) {
       double beta;
       if (c.anneal) {
@@ -243,11 +317,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
                                        (static_cast<double>(steps) / static_cast<double>(maxs));
         beta = 1.0 / kBoltzmann / T;
       }
-      bool accept = de < 0;
-      if (!accept) {
-        const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 0xFFFFFFFEu);
-        accept = UnitOpenRight(Join64(u.x, u.y)) < exp(-de * beta);
-      }
+      const bool accept = de < 0 || s_u < exp(-de * beta);
       const uint64_t ti = trace_base + it;
       if (ti < c.trace_cap) {
         const uint64_t o = static_cast<uint64_t>(r) * c.trace_cap + ti;
@@ -257,18 +327,22 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
         c.tr_acc[o] = accept ? 1 : 0;
       }
       if (accept) {
-        const uint8_t a = occ[p];
-        occ[p] = occ[q];
-        occ[q] = a;
+        occ[p] = static_cast<uint8_t>(eq);
+        occ[q] = static_cast<uint8_t>(ep);
         energy += de;
         abse += de;
         ++acc;
       }
-      if (c.anneal) AnnealUpdate(A, accept, steps, maxs, energy);
-      ++steps;
-      if (steps > maxs) s_done = 1;
+      if (c.anneal) AnnealUpdate(A, sched, accept, steps, energy);
+      if (steps + 1 > maxs) s_done = 1;
     }
+    ++steps;
     __syncthreads();
+  }
+  if constexpr (kLdsOcc) {
+    __syncthreads();
+    uint32_t *dst = reinterpret_cast<uint32_t *>(c.occ + c.off[r]);
+    for (uint64_t i = tid; i < N / 4; i += kThreads) dst[i] = occ_words[i];
   }
   if (tid == 0) {
     c.steps[r] = steps;
@@ -383,10 +457,36 @@ hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off
   return hipGetLastError();
 }
 
+// The occupancy goes to LDS when the largest replica fits beside the static LDS of the kernel.
 hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_t trace_base,
                      hipStream_t s) {
   if (c.R <= 0 || trials == 0) return hipSuccess;
-  hipLaunchKernelGGL(CmcKernel, dim3(c.R), dim3(kThreads), 0, s, t, c, trials, trace_base);
+  static size_t lds_block = 0;
+  static size_t stat = 0;
+  if (!lds_block) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    hipFuncAttributes fa;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
+        hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(CmcKernel<true>)) != hipSuccess)
+      return hipErrorInvalidDevice;
+    lds_block = p.sharedMemPerBlock;
+    stat = fa.sharedSizeBytes;
+  }
+  const size_t dyn = static_cast<size_t>(c.max_sites);
+  if (c.lds_occupancy && dyn % 4 == 0 && stat + dyn <= lds_block) {
+    if (dyn > 65536 - stat) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(CmcKernel<true>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(dyn));
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(CmcKernel<true>, dim3(c.R), dim3(kThreads), dyn, s, t, c, trials,
+                       trace_base);
+  } else {
+    hipLaunchKernelGGL(CmcKernel<false>, dim3(c.R), dim3(kThreads), 0, s, t, c, trials,
+                       trace_base);
+  }
   return hipGetLastError();
 }
 

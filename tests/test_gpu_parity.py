@@ -209,23 +209,30 @@ def test_kmc_options_match_oracle(torch, coeffs):
         np.testing.assert_allclose(gst["temperature"][r], st["temperature"], rtol=1e-12)
 
 
-def test_cmc_trajectory_matches_oracle(torch, coeffs):
+@pytest.mark.parametrize("lds", [True, False])
+def test_cmc_trajectory_matches_oracle(torch, coeffs, lds):
+    """Occupancy held in LDS during the run (replicas of different sizes) or in HBM."""
     import latticemontecarlo_amd as L
     import oracle as O
     path, els = coeffs["e0_3n"]
-    n, R, trials = 5, 3, 600
+    sizes, trials = [5, 4, 6], 600
+    R = len(sizes)
     gp = L.Model(path, els, L.PAIRSITE)
     ge = L.Model(path, els, L.ENERGY)
-    occs = [random_alloc(n, gp.species, 0.05, seed=900 + r, vacancy=False) for r in range(R)]
+    occs = [random_alloc(n, gp.species, 0.05, seed=900 + r, vacancy=False)
+            for r, n in enumerate(sizes)]
     seed = 42
-    drv = L.CanonicalMc(gp, ge, [n] * R, occs, 800.0, 300.0, seed, maximum_steps=trials)
+    print(f"cmc sizes={sizes} lds={lds}", flush=True)
+    drv = L.CanonicalMc(gp, ge, sizes, occs, 800.0, 300.0, seed, maximum_steps=trials)
+    drv.set_lds_occupancy(lds)
     drv.set_trace(trials)
-    drv.run(trials)
+    drv.run(trials // 3)
+    drv.run(trials - trials // 3)
     gst, gtr, gocc = drv.state(), drv.trace(), drv.occupancy()
-    lat = O.Lattice(n)
-    op = O.Model(lat, path, els, O.PAIRSITE)
-    oe = O.Model(lat, path, els, O.ENERGY)
-    for r in range(R):
+    for r, n in enumerate(sizes):
+        lat = O.Lattice(n)
+        op = O.Model(lat, path, els, O.PAIRSITE)
+        oe = O.Model(lat, path, els, O.ENERGY)
         c = O.Cmc(op, oe, lat, occs[r], 800.0, 300.0, seed, replica=r)
         tr = c.run(trials, trials, trace=True)
         st = c.state()
