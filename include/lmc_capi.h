@@ -110,6 +110,52 @@ int lmc_jump_symmetries(int32_t *perm);
 int32_t lmc_jump_orbit_clusters(int32_t n, int64_t v, int64_t nb, int32_t which, int64_t *out,
                                 int32_t cap);
 
+/* ---- Configuration files (host) -----------------------------------------------------------
+ * A configuration of the cubic n x n x n-cell FCC box: the element of every atom, the atom on
+ * every lattice site (lattice ids x-major, the order ReassignLatticeVector gives,
+ * Config.cpp:466-552) and every atom's periodic image map_shift. Atom ids are the file's line
+ * order, as in the reference. Other boxes are rejected with LMC_ERR_ARG. */
+typedef struct lmc_config lmc_config;
+/* cfg::GenerateFCC({n, n, n}, element) (Config.cpp:1060-1095): basis 4.046 n, atom ids in
+ * GenerateFCC's creation order (cells z-major, basis sites (0,0,0) (1,1,0) (1,0,1) (0,1,1) in a/2
+ * units), map_shift zero. */
+int lmc_config_generate(int32_t n, const char *element, lmc_config **out);
+/* Config::ReadConfig(filename) followed by ReassignLatticeVector (Config.cpp:554-661, gzip by
+ * the .gz extension). "Cannot open <filename>" as the reference; unknown element strings are X
+ * (Element.hpp:25-26). */
+int lmc_config_read(const char *filename, lmc_config **out);
+/* Config::ReadMap(lattice_filename, element_filename, map_filename) (Config.cpp:814-885). The
+ * lattice file's positions must be in x-major order (as WriteLattice writes them after
+ * ReassignLatticeVector); its neighbour lists are implied by the positions. Map errors carry the
+ * reference's messages ("Lattice id out of bounds in map file: ", "Duplicate lattice id in map
+ * file: "). */
+int lmc_config_read_map(const char *lattice_filename, const char *element_filename,
+                        const char *map_filename, lmc_config **out);
+void lmc_config_free(lmc_config *c);
+/* Cells per edge, number of sites and the row-major 3 x 3 basis (A); any pointer may be null. */
+int lmc_config_info(const lmc_config *c, int32_t *ncell, int64_t *num_sites, double *basis);
+/* Species code of every lattice id for the species list species[n_species] (a model's sorted
+ * species with X, see lmc_model_species_name): codes[site] indexes that list. LMC_ERR_ARG if an element
+ * of the configuration is not in the list. */
+int lmc_config_codes(const lmc_config *c, int32_t n_species, const char *const *species,
+                     uint8_t *codes);
+/* Atom on every lattice site, lattice_to_atom[num_sites], and map_shift[num_sites][3] by atom
+ * id (Config::GetLatticeToAtomVector, GetMapShiftList); either pointer may be null. */
+int lmc_config_atoms(const lmc_config *c, int64_t *lattice_to_atom, int32_t *map_shift);
+/* The state after a run: species codes by lattice id and, optionally (null keeps the current
+ * ones), the atom on every site (a permutation of the atom ids) and map_shift by atom id. Each
+ * atom takes the element of its site. */
+int lmc_config_update(lmc_config *c, int32_t n_species, const char *const *species,
+                      const uint8_t *codes, const int64_t *lattice_to_atom,
+                      const int32_t *map_shift);
+/* Config::WriteConfig(filename) (Config.cpp:663-710, gzip by the .gz extension): the same text,
+ * byte for byte, as the reference writes for the same configuration. */
+int lmc_config_write(const lmc_config *c, const char *filename);
+/* Config::WriteLattice / WriteElement / WriteMap (Config.cpp:887-922); a null name skips that
+ * file. */
+int lmc_config_write_map(const lmc_config *c, const char *lattice_filename,
+                         const char *element_filename, const char *map_filename);
+
 /* ---- Stateless evaluation on the GPU ------------------------------------------------------
  * The 12 vacancy jumps of each replica, in the reference's event order (neighbour ids
  * ascending): replaces the 12 calls of GetBarrierAndDiffFromLatticeIdPair made by

@@ -4,6 +4,7 @@ The compute path is liblmc.so (hand-written gfx950 HIP kernels behind the C ABI 
 include/lmc_capi.h); this package is the Python mirror of the reference's predictor and driver
 interface over that ABI.
 """
+from . import cfg  # noqa: F401,E402
 from .api import (E0, ENERGY, PAIRSITE, QUARTIC, CanonicalMc, KineticMcChain, KineticMcFirst, Model,  # noqa: F401,E501
                   ReplicaBatch, SimulatedAnnealing, Surrogate, augment, eval_jumps, fcc_neighbors, fcc_num_sites,
                   jump_environment, jump_orbit_clusters, jump_samples, jump_state_clusters, jump_symmetries,

@@ -8,7 +8,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblmc.so")
 
-SOURCES = ["lmc_tables.cpp", "lmc_model.cpp", "lmc_capi.cpp", "lmc_jump_kernels.hip",
+SOURCES = ["lmc_tables.cpp", "lmc_model.cpp", "lmc_capi.cpp", "lmc_config_io.cpp", "lmc_jump_kernels.hip",
            "lmc_site_kernels.hip", "lmc_surrogate.hip", "lmc_surrogate_kmc.hip"]
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
@@ -20,7 +20,7 @@ FLAGS = [
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     cmd = [hipcc, *FLAGS, "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", LIB + ".tmp"]
+           *[os.path.join(CSRC, s) for s in SOURCES], "-lz", "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

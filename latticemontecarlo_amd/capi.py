@@ -57,6 +57,16 @@ SIGNATURES = {
     "lmc_model_species_code": (_i32, [_vp, C.c_char_p]),
     "lmc_model_species_name": (C.c_char_p, [_vp, _i32]),
     "lmc_model_set_formulation": (C.c_int, [_vp, _i32]),
+    "lmc_config_generate": (C.c_int, [_i32, C.c_char_p, C.POINTER(_vp)]),
+    "lmc_config_read": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "lmc_config_read_map": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
+    "lmc_config_free": (None, [_vp]),
+    "lmc_config_info": (C.c_int, [_vp, _i32p, _i64p, _f64p]),
+    "lmc_config_codes": (C.c_int, [_vp, _i32, C.POINTER(C.c_char_p), _u8p]),
+    "lmc_config_atoms": (C.c_int, [_vp, _i64p, _i32p]),
+    "lmc_config_update": (C.c_int, [_vp, _i32, C.POINTER(C.c_char_p), _u8p, _i64p, _i32p]),
+    "lmc_config_write": (C.c_int, [_vp, C.c_char_p]),
+    "lmc_config_write_map": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p]),
     "lmc_model_layout": (C.c_int, [_vp, _i32p, _i32p, _i32p]),
     "lmc_model_eci_key": (C.c_int, [_vp, _i32, _i32p, _i32p]),
     "lmc_model_orbits": (C.c_int, [_vp, _i32, _i32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p]),

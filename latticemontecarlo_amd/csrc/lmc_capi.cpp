@@ -16,6 +16,8 @@
 #include <vector>
 
 #include "lmc_device.h"
+#include "lmc_element.h"
+#include "lmc_error.h"
 #include "lmc_model.h"
 #include "lmc_rng.h"
 #include "lmc_surrogate.h"
@@ -27,39 +29,11 @@ struct lmc_model {
 
 namespace {
 
-thread_local std::string g_error;
-
-int Fail(int code, const std::string &msg) {
-  g_error = msg;
-  return code;
-}
-
-struct HipError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct ModelError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-void Check(hipError_t e, const char *what) {
-  if (e != hipSuccess)
-    throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-int Guard(F &&f) {
-  try {
-    return f();
-  } catch (const HipError &e) {
-    return Fail(LMC_ERR_HIP, e.what());
-  } catch (const ModelError &e) {
-    return Fail(LMC_ERR_MODEL, e.what());
-  } catch (const std::invalid_argument &e) {
-    return Fail(LMC_ERR_ARG, e.what());
-  } catch (const std::exception &e) {
-    return Fail(LMC_ERR_IO, e.what());
-  }
-}
+using lmc::Check;
+using lmc::Fail;
+using lmc::Guard;
+using lmc::HipError;
+using lmc::ModelError;
 
 int32_t PackOff(const lmc::I3 &o) {
   return (o.x & 0xff) | ((o.y & 0xff) << 8) | ((o.z & 0xff) << 16);
@@ -495,14 +469,7 @@ void ToHost(T *dst, const T *src, size_t n) {
   if (dst && n) Check(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 
-double ElementMass(const std::string &e) {  // Element::GetMass (Element.hpp:71-88)
-  if (e == "Al") return 26.98;
-  if (e == "Mg") return 24.31;
-  if (e == "Zn") return 65.38;
-  if (e == "Cu") return 63.55;
-  if (e == "Sn") return 118.71;
-  return 0.0;
-}
+using lmc::ElementMass;
 
 struct Batch {
   std::vector<int64_t> off;
@@ -616,7 +583,7 @@ struct lmc_cmc {
 
 extern "C" {
 
-const char *lmc_last_error(void) { return g_error.c_str(); }
+const char *lmc_last_error(void) { return lmc::g_last_error.c_str(); }
 int lmc_version(void) { return 1; }
 
 int lmc_device_alloc(uint64_t bytes, void **d_ptr) {
