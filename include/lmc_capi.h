@@ -262,6 +262,16 @@ int lmc_kmc_state(lmc_kmc *k, uint64_t *steps, double *time, double *energy, dou
 int lmc_kmc_occupancy(lmc_kmc *k, uint8_t *occ);
 /* Trace arrays [R][capacity]. */
 int lmc_kmc_trace(lmc_kmc *k, int64_t *sel, double *ea, double *de, double *time);
+/* Atom tracking, replacing the atom ids and map_shift that Config::LatticeJump keeps
+ * (Config.cpp:431-456; lattice_to_atom_vector_, map_shift_list_): lattice_to_atom[total] gives
+ * the atom on every site of the batch, ids local to each replica (a permutation of 0..N_r-1, as
+ * lmc_config_atoms returns them), and map_shift[total][3] the periodic image of every atom (row
+ * offset_r + atom id; null: zero). Every jump then swaps the two atoms and moves the image change
+ * of the vacancy -> target displacement from the vacancy's atom to the migrating one. Null
+ * lattice_to_atom turns tracking off. Synchronous. */
+int lmc_kmc_set_atoms(lmc_kmc *k, const int64_t *lattice_to_atom, const int32_t *map_shift);
+/* The tracked atom ids and map_shift, in the layout of lmc_kmc_set_atoms (either may be null). */
+int lmc_kmc_atoms(lmc_kmc *k, int64_t *lattice_to_atom, int32_t *map_shift);
 /* Device views of the batch occupancy (for callers that evaluate it with lmc_eval_jumps). */
 int lmc_kmc_device_views(lmc_kmc *k, uint8_t **d_occ, const int64_t **d_offset,
                          const int32_t **d_ncell, const int64_t **d_vac);
@@ -306,6 +316,10 @@ int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energ
                   uint64_t *accepted);
 int lmc_cmc_occupancy(lmc_cmc *c, uint8_t *occ);
 int lmc_cmc_trace(lmc_cmc *c, int64_t *p, int64_t *q, double *de, int8_t *accepted);
+/* Atom tracking as lmc_kmc_set_atoms / lmc_kmc_atoms: an accepted swap of p and q is
+ * Config::LatticeJump({p, q}) (CanonicalMcAbstract.cpp:86-100). */
+int lmc_cmc_set_atoms(lmc_cmc *c, const int64_t *lattice_to_atom, const int32_t *map_shift);
+int lmc_cmc_atoms(lmc_cmc *c, int64_t *lattice_to_atom, int32_t *map_shift);
 
 /* ---- Learned surrogate of the jump evaluation ------------------------------------------
  * No reference counterpart: an ensemble of BF16 MLPs (one-hot jump environment -> 128 -> 128

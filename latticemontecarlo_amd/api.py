@@ -206,6 +206,35 @@ def total_energy(model, batch, stream=None):
     return e
 
 
+def _set_atoms(fn, h, ncell, lattice_to_atom, map_shift):
+    """Atom tracking of a driver (lmc_kmc_set_atoms / lmc_cmc_set_atoms): per-replica lists of
+    the atom on every site (a permutation of that replica's atom ids) and map_shift [N, 3] by
+    atom (None: zero); lattice_to_atom None turns tracking off."""
+    if lattice_to_atom is None:
+        check(fn(h, None, None))
+        return
+    sizes = 4 * np.asarray(ncell, np.int64) ** 3
+    l2a = np.ascontiguousarray(np.concatenate([np.asarray(a, np.int64) for a in lattice_to_atom]))
+    if l2a.size != sizes.sum():
+        raise ValueError(f"lattice_to_atom has {l2a.size} entries, expected {int(sizes.sum())}")
+    sh = None
+    if map_shift is not None:
+        sh = np.ascontiguousarray(np.concatenate([np.asarray(m, np.int32).reshape(-1, 3)
+                                                  for m in map_shift]))
+        if sh.shape[0] != l2a.size:
+            raise ValueError("map_shift has the wrong size")
+    check(fn(h, ptr(l2a, capi._i64p), None if sh is None else ptr(sh, capi._i32p)))
+
+
+def _atoms(fn, h, ncell):
+    sizes = 4 * np.asarray(ncell, np.int64) ** 3
+    l2a = np.zeros(int(sizes.sum()), np.int64)
+    sh = np.zeros((int(sizes.sum()), 3), np.int32)
+    check(fn(h, ptr(l2a, capi._i64p), ptr(sh, capi._i32p)))
+    cut = np.cumsum(sizes)[:-1]
+    return np.split(l2a, cut), np.split(sh, cut)
+
+
 class KineticMcFirst:
     """Batched KineticMcFirst driver: R independent replicas advanced on the GPU."""
 
@@ -293,6 +322,15 @@ class KineticMcFirst:
                                   ptr(out["vacancy"], capi._i64p), ptr(out["disp"], capi._i64p),
                                   ptr(out["com"], capi._f64p), ptr(out["escaped"], capi._i32p)))
         return out
+
+    def set_atoms(self, lattice_to_atom, map_shift=None):
+        """Track atom ids and map_shift through the jumps (Config::LatticeJump); per-replica
+        lists as cfg.Config.atoms() returns them. None turns tracking off."""
+        _set_atoms(lib().lmc_kmc_set_atoms, self.h, self.ncell, lattice_to_atom, map_shift)
+
+    def atoms(self):
+        """(per-replica lattice_to_atom, per-replica map_shift [N, 3])."""
+        return _atoms(lib().lmc_kmc_atoms, self.h, self.ncell)
 
     def occupancy(self):
         out = np.zeros(self.total_sites, np.uint8)
@@ -411,6 +449,14 @@ class CanonicalMc:
                                   ptr(out["abs_energy"], capi._f64p),
                                   ptr(out["accepted"], capi._u64p)))
         return out
+
+    def set_atoms(self, lattice_to_atom, map_shift=None):
+        """Track atom ids and map_shift through accepted swaps (Config::LatticeJump)."""
+        _set_atoms(lib().lmc_cmc_set_atoms, self.h, self.ncell, lattice_to_atom, map_shift)
+
+    def atoms(self):
+        """(per-replica lattice_to_atom, per-replica map_shift [N, 3])."""
+        return _atoms(lib().lmc_cmc_atoms, self.h, self.ncell)
 
     def occupancy(self):
         out = np.zeros(self.total_sites, np.uint8)

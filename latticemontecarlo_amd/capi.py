@@ -105,6 +105,10 @@ SIGNATURES = {
     "lmc_cmc_set_trace": (C.c_int, [_vp, _u64]),
     "lmc_cmc_run": (C.c_int, [_vp, _u64, _vp]),
     "lmc_cmc_set_annealing": (C.c_int, [_vp, _i32]),
+    "lmc_kmc_set_atoms": (C.c_int, [_vp, _i64p, _i32p]),
+    "lmc_kmc_atoms": (C.c_int, [_vp, _i64p, _i32p]),
+    "lmc_cmc_set_atoms": (C.c_int, [_vp, _i64p, _i32p]),
+    "lmc_cmc_atoms": (C.c_int, [_vp, _i64p, _i32p]),
     "lmc_cmc_set_lds_occupancy": (C.c_int, [_vp, _i32]),
     "lmc_cmc_annealing_state": (C.c_int, [_vp, _vp, _vp, _vp]),
     "lmc_cmc_state": (C.c_int, [_vp, _u64p, _f64p, _f64p, _u64p]),

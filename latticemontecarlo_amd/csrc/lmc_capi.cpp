@@ -499,6 +499,67 @@ void ComputeTotalEnergy(const lmc_model *em, const uint8_t *d_occ, const int64_t
 
 }  // namespace
 
+// Host copies of a batch's offsets and the total site count.
+std::vector<int64_t> HostOffsets(int32_t R, const int64_t *d_off, const int32_t *d_ncell,
+                                 int64_t &total) {
+  std::vector<int64_t> off(R);
+  std::vector<int32_t> nc(R);
+  ToHost(off.data(), d_off, off.size());
+  ToHost(nc.data(), d_ncell, nc.size());
+  total = off.back() + 4ll * nc.back() * nc.back() * nc.back();
+  off.push_back(total);
+  return off;
+}
+
+// Device copies of the atom on every site (ids local to each replica, int32) and map_shift by
+// atom; a null lattice_to_atom turns tracking off.
+void UploadAtoms(int32_t R, const int64_t *d_off, const int32_t *d_ncell, const int64_t *l2a,
+                 const int32_t *map_shift, int32_t *&d_atom, int32_t *&d_shift) {
+  if (!l2a) {
+    DevFree(d_atom);
+    DevFree(d_shift);
+    return;
+  }
+  int64_t total = 0;
+  const std::vector<int64_t> off = HostOffsets(R, d_off, d_ncell, total);
+  std::vector<int32_t> a(static_cast<size_t>(total));
+  std::vector<char> seen;
+  for (int r = 0; r < R; ++r) {
+    const int64_t N = off[r + 1] - off[r];
+    if (N > std::numeric_limits<int32_t>::max())
+      throw std::invalid_argument("atom tracking needs fewer than 2^31 sites per replica");
+    seen.assign(static_cast<size_t>(N), 0);
+    for (int64_t s = 0; s < N; ++s) {
+      const int64_t v = l2a[off[r] + s];
+      if (v < 0 || v >= N || seen[static_cast<size_t>(v)])
+        throw std::invalid_argument("lattice_to_atom of replica " + std::to_string(r) +
+                                    " is not a permutation of its atom ids");
+      seen[static_cast<size_t>(v)] = 1;
+      a[static_cast<size_t>(off[r] + s)] = static_cast<int32_t>(v);
+    }
+  }
+  std::vector<int32_t> sh = map_shift ? std::vector<int32_t>(map_shift, map_shift + 3 * total)
+                                      : std::vector<int32_t>(static_cast<size_t>(3 * total), 0);
+  DevFree(d_atom);
+  DevFree(d_shift);
+  d_atom = DevCopy(a);
+  d_shift = DevCopy(sh);
+}
+
+void DownloadAtoms(int32_t R, const int64_t *d_off, const int32_t *d_ncell, const int32_t *d_atom,
+                   const int32_t *d_shift, int64_t *l2a, int32_t *map_shift) {
+  if (!d_atom) throw std::invalid_argument("atom tracking is off");
+  Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  int64_t total = 0;
+  (void)HostOffsets(R, d_off, d_ncell, total);
+  if (l2a) {
+    std::vector<int32_t> a(static_cast<size_t>(total));
+    ToHost(a.data(), d_atom, a.size());
+    std::copy(a.begin(), a.end(), l2a);
+  }
+  if (map_shift) ToHost(map_shift, d_shift, static_cast<size_t>(3 * total));
+}
+
 struct lmc_kmc {
   const lmc_model *bm = nullptr;
   lmc::KmcDev d;
@@ -546,6 +607,8 @@ struct lmc_kmc {
     DevFree(d.com);
     DevFree(d.error);
     DevFree(d.prev_j);
+    DevFree(d.atom);
+    DevFree(d.shift);
     DevFree(d.tt_time);
     DevFree(d.tt_temp);
     DevFree(d.tr_sel);
@@ -561,6 +624,8 @@ struct lmc_cmc {
   uint64_t trace_pos = 0;
   ~lmc_cmc() {
     DevFree(d.anneal);
+    DevFree(d.atom);
+    DevFree(d.shift);
     DevFree(d.occ);
     DevFree(d.off);
     DevFree(d.ncell);
@@ -1298,6 +1363,24 @@ int lmc_kmc_trace(lmc_kmc *k, int64_t *sel, double *ea, double *de, double *time
   });
 }
 
+int lmc_kmc_set_atoms(lmc_kmc *k, const int64_t *lattice_to_atom, const int32_t *map_shift) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    UploadAtoms(k->d.R, k->d.off, k->d.ncell, lattice_to_atom, map_shift, k->d.atom, k->d.shift);
+    k->DropGraph();
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_atoms(lmc_kmc *k, int64_t *lattice_to_atom, int32_t *map_shift) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    DownloadAtoms(k->d.R, k->d.off, k->d.ncell, k->d.atom, k->d.shift, lattice_to_atom, map_shift);
+    return LMC_OK;
+  });
+}
+
 int lmc_kmc_device_views(lmc_kmc *k, uint8_t **d_occ, const int64_t **d_offset,
                          const int32_t **d_ncell, const int64_t **d_vac) {
   if (!k) return Fail(LMC_ERR_ARG, "null driver");
@@ -1387,6 +1470,23 @@ int lmc_cmc_set_trace(lmc_cmc *c, uint64_t capacity) {
       c->d.tr_acc = DevAlloc<int8_t>(n);
       Check(hipMemset(c->d.tr_p, 0xff, n * sizeof(int64_t)), "hipMemset");
     }
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_set_atoms(lmc_cmc *c, const int64_t *lattice_to_atom, const int32_t *map_shift) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    UploadAtoms(c->d.R, c->d.off, c->d.ncell, lattice_to_atom, map_shift, c->d.atom, c->d.shift);
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_atoms(lmc_cmc *c, int64_t *lattice_to_atom, int32_t *map_shift) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    DownloadAtoms(c->d.R, c->d.off, c->d.ncell, c->d.atom, c->d.shift, lattice_to_atom, map_shift);
     return LMC_OK;
   });
 }

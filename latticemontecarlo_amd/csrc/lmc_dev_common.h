@@ -51,6 +51,30 @@ __device__ __forceinline__ bool ValidSite(int n, int64_t site) {
   return n >= 4 && site >= 0 && site < 4ll * n * n * n;
 }
 
+// Image change of one axis for a displacement of d grid steps (a/2) from lhs to rhs on the 2n
+// axis: Config::LatticeJump's floor(shift + 0.5) for |shift| > 0.5 + 1e-8 with shift = d / 2n
+// (Config.cpp:438-444), exact for |d| < 2n.
+__device__ __forceinline__ int ImageChange(int d, int n) { return (d > n) - (d < -n); }
+
+// Config::LatticeJump (Config.cpp:431-456) on the atom ids and map_shift of one replica: the
+// atoms at lattice ids lhs and rhs swap places; the lhs atom's map_shift loses the image change
+// (icx, icy, icz) of the displacement lhs -> rhs and the rhs atom's gains it.
+__device__ __forceinline__ void AtomJump(int32_t *atom, int32_t *shift, int64_t lhs, int64_t rhs,
+                                         int icx, int icy, int icz) {
+  const int32_t al = atom[lhs], ar = atom[rhs];
+  atom[lhs] = ar;
+  atom[rhs] = al;
+  if (icx | icy | icz) {
+    int32_t *sl = shift + 3 * static_cast<int64_t>(al), *sr = shift + 3 * static_cast<int64_t>(ar);
+    sl[0] -= icx;
+    sl[1] -= icy;
+    sl[2] -= icz;
+    sr[0] += icx;
+    sr[1] += icy;
+    sr[2] += icz;
+  }
+}
+
 // v of lane l - kShift within the lane's row of 16 (0 past the row start): DPP row_shr on both
 // halves of the double.
 template <int kShift>

@@ -98,6 +98,9 @@ struct KmcDev {
   double *com = nullptr;        // [R][3] solute centre-of-mass trajectory
   int32_t *error = nullptr;     // [1] first error code
   int64_t *prev_j = nullptr;    // [R] chain steps: the site the vacancy came from
+  // optional atom tracking: atom id on every site (ids local to the replica) and map_shift
+  // [site offset + atom id][3]
+  int32_t *atom = nullptr, *shift = nullptr;
   // optional temperature schedule (shared)
   int32_t n_tt = 0;
   double *tt_time = nullptr, *tt_temp = nullptr;
@@ -120,6 +123,7 @@ struct CmcDev {
   int32_t R = 0;
   int64_t max_sites = 0;   // largest replica, in sites
   int lds_occupancy = 1;   // hold each replica's occupancy in LDS while it runs, where it fits
+  int32_t *atom = nullptr, *shift = nullptr;  // optional atom tracking, as in KmcDev
   CmcAnneal *anneal = nullptr;  // [R]; null: the linear ramp t0 -> t1
   uint8_t *occ = nullptr;
   int64_t *off = nullptr;

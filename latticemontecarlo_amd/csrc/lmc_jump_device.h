@@ -667,6 +667,13 @@ __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint
     Q.com[1] -= dy * kHalfLattice * wgt;
     Q.com[2] -= dz * kHalfLattice * wgt;
   }
+  if (k.atom) {  // the vacancy at Q.vac is the lhs of the reference's jump pair
+    const int64_t o = k.off[r];
+    const int m = Q.m2, h = m >> 1;
+    AtomJump(k.atom + o, k.shift + 3 * o, Q.vac, jump,
+             ImageChange(Wrap(Q.vx + dx, m) - Q.vx, h), ImageChange(Wrap(Q.vy + dy, m) - Q.vy, h),
+             ImageChange(Wrap(Q.vz + dz, m) - Q.vz, h));
+  }
   Q.disp[0] += dx;
   Q.disp[1] += dy;
   Q.disp[2] += dz;

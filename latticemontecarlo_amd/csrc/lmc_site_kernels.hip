@@ -327,6 +327,12 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
         c.tr_acc[o] = accept ? 1 : 0;
       }
       if (accept) {
+        if (c.atom) {
+          const XYZ a = ToXYZ(p, n), b = ToXYZ(q, n);
+          const int64_t o = c.off[r];
+          AtomJump(c.atom + o, c.shift + 3 * o, p, q, ImageChange(b.X - a.X, n),
+                   ImageChange(b.Y - a.Y, n), ImageChange(b.Z - a.Z, n));
+        }
         occ[p] = static_cast<uint8_t>(eq);
         occ[q] = static_cast<uint8_t>(ep);
         energy += de;
