@@ -272,6 +272,23 @@ int lmc_kmc_trace(lmc_kmc *k, int64_t *sel, double *ea, double *de, double *time
 int lmc_kmc_set_atoms(lmc_kmc *k, const int64_t *lattice_to_atom, const int32_t *map_shift);
 /* The tracked atom ids and map_shift, in the layout of lmc_kmc_set_atoms (either may be null). */
 int lmc_kmc_atoms(lmc_kmc *k, int64_t *lattice_to_atom, int32_t *map_shift);
+/* kmc_log.txt, replacing KineticMcFirstAbstract::Dump's log (KineticMcAbstract.cpp:59-104):
+ * with log_dump_steps > 0 the kernels keep a record at the reference's cadence (every step below
+ * step 99, then every 10^(floor(log10(steps + 1)) - 1) steps up to log_dump_steps, and every
+ * log_dump_steps steps past 10 log_dump_steps), up to `capacity` records per replica between
+ * writes. A record is taken after the step's event is chosen and before it is applied: steps,
+ * time, temperature, energy, Ea, dE, the migrating atom (-1 without lmc_kmc_set_atoms), the
+ * vacancy's unwrapped Cartesian position (box_edge[r] A, null: 4.046 n, times the relative
+ * position plus the vacancy atom's map_shift) and, with solute_disp, the solute centre of mass.
+ * log_dump_steps = 0 turns it off. Synchronous. */
+int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
+                    const double *box_edge);
+/* Appends the records taken since the last call to filenames[r] (null: skip and drop that
+ * replica's records) in the reference's text format: tab-separated columns, precision 16, the
+ * default float format up to the first vector and std::fixed after it, the header line and a
+ * truncated file for a run from step 0, and for a restarted run an appended file without its
+ * first dump. LMC_ERR_ARG after writing if records were dropped for lack of capacity. */
+int lmc_kmc_write_log(lmc_kmc *k, const char *const *filenames);
 /* Device views of the batch occupancy (for callers that evaluate it with lmc_eval_jumps). */
 int lmc_kmc_device_views(lmc_kmc *k, uint8_t **d_occ, const int64_t **d_offset,
                          const int32_t **d_ncell, const int64_t **d_vac);

@@ -256,9 +256,9 @@ class KineticMcFirst:
             p.restart_energy = float(np.broadcast_to(restart_energy, (R,))[r])
             p.restart_time = float(np.broadcast_to(restart_time, (R,))[r])
             p.maximum_steps = int(np.broadcast_to(maximum_steps, (R,))[r])
-            p.rate_corrector = int(rate_corrector)
-            p.early_stop = int(early_stop)
-            p.solute_disp = int(solute_disp)
+            p.rate_corrector = int(np.broadcast_to(rate_corrector, (R,))[r])
+            p.early_stop = int(np.broadcast_to(early_stop, (R,))[r])
+            p.solute_disp = int(np.broadcast_to(solute_disp, (R,))[r])
         occ = np.ascontiguousarray(np.concatenate([np.asarray(o, np.uint8) for o in occ_list]))
         self.total_sites = occ.size
         h = C.c_void_p()
@@ -331,6 +331,25 @@ class KineticMcFirst:
     def atoms(self):
         """(per-replica lattice_to_atom, per-replica map_shift [N, 3])."""
         return _atoms(lib().lmc_kmc_atoms, self.h, self.ncell)
+
+    def set_log(self, log_dump_steps, capacity=100000, box_edge=None):
+        """Keep kmc_log.txt records at the reference's dump cadence (0 turns it off); box_edge:
+        per-replica cubic box edge in A (default 4.046 n)."""
+        edge = None if box_edge is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(box_edge, np.float64), (self.R,)))
+        check(lib().lmc_kmc_set_log(self.h, int(log_dump_steps), int(capacity),
+                                    None if edge is None else ptr(edge, capi._f64p)))
+        self._log_edge = edge
+
+    def write_log(self, filenames):
+        """Append the records taken since the last call to one kmc_log.txt-format file per
+        replica (None skips a replica)."""
+        if isinstance(filenames, (str, bytes)) or hasattr(filenames, "__fspath__"):
+            filenames = [filenames]
+        if len(filenames) != self.R:
+            raise ValueError(f"need {self.R} file names")
+        arr = (C.c_char_p * self.R)(*[None if f is None else str(f).encode() for f in filenames])
+        check(lib().lmc_kmc_write_log(self.h, arr))
 
     def occupancy(self):
         out = np.zeros(self.total_sites, np.uint8)

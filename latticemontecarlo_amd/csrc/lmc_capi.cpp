@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -564,6 +565,10 @@ struct lmc_kmc {
   const lmc_model *bm = nullptr;
   lmc::KmcDev d;
   uint64_t trace_pos = 0;
+  // kmc_log.txt writing: per replica the starting step, the solute column switch, and the
+  // stream state (file begun, std::fixed set by the first vector written)
+  std::vector<uint64_t> start_steps;
+  std::vector<char> solute_disp, log_begun, log_fixed, log_skip_first;
   int64_t total_sites = 0;
   int formulation = 0;
   // surrogate evaluation (optional)
@@ -609,6 +614,9 @@ struct lmc_kmc {
     DevFree(d.prev_j);
     DevFree(d.atom);
     DevFree(d.shift);
+    DevFree(d.log_n);
+    DevFree(d.log);
+    DevFree(d.log_edge);
     DevFree(d.tt_time);
     DevFree(d.tt_temp);
     DevFree(d.tr_sel);
@@ -1050,6 +1058,9 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
     }
     for (int c = 0; c < m.S && c < lmc::kMaxSpecies; ++c)
       k->d.mass[c] = ElementMass(m.eci.species[c]);
+    k->start_steps = steps;
+    k->solute_disp.resize(R);
+    for (int r = 0; r < R; ++r) k->solute_disp[r] = params[r].solute_disp ? 1 : 0;
     auto &d = k->d;
     d.occ = DevCopy(std::vector<uint8_t>(occ, occ + b.total));
     d.off = DevCopy(b.off);
@@ -1377,6 +1388,117 @@ int lmc_kmc_atoms(lmc_kmc *k, int64_t *lattice_to_atom, int32_t *map_shift) {
   return Guard([&] {
     if (!k) throw std::invalid_argument("null driver");
     DownloadAtoms(k->d.R, k->d.off, k->d.ncell, k->d.atom, k->d.shift, lattice_to_atom, map_shift);
+    return LMC_OK;
+  });
+}
+
+int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
+                    const double *box_edge) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const int R = k->d.R;
+    DevFree(k->d.log_n);
+    DevFree(k->d.log);
+    DevFree(k->d.log_edge);
+    k->d.log_every = 0;
+    k->d.log_cap = 0;
+    k->DropGraph();
+    if (!log_dump_steps) return LMC_OK;
+    if (!capacity) throw std::invalid_argument("log capacity must be positive");
+    std::vector<int32_t> nc(R);
+    ToHost(nc.data(), k->d.ncell, nc.size());
+    std::vector<double> edge(R);
+    for (int r = 0; r < R; ++r) {
+      edge[r] = box_edge ? box_edge[r] : 4.046 * nc[r];  // GenerateFCC's basis (Constants.hpp:6)
+      if (!(edge[r] > 0)) throw std::invalid_argument("box edges must be positive");
+    }
+    k->d.log_n = DevCopy(std::vector<uint64_t>(R, 0));
+    k->d.log = DevAlloc<lmc::KmcLogRecord>(static_cast<size_t>(capacity) * R);
+    k->d.log_edge = DevCopy(edge);
+    k->d.log_every = log_dump_steps;
+    k->d.log_cap = capacity;
+    k->log_begun.assign(R, 0);
+    k->log_fixed.assign(R, 0);
+    // a restarted run skips its first dump (McAbstract's is_restarted_, KineticMcAbstract.cpp:60-63)
+    k->log_skip_first.resize(R);
+    for (int r = 0; r < R; ++r) k->log_skip_first[r] = k->start_steps[r] > 0 ? 1 : 0;
+    return LMC_OK;
+  });
+}
+
+namespace {
+// A double as McAbstract's log stream prints it: precision(16), the default float format until
+// the first vector has set std::fixed (VectorMatrix.hpp:41-51).
+void PutDouble(std::string &o, double v, bool fixed) {
+  char buf[64];
+  const int n = std::snprintf(buf, sizeof(buf), fixed ? "%.16f" : "%.16g", v);
+  o.append(buf, static_cast<size_t>(n));
+}
+}  // namespace
+
+int lmc_kmc_write_log(lmc_kmc *k, const char *const *filenames) {
+  return Guard([&] {
+    if (!k || !filenames) throw std::invalid_argument("null argument");
+    if (!k->d.log_every) throw std::invalid_argument("logging is off");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const int R = k->d.R;
+    std::vector<uint64_t> cnt(R);
+    ToHost(cnt.data(), k->d.log_n, cnt.size());
+    std::vector<lmc::KmcLogRecord> rec(static_cast<size_t>(k->d.log_cap) * R);
+    ToHost(rec.data(), k->d.log, rec.size());
+    Check(hipMemset(k->d.log_n, 0, sizeof(uint64_t) * R), "hipMemset");
+    uint64_t lost = 0;
+    for (int r = 0; r < R; ++r) {
+      if (!filenames[r]) continue;
+      std::string o;
+      const bool fresh = !k->log_begun[r];
+      if (fresh && k->start_steps[r] == 0) {  // header of a run from step 0 (Dump, :64-70)
+        o += "steps\ttime\ttemperature\tenergy\tEa\tdE\tselected\tvac1\tvac2\tvac3";
+        if (k->solute_disp[r]) o += "\tsolute_com1\tsolute_com2\tsolute_com3";
+        o += '\n';
+      }
+      const uint64_t n = std::min(cnt[r], k->d.log_cap);
+      lost += cnt[r] - n;
+      for (uint64_t i = 0; i < n; ++i) {
+        const lmc::KmcLogRecord &L = rec[static_cast<size_t>(r) * k->d.log_cap + i];
+        if (k->log_skip_first[r]) {
+          k->log_skip_first[r] = 0;
+          if (L.steps == k->start_steps[r]) continue;
+        }
+        const bool fx = k->log_fixed[r] != 0;
+        o += std::to_string(L.steps);
+        for (double v : {L.time, L.temperature, L.energy, L.ea, L.de}) {
+          o += '\t';
+          PutDouble(o, v, fx);
+        }
+        o += '\t';
+        o += std::to_string(L.atom);
+        o += '\t';
+        for (int d = 0; d < 3; ++d) {
+          if (d) o += ' ';
+          PutDouble(o, L.vac[d], true);
+        }
+        k->log_fixed[r] = 1;
+        if (k->solute_disp[r]) {
+          o += '\t';
+          for (int d = 0; d < 3; ++d) {
+            if (d) o += ' ';
+            PutDouble(o, L.com[d], true);
+          }
+        }
+        o += '\n';
+      }
+      // a run from step 0 truncates (std::ofstream::out); a restarted one appends
+      std::FILE *f = std::fopen(filenames[r], fresh && k->start_steps[r] == 0 ? "wb" : "ab");
+      if (!f) throw std::runtime_error(std::string("Cannot open ") + filenames[r]);
+      const bool ok = std::fwrite(o.data(), 1, o.size(), f) == o.size();
+      if (std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("Cannot write ") + filenames[r]);
+      k->log_begun[r] = 1;
+    }
+    if (lost)
+      throw std::invalid_argument(std::to_string(lost) +
+                                  " log records exceeded the capacity and were dropped");
     return LMC_OK;
   });
 }

@@ -79,6 +79,16 @@ struct DeviceTables {
 const KTables &Upload(Model &m);
 
 // Per-replica KMC state, structure of arrays in device memory.
+// One line of kmc_log.txt (KineticMcFirstAbstract::Dump, KineticMcAbstract.cpp:59-104), taken
+// after the step's event is chosen and before it is applied.
+struct KmcLogRecord {
+  uint64_t steps;
+  int64_t atom;  // the migrating atom (-1 without atom tracking)
+  double time, temperature, energy, ea, de;
+  double vac[3];  // unwrapped Cartesian position of the vacancy (A)
+  double com[3];  // solute centre-of-mass trajectory
+};
+
 struct KmcDev {
   int32_t R = 0;
   uint8_t *occ = nullptr;       // concatenated occupancy
@@ -101,6 +111,11 @@ struct KmcDev {
   // optional atom tracking: atom id on every site (ids local to the replica) and map_shift
   // [site offset + atom id][3]
   int32_t *atom = nullptr, *shift = nullptr;
+  // optional log records [R][log_cap] at the reference's dump cadence of log_every
+  uint64_t log_every = 0, log_cap = 0;
+  uint64_t *log_n = nullptr;      // [R] records taken since the last drain
+  KmcLogRecord *log = nullptr;
+  double *log_edge = nullptr;     // [R] cubic box edge (A)
   // optional temperature schedule (shared)
   int32_t n_tt = 0;
   double *tt_time = nullptr, *tt_temp = nullptr;

@@ -641,6 +641,49 @@ __device__ __forceinline__ bool ChooseEvent(JumpLds &L, const ReplicaLds &Q, int
   return true;
 }
 
+// The dump cadence of KineticMcFirstAbstract::Dump (KineticMcAbstract.cpp:83-98): every
+// log_every steps past 10 log_every; below that every 10^(floor(log10(steps + 1)) - 1) steps
+// (every step below 99), at most log_every.
+__device__ bool LogDue(uint64_t steps, uint64_t log_every) {
+  uint64_t every = log_every;
+  if (steps <= 10 * log_every) {
+    uint64_t p = 1;
+    for (uint64_t t = steps + 1; t >= 100; t /= 10) p *= 10;
+    every = p < log_every ? p : log_every;
+  }
+  return steps % every == 0;
+}
+
+// The kmc_log.txt record of the step about to jump the vacancy to `jump` (one lane).
+__device__ void LogStep(const KmcDev &k, const ReplicaLds &Q, int r, int64_t jump, double ea,
+                        double de) {
+  const uint64_t i = k.log_n[r];
+  k.log_n[r] = i + 1;  // counts past the capacity too, so the host can report the loss
+  if (i >= k.log_cap) return;
+  KmcLogRecord &L = k.log[static_cast<uint64_t>(r) * k.log_cap + i];
+  L.steps = Q.steps;
+  L.time = Q.time;
+  L.temperature = Q.T;
+  L.energy = Q.energy;
+  L.ea = ea;
+  L.de = de;
+  const int64_t o = k.off[r];
+  const int32_t *sv = nullptr;
+  L.atom = -1;
+  if (k.atom) {
+    L.atom = k.atom[o + jump];
+    sv = k.shift + 3 * (o + k.atom[o + Q.vac]);
+  }
+  // Config::GetUnwrappedCartesianPositionOfLattice (Config.cpp:280-291) on the cubic box
+  const double m = Q.m2, edge = k.log_edge[r];
+  L.vac[0] = (Q.vx / m + (sv ? sv[0] : 0)) * edge;
+  L.vac[1] = (Q.vy / m + (sv ? sv[1] : 0)) * edge;
+  L.vac[2] = (Q.vz / m + (sv ? sv[2] : 0)) * edge;
+  L.com[0] = Q.com[0];
+  L.com[1] = Q.com[1];
+  L.com[2] = Q.com[2];
+}
+
 // One lane: advances the clock and applies a jump of the vacancy to site `jump` (offset
 // dx, dy, dz; migrating species mig) with barrier ea and energy change de
 // (KineticMcAbstract.cpp:147-181). `escaped`: IsEscaped of the step.
@@ -651,6 +694,7 @@ __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint
     Q.esc = 1;
     Q.steps = Q.max_steps + 1;
   }
+  if (k.log_every && LogDue(Q.steps, k.log_every)) LogStep(k, Q, r, jump, ea, de);
   if (trace_index < k.trace_cap) {
     const uint64_t o = static_cast<uint64_t>(r) * k.trace_cap + trace_index;
     k.tr_sel[o] = jump;

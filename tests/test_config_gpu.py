@@ -155,3 +155,110 @@ def test_kmc_resume_from_cfg_is_bit_exact(torch, coeffs, tmp_path):
     (ra,), (rs,) = rest.atoms()
     (fa,), (fs,) = full.atoms()
     assert np.array_equal(ra, fa) and np.array_equal(rs, fs)
+
+
+def _reference_due(steps, L):
+    """KineticMcFirstAbstract::Dump's cadence restated with its floating-point log10
+    (KineticMcAbstract.cpp:83-98); a negative exponent casts to 0 and is raised to 1."""
+    import math
+    if steps > 10 * L:
+        every = L
+    else:
+        q = int(math.log10(steps + 1) - 1)  # truncation toward zero, as the cast
+        every = int(10.0 ** q) if q >= 0 else 0
+        every = min(max(every, 1), L)
+    return steps % every == 0
+
+
+def _expected_log(n, occ, l2a, sh, tr, steps0, e0, t0, T, L, sp, first_fixed, skip_first):
+    """kmc_log.txt lines of a trajectory from its trace (without the COM columns)."""
+    o = CO.generate_fcc(n, "Al")
+    o.reassign()
+    o.atom_to_lattice = [0] * len(l2a)
+    for s, a in enumerate(l2a):
+        o.atom_to_lattice[int(a)] = s
+    o.map_shift = [list(map(int, r)) for r in sh]
+    vac = int(np.flatnonzero(occ == sp.index("X"))[0])
+    edge, m = 4.046 * n, float(2 * n)
+    fixed = first_fixed
+    energy, time, lines = e0, t0, []
+    for i, (s, ea, de, tnew) in enumerate(zip(tr["sel"], tr["ea"], tr["de"], tr["time"])):
+        step = steps0 + i
+        if _reference_due(step, L) and not (skip_first and i == 0):
+            l2a_now = o.lattice_to_atom
+            X = vac // (2 * n * n)
+            Y = (vac - X * 2 * n * n) // n
+            Z = 2 * (vac - X * 2 * n * n - Y * n) + ((X + Y) & 1)
+            sv = o.map_shift[l2a_now[vac]]
+            f = "%.16f" if fixed else "%.16g"
+            vals = "\t".join(f % v for v in (time, T, energy, ea, de))
+            pos = " ".join("%.16f" % ((c / m + sv[d]) * edge) for d, c in enumerate((X, Y, Z)))
+            lines.append(f"{step}\t{vals}\t{l2a_now[int(s)]}\t{pos}")
+            fixed = True
+        o.lattice_jump(vac, int(s))
+        vac = int(s)
+        energy += de
+        time = tnew
+    return lines
+
+
+def test_kmc_log_matches_reference_format(torch, coeffs, tmp_path):
+    """kmc_log.txt written from the GPU records against the lines rebuilt from the trace with the
+    reference's floating-point cadence and stream formats, over two writes; a restarted run
+    appends without a header and without its first dump. The solute COM columns are compared
+    to 1e-9 A."""
+    import latticemontecarlo_amd as L
+    path, els = coeffs["e0_3"]
+    n, R, k, Ld, T = 4, 2, 700, 20, 700.0
+    gb = L.Model(path, els, L.E0)
+    sp = gb.species
+    occs = [random_alloc(n, sp, 0.08, seed=60 + r) for r in range(R)]
+    l2a, sh = _start_atoms(n, R, seed=21)
+    print(f"kmc log n={n} R={R} steps={2 * k}", flush=True)
+    drv = L.KineticMcFirst(gb, None, [n] * R, occs, T, 8, maximum_steps=10 ** 9,
+                           solute_disp=[False, True])
+    drv.set_atoms(l2a, sh)
+    drv.set_log(Ld, capacity=400)
+    drv.set_trace(2 * k)
+    files = [tmp_path / f"kmc_log_{r}.txt" for r in range(R)]
+    drv.run(k)
+    drv.write_log(files)
+    drv.run(k)
+    drv.write_log(files)
+    tr = drv.trace()
+    st = drv.state()
+    (ga, gs) = drv.atoms()
+    for r in range(R):
+        got = files[r].read_text().split("\n")
+        hdr = "steps\ttime\ttemperature\tenergy\tEa\tdE\tselected\tvac1\tvac2\tvac3"
+        assert got[0] == hdr + ("\tsolute_com1\tsolute_com2\tsolute_com3" if r == 1 else "")
+        assert got[-1] == ""
+        want = _expected_log(n, occs[r], l2a[r], sh[r], {key: v[r] for key, v in tr.items()},
+                             0, 0.0, 0.0, T, Ld, sp, False, False)
+        body = got[1:-1]
+        assert len(body) == len(want) and len(want) > 100
+        if r == 1:
+            com = np.array([[float(x) for x in ln.split("\t")[8].split()] for ln in body])
+            assert np.abs(com).max() > 0
+            body = ["\t".join(ln.split("\t")[:8]) for ln in body]
+        assert body == want
+    # restart from the final state: appended lines, no header, no first dump
+    occ_now = drv.occupancy()
+    rest = L.KineticMcFirst(gb, None, [n] * R, occ_now, T, 8, maximum_steps=10 ** 9,
+                            restart_steps=[int(x) for x in st["steps"]],
+                            restart_time=list(st["time"]), restart_energy=list(st["energy"]))
+    rest.set_atoms(ga, gs)
+    rest.set_log(Ld, capacity=100)
+    rest.set_trace(60)
+    rest.run(60)
+    before = [f.read_text() for f in files]
+    rest.write_log(files)
+    rtr = rest.trace()
+    for r in range(R):
+        added = files[r].read_text()[len(before[r]):].split("\n")[:-1]
+        want = _expected_log(n, occ_now[r], ga[r], gs[r], {key: v[r] for key, v in rtr.items()},
+                             int(st["steps"][r]), float(st["energy"][r]), float(st["time"][r]), T,
+                             Ld, sp, False, True)
+        if r == 1:
+            added = ["\t".join(ln.split("\t")[:8]) for ln in added]
+        assert added == want and len(want) == 2  # steps 1420 and 1440; 1400 is skipped
