@@ -2,7 +2,8 @@
 //
 // Names, argument meanings and exceptions follow zhucongx/LatticeMonteCarlo so that host code
 // written against it compiles with few changes:
-//   cfg::Config, cfg::GenerateFCC                  lmc/cfg/include/Config.h:15-115
+//   cfg::Config, cfg::GenerateFCC                  lmc/cfg/include/Config.h:15-115 (with
+//                                                  ReadConfig/WriteConfig and the map files)
 //   pred::EnergyPredictor                          lmc/pred/include/EnergyPredictor.h:16-25
 //   pred::VacancyMigrationPredictorE0/Quartic      VacancyMigrationPredictorE0.h:15-24,
 //                                                  VacancyMigrationPredictorQuartic.h:18-27
@@ -93,44 +94,129 @@ class ModelHandle {
 
 namespace cfg {
 
-// An n x n x n-cell FCC box (a = 4.046 A) with one element per lattice site, ids in the
-// reference's x-major order (Config.cpp:466-470). The species codes of the element set index the
-// engine's tables; a device copy is made on first use and refreshed after every change.
+namespace detail {
+struct FreeConfig {
+  void operator()(lmc_config *c) const { lmc_config_free(c); }
+};
+using ConfigHandle = std::unique_ptr<lmc_config, FreeConfig>;
+inline std::vector<const char *> Names(const std::vector<std::string> &v) {
+  std::vector<const char *> out;
+  for (const auto &s : v) out.push_back(s.c_str());
+  return out;
+}
+// Every element name the reference knows (Element.hpp:7), sorted.
+inline std::vector<std::string> AllElements() {
+  std::vector<std::string> v{"Al", "Cu", "Mg", "Sn", "X", "Zn", "pAl", "pCu", "pMg", "pSn", "pZn"};
+  std::sort(v.begin(), v.end());
+  return v;
+}
+}  // namespace detail
+
+// An n x n x n-cell FCC box with one atom per lattice site, lattice ids in the reference's
+// x-major order (ReassignLatticeVector, Config.cpp:466-552). It keeps the element of every site
+// (species codes of sorted(element_set + "X"), which index the engine's tables), the atom on
+// every site and every atom's map_shift. A device copy of the codes is made on first use and
+// refreshed after every change.
 class Config {
  public:
+  // GenerateFCC (Config.cpp:1060-1095): basis 4.046 n, atoms numbered as GenerateFCC creates them.
   Config(int n, std::set<std::string> element_set_without_vacancy, const std::string &element)
       : n_(n) {
-    element_set_without_vacancy.erase("X");
-    species_.assign(element_set_without_vacancy.begin(), element_set_without_vacancy.end());
-    species_.push_back("X");
-    std::sort(species_.begin(), species_.end());
-    if (n < 4) throw std::invalid_argument("FCC box needs at least 4 cells per side");
+    SetSpecies(std::move(element_set_without_vacancy));
+    lmc_config *c = nullptr;
+    lmc::Check(lmc_config_generate(n, element.c_str(), &c));
+    detail::ConfigHandle h(c);
     codes_.assign(static_cast<size_t>(lmc_fcc_num_sites(n)), CodeOf(element));
+    LoadAtoms(h.get());
   }
   // Copies get their own device buffer.
-  Config(const Config &o) : n_(o.n_), species_(o.species_), codes_(o.codes_) {}
+  Config(const Config &o)
+      : n_(o.n_), basis_(o.basis_), species_(o.species_), codes_(o.codes_), atoms_(o.atoms_),
+        shifts_(o.shifts_) {}
   Config &operator=(const Config &o) {
     n_ = o.n_;
+    basis_ = o.basis_;
     species_ = o.species_;
     codes_ = o.codes_;
+    atoms_ = o.atoms_;
+    shifts_ = o.shifts_;
     dirty_ = true;
     dev_.reset();
     return *this;
   }
   Config(Config &&) = default;
   Config &operator=(Config &&) = default;
+
+  // Config::ReadConfig(filename) + ReassignLatticeVector (Config.cpp:554-661; gzip by the .gz
+  // extension). The species list is sorted(element_set + "X"); an empty element_set takes the
+  // elements found in the file.
+  static Config ReadConfig(const std::string &filename, std::set<std::string> element_set = {}) {
+    lmc_config *c = nullptr;
+    lmc::Check(lmc_config_read(filename.c_str(), &c));
+    return FromHandle(detail::ConfigHandle(c), std::move(element_set));
+  }
+  // Config::ReadMap(lattice_filename, element_filename, map_filename) (Config.cpp:814-885).
+  static Config ReadMap(const std::string &lattice_filename, const std::string &element_filename,
+                        const std::string &map_filename, std::set<std::string> element_set = {}) {
+    lmc_config *c = nullptr;
+    lmc::Check(lmc_config_read_map(lattice_filename.c_str(), element_filename.c_str(),
+                                   map_filename.c_str(), &c));
+    return FromHandle(detail::ConfigHandle(c), std::move(element_set));
+  }
+  // Config::WriteConfig (Config.cpp:663-710): the reference's text, gzip by the .gz extension.
+  void WriteConfig(const std::string &filename) const {
+    lmc::Check(lmc_config_write(Handle().get(), filename.c_str()));
+  }
+  // Config::WriteLattice / WriteElement / WriteMap (Config.cpp:887-922).
+  void WriteLattice(const std::string &filename) const {
+    lmc::Check(lmc_config_write_map(Handle().get(), filename.c_str(), nullptr, nullptr));
+  }
+  void WriteElement(const std::string &filename) const {
+    lmc::Check(lmc_config_write_map(Handle().get(), nullptr, filename.c_str(), nullptr));
+  }
+  void WriteMap(const std::string &filename) const {
+    lmc::Check(lmc_config_write_map(Handle().get(), nullptr, nullptr, filename.c_str()));
+  }
+
   [[nodiscard]] size_t GetNumAtoms() const { return codes_.size(); }
   [[nodiscard]] int GetNumCells() const { return n_; }
+  [[nodiscard]] const std::vector<double> &GetBasis() const { return basis_; }
   [[nodiscard]] std::string GetElementAtLatticeId(size_t lattice_id) const {
     return species_.at(codes_.at(lattice_id));
   }
+  [[nodiscard]] size_t GetAtomIdFromLatticeId(size_t lattice_id) const {
+    return static_cast<size_t>(atoms_.at(lattice_id));
+  }
+  // lattice id -> atom id, and map_shift [atom id * 3 + axis]
+  [[nodiscard]] const std::vector<int64_t> &GetLatticeToAtomVector() const { return atoms_; }
+  [[nodiscard]] const std::vector<int32_t> &GetMapShiftList() const { return shifts_; }
   void SetAtomElementTypeAtLattice(size_t lattice_id, const std::string &element) {
     codes_.at(lattice_id) = CodeOf(element);
     dirty_ = true;
   }
-  // Config::LatticeJump (Config.cpp:431-456): exchanges the two sites' atoms.
+  // Config::LatticeJump (Config.cpp:431-456): exchanges the two sites' atoms and moves the image
+  // change of the displacement lhs -> rhs from the lhs atom's map_shift to the rhs atom's.
   void LatticeJump(const std::pair<size_t, size_t> &lattice_id_jump_pair) {
-    std::swap(codes_.at(lattice_id_jump_pair.first), codes_.at(lattice_id_jump_pair.second));
+    const size_t l = lattice_id_jump_pair.first, r = lattice_id_jump_pair.second;
+    std::swap(codes_.at(l), codes_.at(r));
+    auto xyz = [this](size_t id, int *c) {
+      const int64_t plane = 2ll * n_ * n_;
+      c[0] = static_cast<int>(static_cast<int64_t>(id) / plane);
+      const int64_t rest = static_cast<int64_t>(id) - c[0] * plane;
+      c[1] = static_cast<int>(rest / n_);
+      c[2] = 2 * static_cast<int>(rest - static_cast<int64_t>(c[1]) * n_) + ((c[0] + c[1]) & 1);
+    };
+    int a[3], b[3];
+    xyz(l, a);
+    xyz(r, b);
+    const int64_t al = atoms_.at(l), ar = atoms_.at(r);
+    for (int d = 0; d < 3; ++d) {
+      const int dd = b[d] - a[d];
+      const int ic = (dd > n_) - (dd < -n_);  // floor(shift + 0.5) for |shift| > 0.5, shift = dd/2n
+      shifts_[3 * al + d] -= ic;
+      shifts_[3 * ar + d] += ic;
+    }
+    std::swap(atoms_[l], atoms_[r]);
     dirty_ = true;
   }
   [[nodiscard]] size_t GetVacancyLatticeId() const {
@@ -153,6 +239,15 @@ class Config {
   // Species codes by lattice id, in the order of sorted(element_set + "X").
   [[nodiscard]] const std::vector<uint8_t> &GetCodes() const { return codes_; }
   [[nodiscard]] const std::vector<std::string> &GetSpecies() const { return species_; }
+  // The state after a run on the GPU: codes by lattice id and, when the run tracked atoms, the
+  // atom ids and map_shift (empty vectors keep the current ones).
+  void SetState(const uint8_t *codes, const std::vector<int64_t> &atoms,
+                const std::vector<int32_t> &shifts) {
+    std::copy(codes, codes + codes_.size(), codes_.begin());
+    if (!atoms.empty()) atoms_ = atoms;
+    if (!shifts.empty()) shifts_ = shifts;
+    dirty_ = true;
+  }
   [[nodiscard]] const uint8_t *DeviceOccupancy() const {
     if (dirty_ || !dev_) {
       if (!dev_) dev_ = std::make_unique<lmc::DeviceBuffer>(codes_.size());
@@ -170,9 +265,58 @@ class Config {
   }
 
  private:
-  int n_;
+  Config() = default;
+  void SetSpecies(std::set<std::string> element_set_without_vacancy) {
+    element_set_without_vacancy.erase("X");
+    species_.assign(element_set_without_vacancy.begin(), element_set_without_vacancy.end());
+    species_.push_back("X");
+    std::sort(species_.begin(), species_.end());
+  }
+  void LoadAtoms(lmc_config *c) {
+    int32_t n = 0;
+    int64_t N = 0;
+    basis_.assign(9, 0.0);
+    lmc::Check(lmc_config_info(c, &n, &N, basis_.data()));
+    n_ = n;
+    atoms_.resize(static_cast<size_t>(N));
+    shifts_.resize(static_cast<size_t>(3 * N));
+    lmc::Check(lmc_config_atoms(c, atoms_.data(), shifts_.data()));
+  }
+  static Config FromHandle(detail::ConfigHandle h, std::set<std::string> element_set) {
+    Config out;
+    out.LoadAtoms(h.get());
+    if (element_set.empty()) {  // the elements present in the file
+      const auto all = detail::AllElements();
+      const auto names = detail::Names(all);
+      std::vector<uint8_t> c(out.atoms_.size());
+      lmc::Check(lmc_config_codes(h.get(), static_cast<int32_t>(names.size()), names.data(), c.data()));
+      for (uint8_t v : c) element_set.insert(all[v]);
+    }
+    out.SetSpecies(std::move(element_set));
+    const auto names = detail::Names(out.species_);
+    out.codes_.resize(out.atoms_.size());
+    lmc::Check(lmc_config_codes(h.get(), static_cast<int32_t>(names.size()), names.data(),
+                                out.codes_.data()));
+    return out;
+  }
+  // A configuration handle of the current state (the box edge from the basis).
+  detail::ConfigHandle Handle() const {
+    lmc_config *c = nullptr;
+    lmc::Check(lmc_config_generate(n_, "X", &c));
+    detail::ConfigHandle h(c);
+    const auto names = detail::Names(species_);
+    lmc::Check(lmc_config_update(h.get(), static_cast<int32_t>(names.size()), names.data(),
+                                 codes_.data(), atoms_.data(), shifts_.data()));
+    lmc::Check(lmc_config_set_basis(h.get(), basis_.data()));
+    return h;
+  }
+
+  int n_ = 0;
+  std::vector<double> basis_;
   std::vector<std::string> species_;
   std::vector<uint8_t> codes_;
+  std::vector<int64_t> atoms_;
+  std::vector<int32_t> shifts_;
   mutable bool dirty_ = true;
   mutable std::unique_ptr<lmc::DeviceBuffer> dev_;
 };
@@ -336,6 +480,33 @@ namespace mc {
 // KineticMcFirstOmp for R independent replicas advanced by one persistent kernel. Per-replica
 // parameters follow the reference constructor (temperature, restart_*, maximum_steps,
 // rate_corrector, early_stop, solute_disp); Simulate() runs every replica to maximum_steps.
+namespace detail {
+// Atom ids and map_shift of a batch of configurations, concatenated (the drivers' layout).
+inline void ConcatAtoms(const std::vector<cfg::Config> &configs, std::vector<int64_t> &atoms,
+                        std::vector<int32_t> &shifts) {
+  for (const auto &c : configs) {
+    atoms.insert(atoms.end(), c.GetLatticeToAtomVector().begin(), c.GetLatticeToAtomVector().end());
+    shifts.insert(shifts.end(), c.GetMapShiftList().begin(), c.GetMapShiftList().end());
+  }
+}
+// Splits a batch's occupancy (and tracked atoms) back into its configurations.
+inline void SplitState(std::vector<cfg::Config> &configs, const std::vector<uint8_t> &occ,
+                       const std::vector<int64_t> &atoms, const std::vector<int32_t> &shifts) {
+  size_t at = 0;
+  for (auto &c : configs) {
+    const size_t N = c.GetNumAtoms();
+    std::vector<int64_t> a;
+    std::vector<int32_t> sh;
+    if (!atoms.empty()) {
+      a.assign(atoms.begin() + at, atoms.begin() + at + N);
+      sh.assign(shifts.begin() + 3 * at, shifts.begin() + 3 * (at + N));
+    }
+    c.SetState(occ.data() + at, a, sh);
+    at += N;
+  }
+}
+}  // namespace detail
+
 class KineticMcFirstBatch {
  public:
   KineticMcFirstBatch(const std::vector<cfg::Config> &configs, const std::vector<lmc_kmc_replica> &params,
@@ -354,7 +525,42 @@ class KineticMcFirstBatch {
                               params.data(), &k));
     k_.reset(k);
     R_ = configs.size();
+    total_ = occ.size();
     for (const auto &p : params) max_steps_ = std::max<uint64_t>(max_steps_, p.maximum_steps);
+  }
+  // Track atom ids and map_shift from these configurations on (Config::LatticeJump).
+  void TrackAtoms(const std::vector<cfg::Config> &configs) {
+    std::vector<int64_t> a;
+    std::vector<int32_t> sh;
+    detail::ConcatAtoms(configs, a, sh);
+    if (a.size() != total_) throw std::invalid_argument("configurations differ from the batch");
+    lmc::Check(lmc_kmc_set_atoms(k_.get(), a.data(), sh.data()));
+    tracking_ = true;
+  }
+  // The batch's current state back into its configurations: elements, and atom ids and
+  // map_shift when tracking.
+  void UpdateConfigs(std::vector<cfg::Config> &configs) const {
+    std::vector<uint8_t> occ(total_);
+    lmc::Check(lmc_kmc_occupancy(k_.get(), occ.data()));
+    std::vector<int64_t> a;
+    std::vector<int32_t> sh;
+    if (tracking_) {
+      a.resize(total_);
+      sh.resize(3 * total_);
+      lmc::Check(lmc_kmc_atoms(k_.get(), a.data(), sh.data()));
+    }
+    detail::SplitState(configs, occ, a, sh);
+  }
+  // kmc_log.txt records at the reference's dump cadence, written by WriteLog (one file per
+  // replica, appended; see lmc_kmc_set_log).
+  void SetLog(uint64_t log_dump_steps, uint64_t capacity = 100000) {
+    lmc::Check(lmc_kmc_set_log(k_.get(), log_dump_steps, capacity, nullptr));
+  }
+  void WriteLog(const std::vector<std::string> &filenames) {
+    if (filenames.size() != R_) throw std::invalid_argument("one log file per replica");
+    std::vector<const char *> f;
+    for (const auto &s : filenames) f.push_back(s.c_str());
+    lmc::Check(lmc_kmc_write_log(k_.get(), f.data()));
   }
   void SetTimeTemperature(const std::vector<double> &time, const std::vector<double> &temperature) {
     lmc::Check(lmc_kmc_set_schedule(k_.get(), static_cast<int32_t>(time.size()), time.data(),
@@ -424,8 +630,9 @@ class KineticMcFirstBatch {
     void operator()(lmc_kmc *k) const { lmc_kmc_free(k); }
   };
   std::unique_ptr<lmc_kmc, Free> k_;
-  size_t R_ = 0;
+  size_t R_ = 0, total_ = 0;
   uint64_t max_steps_ = 0;
+  bool tracking_ = false;
 };
 
 // KineticMcChainOmpi for R independent replicas: second-order residence-time steps.
@@ -458,8 +665,30 @@ class CanonicalMcBatch {
                               params.data(), &c));
     c_.reset(c);
     R_ = configs.size();
+    total_ = occ.size();
   }
   void Run(uint64_t trials, void *stream = nullptr) { lmc::Check(lmc_cmc_run(c_.get(), trials, stream)); }
+  // Track atom ids and map_shift through accepted swaps (Config::LatticeJump).
+  void TrackAtoms(const std::vector<cfg::Config> &configs) {
+    std::vector<int64_t> a;
+    std::vector<int32_t> sh;
+    detail::ConcatAtoms(configs, a, sh);
+    if (a.size() != total_) throw std::invalid_argument("configurations differ from the batch");
+    lmc::Check(lmc_cmc_set_atoms(c_.get(), a.data(), sh.data()));
+    tracking_ = true;
+  }
+  void UpdateConfigs(std::vector<cfg::Config> &configs) const {
+    std::vector<uint8_t> occ(total_);
+    lmc::Check(lmc_cmc_occupancy(c_.get(), occ.data()));
+    std::vector<int64_t> a;
+    std::vector<int32_t> sh;
+    if (tracking_) {
+      a.resize(total_);
+      sh.resize(3 * total_);
+      lmc::Check(lmc_cmc_atoms(c_.get(), a.data(), sh.data()));
+    }
+    detail::SplitState(configs, occ, a, sh);
+  }
   // Occupancy in LDS during runs where it fits (default), or in HBM; results are identical.
   void SetLdsOccupancy(bool on) { lmc::Check(lmc_cmc_set_lds_occupancy(c_.get(), on ? 1 : 0)); }
   // SimulatedAnnealing's adaptive schedule from t_start (see lmc_cmc_set_annealing).
@@ -485,7 +714,8 @@ class CanonicalMcBatch {
     void operator()(lmc_cmc *c) const { lmc_cmc_free(c); }
   };
   std::unique_ptr<lmc_cmc, Free> c_;
-  size_t R_ = 0;
+  size_t R_ = 0, total_ = 0;
+  bool tracking_ = false;
 };
 
 // SimulatedAnnealing (SimulatedAnnealing.h) for R independent replicas: swap trials with the

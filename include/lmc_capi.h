@@ -148,6 +148,8 @@ int lmc_config_atoms(const lmc_config *c, int64_t *lattice_to_atom, int32_t *map
 int lmc_config_update(lmc_config *c, int32_t n_species, const char *const *species,
                       const uint8_t *codes, const int64_t *lattice_to_atom,
                       const int32_t *map_shift);
+/* The row-major 3 x 3 basis (A) of a cubic box: diagonal, equal edges. */
+int lmc_config_set_basis(lmc_config *c, const double *basis);
 /* Config::WriteConfig(filename) (Config.cpp:663-710, gzip by the .gz extension): the same text,
  * byte for byte, as the reference writes for the same configuration. */
 int lmc_config_write(const lmc_config *c, const char *filename);

@@ -433,6 +433,15 @@ int lmc_config_update(lmc_config *c, int32_t n_species, const char *const *speci
   });
 }
 
+int lmc_config_set_basis(lmc_config *c, const double *basis) {
+  return Guard([&] {
+    if (!c || !basis) throw std::invalid_argument("null argument");
+    CheckBasis(basis, "basis");
+    std::memcpy(c->basis, basis, sizeof(c->basis));
+    return LMC_OK;
+  });
+}
+
 int lmc_config_write(const lmc_config *c, const char *filename) {
   return Guard([&] {
     if (!c || !filename) throw std::invalid_argument("null argument");

@@ -1,8 +1,8 @@
 // Host program written against the C++ mirror (include/lmc.hpp) the way reference host code
 // uses pred::/mc:: objects. Driven by tests/test_cpp_mirror.py.
 //
-//   mirror_check cpu <json>                                   host-only checks, prints "ok"
-//   mirror_check gpu <n> <occ.bin> <e0.json> <pairsite.json>  one JSON line per result
+//   mirror_check cpu <json> <dir>                             host-only checks, prints "ok"
+//   mirror_check gpu <n> <occ.bin> <e0.json> <pairsite.json> <dir>  one JSON line per result
 #include <cstdio>
 #include <fstream>
 #include <iterator>
@@ -19,7 +19,7 @@ int Fail(const char *what) {
   return 1;
 }
 
-int CpuChecks(const std::string &json) {
+int CpuChecks(const std::string &json, const std::string &dir) {
   const std::set<std::string> els{"Al", "Mg", "Zn"};
   cfg::Config c = cfg::GenerateFCC(5, els, "Al");
   if (c.GetNumAtoms() != 500) return Fail("site count");
@@ -45,11 +45,45 @@ int CpuChecks(const std::string &json) {
   }
   pred::VacancyMigrationPredictorE0 e0(json, c, els);  // loads and derives tables on the host
   (void)e0;
+  // atoms and map_shift: GenerateFCC numbering, a jump across the x boundary
+  // lattice 1 is (X, Y, Z) = (0, 0, 2): cell (0, 0, 1), basis site 0, atom 4 * 25
+  if (c.GetAtomIdFromLatticeId(0) != 0 || c.GetAtomIdFromLatticeId(1) != 100) return Fail("atom ids");
+  const size_t across = 9 * 2 * 5 * 5 + 1 * 5;  // (X, Y, Z) = (9, 1, 0): a first neighbour of 0
+  const size_t a0 = c.GetAtomIdFromLatticeId(0), a1 = c.GetAtomIdFromLatticeId(across);
+  c.LatticeJump({0, across});
+  const auto &sh = c.GetMapShiftList();
+  if (c.GetAtomIdFromLatticeId(across) != a0 || sh[3 * a0] != -1 || sh[3 * a1] != 1 ||
+      sh[3 * a0 + 1] != 0 || sh[3 * a1 + 2] != 0)
+    return Fail("map_shift");
+  // WriteConfig / ReadConfig and the map files
+  c.WriteConfig(dir + "/c.cfg.gz");
+  const cfg::Config r = cfg::Config::ReadConfig(dir + "/c.cfg.gz", els);
+  if (r.GetCodes() != c.GetCodes() || r.GetLatticeToAtomVector() != c.GetLatticeToAtomVector() ||
+      r.GetMapShiftList() != c.GetMapShiftList() || r.GetSpecies() != c.GetSpecies() ||
+      r.GetBasis() != c.GetBasis())
+    return Fail("cfg round trip");
+  const cfg::Config r2 = cfg::Config::ReadConfig(dir + "/c.cfg.gz", {"Al", "Cu", "Mg", "Zn"});
+  if (r2.GetSpecies().size() != 5 || r2.GetElementAtLatticeId(4) != "X") return Fail("species set");
+  const cfg::Config r3 = cfg::Config::ReadConfig(dir + "/c.cfg.gz");  // the elements in the file
+  if (r3.GetSpecies() != std::vector<std::string>{"Al", "X"}) return Fail("file species");
+  c.WriteLattice(dir + "/lattice.txt");
+  c.WriteElement(dir + "/element.txt");
+  c.WriteMap(dir + "/map.txt");
+  const cfg::Config m = cfg::Config::ReadMap(dir + "/lattice.txt", dir + "/element.txt", dir + "/map.txt", els);
+  if (m.GetCodes() != c.GetCodes() || m.GetLatticeToAtomVector() != c.GetLatticeToAtomVector())
+    return Fail("map round trip");
+  try {
+    (void)cfg::Config::ReadConfig(dir + "/none.cfg");
+    return Fail("missing cfg accepted");
+  } catch (const std::runtime_error &e) {
+    if (std::string(e.what()).find("Cannot open") == std::string::npos) return Fail("cfg io message");
+  }
   std::printf("ok\n");
   return 0;
 }
 
-int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const std::string &ps_json) {
+int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const std::string &ps_json,
+           const std::string &dir) {
   const std::set<std::string> els{"Al", "Mg", "Zn"};
   std::ifstream f(occ_path, std::ios::binary);
   std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
@@ -81,6 +115,19 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
   const auto s = kmc.GetState();
   std::printf("{\"kmc_steps\": %llu, \"kmc_time\": %.17g, \"kmc_vacancy\": %lld}\n",
               static_cast<unsigned long long>(s.steps[0]), s.time[0], static_cast<long long>(s.vacancy[0]));
+  // the same 201 steps with atom tracking and the log, the end state written as a .cfg.gz
+  {
+    std::vector<cfg::Config> cs{c};
+    mc::KineticMcFirstBatch t(cs, {prm}, barrier, &energy);
+    t.TrackAtoms(cs);
+    t.SetLog(10, 1000);
+    t.Simulate();
+    t.UpdateConfigs(cs);
+    t.WriteLog({dir + "/kmc_log.txt"});
+    cs[0].WriteConfig(dir + "/end.cfg.gz");
+    std::printf("{\"tracked\": \"%s\", \"log\": \"%s\"}\n", (dir + "/end.cfg.gz").c_str(),
+                (dir + "/kmc_log.txt").c_str());
+  }
   // the same 201 steps replayed from a captured graph of three launches
   mc::KineticMcFirstBatch g({c}, {prm}, barrier, &energy);
   g.RunGraph(67, 3);
@@ -119,9 +166,9 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
 int main(int argc, char **argv) {
   try {
     const std::string mode = argc > 1 ? argv[1] : "";
-    if (mode == "cpu" && argc == 3) return CpuChecks(argv[2]);
-    if (mode == "gpu" && argc == 6) return GpuRun(std::stoi(argv[2]), argv[3], argv[4], argv[5]);
-    std::printf("usage: mirror_check cpu <json> | gpu <n> <occ.bin> <e0.json> <pairsite.json>\n");
+    if (mode == "cpu" && argc == 4) return CpuChecks(argv[2], argv[3]);
+    if (mode == "gpu" && argc == 7) return GpuRun(std::stoi(argv[2]), argv[3], argv[4], argv[5], argv[6]);
+    std::printf("usage: mirror_check cpu <json> <dir> | gpu <n> <occ.bin> <e0.json> <pairsite.json> <dir>\n");
     return 2;
   } catch (const std::exception &e) {
     std::printf("FAIL exception: %s\n", e.what());

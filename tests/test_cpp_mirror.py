@@ -22,9 +22,9 @@ def exe(tmp_path_factory):
     return str(out)
 
 
-def test_mirror_host_behaviour(exe, coeffs):
+def test_mirror_host_behaviour(exe, coeffs, tmp_path):
     path, _ = coeffs["e0_3"]
-    r = subprocess.run([exe, "cpu", path], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "cpu", path, str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
 
 
@@ -38,8 +38,8 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
     occ_file = tmp_path / "occ.bin"
     occ.astype(np.uint8).tofile(occ_file)
     print(f"mirror_check gpu n={n} seed=31", flush=True)
-    r = subprocess.run([exe, "gpu", str(n), str(occ_file), path, path], capture_output=True,
-                       text=True, timeout=120)
+    r = subprocess.run([exe, "gpu", str(n), str(occ_file), path, path, str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = [json.loads(line) for line in r.stdout.splitlines()]
     lat = O.Lattice(n)
@@ -64,6 +64,28 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
     assert k["kmc_steps"] == st["steps"] == 201
     assert k["kmc_vacancy"] == st["vacancy"]
     assert abs(k["kmc_time"] - st["time"]) <= 1e-12 * abs(st["time"])
+    # tracked run written through cfg::Config::WriteConfig: atom ids and map_shift against the
+    # oracle's LatticeJump replayed over the oracle's own trajectory; the log's line count
+    import cfg_oracle as CO
+    t = next(x for x in rows if "tracked" in x)
+    ref = O.Kmc(ob, oe, lat, occ, 600.0, 99, replica=0)
+    otr = ref.run(201, maximum_steps=200, trace=True)
+    o = CO.generate_fcc(n, "Al")
+    o.reassign()
+    for s_, a in enumerate(o.lattice_to_atom):
+        o.elements[a] = species[occ[s_]]
+    vac = int(np.flatnonzero(occ == species.index("X"))[0])
+    for s_ in otr["sel"]:
+        o.lattice_jump(vac, int(s_))
+        vac = int(s_)
+    w = CO.read_file(t["tracked"])
+    w.reassign()
+    assert w.lattice_to_atom == o.lattice_to_atom and w.map_shift == o.map_shift
+    assert w.elements == o.elements
+    lines = open(t["log"]).read().split("\n")
+    assert lines[0].startswith("steps\ttime") and lines[-1] == ""
+    # every step below 99, then every 10 (steps 100, 110, ..., 200)
+    assert len(lines) - 2 == 99 + 11
     g = next(x for x in rows if "graph_steps" in x)  # graph replay: bit-identical to eager
     assert (g["graph_steps"], g["graph_vacancy"], g["graph_time"]) == (
         k["kmc_steps"], k["kmc_vacancy"], k["kmc_time"])
