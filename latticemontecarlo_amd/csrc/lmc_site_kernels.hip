@@ -217,43 +217,108 @@ __device__ __forceinline__ int64_t ReadLane64(int64_t v, int lane) {
       static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
   return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
 }
+__device__ __forceinline__ XYZ ReadLaneXYZ(const XYZ &v, int lane) {
+  return {__builtin_amdgcn_readlane(v.X, lane), __builtin_amdgcn_readlane(v.Y, lane),
+          __builtin_amdgcn_readlane(v.Z, lane)};
+}
 
-// The pair of step `steps`: the first attempt att = 0, 1, ... whose Philox draw (counter 2 att +
-// 1) puts different species on p and q (CanonicalMcAbstract.cpp:43-101 redraws until they
-// differ). Lane l of wave 0 tests attempt 64k + l in round k; the lowest hit of the first round
-// with one is the sequential answer. Returns false after kMaxPairAttempts attempts.
-__device__ bool DrawPair(const CmcDev &c, int r, uint64_t steps, const uint8_t *occ, uint64_t N,
-                         int lane, int64_t &p, int64_t &q) {
-  for (uint32_t base = 0; base < kMaxPairAttempts; base += 64) {
-    const uint32_t att = base + static_cast<uint32_t>(lane);
-    const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 2u * att + 1u);
-    const int64_t a = static_cast<int64_t>(MulHi64(Join64(u.x, u.y), N));
-    const int64_t b = static_cast<int64_t>(MulHi64(Join64(u.z, u.w), N));
-    const uint64_t m = __ballot(occ[a] != occ[b]);
-    if (m) {
-      const int w = __builtin_ctzll(m);
-      p = ReadLane64(a, w);
-      q = ReadLane64(b, w);
-      return true;
-    }
+// ToXYZ with 32-bit divisions: every box has 4 n^3 <= 4e9 < 2^32 sites (n <= 1000).
+__device__ __forceinline__ XYZ ToXYZ32(uint32_t id, uint32_t n) {
+  const uint32_t plane = 2u * n * n;
+  const uint32_t X = id / plane;
+  const uint32_t rest = id - X * plane;
+  const uint32_t Y = rest / n;
+  const uint32_t Zh = rest - Y * n;
+  return {static_cast<int>(X), static_cast<int>(Y), static_cast<int>(2 * Zh + ((X + Y) & 1))};
+}
+
+// Draw attempt att of step `steps` (Philox counter 2 att + 1): the two sites and their
+// coordinates. CanonicalMcAbstract.cpp:43-101 redraws until the species differ; lane l of a
+// wave holds attempt base + l, so the lowest lane of the first round with a hit is the
+// sequential answer.
+struct PairDraw {
+  int64_t a, b;
+  XYZ xa, xb;
+};
+__device__ __forceinline__ PairDraw DrawAttempt(const CmcDev &c, int r, uint64_t steps,
+                                                uint32_t att, uint64_t N, int n) {
+  const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 2u * att + 1u);
+  PairDraw d;
+  d.a = static_cast<int64_t>(MulHi64(Join64(u.x, u.y), N));
+  d.b = static_cast<int64_t>(MulHi64(Join64(u.z, u.w), N));
+  d.xa = ToXYZ32(static_cast<uint32_t>(d.a), static_cast<uint32_t>(n));
+  d.xb = ToXYZ32(static_cast<uint32_t>(d.b), static_cast<uint32_t>(n));
+  return d;
+}
+
+// Metropolis uniform of step `steps`.
+__device__ __forceinline__ double TrialUniform(const CmcDev &c, int r, uint64_t steps) {
+  const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 0xFFFFFFFEu);
+  return UnitOpenRight(Join64(u.x, u.y));
+}
+
+// Cluster `tid` of the site table, decoded once: the weight index of centre species x with the
+// other sites' species e1, e2 is base + x kx + e1 k1 + e2 k2 (SiteWeight's three cases).
+struct SiteCluster {
+  int base, kx, k1, k2;
+  int o1, o2;      // environment slots read (0 where unused)
+  int r1, r2;      // raw slots (63 = none), for the partner test
+  bool live;
+};
+__device__ SiteCluster DecodeSiteCluster(const KTables &t, const SiteLds &L, int i) {
+  SiteCluster q{};
+  q.live = i < t.n_site;
+  if (!q.live) return q;
+  const uint32_t d = L.cl[i];
+  const int S = t.S, label = d & 7;
+  q.r1 = (d >> 4) & 63;
+  q.r2 = (d >> 10) & 63;
+  if (label == 0) {
+    q.kx = 1;
+  } else if (label < 4) {
+    q.base = S + (label - 1) * S * S;
+    q.kx = S;
+    q.k1 = 1;
+    q.o1 = q.r1;
+  } else {
+    q.base = S + 3 * S * S + (label - 4) * S * S * S;
+    q.kx = S * S;
+    q.k1 = S;
+    q.k2 = 1;
+    q.o1 = q.r1;
+    q.o2 = q.r2;
   }
-  return false;
+  return q;
+}
+// w(x -> en) - w(x -> eo) of the cluster in environment `which`, 0 when it contains slot skip.
+__device__ __forceinline__ double ClusterDelta(const SiteLds &L, const SiteCluster &q, int which,
+                                               int eo, int en, int skip) {
+  if (!q.live || (skip >= 0 && (q.r1 == skip || q.r2 == skip))) return 0.0;
+  const uint8_t *e = L.env[which];
+  const int rest = q.base + e[q.o1] * q.k1 + e[q.o2] * q.k2;
+  return L.w[rest + en * q.kx] - L.w[rest + eo * q.kx];
 }
 
 // CanonicalMcSerial (CanonicalMcSerial.cpp:40-51, CanonicalMcAbstract.cpp:43-101) with an
 // optional linear temperature ramp t0 -> t1 over maximum_steps, or the adaptive annealing
 // schedule. One workgroup per replica. kLdsOcc: the replica's occupancy (N bytes, N a multiple
 // of 4) lives in dynamic LDS for the launch and is written back at its end.
-// Per trial: wave 0 draws the pair while lane 0 of wave 1 draws the Metropolis uniform; waves 0
-// and 1 gather the two environments; all waves sum the clusters; thread 0 accepts and swaps.
+// Per trial, three barriers:
+//   1. wave 0 finds the pair among its 64 precomputed draw attempts (ballot), gathers both
+//      43-site environments and both partner slots;
+//   2. every thread adds its cluster's change for both sites; DPP wave sums;
+//   3. lane 0 of wave 3 accepts or rejects and swaps, while wave 0 draws the next trial's 64
+//      attempts (sites and coordinates) and lane 0 of wave 1 its Metropolis uniform.
 template <bool kLdsOcc>
 __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint64_t ntrials,
                                                       uint64_t trace_base) {
   __shared__ SiteLds L;
   __shared__ int64_t s_p, s_q;
-  __shared__ int s_done;
-  __shared__ double s_u;
+  __shared__ XYZ s_xp, s_xq;
+  __shared__ int s_ep, s_eq, s_done;
+  __shared__ double s_u[2];
   extern __shared__ uint32_t occ_words[];
+  constexpr int kAcceptThread = 3 * 64;
   const int r = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   LoadSiteTables(t, L);
@@ -271,43 +336,80 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
   double energy = 0, abse = 0;
   CmcAnneal A{};
   AnnealSchedule sched{};
-  if (tid == 0) {
+  if (tid == kAcceptThread) {
     acc = c.accepted[r];
     energy = c.energy[r];
     abse = c.abs_energy[r];
-    s_done = steps > maxs ? 1 : 0;
     if (c.anneal) {
       A = c.anneal[r];
       sched = MakeAnnealSchedule(maxs);
     }
   }
+  if (tid == 0) s_done = steps > maxs ? 1 : 0;
+  PairDraw cand{};
+  int ox = 0, oy = 0, oz = 0;  // wave 0: the lane's environment offset
+  if (wave == 0) {
+    cand = DrawAttempt(c, r, steps, static_cast<uint32_t>(lane), N, n);
+    if (lane < kSiteSlots) Unpack(t.site_off[lane], ox, oy, oz);
+  } else if (tid == 64) {
+    s_u[steps & 1] = TrialUniform(c, r, steps);
+  }
   __syncthreads();
+  const SiteCluster cl = DecodeSiteCluster(t, L, tid);
   for (uint64_t it = 0; it < ntrials; ++it) {
     if (s_done) break;
     if (wave == 0) {
-      int64_t p = 0, q = 0;
-      const bool ok = DrawPair(c, r, steps, occ, N, lane, p, q);
-      if (lane == 0) {
-        if (ok) {
+      uint64_t m = __ballot(occ[cand.a] != occ[cand.b]);
+      for (uint32_t base = 64; !m && base < kMaxPairAttempts; base += 64) {
+        cand = DrawAttempt(c, r, steps, base + static_cast<uint32_t>(lane), N, n);
+        m = __ballot(occ[cand.a] != occ[cand.b]);
+      }
+      if (m) {
+        const int w = __builtin_ctzll(m);
+        const int64_t p = ReadLane64(cand.a, w), q = ReadLane64(cand.b, w);
+        const XYZ xp = ReadLaneXYZ(cand.xa, w), xq = ReadLaneXYZ(cand.xb, w);
+        bool hq = false, hp = false;
+        if (lane < kSiteSlots) {
+          const uint8_t vp = occ[SiteAt(xp, n, ox, oy, oz)];
+          const uint8_t vq = occ[SiteAt(xq, n, ox, oy, oz)];
+          L.env[0][lane] = vp < t.S ? vp : 0;  // codes are validated at creation
+          L.env[1][lane] = vq < t.S ? vq : 0;
+          if (lane >= 1) {  // partner slots: the first offset equal to the minimum image
+            hq = ox == MinImage(xq.X - xp.X, n) && oy == MinImage(xq.Y - xp.Y, n) &&
+                 oz == MinImage(xq.Z - xp.Z, n);
+            hp = ox == MinImage(xp.X - xq.X, n) && oy == MinImage(xp.Y - xq.Y, n) &&
+                 oz == MinImage(xp.Z - xq.Z, n);
+          }
+        }
+        const uint64_t mq = __ballot(hq), mp = __ballot(hp);
+        if (lane == 0) {
+          L.slot[0] = mq ? __builtin_ctzll(mq) : -1;
+          L.slot[1] = mp ? __builtin_ctzll(mp) : -1;
           s_p = p;
           s_q = q;
-        } else {
-          atomicOr(c.error, 2);
-          s_done = 1;
+          s_xp = xp;
+          s_xq = xq;
+          s_ep = occ[p];
+          s_eq = occ[q];
         }
+      } else if (lane == 0) {
+        atomicOr(c.error, 2);
+        s_done = 1;
       }
-    } else if (tid == 64) {
-      const U4 u = Philox4x32_10(c.seed[r], steps, c.rid[r], 0xFFFFFFFEu);
-      s_u = UnitOpenRight(Join64(u.x, u.y));
     }
     __syncthreads();
     if (s_done) break;
-    const int64_t p = s_p, q = s_q;
-    const int ep = occ[p], eq = occ[q];
-    GatherPair(L, occ, n, p, q, t.S);
+    {
+      const int ep = s_ep, eq = s_eq;
+      double v = ClusterDelta(L, cl, 0, ep, eq, L.slot[0]) + ClusterDelta(L, cl, 1, eq, ep, L.slot[1]);
+      v = WaveSum(v);
+      if (lane == 0) L.part[wave] = v;
+    }
     __syncthreads();
-    const double de = SwapDe(t, L, ep, eq);
-    if (tid == 0) {
+    if (tid == kAcceptThread) {
+      double de = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) de += L.part[w];
+      const int64_t p = s_p, q = s_q;
       double beta;
       if (c.anneal) {
         if (energy < A.lowest - 1e-4) A.lowest = energy;  // Dump (SimulatedAnnealing.cpp:80-97)
@@ -317,7 +419,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
                                        (static_cast<double>(steps) / static_cast<double>(maxs));
         beta = 1.0 / kBoltzmann / T;
       }
-      const bool accept = de < 0 || s_u < exp(-de * beta);
+      const bool accept = de < 0 || s_u[steps & 1] < exp(-de * beta);
       const uint64_t ti = trace_base + it;
       if (ti < c.trace_cap) {
         const uint64_t o = static_cast<uint64_t>(r) * c.trace_cap + ti;
@@ -328,19 +430,23 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
       }
       if (accept) {
         if (c.atom) {
-          const XYZ a = ToXYZ(p, n), b = ToXYZ(q, n);
+          const XYZ a = s_xp, b = s_xq;
           const int64_t o = c.off[r];
           AtomJump(c.atom + o, c.shift + 3 * o, p, q, ImageChange(b.X - a.X, n),
                    ImageChange(b.Y - a.Y, n), ImageChange(b.Z - a.Z, n));
         }
-        occ[p] = static_cast<uint8_t>(eq);
-        occ[q] = static_cast<uint8_t>(ep);
+        occ[p] = static_cast<uint8_t>(s_eq);
+        occ[q] = static_cast<uint8_t>(s_ep);
         energy += de;
         abse += de;
         ++acc;
       }
       if (c.anneal) AnnealUpdate(A, sched, accept, steps, energy);
       if (steps + 1 > maxs) s_done = 1;
+    } else if (wave == 0) {
+      cand = DrawAttempt(c, r, steps + 1, static_cast<uint32_t>(lane), N, n);
+    } else if (tid == 64) {
+      s_u[(steps + 1) & 1] = TrialUniform(c, r, steps + 1);
     }
     ++steps;
     __syncthreads();
@@ -350,7 +456,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     uint32_t *dst = reinterpret_cast<uint32_t *>(c.occ + c.off[r]);
     for (uint64_t i = tid; i < N / 4; i += kThreads) dst[i] = occ_words[i];
   }
-  if (tid == 0) {
+  if (tid == kAcceptThread) {
     c.steps[r] = steps;
     c.accepted[r] = acc;
     c.energy[r] = energy;
