@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <set>
 #include <stdexcept>
@@ -355,6 +356,44 @@ class EnergyPredictor {
     one.out.Download(&e, sizeof(e));
     lmc::Check(lmc_check_errors(m_.get()));
     return e;
+  }
+  // GetEncode (EnergyPredictor.cpp:40-96): counts / normalisers in ECI key order.
+  [[nodiscard]] std::vector<double> GetEncode(const cfg::Config &config) const {
+    detail::One one(config.GetNumCells());
+    int32_t n_eci = 0;
+    lmc::Check(lmc_model_layout(m_.get(), &n_eci, nullptr, nullptr));
+    lmc::DeviceBuffer out(sizeof(double) * static_cast<size_t>(n_eci));
+    lmc::Check(lmc_total_encode(m_.get(), config.DeviceOccupancy(), one.off.as<int64_t>(),
+                                one.ncell.as<int32_t>(), 1, out.as<double>(), nullptr));
+    std::vector<double> enc(static_cast<size_t>(n_eci));
+    out.Download(enc.data(), sizeof(double) * enc.size());
+    lmc::Check(lmc_check_errors(m_.get()));
+    return enc;
+  }
+  // GetEnergyOfCluster (EnergyPredictor.cpp:179-187): the clusters inside the atoms' three
+  // neighbour shells.
+  [[nodiscard]] double GetEnergyOfCluster(const cfg::Config &config,
+                                          const std::vector<size_t> &atom_id_list) const {
+    const auto &l2a = config.GetLatticeToAtomVector();
+    std::vector<int64_t> site_of(l2a.size());
+    for (size_t s = 0; s < l2a.size(); ++s) site_of[static_cast<size_t>(l2a[s])] = static_cast<int64_t>(s);
+    std::vector<int64_t> sites;
+    for (size_t a : atom_id_list) sites.push_back(site_of.at(a));
+    double e = 0;
+    lmc::Check(lmc_cluster_energy(m_.get(), config.DeviceOccupancy(), config.GetNumCells(),
+                                  sites.data(), static_cast<int32_t>(sites.size()), nullptr, &e));
+    lmc::Check(lmc_check_errors(m_.get()));
+    return e;
+  }
+  // GetChemicalPotential(solvent) (EnergyPredictor.cpp:194-212), by element string.
+  [[nodiscard]] std::map<std::string, double> GetChemicalPotential(const std::string &solvent) const {
+    const int32_t S = lmc_model_num_species(m_.get());
+    std::vector<double> mu(static_cast<size_t>(S));
+    lmc::Check(lmc_chemical_potential(m_.get(), solvent.c_str(), mu.data()));
+    lmc::Check(lmc_check_errors(m_.get()));
+    std::map<std::string, double> out;
+    for (int32_t c = 0; c < S; ++c) out[lmc_model_species_name(m_.get(), c)] = mu[static_cast<size_t>(c)];
+    return out;
   }
   [[nodiscard]] lmc_model *handle() const { return m_.get(); }
 

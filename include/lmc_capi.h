@@ -197,6 +197,23 @@ int lmc_site_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offse
  * (lmc/pred/src/EnergyPredictor.cpp:173-177). */
 int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
                      const int32_t *d_ncell, int32_t n_replicas, double *d_energy, void *stream);
+/* EnergyPredictor::GetEncode (lmc/pred/src/EnergyPredictor.cpp:40-96): per replica the ordered
+ * cluster counts divided by the total-energy normalisers, d_encode[R][n_eci] (n_eci from
+ * lmc_model_layout, in ECI key order); theta . encode is lmc_total_energy's value. */
+int lmc_total_encode(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                     const int32_t *d_ncell, int32_t n_replicas, double *d_encode, void *stream);
+/* EnergyPredictor::GetEncodeOfCluster / GetEnergyOfCluster (EnergyPredictor.cpp:97-187): the
+ * encode (host encode[n_eci], may be null) and energy (host, may be null) of the clusters whose
+ * sites all lie in the union of sites[k] and their three neighbour shells, in the device
+ * occupancy d_occ of one n^3-cell box. Sites are lattice ids (the reference takes atom ids).
+ * Synchronous. */
+int lmc_cluster_energy(const lmc_model *m, const uint8_t *d_occ, int32_t n, const int64_t *sites,
+                       int32_t k, double *encode, double *energy);
+/* EnergyPredictor::GetChemicalPotential(solvent) (EnergyPredictor.cpp:194-212): mu[code] =
+ * E(15^3-cell solvent box with one site of that species) - E(pure solvent box) for every other
+ * species of the model, X included, and 0 for the solvent; the S boxes are one GPU batch.
+ * Synchronous. */
+int lmc_chemical_potential(const lmc_model *m, const char *solvent, double *mu);
 /* Returns LMC_ERR_MODEL (and clears the flag) if a kernel of model m met an unknown cluster
  * type or an element without barrier parameters since the last check. Synchronous. */
 int lmc_check_errors(const lmc_model *m);

@@ -206,6 +206,41 @@ def total_energy(model, batch, stream=None):
     return e
 
 
+def total_encode(model, batch, stream=None):
+    """EnergyPredictor::GetEncode of every replica: [R, n_eci] float64 device tensor (ECI key
+    order, counts divided by the total-energy normalisers)."""
+    torch = _torch()
+    n_eci = C.c_int32()
+    check(lib().lmc_model_layout(model.h, C.byref(n_eci), None, None))
+    out = torch.empty((batch.R, n_eci.value), dtype=torch.float64, device=batch.occ.device)
+    check(lib().lmc_total_encode(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
+                                 batch.R, dptr(out), capi.stream_ptr(stream)))
+    return out
+
+
+def cluster_energy(model, occ, n, sites):
+    """EnergyPredictor::GetEnergyOfCluster / GetEncodeOfCluster of lattice ids `sites` in one
+    n^3-cell occupancy (uint8 device tensor): (energy, encode [n_eci])."""
+    torch = _torch()
+    occ = torch.as_tensor(occ).to(dtype=torch.uint8, device="cuda").contiguous()
+    s = np.ascontiguousarray(np.asarray(sites, np.int64).ravel())
+    n_eci = C.c_int32()
+    check(lib().lmc_model_layout(model.h, C.byref(n_eci), None, None))
+    enc = np.zeros(n_eci.value)
+    e = C.c_double()
+    check(lib().lmc_cluster_energy(model.h, C.c_void_p(occ.data_ptr()), int(n),
+                                   ptr(s, capi._i64p), s.size, ptr(enc, capi._f64p), C.byref(e)))
+    return e.value, enc
+
+
+def chemical_potential(model, solvent):
+    """EnergyPredictor::GetChemicalPotential(solvent): {element: mu} for every species of the
+    model (X included; 0 for the solvent), from 15^3-cell boxes on the GPU."""
+    mu = np.zeros(len(model.species))
+    check(lib().lmc_chemical_potential(model.h, solvent.encode(), ptr(mu, capi._f64p)))
+    return {e: float(mu[i]) for i, e in enumerate(model.species)}
+
+
 def _set_atoms(fn, h, ncell, lattice_to_atom, map_shift):
     """Atom tracking of a driver (lmc_kmc_set_atoms / lmc_cmc_set_atoms): per-replica lists of
     the atom on every site (a permutation of that replica's atom ids) and map_shift [N, 3] by

@@ -498,8 +498,6 @@ void ComputeTotalEnergy(const lmc_model *em, const uint8_t *d_occ, const int64_t
   Check(e, "total energy kernel");
 }
 
-}  // namespace
-
 // Host copies of a batch's offsets and the total site count.
 std::vector<int64_t> HostOffsets(int32_t R, const int64_t *d_off, const int32_t *d_ncell,
                                  int64_t &total) {
@@ -560,6 +558,8 @@ void DownloadAtoms(int32_t R, const int64_t *d_off, const int32_t *d_ncell, cons
   }
   if (map_shift) ToHost(map_shift, d_shift, static_cast<size_t>(3 * total));
 }
+
+}  // namespace
 
 struct lmc_kmc {
   const lmc_model *bm = nullptr;
@@ -991,6 +991,106 @@ int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_
   return Guard([&] {
     RequireModel(m);
     ComputeTotalEnergy(m, d_occ, d_offset, d_ncell, n_replicas, d_energy, S(stream));
+    return LMC_OK;
+  });
+}
+
+int lmc_total_encode(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                     const int32_t *d_ncell, int32_t n_replicas, double *d_encode, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    if (n_replicas <= 0 || !d_encode) throw std::invalid_argument("bad argument");
+    const auto &t = lmc::Upload(*m->m);
+    unsigned long long *counts =
+        DevAlloc<unsigned long long>(static_cast<size_t>(n_replicas) * t.n_eci);
+    const hipError_t e = lmc::LaunchTotalEncode(t, d_occ, d_offset, d_ncell, n_replicas, counts,
+                                                d_encode, nullptr, S(stream));
+    (void)hipStreamSynchronize(S(stream));
+    (void)hipFree(counts);
+    Check(e, "encode kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_cluster_energy(const lmc_model *m, const uint8_t *d_occ, int32_t n, const int64_t *sites,
+                       int32_t k, double *encode, double *energy) {
+  return Guard([&] {
+    RequireModel(m);
+    if (!d_occ || !sites || k <= 0 || n < 4 || n > 1000) throw std::invalid_argument("bad argument");
+    const int64_t N = 4ll * n * n * n;
+    std::vector<int64_t> set;
+    for (int32_t i = 0; i < k; ++i) {
+      if (sites[i] < 0 || sites[i] >= N) throw std::invalid_argument("site out of range");
+      set.push_back(sites[i]);
+      for (int shell = 1; shell <= 3; ++shell) {
+        int64_t nb[24];
+        const int c = lmc_fcc_neighbors(n, sites[i], shell, nb);
+        set.insert(set.end(), nb, nb + c);
+      }
+    }
+    std::sort(set.begin(), set.end());
+    set.erase(std::unique(set.begin(), set.end()), set.end());
+    if (set.size() > static_cast<size_t>(std::numeric_limits<int32_t>::max()))
+      throw std::invalid_argument("too many sites");
+    const auto &t = lmc::Upload(*m->m);
+    int64_t *d_set = DevCopy(set);
+    unsigned long long *counts = DevAlloc<unsigned long long>(static_cast<size_t>(t.n_eci));
+    double *d_enc = DevAlloc<double>(static_cast<size_t>(t.n_eci));
+    double *d_e = DevAlloc<double>(1);
+    hipError_t e = lmc::LaunchClusterEncode(t, d_occ, n, d_set, static_cast<int32_t>(set.size()),
+                                            counts, d_enc, d_e, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && encode)
+      e = hipMemcpy(encode, d_enc, sizeof(double) * t.n_eci, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && energy) e = hipMemcpy(energy, d_e, sizeof(double), hipMemcpyDeviceToHost);
+    DevFree(d_set);
+    DevFree(counts);
+    DevFree(d_enc);
+    DevFree(d_e);
+    Check(e, "cluster encode kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_chemical_potential(const lmc_model *m, const char *solvent, double *mu) {
+  return Guard([&] {
+    RequireModel(m);
+    if (!solvent || !mu) throw std::invalid_argument("null argument");
+    const Model &md = *m->m;
+    int sv = -1;
+    for (int c = 0; c < md.S; ++c)
+      if (md.eci.species[c] == lmc::CanonicalElement(solvent)) sv = c;
+    if (sv < 0) throw std::invalid_argument(std::string("solvent ") + solvent + " is not in the model");
+    // GenerateFCC({15, 15, 15}, solvent), and per other species the same box with atom 0 (lattice
+    // site 0) replaced (EnergyPredictor.cpp:194-212): one batch of S replicas on the GPU.
+    const int n = 15, S_ = md.S;
+    const int64_t N = 4ll * n * n * n;
+    std::vector<uint8_t> occ(static_cast<size_t>(N * S_), static_cast<uint8_t>(sv));
+    std::vector<int64_t> off(S_);
+    for (int c = 0; c < S_; ++c) {
+      off[c] = c * N;
+      occ[static_cast<size_t>(c * N)] = static_cast<uint8_t>(c);
+    }
+    uint8_t *d_occ = DevCopy(occ);
+    int64_t *d_off = DevCopy(off);
+    int32_t *d_nc = DevCopy(std::vector<int32_t>(S_, n));
+    double *d_e = DevAlloc<double>(S_);
+    std::vector<double> e(S_);
+    try {
+      ComputeTotalEnergy(m, d_occ, d_off, d_nc, S_, d_e, nullptr);
+      ToHost(e.data(), d_e, e.size());
+    } catch (...) {
+      DevFree(d_occ);
+      DevFree(d_off);
+      DevFree(d_nc);
+      DevFree(d_e);
+      throw;
+    }
+    DevFree(d_occ);
+    DevFree(d_off);
+    DevFree(d_nc);
+    DevFree(d_e);
+    for (int c = 0; c < S_; ++c) mu[c] = c == sv ? 0.0 : e[c] - e[sv];
     return LMC_OK;
   });
 }

@@ -180,6 +180,14 @@ hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off
                         int32_t R, double *de, hipStream_t s);
 hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_t trace_base,
                      hipStream_t s);
+// GetEncode of every replica: encode[R][n_eci] and energy[R] (either may be null).
+hipError_t LaunchTotalEncode(const KTables &t, const uint8_t *occ, const int64_t *off,
+                             const int32_t *ncell, int32_t R, unsigned long long *counts,
+                             double *encode, double *energy, hipStream_t s);
+// GetEncodeOfCluster over the sorted unique sites set[m] of one n^3-cell occupancy.
+hipError_t LaunchClusterEncode(const KTables &t, const uint8_t *occ, int32_t n, const int64_t *set,
+                               int32_t m, unsigned long long *counts, double *encode,
+                               double *energy, hipStream_t s);
 hipError_t LaunchTotalEnergy(const KTables &t, const uint8_t *occ, const int64_t *off,
                              const int32_t *ncell, int32_t R, unsigned long long *counts,
                              double *energy, hipStream_t s);

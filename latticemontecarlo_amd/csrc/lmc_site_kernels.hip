@@ -2,6 +2,8 @@
 // CanonicalMc driver and the full-lattice cluster-expansion energy (EnergyPredictor).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lmc_dev_common.h"
 #include "lmc_device.h"
 #include "lmc_rng.h"
@@ -534,6 +536,85 @@ __global__ __launch_bounds__(kThreads) void TotalCountsKernel(KTables t, const u
     if (hist[i]) atomicAdd(&counts[static_cast<int64_t>(r) * t.n_eci + i], hist[i]);
 }
 
+// Membership of site s in the sorted list set[m].
+__device__ __forceinline__ bool InSet(const int64_t *set, int m, int64_t s) {
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (set[mid] < s) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < m && set[lo] == s;
+}
+
+// EnergyPredictor::GetEncodeOfCluster (EnergyPredictor.cpp:97-172): the ordered cluster counts
+// of the sites in set[m] (sorted, unique) whose member sites all lie in the set, enumerated as in
+// GetEncode: from every first site, its neighbours, and for 1NN neighbours the triplets closed
+// through the second site's 1NN and 2NN shells.
+__global__ __launch_bounds__(kThreads) void ClusterCountsKernel(KTables t, const uint8_t *occ, int n,
+                                                                const int64_t *set, int m,
+                                                                unsigned long long *counts) {
+  __shared__ unsigned int hist[512];
+  for (int i = threadIdx.x; i < t.n_eci; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const int S = t.S;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const int64_t s1 = set[i];
+    const XYZ c = ToXYZ(s1, n);
+    const int e1 = Code(occ[s1], S);
+    atomicAdd(&hist[t.eci1[e1]], 1u);
+    for (int a = 0; a < 12; ++a) {
+      const int ax = c_nn1[a][0], ay = c_nn1[a][1], az = c_nn1[a][2];
+      const int64_t s2 = SiteAt(c, n, ax, ay, az);
+      if (!InSet(set, m, s2)) continue;
+      const int e2 = Code(occ[s2], S);
+      atomicAdd(&hist[t.eci2[(0 * S + e1) * S + e2]], 1u);
+      for (int b = 0; b < 12; ++b) {
+        const int x = ax + c_nn1[b][0], y = ay + c_nn1[b][1], z = az + c_nn1[b][2];
+        const int sh = Shell(x, y, z);
+        if (!sh) continue;
+        const int64_t s3 = SiteAt(c, n, x, y, z);
+        if (!InSet(set, m, s3)) continue;
+        const int e3 = Code(occ[s3], S);
+        atomicAdd(&hist[t.eci3[(sh - 1) * S * S * S + (e1 * S + e2) * S + e3]], 1u);
+      }
+      for (int b = 0; b < 6; ++b) {
+        const int x = ax + c_nn2[b][0], y = ay + c_nn2[b][1], z = az + c_nn2[b][2];
+        if (Shell(x, y, z) != 3) continue;
+        const int64_t s3 = SiteAt(c, n, x, y, z);
+        if (!InSet(set, m, s3)) continue;
+        const int e3 = Code(occ[s3], S);
+        atomicAdd(&hist[t.eci3[3 * S * S * S + (e1 * S + e2) * S + e3]], 1u);
+      }
+    }
+    for (int a = 0; a < 6; ++a) {
+      const int64_t s2 = SiteAt(c, n, c_nn2[a][0], c_nn2[a][1], c_nn2[a][2]);
+      if (InSet(set, m, s2)) atomicAdd(&hist[t.eci2[(1 * S + e1) * S + Code(occ[s2], S)]], 1u);
+    }
+    for (int a = 0; a < 24; ++a) {
+      const int64_t s2 = SiteAt(c, n, c_nn3[a][0], c_nn3[a][1], c_nn3[a][2]);
+      if (InSet(set, m, s2)) atomicAdd(&hist[t.eci2[(2 * S + e1) * S + Code(occ[s2], S)]], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < t.n_eci; i += blockDim.x)
+    if (hist[i]) atomicAdd(&counts[i], hist[i]);
+}
+
+// The encode count / N_label of each replica (EnergyPredictor.cpp:89-95) and theta . encode.
+__global__ void EncodeKernel(KTables t, const unsigned long long *counts, int32_t R,
+                             double *encode, double *energy) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  double e = 0.0;
+  for (int i = 0; i < t.n_eci; ++i) {
+    const double x = static_cast<double>(counts[static_cast<int64_t>(r) * t.n_eci + i]) / t.norm_tot[i];
+    if (encode) encode[static_cast<int64_t>(r) * t.n_eci + i] = x;
+    e += t.theta[i] * x;
+  }
+  if (energy) energy[r] = e;
+}
+
 __global__ void TotalEnergyKernel(KTables t, const unsigned long long *counts, int32_t R,
                                   double *energy) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -599,6 +680,38 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
     hipLaunchKernelGGL(CmcKernel<false>, dim3(c.R), dim3(kThreads), 0, s, t, c, trials,
                        trace_base);
   }
+  return hipGetLastError();
+}
+
+hipError_t LaunchTotalEncode(const KTables &t, const uint8_t *occ, const int64_t *off,
+                             const int32_t *ncell, int32_t R, unsigned long long *counts,
+                             double *encode, double *energy, hipStream_t s) {
+  if (R <= 0) return hipSuccess;
+  const int64_t nc = static_cast<int64_t>(R) * t.n_eci;
+  hipLaunchKernelGGL(ZeroKernel, dim3((nc + 255) / 256), dim3(256), 0, s, counts, nc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(TotalCountsKernel, dim3(64, R), dim3(kThreads), 0, s, t, occ, off, ncell,
+                     counts);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(EncodeKernel, dim3((R + 63) / 64), dim3(64), 0, s, t, counts, R, encode, energy);
+  return hipGetLastError();
+}
+
+hipError_t LaunchClusterEncode(const KTables &t, const uint8_t *occ, int32_t n, const int64_t *set,
+                               int32_t m, unsigned long long *counts, double *encode,
+                               double *energy, hipStream_t s) {
+  hipLaunchKernelGGL(ZeroKernel, dim3((t.n_eci + 255) / 256), dim3(256), 0, s, counts,
+                     static_cast<int64_t>(t.n_eci));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int blocks = std::max(1, std::min(64, (m + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL(ClusterCountsKernel, dim3(blocks), dim3(kThreads), 0, s, t, occ, n, set, m,
+                     counts);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(EncodeKernel, dim3(1), dim3(64), 0, s, t, counts, 1, encode, energy);
   return hipGetLastError();
 }
 

@@ -100,6 +100,14 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
     std::printf("{\"jump\": [%zu, %zu], \"ea\": %.17g, \"de\": %.17g}\n", v, nb, ea, de);
   }
   std::printf("{\"energy\": %.17g}\n", energy.GetEnergy(c));
+  {
+    const auto enc = energy.GetEncode(c);
+    const auto mu = energy.GetChemicalPotential("Al");
+    const double ec = energy.GetEnergyOfCluster(c, {c.GetAtomIdFromLatticeId(v)});
+    std::printf("{\"encode_len\": %zu, \"encode0\": %.17g, \"mu_Mg\": %.17g, \"mu_X\": %.17g, "
+                "\"cluster_energy\": %.17g}\n",
+                enc.size(), enc[0], mu.at("Mg"), mu.at("X"), ec);
+  }
   const size_t p = (v + 11) % c.GetNumAtoms(), q = (v + 137) % c.GetNumAtoms();
   std::printf("{\"swap\": [%zu, %zu], \"de\": %.17g}\n", p, q,
               pairsite.GetDeFromLatticeIdPair(c, {p, q}));

@@ -51,6 +51,16 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
     for x in jumps:
         ea, de = ob.barrier_de(occ, *x["jump"])
         assert abs(x["ea"] - ea) <= 1e-9 and abs(x["de"] - de) <= 1e-9
+    x_enc = next(x for x in rows if "encode_len" in x)  # EnergyPredictor extras, C++ mirror
+    import latticemontecarlo_amd as L
+    import torch
+    gm = L.Model(path, els, L.ENERGY)
+    ref_enc = L.total_encode(gm, L.ReplicaBatch([n], [occ])).cpu().numpy()[0]
+    assert x_enc["encode_len"] == ref_enc.size and x_enc["encode0"] == ref_enc[0]
+    mu = L.chemical_potential(gm, "Al")
+    assert x_enc["mu_Mg"] == mu["Mg"] and x_enc["mu_X"] == mu["X"]
+    v0 = int(np.flatnonzero(occ == species.index("X"))[0])
+    assert x_enc["cluster_energy"] == L.cluster_energy(gm, torch.from_numpy(occ), n, [v0])[0]
     e = next(x for x in rows if "energy" in x)["energy"]
     assert abs(e - oe.total_energy(occ)) <= 1e-9 * max(1.0, abs(e))
     sw = next(x for x in rows if "swap" in x)
