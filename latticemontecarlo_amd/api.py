@@ -531,14 +531,34 @@ class CanonicalMc:
 
 class SimulatedAnnealing(CanonicalMc):
     """Batched SimulatedAnnealing (SimulatedAnnealing.cpp:99-185): swap trials with the
-    reference's adaptive schedule from T = t_start over maximum_steps trials; energies relative to
-    the starting configuration. Arguments as CanonicalMc without t_end."""
+    reference's adaptive schedule from T = t_start over maximum_steps trials. Energies start at
+    restart_energy (default 0); with solvent given (and an energy model), at the reference's
+    starting energy (reference_energy). Arguments as CanonicalMc without t_end."""
 
     def __init__(self, pairsite_model, energy_model, ncell, occ_list, t_start, seed, maximum_steps,
-                 **kwargs):
+                 solvent=None, **kwargs):
+        if solvent is not None:
+            if energy_model is None:
+                raise ValueError("the reference's starting energy needs an energy model")
+            kwargs["restart_energy"] = [
+                SimulatedAnnealing.reference_energy(energy_model, n, o, solvent)
+                for n, o in zip(np.broadcast_to(ncell, (len(occ_list),)), occ_list)]
         super().__init__(pairsite_model, energy_model, ncell, occ_list, t_start, t_start, seed,
                          maximum_steps, **kwargs)
         check(lib().lmc_cmc_set_annealing(self.h, 1))
+
+    @staticmethod
+    def reference_energy(energy_model, n, occ, solvent):
+        """The reference's starting energy (SimulatedAnnealing.cpp:59-72): E(config) - E(pure
+        solvent box of the same size) - sum over solute atoms of their chemical potentials."""
+        sp = energy_model.species
+        occ = np.asarray(occ, np.uint8)
+        pure = np.full_like(occ, sp.index(solvent))
+        e = total_energy(energy_model, ReplicaBatch([int(n), int(n)], [occ, pure])).cpu().numpy()
+        mu = chemical_potential(energy_model, solvent)
+        counts = np.bincount(occ, minlength=len(sp))
+        solution = sum(mu[el] * float(counts[c]) for c, el in enumerate(sp) if el != solvent)
+        return float(e[0] - e[1] - solution)
 
     def annealing_state(self):
         R = self.R

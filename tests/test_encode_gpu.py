@@ -140,3 +140,32 @@ def test_chemical_potential_matches_oracle(torch, coeffs):
         o = base.copy()
         o[0] = sp.index(el)
         assert abs(mu[el] - (oe.total_energy(o) - e0)) <= 1e-9
+
+
+def test_annealing_reference_starting_energy(torch, coeffs):
+    """SimulatedAnnealing's starting energy (SimulatedAnnealing.cpp:59-72) from the oracle's
+    total energies: E(config) - E(pure solvent) - sum of solute chemical potentials."""
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs["e0_3n"]
+    n = 5
+    ge = L.Model(path, els, L.ENERGY)
+    gp = L.Model(path, els, L.PAIRSITE)
+    sp = ge.species
+    occ = random_alloc(n, sp, 0.06, seed=12, vacancy=False)
+    sa = L.SimulatedAnnealing(gp, ge, [n], [occ], 900.0, 3, maximum_steps=100, solvent="Al")
+    e_start = sa.state()["energy"][0]
+    oe = O.Model(O.Lattice(n), path, els, O.ENERGY)
+    o15 = O.Model(O.Lattice(15), path, els, O.ENERGY)
+    base = np.full(4 * 15 ** 3, sp.index("Al"), np.uint8)
+    e15 = o15.total_energy(base)
+    mu = {}
+    for el in ("Mg", "Zn"):
+        o = base.copy()
+        o[0] = sp.index(el)
+        mu[el] = o15.total_energy(o) - e15
+    ref = (oe.total_energy(occ) - oe.total_energy(np.full_like(occ, sp.index("Al")))
+           - sum(mu[el] * int((occ == sp.index(el)).sum()) for el in ("Mg", "Zn")))
+    assert abs(e_start - ref) <= 1e-9 * max(1.0, abs(ref))
+    sa.run(100)
+    assert sa.state()["steps"][0] == 100
