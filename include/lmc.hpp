@@ -188,6 +188,12 @@ class Config {
   [[nodiscard]] size_t GetAtomIdFromLatticeId(size_t lattice_id) const {
     return static_cast<size_t>(atoms_.at(lattice_id));
   }
+  // Config::GetLatticeIdFromAtomId (Config.cpp:124-126): a search over the lattice.
+  [[nodiscard]] size_t GetLatticeIdFromAtomId(size_t atom_id) const {
+    for (size_t s = 0; s < atoms_.size(); ++s)
+      if (static_cast<size_t>(atoms_[s]) == atom_id) return s;
+    throw std::out_of_range("atom id " + std::to_string(atom_id));
+  }
   // lattice id -> atom id, and map_shift [atom id * 3 + axis]
   [[nodiscard]] const std::vector<int64_t> &GetLatticeToAtomVector() const { return atoms_; }
   [[nodiscard]] const std::vector<int32_t> &GetMapShiftList() const { return shifts_; }
@@ -418,6 +424,13 @@ class VacancyMigrationPredictor {
     lmc::Check(lmc_check_errors(m_.get()));
     return {ea, de};
   }
+  // GetBarrierAndDiffFromAtomIdPair: the same jump named by the atoms on the two sites.
+  [[nodiscard]] std::pair<double, double> GetBarrierAndDiffFromAtomIdPair(
+      const cfg::Config &config, const std::pair<size_t, size_t> &atom_id_jump_pair) const {
+    return GetBarrierAndDiffFromLatticeIdPair(
+        config, {config.GetLatticeIdFromAtomId(atom_id_jump_pair.first),
+                 config.GetLatticeIdFromAtomId(atom_id_jump_pair.second)});
+  }
   [[nodiscard]] lmc_model *handle() const { return m_.get(); }
 
  private:
@@ -438,6 +451,22 @@ class VacancyMigrationPredictorQuartic : public VacancyMigrationPredictor {
                                    const std::set<std::string> &element_set)
       : VacancyMigrationPredictor(predictor_filename, reference_config, element_set,
                                   LMC_MODEL_QUARTIC) {}
+};
+
+// The ...Lru predictors (VacancyMigrationPredictorE0Lru.h, VacancyMigrationPredictorQuarticLru.h)
+// take a cache size; every jump is recomputed on the GPU, so it is accepted and unused.
+class VacancyMigrationPredictorE0Lru : public VacancyMigrationPredictorE0 {
+ public:
+  VacancyMigrationPredictorE0Lru(const std::string &predictor_filename, const cfg::Config &reference_config,
+                                 const std::set<std::string> &element_set, size_t /*cache_size*/)
+      : VacancyMigrationPredictorE0(predictor_filename, reference_config, element_set) {}
+};
+class VacancyMigrationPredictorQuarticLru : public VacancyMigrationPredictorQuartic {
+ public:
+  VacancyMigrationPredictorQuarticLru(const std::string &predictor_filename,
+                                      const cfg::Config &reference_config,
+                                      const std::set<std::string> &element_set, size_t /*cache_size*/)
+      : VacancyMigrationPredictorQuartic(predictor_filename, reference_config, element_set) {}
 };
 
 class EnergyChangePredictorPairSite {
@@ -468,6 +497,17 @@ class EnergyChangePredictorPairSite {
                            one.ncell.as<int32_t>(), one.idx.as<int64_t>(), one.code.as<uint8_t>(), 1,
                            one.out.as<double>(), nullptr));
     return Result(one);
+  }
+  // GetDeFromAtomIdPair / GetDeFromAtomIdSite (EnergyChangePredictorPairSite.h:20-25): the same
+  // changes named by atom ids.
+  [[nodiscard]] double GetDeFromAtomIdPair(const cfg::Config &config,
+                                           const std::pair<size_t, size_t> &atom_id_jump_pair) const {
+    return GetDeFromLatticeIdPair(config, {config.GetLatticeIdFromAtomId(atom_id_jump_pair.first),
+                                           config.GetLatticeIdFromAtomId(atom_id_jump_pair.second)});
+  }
+  [[nodiscard]] double GetDeFromAtomIdSite(const cfg::Config &config, size_t atom_id,
+                                           const std::string &new_element) const {
+    return GetDeFromLatticeIdSite(config, config.GetLatticeIdFromAtomId(atom_id), new_element);
   }
   [[nodiscard]] lmc_model *handle() const { return m_.get(); }
 

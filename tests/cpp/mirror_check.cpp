@@ -113,6 +113,18 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
               pairsite.GetDeFromLatticeIdPair(c, {p, q}));
   std::printf("{\"site\": %zu, \"element\": \"Mg\", \"de\": %.17g}\n", p,
               pairsite.GetDeFromLatticeIdSite(c, p, "Mg"));
+  {  // the atom-id forms name the same sites
+    const size_t ap = c.GetAtomIdFromLatticeId(p), aq = c.GetAtomIdFromLatticeId(q);
+    const size_t av = c.GetAtomIdFromLatticeId(v);
+    const size_t nb0 = c.GetNeighborLatticeIdVector(v, 1)[0];
+    const pred::VacancyMigrationPredictorE0Lru lru(e0_json, c, els, 100000);
+    const bool same =
+        pairsite.GetDeFromAtomIdPair(c, {ap, aq}) == pairsite.GetDeFromLatticeIdPair(c, {p, q}) &&
+        pairsite.GetDeFromAtomIdSite(c, ap, "Mg") == pairsite.GetDeFromLatticeIdSite(c, p, "Mg") &&
+        lru.GetBarrierAndDiffFromAtomIdPair(c, {av, c.GetAtomIdFromLatticeId(nb0)}) ==
+            barrier.GetBarrierAndDiffFromLatticeIdPair(c, {v, nb0});
+    std::printf("{\"atom_forms\": %s}\n", same ? "true" : "false");
+  }
   lmc_kmc_replica prm{};
   prm.temperature = 600.0;
   prm.seed = 99;

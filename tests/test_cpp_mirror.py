@@ -51,6 +51,7 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
     for x in jumps:
         ea, de = ob.barrier_de(occ, *x["jump"])
         assert abs(x["ea"] - ea) <= 1e-9 and abs(x["de"] - de) <= 1e-9
+    assert next(x for x in rows if "atom_forms" in x)["atom_forms"] is True
     x_enc = next(x for x in rows if "encode_len" in x)  # EnergyPredictor extras, C++ mirror
     import latticemontecarlo_amd as L
     import torch
