@@ -1247,6 +1247,9 @@ int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *
     lmc::KmcDev s = k->d;
     s.trace_cap = 0;
     s.occ = nullptr;
+    s.log_every = 0;  // no log records from the timing runs
+    s.log = nullptr;
+    s.log_n = nullptr;
     std::vector<void *> owned;
     auto clone = [&](auto *&dst, auto *src, size_t n) {
       using T = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
@@ -1277,6 +1280,10 @@ int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *
         clone(s.disp, k->d.disp, 3 * R);
         clone(s.com, k->d.com, 3 * R);
         clone(s.error, k->d.error, 1);
+        if (k->d.atom) {  // tracked atoms are written by every jump
+          clone(s.atom, k->d.atom, static_cast<size_t>(k->total_sites));
+          clone(s.shift, k->d.shift, 3 * static_cast<size_t>(k->total_sites));
+        }
         Check(hipEventRecord(e0, nullptr), "hipEventRecord");
         Check(lmc::LaunchKmc(t, s, static_cast<uint64_t>(steps), 0, nullptr, f),
               "autotune KMC launch");

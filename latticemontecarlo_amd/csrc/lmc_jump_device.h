@@ -687,6 +687,9 @@ __device__ void LogStep(const KmcDev &k, const ReplicaLds &Q, int r, int64_t jum
 // One lane: advances the clock and applies a jump of the vacancy to site `jump` (offset
 // dx, dy, dz; migrating species mig) with barrier ea and energy change de
 // (KineticMcAbstract.cpp:147-181). `escaped`: IsEscaped of the step.
+// kRecord: atom tracking and log records are compiled in (taken when k.atom / k.log_every are
+// set); kernels launched without either use the variant without them.
+template <bool kRecord = true>
 __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint8_t *occ, int r,
                           uint64_t trace_index, int64_t jump, int dx, int dy, int dz, int mig,
                           double ea, double de, double dt, bool escaped) {
@@ -694,7 +697,7 @@ __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint
     Q.esc = 1;
     Q.steps = Q.max_steps + 1;
   }
-  if (k.log_every && LogDue(Q.steps, k.log_every)) LogStep(k, Q, r, jump, ea, de);
+  if (kRecord && k.log_every && LogDue(Q.steps, k.log_every)) LogStep(k, Q, r, jump, ea, de);
   if (trace_index < k.trace_cap) {
     const uint64_t o = static_cast<uint64_t>(r) * k.trace_cap + trace_index;
     k.tr_sel[o] = jump;
@@ -711,7 +714,7 @@ __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint
     Q.com[1] -= dy * kHalfLattice * wgt;
     Q.com[2] -= dz * kHalfLattice * wgt;
   }
-  if (k.atom) {  // the vacancy at Q.vac is the lhs of the reference's jump pair
+  if (kRecord && k.atom) {  // the vacancy at Q.vac is the lhs of the reference's jump pair
     const int64_t o = k.off[r];
     const int m = Q.m2, h = m >> 1;
     AtomJump(k.atom + o, k.shift + 3 * o, Q.vac, jump,
@@ -732,6 +735,7 @@ __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint
 }
 
 // One lane: applies the chosen event `sel` of the first-order step with energy change de.
+template <bool kRecord = true>
 __device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, ReplicaLds &Q,
                            uint8_t *occ, int r, uint64_t trace_index, int sel, double dt,
                            double de) {
@@ -751,10 +755,11 @@ __device__ void ApplyEvent(const KTables &t, const KmcDev &k, const JumpLds &L, 
   const int dir = L.dir[sel];
   int dx, dy, dz;
   Unpack16(L.off[dir * kEnvSlots + t.n_slot], dx, dy, dz);
-  ApplyJump(t, k, Q, occ, r, trace_index, L.snb[sel], dx, dy, dz, L.env[dir][t.n_slot], L.ea[sel],
+  ApplyJump<kRecord>(t, k, Q, occ, r, trace_index, L.snb[sel], dx, dy, dz, L.env[dir][t.n_slot], L.ea[sel],
             de, dt, all);
 }
 
+template <bool kRecord = true>
 __device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, ReplicaLds &Q,
                                uint8_t *occ, int r, uint64_t trace_index, int tid) {
   const int i = tid;
@@ -763,7 +768,7 @@ __device__ void SelectAndApply(const KTables &t, const KmcDev &k, JumpLds &L, Re
   int sel;
   double dt;
   if (!ChooseEvent(L, Q, i, sel, dt)) return;
-  ApplyEvent(t, k, L, Q, occ, r, trace_index, sel, dt, L.de[sel]);
+  ApplyEvent<kRecord>(t, k, L, Q, occ, r, trace_index, sel, dt, L.de[sel]);
 }
 
 // ---- Second-order (chain) steps: KineticMcChainOmpi (lmc/mc/src/KineticMcChainOmpi.cpp) ----
@@ -828,6 +833,7 @@ __device__ void ChainPass(JumpLds &L, ChainLds &C, const ReplicaLds &Q, int i) {
 
 // Lane 0: second-order probabilities and time (KineticMcChainOmpi.cpp:93-151; the rank sums in
 // neighbour order), the selection with u2 (KineticMcAbstract.cpp:106-123) and the jump.
+template <bool kRecord = true>
 __device__ void ChainSelectAndApply(const KTables &t, const KmcDev &k, const JumpLds &L,
                                     ChainLds &C, ReplicaLds &Q, uint8_t *occ, int n, int r,
                                     uint64_t trace_index) {
@@ -883,7 +889,7 @@ __device__ void ChainSelectAndApply(const KTables &t, const KmcDev &k, const Jum
   int dx, dy, dz;
   Unpack16(C.soff[sel], dx, dy, dz);
   const int64_t from = Q.vac;
-  ApplyJump(t, k, Q, occ, r, trace_index, C.sid[sel], dx, dy, dz, C.sm[sel], C.barrier[sel],
+  ApplyJump<kRecord>(t, k, Q, occ, r, trace_index, C.sid[sel], dx, dy, dz, C.sm[sel], C.barrier[sel],
             C.de[sel], dt, all);
   C.prev = from;  // KineticMcChainAbstract::OneStepSimulation (KineticMcAbstract.cpp:260-263)
 }

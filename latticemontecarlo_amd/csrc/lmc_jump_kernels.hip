@@ -23,6 +23,8 @@
 // fit the workgroup's share of the CU's LDS, otherwise they are read from global memory.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lmc_jump_device.h"
 
 namespace lmc {
@@ -122,7 +124,7 @@ __device__ __forceinline__ void UpdateTemperature(const KmcDev &k, ReplicaLds &Q
 }
 
 // Persistent KMC: one 256-thread workgroup per replica, up to four per CU.
-template <bool kSolvent, bool kBL>
+template <bool kSolvent, bool kBL, bool kRecord>
 __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout lay, KmcDev k,
                                                       uint64_t nsteps, uint64_t trace_base) {
   __shared__ JumpLds L;
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
     if (Q.done) break;
     UpdateTemperature(k, Q, tid);
     EvaluateJumps<kSolvent>(t, L, T, occ, n, VacancyXYZ(Q), sv, Q.beta, k.error, tid, &Q);
-    SelectAndApply(t, k, L, Q, occ, r, trace_base + it, tid);
+    SelectAndApply<kRecord>(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }
   if (tid == 0) StoreReplica(k, r, Q);
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
 
 // Persistent KMC, wide shape: one 768-thread workgroup per replica (wave k evaluates event k),
 // for batches of at most one replica per CU, where a step's latency sets the rate.
-template <bool kBL>
+template <bool kBL, bool kRecord>
 __global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynLayout lay,
                                                                  KmcDev k, uint64_t nsteps,
                                                                  uint64_t trace_base) {
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynL
     __syncthreads();
     WideEvent(t, L, T, Q.beta, k.error, tid);
     __syncthreads();
-    SelectAndApply(t, k, L, Q, occ, r, trace_base + it, tid);
+    SelectAndApply<kRecord>(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }
   if (tid == 0) StoreReplica(k, r, Q);
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynL
 // Each step evaluates the 12 jumps out of every first neighbour i of the vacancy k in the
 // configuration after the jump k -> i (12 passes of the dense evaluation: vacancy at i, site k
 // holding i's species), then lane 0 forms the second-order probabilities and time and jumps.
-template <bool kBL>
+template <bool kBL, bool kRecord>
 __global__ __launch_bounds__(kThreads, 4) void KmcChainKernel(KTables t, DynLayout lay, KmcDev k,
                                                            uint64_t nsteps, uint64_t trace_base) {
   __shared__ JumpLds L;
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcChainKernel(KTables t, DynLayo
       if (tid == 0) ChainPass(L, C, Q, i);
       __syncthreads();
     }
-    if (tid == 0) ChainSelectAndApply(t, k, L, C, Q, occ, n, r, trace_base + it);
+    if (tid == 0) ChainSelectAndApply<kRecord>(t, k, L, C, Q, occ, n, r, trace_base + it);
     __syncthreads();
   }
   if (tid == 0) {
@@ -338,25 +340,34 @@ hipError_t LaunchVacancyScan(const KTables &t, const uint8_t *occ, int32_t n, in
 // The instantiation of form (0 dense, 1 solvent baseline, 2 dense wide, 3 second-order chain
 // steps) and LDS placement, with its block size; f(kernel, threads).
 template <typename F>
-hipError_t WithKmcKernel(int form, bool b1_lds, F &&f) {
-  if (form == 3) return b1_lds ? f(KmcChainKernel<true>, kThreads) : f(KmcChainKernel<false>, kThreads);
-  if (form == 2) return b1_lds ? f(KmcWideKernel<true>, kWideThreads) : f(KmcWideKernel<false>, kWideThreads);
-  if (form == 1) return b1_lds ? f(KmcKernel<true, true>, kThreads) : f(KmcKernel<true, false>, kThreads);
-  return b1_lds ? f(KmcKernel<false, true>, kThreads) : f(KmcKernel<false, false>, kThreads);
+hipError_t WithKmcKernel(int form, bool b1_lds, bool record, F &&f) {
+  auto pick = [&](auto rec) {
+    constexpr bool kR = decltype(rec)::value;
+    if (form == 3)
+      return b1_lds ? f(KmcChainKernel<true, kR>, kThreads) : f(KmcChainKernel<false, kR>, kThreads);
+    if (form == 2)
+      return b1_lds ? f(KmcWideKernel<true, kR>, kWideThreads) : f(KmcWideKernel<false, kR>, kWideThreads);
+    if (form == 1)
+      return b1_lds ? f(KmcKernel<true, true, kR>, kThreads) : f(KmcKernel<true, false, kR>, kThreads);
+    return b1_lds ? f(KmcKernel<false, true, kR>, kThreads) : f(KmcKernel<false, false, kR>, kThreads);
+  };
+  return record ? pick(std::true_type{}) : pick(std::false_type{});
 }
 
 hipError_t PrepareKmc(const KTables &t, int32_t R, int form) {
   if (R <= 0) return hipSuccess;
   const DynLayout lay = PlanLds(t, R, form);
-  return WithKmcKernel(form, lay.b1_lds != 0,
-                       [&](auto kernel, int) { return AllowLds(kernel, lay.bytes); });
+  auto allow = [&](auto kernel, int) { return AllowLds(kernel, lay.bytes); };
+  const hipError_t e = WithKmcKernel(form, lay.b1_lds != 0, false, allow);
+  return e != hipSuccess ? e : WithKmcKernel(form, lay.b1_lds != 0, true, allow);
 }
 
 hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t trace_base,
                      hipStream_t s, int form) {
   if (k.R <= 0 || steps == 0) return hipSuccess;
   const DynLayout lay = PlanLds(t, k.R, form);
-  return WithKmcKernel(form, lay.b1_lds != 0, [&](auto kernel, int threads) {
+  const bool record = k.atom != nullptr || k.log_every != 0;
+  return WithKmcKernel(form, lay.b1_lds != 0, record, [&](auto kernel, int threads) {
     hipError_t e = AllowLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kernel, dim3(k.R), dim3(threads), lay.bytes, s, t, lay, k, steps,

@@ -177,15 +177,22 @@ def test_chain_kmc_trajectory_matches_oracle(torch, coeffs, name, kind, opts):
     _compare_kmc(gst, gtr, gocc, refs, steps)
 
 
-def test_kmc_autotune_keeps_results(torch, coeffs):
-    """The autotuner times every kernel shape without changing the replicas' state."""
+def test_kmc_autotune_keeps_results(torch, coeffs, tmp_path):
+    """The autotuner times every kernel shape without changing the replicas' state, tracked
+    atoms or log records."""
     import latticemontecarlo_amd as L
     path, els = coeffs["e0_3"]
     gb = L.Model(path, els, L.E0)
     n, R = 6, 8
+    N = 4 * n ** 3
     occs = [random_alloc(n, gb.species, 0.03, seed=70 + r) for r in range(R)]
+    rng = np.random.default_rng(4)
+    l2a = [rng.permutation(N) for _ in range(R)]
     a = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
     b = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 6)
+    for d in (a, b):
+        d.set_atoms(l2a)
+        d.set_log(10, capacity=1000)
     info = a.autotune(steps=50)
     assert info["formulation"] in (0, 1, 2) and info["ms_dense"] > 0 and info["ms_solvent"] > 0
     assert info["ms_wide"] > 0  # 8 replicas: every one has a CU of its own
@@ -195,6 +202,16 @@ def test_kmc_autotune_keeps_results(torch, coeffs):
     assert np.array_equal(sa["vacancy"], sb["vacancy"])
     assert np.array_equal(sa["steps"], sb["steps"])
     np.testing.assert_allclose(sa["time"], sb["time"], rtol=1e-12)
+    (aa, ash), (ba, bsh) = a.atoms(), b.atoms()
+    assert all(np.array_equal(x, y) for x, y in zip(aa, ba))
+    assert all(np.array_equal(x, y) for x, y in zip(ash, bsh))
+    fa, fb = tmp_path / "a.txt", tmp_path / "b.txt"
+    a.write_log([fa] + [None] * (R - 1))
+    b.write_log([fb] + [None] * (R - 1))
+    la, lb = fa.read_text().split("\n"), fb.read_text().split("\n")
+    assert len(la) == len(lb) == 99 + 10 + 2  # header, 109 records (0-98, 100, 110-190), end
+    # same steps, atoms and positions; times agree to rounding (shapes sum in other orders)
+    assert [x.split("\t")[0::6] for x in la] == [x.split("\t")[0::6] for x in lb]
 
 
 def test_kmc_options_match_oracle(torch, coeffs):
