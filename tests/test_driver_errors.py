@@ -1,0 +1,74 @@
+"""Argument checks of the batch drivers (lmc_kmc_create, lmc_cmc_create): empty and ragged
+batches, box sizes out of range, species codes, vacancy counts and parameters. Every one fails
+on the host before any device memory is touched, so these run without a GPU."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import random_alloc
+
+import latticemontecarlo_amd as L
+from latticemontecarlo_amd import capi
+
+
+@pytest.fixture(scope="module")
+def models(coeffs):
+    path, els = coeffs["e0_3"]
+    return L.Model(path, els, L.E0), L.Model(path, els, L.PAIRSITE)
+
+
+def _kmc(model, ncell, occs, **kw):
+    return L.KineticMcFirst(model, None, ncell, occs, kw.pop("T", 500.0), 1, maximum_steps=10,
+                            **kw)
+
+
+def test_kmc_rejects_bad_batches(models):
+    gb, _ = models
+    sp = gb.species
+    good = random_alloc(4, sp, 0.05, seed=1)
+    h = C.c_void_p()
+    params = (capi.KmcReplica * 1)()
+    nc = np.array([4], np.int32)
+    with pytest.raises(capi.LmcError, match="null argument"):
+        capi.check(capi.lib().lmc_kmc_create(gb.h, None, 1, None, None, params, C.byref(h)))
+    with pytest.raises(capi.LmcError, match="n_replicas must be positive"):  # an empty batch
+        capi.check(capi.lib().lmc_kmc_create(gb.h, None, 0, capi.ptr(nc, capi._i32p),
+                                             capi.ptr(good, capi._u8p), params, C.byref(h)))
+    for n in (3, 1001):
+        with pytest.raises(capi.LmcError, match=r"lattice size n must be in \[4, 1000\]"):
+            _kmc(gb, [4, n], [good, good])
+    bad = good.copy()
+    bad[7] = len(sp)
+    with pytest.raises(capi.LmcError, match="species code >= S"):
+        _kmc(gb, [4], [bad])
+    none = np.where(good == sp.index("X"), sp.index("Al"), good).astype(np.uint8)
+    with pytest.raises(capi.LmcError, match="exactly one vacancy"):
+        _kmc(gb, [4], [none])
+    two = good.copy()
+    two[np.flatnonzero(good != sp.index("X"))[0]] = sp.index("X")
+    with pytest.raises(capi.LmcError, match="exactly one vacancy"):
+        _kmc(gb, [4, 4], [good, two])  # the ragged second replica is the bad one
+    with pytest.raises(capi.LmcError, match="temperature must be positive"):
+        _kmc(gb, [4], [good], T=0.0)
+
+
+def test_cmc_rejects_bad_batches(models):
+    _, gp = models
+    sp = gp.species
+    good = random_alloc(4, sp, 0.1, seed=2, vacancy=False)
+    one = np.full_like(good, sp.index("Al"))
+    with pytest.raises(capi.LmcError, match="at least two species"):
+        L.CanonicalMc(gp, None, [4, 4], [good, one], 500.0, 500.0, 1, maximum_steps=10)
+    with pytest.raises(capi.LmcError, match="must be positive"):
+        L.CanonicalMc(gp, None, [4], [good], 500.0, 500.0, 1, maximum_steps=0)
+    with pytest.raises(capi.LmcError, match="must be positive"):
+        L.CanonicalMc(gp, None, [4], [good], -1.0, 500.0, 1, maximum_steps=10)
+
+
+def test_kmc_needs_a_barrier_model(models, coeffs):
+    path, els = coeffs["e0_3"]
+    ge = L.Model(path, els, L.ENERGY)
+    occ = random_alloc(4, ge.species, 0.05, seed=3)
+    with pytest.raises(capi.LmcError, match="E0 or Quartic"):
+        _kmc(ge, [4], [occ])
