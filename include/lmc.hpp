@@ -168,6 +168,10 @@ class Config {
   void WriteConfig(const std::string &filename) const {
     lmc::Check(lmc_config_write(Handle().get(), filename.c_str()));
   }
+  // Config::WriteExtendedXyz(filename, {}, {}) (Config.cpp:712-809).
+  void WriteExtendedXyz(const std::string &filename) const {
+    lmc::Check(lmc_config_write_xyz(Handle().get(), filename.c_str()));
+  }
   // Config::WriteLattice / WriteElement / WriteMap (Config.cpp:887-922).
   void WriteLattice(const std::string &filename) const {
     lmc::Check(lmc_config_write_map(Handle().get(), filename.c_str(), nullptr, nullptr));

@@ -153,6 +153,10 @@ int lmc_config_set_basis(lmc_config *c, const double *basis);
 /* Config::WriteConfig(filename) (Config.cpp:663-710, gzip by the .gz extension): the same text,
  * byte for byte, as the reference writes for the same configuration. */
 int lmc_config_write(const lmc_config *c, const char *filename);
+/* Config::WriteExtendedXyz(filename, {}, {}) (Config.cpp:712-809): extended XYZ with the
+ * lattice, periodic flags and per atom (atom-id order) species, Cartesian position and map_shift;
+ * gzip by the .gz extension. Auxiliary and global lists are not written. */
+int lmc_config_write_xyz(const lmc_config *c, const char *filename);
 /* Config::WriteLattice / WriteElement / WriteMap (Config.cpp:887-922); a null name skips that
  * file. */
 int lmc_config_write_map(const lmc_config *c, const char *lattice_filename,

@@ -70,6 +70,7 @@ SIGNATURES = {
     "lmc_config_update": (C.c_int, [_vp, _i32, C.POINTER(C.c_char_p), _u8p, _i64p, _i32p]),
     "lmc_config_set_basis": (C.c_int, [_vp, _f64p]),
     "lmc_config_write": (C.c_int, [_vp, C.c_char_p]),
+    "lmc_config_write_xyz": (C.c_int, [_vp, C.c_char_p]),
     "lmc_config_write_map": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.c_char_p]),
     "lmc_model_layout": (C.c_int, [_vp, _i32p, _i32p, _i32p]),
     "lmc_model_eci_key": (C.c_int, [_vp, _i32, _i32p, _i32p]),

@@ -82,6 +82,10 @@ class Config:
         """Config::WriteConfig: .cfg, or gzip when the name ends in .gz."""
         check(lib().lmc_config_write(self.h, str(filename).encode()))
 
+    def write_xyz(self, filename):
+        """Config::WriteExtendedXyz without auxiliary lists (.gz by extension)."""
+        check(lib().lmc_config_write_xyz(self.h, str(filename).encode()))
+
     def write_map(self, lattice_filename=None, element_filename=None, map_filename=None):
         """Config::WriteLattice / WriteElement / WriteMap (None skips a file)."""
         enc = lambda f: None if f is None else str(f).encode()  # noqa: E731

@@ -450,6 +450,38 @@ int lmc_config_write(const lmc_config *c, const char *filename) {
   });
 }
 
+int lmc_config_write_xyz(const lmc_config *c, const char *filename) {
+  return Guard([&] {
+    if (!c || !filename) throw std::invalid_argument("null argument");
+    // Config::WriteExtendedXyz without auxiliary or global lists (Config.cpp:712-809):
+    // precision(16), every double through a vector (std::fixed), atoms in atom-id order with
+    // their Cartesian position (relative position times the basis) and map_shift.
+    const size_t N = c->atom.size();
+    std::vector<int64_t> site_of(N);
+    for (size_t s = 0; s < N; ++s) site_of[static_cast<size_t>(c->atom[s])] = static_cast<int64_t>(s);
+    std::string o = std::to_string(N) + "\nLattice=\"";
+    char buf[160];
+    for (int r = 0; r < 3; ++r) {
+      const int k = std::snprintf(buf, sizeof(buf), "%s%.16f %.16f %.16f", r ? " " : "",
+                                  c->basis[3 * r], c->basis[3 * r + 1], c->basis[3 * r + 2]);
+      o.append(buf, static_cast<size_t>(k));
+    }
+    o += "\" pbc=\"T T T\" Properties=species:S:1:pos:R:3:map_shift:I:3\n";
+    for (size_t a = 0; a < N; ++a) {
+      double rel[3], cart[3];
+      PositionOfSite(site_of[a], c->n, rel);
+      for (int d = 0; d < 3; ++d)  // Vector_d * Matrix_d (VectorMatrix.hpp:275-280)
+        cart[d] = rel[0] * c->basis[d] + rel[1] * c->basis[3 + d] + rel[2] * c->basis[6 + d];
+      o += c->element[a];
+      const int k = std::snprintf(buf, sizeof(buf), " %.16f %.16f %.16f %d %d %d \n", cart[0], cart[1],
+                                  cart[2], c->shift[3 * a], c->shift[3 * a + 1], c->shift[3 * a + 2]);
+      o.append(buf, static_cast<size_t>(k));
+    }
+    Spill(filename, o);
+    return LMC_OK;
+  });
+}
+
 int lmc_config_write_map(const lmc_config *c, const char *lattice_filename,
                          const char *element_filename, const char *map_filename) {
   return Guard([&] {

@@ -93,6 +93,18 @@ class OracleConfig:
             fixed = True
         return "".join(out)
 
+    def write_xyz_text(self):
+        """WriteExtendedXyz with no auxiliary or global lists (Config.cpp:712-809)."""
+        B = self.basis
+        lat = " ".join("%.16f %.16f %.16f" % tuple(B[r]) for r in range(3))
+        out = [f"{len(self.elements)}\n",
+               f'Lattice="{lat}" pbc="T T T" Properties=species:S:1:pos:R:3:map_shift:I:3\n']
+        for a, e in enumerate(self.elements):
+            p = self.positions[self.atom_to_lattice[a]]
+            cart = [p[0] * B[0, i] + p[1] * B[1, i] + p[2] * B[2, i] for i in range(3)]
+            out.append(f"{e} " + "%.16f %.16f %.16f" % tuple(cart) + " %d %d %d \n" % tuple(self.map_shift[a]))
+        return "".join(out)
+
     def neighbour_shells(self, s):
         """Ids of the 1st/2nd/3rd neighbour shells of lattice s, ascending (UpdateNeighbors),
         by minimum-image distances in units of a/2."""

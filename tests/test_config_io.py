@@ -195,3 +195,18 @@ def test_errors(tmp_path):
         CF.read_config(tmp_path / "off.cfg")
     with pytest.raises(capi.LmcError, match="not in the species list"):
         c.codes(["Mg", "X", "Zn"])
+
+
+def test_extended_xyz_matches_reference_route(tmp_path):
+    """WriteExtendedXyz of a scrambled configuration (plain and gzip) against the oracle."""
+    o = _scrambled(4, seed=5)
+    f = tmp_path / "s.cfg"
+    f.write_text(o.write_text())
+    c = CF.read_config(f)
+    c.write_xyz(tmp_path / "s.xyz")
+    c.write_xyz(tmp_path / "s.xyz.gz")
+    r = CO.read_text(o.write_text())
+    r.reassign()
+    want = r.write_xyz_text()
+    assert (tmp_path / "s.xyz").read_text() == want
+    assert gzip.open(tmp_path / "s.xyz.gz", "rt").read() == want
