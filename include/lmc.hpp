@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <limits>
 #include <map>
 #include <memory>
 #include <set>
@@ -829,6 +830,74 @@ class ExitTime {
         solvent_(config.CodeOf(solvent_element)),
         beta_escape_(1.0 / 8.617333262145e-5 / escape_temperature),
         predictor_(vacancy_migration_predictor) {}
+  // The reference's constructor (ExitTime.cpp:6-23), elements as strings; the pair-site
+  // predictor and chemical potentials serve GetProfileEnergy and GetBindingEnergy.
+  ExitTime(const cfg::Config &config, const std::string &solvent_element,
+           const std::set<std::string> & /*element_set*/, double escape_temperature,
+           double /*current_temperature*/, const pred::EnergyPredictor & /*energy_predictor*/,
+           const pred::VacancyMigrationPredictor &vacancy_migration_predictor,
+           const pred::EnergyChangePredictorPairSite &energy_change_predictor_site,
+           std::map<std::string, double> chemical_potential)
+      : ExitTime(config, solvent_element, escape_temperature, vacancy_migration_predictor) {
+    pairsite_ = &energy_change_predictor_site;
+    mu_ = std::move(chemical_potential);
+  }
+
+  // GetProfileEnergy (ExitTime.cpp:473-512), by lattice id.
+  [[nodiscard]] std::vector<double> GetProfileEnergy() const {
+    const size_t N = config_.GetNumAtoms();
+    const cfg::Config c = WithoutVacancy();
+    const uint8_t x = config_.CodeOf("X");
+    std::vector<int64_t> sites(N);
+    std::vector<uint8_t> from(c.GetCodes()), to(N, x);
+    for (size_t i = 0; i < N; ++i) sites[i] = static_cast<int64_t>(i);
+    std::vector<double> de = SiteChange(c, sites, from, to);
+    std::vector<double> prof(N);
+    for (size_t i = 0; i < N; ++i) prof[i] = de[i] + Mu(c.GetSpecies()[from[i]]) - Mu("X");
+    const int64_t v = Vacancy();
+    if (v >= 0) {
+      std::vector<int64_t> vs;
+      std::vector<uint8_t> vf, vt;
+      for (size_t e = 0; e < c.GetSpecies().size(); ++e)
+        if (c.GetSpecies()[e] != "X") {
+          vs.push_back(v);
+          vf.push_back(static_cast<uint8_t>(e));
+          vt.push_back(x);
+        }
+      const std::vector<double> dv = SiteChange(c, vs, vf, vt);
+      double best = -std::numeric_limits<double>::infinity();
+      for (size_t j = 0; j < dv.size(); ++j)
+        best = std::max(best, dv[j] + Mu(c.GetSpecies()[vf[j]]) - Mu("X"));
+      prof[static_cast<size_t>(v)] = best;
+    }
+    return prof;
+  }
+  // GetBindingEnergy (ExitTime.cpp:401-429): per element of the chemical potentials other than
+  // X, by lattice id.
+  [[nodiscard]] std::map<std::string, std::vector<double>> GetBindingEnergy() const {
+    const size_t N = config_.GetNumAtoms();
+    const cfg::Config c = WithoutVacancy();
+    const uint8_t x = config_.CodeOf("X");
+    std::vector<std::string> els;
+    for (const auto &[e, m] : mu_)
+      if (e != "X") els.push_back(e);
+    std::vector<int64_t> sites;
+    std::vector<uint8_t> from, to;
+    for (const auto &e : els)
+      for (size_t i = 0; i < N; ++i) {
+        sites.push_back(static_cast<int64_t>(i));
+        from.push_back(c.CodeOf(e));
+        to.push_back(x);
+      }
+    const std::vector<double> de = SiteChange(c, sites, from, to);
+    std::map<std::string, std::vector<double>> out;
+    for (size_t k = 0; k < els.size(); ++k) {
+      const double pot = Mu("X") - Mu(els[k]);
+      auto &v = out[els[k]];
+      for (size_t i = 0; i < N; ++i) v.push_back(de[k * N + i] - pot);
+    }
+    return out;
+  }
 
   // GetBarrierListAndExitTime (ExitTime.cpp:216-244): per lattice site, the 12 migration
   // barriers of a vacancy placed there (ascending), their mean, and the exit time
@@ -870,10 +939,52 @@ class ExitTime {
   }
 
  private:
+  int64_t Vacancy() const {
+    const uint8_t x = config_.CodeOf("X");
+    for (size_t i = 0; i < config_.GetNumAtoms(); ++i)
+      if (config_.GetCodes()[i] == x) return static_cast<int64_t>(i);
+    return -1;
+  }
+  cfg::Config WithoutVacancy() const {
+    cfg::Config c = config_;
+    const int64_t v = Vacancy();
+    if (v >= 0) c.SetAtomElementTypeAtLattice(static_cast<size_t>(v), config_.GetSpecies()[solvent_]);
+    return c;
+  }
+  double Mu(const std::string &e) const {
+    const auto it = mu_.find(e);
+    if (it == mu_.end()) throw std::invalid_argument("no chemical potential for " + e);
+    return it->second;
+  }
+  // lmc_site_change_de over the shared occupancy of c.
+  std::vector<double> SiteChange(const cfg::Config &c, const std::vector<int64_t> &sites,
+                                 const std::vector<uint8_t> &from, const std::vector<uint8_t> &to) const {
+    if (!pairsite_) throw std::invalid_argument("this analysis needs the pair-site predictor");
+    const size_t K = sites.size();
+    std::vector<int64_t> off(K, 0);
+    std::vector<int32_t> nc(K, c.GetNumCells());
+    lmc::DeviceBuffer d_s(K * 8), d_o(K * 8), d_n(K * 4), d_f(K), d_t(K), d_de(K * 8);
+    d_s.Upload(sites.data(), K * 8);
+    d_o.Upload(off.data(), K * 8);
+    d_n.Upload(nc.data(), K * 4);
+    d_f.Upload(from.data(), K);
+    d_t.Upload(to.data(), K);
+    lmc::Check(lmc_site_change_de(pairsite_->handle(), c.DeviceOccupancy(), d_o.as<int64_t>(),
+                                  d_n.as<int32_t>(), d_s.as<int64_t>(), d_f.as<uint8_t>(),
+                                  d_t.as<uint8_t>(), static_cast<int32_t>(K), d_de.as<double>(),
+                                  nullptr));
+    std::vector<double> de(K);
+    d_de.Download(de.data(), K * 8);
+    lmc::Check(lmc_check_errors(pairsite_->handle()));
+    return de;
+  }
+
   const cfg::Config &config_;
   int32_t solvent_;
   double beta_escape_;
   const pred::VacancyMigrationPredictor &predictor_;
+  const pred::EnergyChangePredictorPairSite *pairsite_ = nullptr;
+  std::map<std::string, double> mu_;
 };
 
 }  // namespace ansys

@@ -197,6 +197,13 @@ int lmc_swap_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offse
 int lmc_site_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
                 const int32_t *d_ncell, const int64_t *d_sites, const uint8_t *d_codes,
                 int32_t n_replicas, double *d_de, void *stream);
+/* Energy change of site d_sites[r] going from species d_from[r] to d_codes[r], the rest of the
+ * occupancy as it is: GetDeFromLatticeIdSite on the configuration with that site set to
+ * d_from[r] first (the ExitTime analyses, ExitTime.cpp:401-512, evaluate every site this way).
+ * Replicas may share one occupancy (equal offsets). */
+int lmc_site_change_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                       const int32_t *d_ncell, const int64_t *d_sites, const uint8_t *d_from,
+                       const uint8_t *d_codes, int32_t n_replicas, double *d_de, void *stream);
 /* Total cluster-expansion energy per replica: EnergyPredictor::GetEnergy
  * (lmc/pred/src/EnergyPredictor.cpp:173-177). */
 int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,

@@ -7,10 +7,11 @@ on the host is small: sorting and averaging the per-site barrier lists, the barr
 of a cluster and its first-neighbour shell, and the absorbing Markov chain of a cluster (a dense
 solve over the cluster's sites).
 
-Differences from the reference: sites are lattice ids (the engine has no separate atom ids),
-sets are iterated in ascending id order instead of hash order (sums differ only by rounding),
-and the base energies of the Markov chain (the reference's profile energies, which need
-chemical potentials) are an argument.
+Profile and binding energies (the Markov chain's base energies) are batched site energy changes
+on the GPU (lmc_site_change_de) with chemical potentials from api.chemical_potential.
+
+Differences from the reference: sites are lattice ids (atom ids live in cfg.Config), and sets
+are iterated in ascending id order instead of hash order (sums differ only by rounding).
 """
 import numpy as np
 
@@ -26,9 +27,10 @@ class ExitTime:
     tensor) of n^3 cells, the barrier model, and the solvent's species code."""
 
     def __init__(self, barrier_model, occ, n, solvent, escape_temperature,
-                 current_temperature=None):
+                 current_temperature=None, pairsite_model=None):
         torch = api._torch()
         self.model = barrier_model
+        self.pairsite = pairsite_model
         self.n = int(n)
         self.N = api.fcc_num_sites(self.n)
         self.solvent = int(solvent)
@@ -75,6 +77,68 @@ class ExitTime:
         pair = {(int(s), int(nb[i, j])): (float(ea[i, j]), float(de[i, j]))
                 for i, s in enumerate(plus) for j in range(12)}
         return pair, nbl, bl, dl
+
+    def _site_change(self, occ, sites, frm, to):
+        """Energy changes of sites[i] going from species frm[i] to to[i] in the device occupancy
+        occ (one batched launch of lmc_site_change_de over a shared occupancy)."""
+        if self.pairsite is None:
+            raise ValueError("this analysis needs the pair-site model (pairsite_model=)")
+        torch = api._torch()
+        K = len(sites)
+        dev = occ.device
+        off = torch.zeros(K, dtype=torch.int64, device=dev)
+        nc = torch.full((K,), self.n, dtype=torch.int32, device=dev)
+        s = torch.as_tensor(np.asarray(sites, np.int64), device=dev)
+        f = torch.as_tensor(np.asarray(frm, np.uint8), device=dev)
+        t = torch.as_tensor(np.asarray(to, np.uint8), device=dev)
+        de = torch.empty(K, dtype=torch.float64, device=dev)
+        api.check(api.lib().lmc_site_change_de(self.pairsite.h, api.dptr(occ), api.dptr(off),
+                                               api.dptr(nc), api.dptr(s), api.dptr(f),
+                                               api.dptr(t), K, api.dptr(de), None))
+        self.pairsite.check_errors()
+        return de.cpu().numpy()
+
+    def _without_vacancy(self):
+        occ = self.occ.clone()
+        if self.vacancy >= 0:
+            occ[self.vacancy] = self.solvent
+        return occ
+
+    def profile_energy(self, chemical_potential):
+        """GetProfileEnergy (ExitTime.cpp:473-512): per site of the configuration whose vacancy
+        holds the solvent, the energy of turning it into a vacancy plus mu(its element) - mu(X);
+        at the vacancy's own site the largest value over the elements it could hold.
+        chemical_potential: {element: mu} (api.chemical_potential)."""
+        sp = self.pairsite.species if self.pairsite is not None else self.model.species
+        x = sp.index("X")
+        occ = self._without_vacancy()
+        host = occ.cpu().numpy()
+        mu = np.array([chemical_potential[e] for e in sp])
+        de = self._site_change(occ, np.arange(self.N), host, np.full(self.N, x))
+        prof = de + mu[host] - mu[x]
+        if self.vacancy >= 0:
+            els = [c for c, e in enumerate(sp) if e != "X"]
+            dv = self._site_change(occ, [self.vacancy] * len(els), els, [x] * len(els))
+            prof[self.vacancy] = max(dv[i] + mu[c] - mu[x] for i, c in enumerate(els))
+        return prof
+
+    def binding_energy(self, chemical_potential):
+        """GetBindingEnergy (ExitTime.cpp:401-429): {element: [N]} over the elements of
+        chemical_potential other than X; entry s is the energy of turning site s, holding that
+        element, into a vacancy minus mu(X) - mu(element), in the configuration whose vacancy
+        holds the solvent."""
+        sp = self.pairsite.species
+        x = sp.index("X")
+        occ = self._without_vacancy()
+        els = [e for e in chemical_potential if e != "X"]
+        codes = np.repeat([sp.index(e) for e in els], self.N)
+        sites = np.tile(np.arange(self.N), len(els))
+        de = self._site_change(occ, sites, codes, np.full(sites.size, x))
+        out = {}
+        for i, e in enumerate(els):
+            pot = chemical_potential["X"] - chemical_potential[e]
+            out[e] = de[i * self.N:(i + 1) * self.N] - pot
+        return out
 
     @staticmethod
     def markov_escape_time(sites, neighbor_lists, barrier_lists, base_energy, beta):

@@ -57,6 +57,7 @@ SIGNATURES = {
     "lmc_model_species_code": (_i32, [_vp, C.c_char_p]),
     "lmc_model_species_name": (C.c_char_p, [_vp, _i32]),
     "lmc_model_set_formulation": (C.c_int, [_vp, _i32]),
+    "lmc_site_change_de": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lmc_total_encode": (C.c_int, [_vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lmc_cluster_energy": (C.c_int, [_vp, _vp, _i32, _i64p, _i32, _f64p, _f64p]),
     "lmc_chemical_potential": (C.c_int, [_vp, C.c_char_p, _f64p]),

@@ -979,8 +979,22 @@ int lmc_site_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offse
   return Guard([&] {
     RequireModel(m);
     const auto &t = lmc::Upload(*m->m);
-    Check(lmc::LaunchSiteDe(t, d_occ, d_offset, d_ncell, d_sites, d_codes, n_replicas, d_de,
-                            S(stream)),
+    Check(lmc::LaunchSiteDe(t, d_occ, d_offset, d_ncell, d_sites, nullptr, d_codes, n_replicas,
+                            d_de, S(stream)),
+          "site dE kernel");
+    return LMC_OK;
+  });
+}
+
+int lmc_site_change_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                       const int32_t *d_ncell, const int64_t *d_sites, const uint8_t *d_from,
+                       const uint8_t *d_codes, int32_t n_replicas, double *d_de, void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    if (!d_from) throw std::invalid_argument("null argument");
+    const auto &t = lmc::Upload(*m->m);
+    Check(lmc::LaunchSiteDe(t, d_occ, d_offset, d_ncell, d_sites, d_from, d_codes, n_replicas,
+                            d_de, S(stream)),
           "site dE kernel");
     return LMC_OK;
   });

@@ -175,9 +175,10 @@ hipError_t PrepareKmc(const KTables &t, int32_t R, int form);
 hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off,
                         const int32_t *ncell, const int64_t *pairs, int32_t R, double *de,
                         hipStream_t s);
+// from: optional species the sites change from (null: the occupancy's).
 hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off,
-                        const int32_t *ncell, const int64_t *sites, const uint8_t *codes,
-                        int32_t R, double *de, hipStream_t s);
+                        const int32_t *ncell, const int64_t *sites, const uint8_t *from,
+                        const uint8_t *codes, int32_t R, double *de, hipStream_t s);
 hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_t trace_base,
                      hipStream_t s);
 // GetEncode of every replica: encode[R][n_eci] and energy[R] (either may be null).

@@ -139,21 +139,23 @@ __global__ __launch_bounds__(kThreads) void SwapDeKernel(KTables t, const uint8_
   if (threadIdx.x == 0) de[r] = v;
 }
 
-// GetDeFromLatticeIdSite (EnergyChangePredictorPairSite.cpp:154-192)
+// GetDeFromLatticeIdSite (EnergyChangePredictorPairSite.cpp:154-192): site sites[r] changes from
+// species from[r] (null: the species it holds) to codes[r], the rest of the occupancy as is.
 __global__ __launch_bounds__(kThreads) void SiteDeKernel(KTables t, const uint8_t *occ,
                                                          const int64_t *off, const int32_t *ncell,
-                                                         const int64_t *sites,
+                                                         const int64_t *sites, const uint8_t *from,
                                                          const uint8_t *codes, double *de) {
   __shared__ SiteLds L;
   const int r = blockIdx.x;
   const uint8_t *o = occ + off[r];
-  if (!ValidSite(ncell[r], sites[r]) || o[sites[r]] >= t.S || codes[r] >= t.S) {
+  if (!ValidSite(ncell[r], sites[r]) || o[sites[r]] >= t.S || codes[r] >= t.S ||
+      (from && from[r] >= t.S)) {
     if (threadIdx.x == 0) de[r] = __builtin_nan("");
     return;
   }
   LoadSiteTables(t, L);
   __syncthreads();
-  const int eo = o[sites[r]], en = codes[r];
+  const int eo = from ? from[r] : o[sites[r]], en = codes[r];
   if (eo == en) {
     if (threadIdx.x == 0) de[r] = 0.0;
     return;
@@ -642,11 +644,11 @@ hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off
 }
 
 hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off,
-                        const int32_t *ncell, const int64_t *sites, const uint8_t *codes,
-                        int32_t R, double *de, hipStream_t s) {
+                        const int32_t *ncell, const int64_t *sites, const uint8_t *from,
+                        const uint8_t *codes, int32_t R, double *de, hipStream_t s) {
   if (R <= 0) return hipSuccess;
-  hipLaunchKernelGGL(SiteDeKernel, dim3(R), dim3(kThreads), 0, s, t, occ, off, ncell, sites, codes,
-                     de);
+  hipLaunchKernelGGL(SiteDeKernel, dim3(R), dim3(kThreads), 0, s, t, occ, off, ncell, sites, from,
+                     codes, de);
   return hipGetLastError();
 }
 

@@ -169,6 +169,14 @@ int GpuRun(int n, const std::string &occ_path, const std::string &e0_json, const
               "\"sur_fallbacks\": %llu, \"sur_mse_size\": %zu}\n",
               static_cast<unsigned long long>(su.steps[0]), su.time[0], static_cast<long long>(su.vacancy[0]),
               static_cast<unsigned long long>(events), static_cast<unsigned long long>(fallbacks), mse.size());
+  {  // profile and binding energies through the reference's ExitTime constructor
+    const ansys::ExitTime full(c, "Al", els, 500.0, 500.0, energy, barrier, pairsite,
+                               energy.GetChemicalPotential("Al"));
+    const auto prof = full.GetProfileEnergy();
+    const auto bind = full.GetBindingEnergy();
+    std::printf("{\"profile_vac\": %.17g, \"profile_7\": %.17g, \"binding_Mg_7\": %.17g}\n",
+                prof[v], prof[7], bind.at("Mg")[7]);
+  }
   // exit-time analysis of the starting configuration: three sites' barrier lists
   const ansys::ExitTime et(c, "Al", 500.0, barrier);
   const auto [lists, mean, exit_time] = et.GetBarrierListAndExitTime();

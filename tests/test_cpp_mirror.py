@@ -52,6 +52,16 @@ def test_mirror_gpu_matches_oracle(exe, coeffs, tmp_path):
         ea, de = ob.barrier_de(occ, *x["jump"])
         assert abs(x["ea"] - ea) <= 1e-9 and abs(x["de"] - de) <= 1e-9
     assert next(x for x in rows if "atom_forms" in x)["atom_forms"] is True
+    from latticemontecarlo_amd import ansys as A
+    pr = next(x for x in rows if "profile_vac" in x)
+    import latticemontecarlo_amd as L
+    gm_, gp_ = L.Model(path, els, L.E0), L.Model(path, els, L.PAIRSITE)
+    mu = L.chemical_potential(L.Model(path, els, L.ENERGY), "Al")
+    et = A.ExitTime(gm_, occ, n, species.index("Al"), 500.0, pairsite_model=gp_)
+    prof, bind = et.profile_energy(mu), et.binding_energy(mu)
+    v0 = int(np.flatnonzero(occ == species.index("X"))[0])
+    assert pr["profile_vac"] == prof[v0] and pr["profile_7"] == prof[7]
+    assert pr["binding_Mg_7"] == bind["Mg"][7]
     x_enc = next(x for x in rows if "encode_len" in x)  # EnergyPredictor extras, C++ mirror
     import latticemontecarlo_amd as L
     import torch

@@ -184,3 +184,46 @@ def _modified(occ, vac, sp, s):
     o[vac] = sp.index("Al")
     o[s] = sp.index("X")
     return o
+
+
+@pytest.mark.gpu
+def test_profile_and_binding_energy_match_oracle(torch, coeffs):
+    """GetProfileEnergy and GetBindingEnergy (ExitTime.cpp:401-512) from batched site energy
+    changes against the oracle's site dE on the same modified configurations."""
+    import latticemontecarlo_amd as L
+    import oracle as O
+    path, els = coeffs["e0_3n"]
+    n = 4
+    gm = L.Model(path, els, L.E0)
+    gp = L.Model(path, els, L.PAIRSITE)
+    ge = L.Model(path, els, L.ENERGY)
+    sp = gp.species
+    occ = random_alloc(n, sp, 0.1, seed=17)
+    vac = int(np.flatnonzero(occ == sp.index("X"))[0])
+    mu = L.chemical_potential(ge, "Al")
+    print(f"profile/binding energies n={n}", flush=True)
+    et = A.ExitTime(gm, occ, n, sp.index("Al"), 500.0, pairsite_model=gp)
+    prof = et.profile_energy(mu)
+    bind = et.binding_energy(mu)
+    op = O.Model(O.Lattice(n), path, els, O.PAIRSITE)
+    base = occ.copy()
+    base[vac] = sp.index("Al")
+    x = sp.index("X")
+    for s in range(et.N):
+        if s == vac:
+            ref = max(op.site_de(_with(base, s, sp.index(e)), s, x) + mu[e] - mu["X"]
+                      for e in sp if e != "X")
+        else:
+            ref = op.site_de(base, s, x) + mu[sp[base[s]]] - mu["X"]
+        assert abs(prof[s] - ref) <= 1e-9, s
+    assert set(bind) == {"Al", "Mg", "Zn"}
+    for e, v in bind.items():
+        for s in range(0, et.N, 7):
+            ref = op.site_de(_with(base, s, sp.index(e)), s, x) - (mu["X"] - mu[e])
+            assert abs(v[s] - ref) <= 1e-9, (e, s)
+
+
+def _with(occ, s, c):
+    o = occ.copy()
+    o[s] = c
+    return o
