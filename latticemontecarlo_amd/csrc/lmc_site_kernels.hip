@@ -16,9 +16,10 @@ struct SiteLds {
   uint32_t cl[kMaxSiteClusters];
   int32_t off[kSiteSlots];
   double w[kMaxW];
-  uint8_t env[2][48];
-  int slot[2];                 // partner slots of a swap (q around p, p around q)
-  double part[kThreads / 64];  // per-wave partial sums
+  uint8_t env[4][48];           // environments of up to two swaps (p, q of each)
+  int slot[4];                  // partner slots (q around p, p around q) of each swap
+  double part[kThreads / 64];   // per-wave partial sums
+  double part2[kThreads / 64];  // the same for the second swap
 };
 
 __device__ void LoadSiteTables(const KTables &t, SiteLds &L) {
@@ -199,11 +200,12 @@ __device__ void AnnealUpdate(CmcAnneal &A, const AnnealSchedule &s, bool accepte
     A.window_trials = 0;
     A.window_accepts = 0;
   }
-  const double acc_est = A.window_trials > 0u ? static_cast<double>(A.window_accepts) /
-                                                     static_cast<double>(A.window_trials)
-                                               : 1.0;
+  // the acceptance estimate last: the division only runs when the other conditions hold
   if (A.reheats < 5 && steps - A.last_improvement >= s.trigger &&
-      steps - A.last_reheat >= s.cooldown && acc_est < 0.05) {
+      steps - A.last_reheat >= s.cooldown &&
+      (A.window_trials > 0u ? static_cast<double>(A.window_accepts) /
+                                  static_cast<double>(A.window_trials)
+                            : 1.0) < 0.05) {
     A.T *= 1.10;
     A.last_improvement = steps;
     A.last_reheat = steps;
@@ -303,24 +305,84 @@ __device__ __forceinline__ double ClusterDelta(const SiteLds &L, const SiteClust
   return L.w[rest + en * q.kx] - L.w[rest + eo * q.kx];
 }
 
+// One trial's pair as found by its wave: sites, coordinates, species, and the winning lane of
+// the first draw round (-1: found in a later round; -2: none within kMaxPairAttempts).
+struct TrialLds {
+  int64_t p, q;
+  XYZ xp, xq;
+  int ep, eq, win;
+};
+
+// Finds the trial's pair among the wave's draw attempts (cand: attempts lane of step `step`,
+// further rounds drawn here), gathers both environments into L.env[e0], L.env[e0 + 1] and both
+// partner slots, and leaves the result in T (lane 0 writes). Called by a whole wave.
+__device__ void FindTrial(const CmcDev &c, int r, uint64_t step, const uint8_t *occ, uint64_t N,
+                          int n, int S, int lane, int ox, int oy, int oz, PairDraw &cand,
+                          SiteLds &L, int e0, TrialLds &T) {
+  uint64_t m = __ballot(occ[cand.a] != occ[cand.b]);
+  const bool first = m != 0;
+  for (uint32_t base = 64; !m && base < kMaxPairAttempts; base += 64) {
+    cand = DrawAttempt(c, r, step, base + static_cast<uint32_t>(lane), N, n);
+    m = __ballot(occ[cand.a] != occ[cand.b]);
+  }
+  if (!m) {
+    if (lane == 0) T.win = -2;
+    return;
+  }
+  const int w = __builtin_ctzll(m);
+  const int64_t p = ReadLane64(cand.a, w), q = ReadLane64(cand.b, w);
+  const XYZ xp = ReadLaneXYZ(cand.xa, w), xq = ReadLaneXYZ(cand.xb, w);
+  bool hq = false, hp = false;
+  if (lane < kSiteSlots) {
+    const uint8_t vp = occ[SiteAt(xp, n, ox, oy, oz)];
+    const uint8_t vq = occ[SiteAt(xq, n, ox, oy, oz)];
+    L.env[e0][lane] = vp < S ? vp : 0;  // codes are validated at creation
+    L.env[e0 + 1][lane] = vq < S ? vq : 0;
+    if (lane >= 1) {  // partner slots: the first offset equal to the minimum image
+      hq = ox == MinImage(xq.X - xp.X, n) && oy == MinImage(xq.Y - xp.Y, n) &&
+           oz == MinImage(xq.Z - xp.Z, n);
+      hp = ox == MinImage(xp.X - xq.X, n) && oy == MinImage(xp.Y - xq.Y, n) &&
+           oz == MinImage(xp.Z - xq.Z, n);
+    }
+  }
+  const uint64_t mq = __ballot(hq), mp = __ballot(hp);
+  if (lane == 0) {
+    L.slot[e0] = mq ? __builtin_ctzll(mq) : -1;
+    L.slot[e0 + 1] = mp ? __builtin_ctzll(mp) : -1;
+    T.p = p;
+    T.q = q;
+    T.xp = xp;
+    T.xq = xq;
+    T.ep = occ[p];
+    T.eq = occ[q];
+    T.win = first ? w : -1;
+  }
+}
+
 // CanonicalMcSerial (CanonicalMcSerial.cpp:40-51, CanonicalMcAbstract.cpp:43-101) with an
 // optional linear temperature ramp t0 -> t1 over maximum_steps, or the adaptive annealing
 // schedule. One workgroup per replica. kLdsOcc: the replica's occupancy (N bytes, N a multiple
 // of 4) lives in dynamic LDS for the launch and is written back at its end.
-// Per trial, three barriers:
-//   1. wave 0 finds the pair among its 64 precomputed draw attempts (ballot), gathers both
-//      43-site environments and both partner slots;
-//   2. every thread adds its cluster's change for both sites; DPP wave sums;
-//   3. lane 0 of wave 3 accepts or rejects and swaps, while wave 0 draws the next trial's 64
-//      attempts (sites and coordinates) and lane 0 of wave 1 its Metropolis uniform.
+// Two trials per round of three barriers: trial A (step s) and, speculatively, trial B (step
+// s + 1) on the same occupancy.
+//   1. wave 0 finds A's pair among its 64 precomputed draw attempts (ballot) and gathers both
+//      environments and partner slots; wave 1 does the same for B;
+//   2. every thread adds its cluster's change for both swaps (DPP wave sums); wave 1 checks
+//      whether A's two sites lie among B's draw attempts up to its winning one or in B's
+//      environments;
+//   3. lane 0 of wave 3 accepts or rejects A, then B unless A was accepted and touched B (B is
+//      then redone as the next round's A, so the trajectory is the sequential one); meanwhile
+//      waves 0 and 1 draw the attempts of steps s + 2 and s + 3 and lane 0 of wave 2 their
+//      Metropolis uniforms.
 template <bool kLdsOcc>
 __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint64_t ntrials,
                                                       uint64_t trace_base) {
   __shared__ SiteLds L;
-  __shared__ int64_t s_p, s_q;
-  __shared__ XYZ s_xp, s_xq;
-  __shared__ int s_ep, s_eq, s_done;
-  __shared__ double s_u[2];
+  __shared__ TrialLds s_tr[2];
+  __shared__ int s_done, s_adv, s_conf;
+  __shared__ double s_u[4], s_beta[4];  // by step & 3; s_beta: the ramp's beta
+  // annealing: T at the start of the round and the betas of A (that T) and B (after A's decay)
+  __shared__ double s_T, s_bA, s_bB;
   extern __shared__ uint32_t occ_words[];
   constexpr int kAcceptThread = 3 * 64;
   const int r = blockIdx.x;
@@ -347,56 +409,77 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     if (c.anneal) {
       A = c.anneal[r];
       sched = MakeAnnealSchedule(maxs);
+      s_T = A.T;
     }
   }
+  const double decay = c.anneal ? MakeAnnealSchedule(maxs).decay : 1.0;
   if (tid == 0) s_done = steps > maxs ? 1 : 0;
   PairDraw cand{};
-  int ox = 0, oy = 0, oz = 0;  // wave 0: the lane's environment offset
-  if (wave == 0) {
-    cand = DrawAttempt(c, r, steps, static_cast<uint32_t>(lane), N, n);
+  int ox = 0, oy = 0, oz = 0;  // waves 0 and 1: the lane's environment offset
+  if (wave < 2) {
+    cand = DrawAttempt(c, r, steps + wave, static_cast<uint32_t>(lane), N, n);
     if (lane < kSiteSlots) Unpack(t.site_off[lane], ox, oy, oz);
-  } else if (tid == 64) {
-    s_u[steps & 1] = TrialUniform(c, r, steps);
+  }
+  // the ramp's beta of a step (CanonicalMcAbstract: beta = 1 / kB / T)
+  auto ramp_beta = [&](uint64_t step) {
+    const double T0 = c.t0[r] + (c.t1[r] - c.t0[r]) *
+                                    (static_cast<double>(step) / static_cast<double>(maxs));
+    return 1.0 / kBoltzmann / T0;
+  };
+  // lane 0 of wave 2: the Metropolis uniforms (and ramp betas) of steps s and s + 1
+  auto draw_ahead = [&](uint64_t step) {
+    s_u[step & 3] = TrialUniform(c, r, step);
+    if (!c.anneal) s_beta[step & 3] = ramp_beta(step);
+  };
+  if (tid == 128) {
+    draw_ahead(steps);
+    draw_ahead(steps + 1);
   }
   __syncthreads();
   const SiteCluster cl = DecodeSiteCluster(t, L, tid);
-  for (uint64_t it = 0; it < ntrials; ++it) {
-    if (s_done) break;
-    if (wave == 0) {
-      uint64_t m = __ballot(occ[cand.a] != occ[cand.b]);
-      for (uint32_t base = 64; !m && base < kMaxPairAttempts; base += 64) {
-        cand = DrawAttempt(c, r, steps, base + static_cast<uint32_t>(lane), N, n);
-        m = __ballot(occ[cand.a] != occ[cand.b]);
+  // Metropolis step of the trial at `step` (accept thread): accept, trace, swap, anneal.
+  // pre_T, pre_beta: a temperature and its beta computed ahead (used when A.T equals pre_T).
+  auto metropolis = [&](uint64_t step, uint64_t trace_index, const TrialLds &T, double de,
+                        double pre_T, double pre_beta) {
+    double beta;
+    if (c.anneal) {
+      if (energy < A.lowest - 1e-4) A.lowest = energy;  // Dump (SimulatedAnnealing.cpp:80-97)
+      beta = A.T == pre_T ? pre_beta : 1.0 / kBoltzmann / fmax(A.T, 1e-12);
+    } else {
+      beta = s_beta[step & 3];
+    }
+    const bool accept = de < 0 || s_u[step & 3] < exp(-de * beta);
+    if (trace_index < c.trace_cap) {
+      const uint64_t o = static_cast<uint64_t>(r) * c.trace_cap + trace_index;
+      c.tr_p[o] = T.p;
+      c.tr_q[o] = T.q;
+      c.tr_de[o] = de;
+      c.tr_acc[o] = accept ? 1 : 0;
+    }
+    if (accept) {
+      if (c.atom) {
+        const int64_t o = c.off[r];
+        AtomJump(c.atom + o, c.shift + 3 * o, T.p, T.q, ImageChange(T.xq.X - T.xp.X, n),
+                 ImageChange(T.xq.Y - T.xp.Y, n), ImageChange(T.xq.Z - T.xp.Z, n));
       }
-      if (m) {
-        const int w = __builtin_ctzll(m);
-        const int64_t p = ReadLane64(cand.a, w), q = ReadLane64(cand.b, w);
-        const XYZ xp = ReadLaneXYZ(cand.xa, w), xq = ReadLaneXYZ(cand.xb, w);
-        bool hq = false, hp = false;
-        if (lane < kSiteSlots) {
-          const uint8_t vp = occ[SiteAt(xp, n, ox, oy, oz)];
-          const uint8_t vq = occ[SiteAt(xq, n, ox, oy, oz)];
-          L.env[0][lane] = vp < t.S ? vp : 0;  // codes are validated at creation
-          L.env[1][lane] = vq < t.S ? vq : 0;
-          if (lane >= 1) {  // partner slots: the first offset equal to the minimum image
-            hq = ox == MinImage(xq.X - xp.X, n) && oy == MinImage(xq.Y - xp.Y, n) &&
-                 oz == MinImage(xq.Z - xp.Z, n);
-            hp = ox == MinImage(xp.X - xq.X, n) && oy == MinImage(xp.Y - xq.Y, n) &&
-                 oz == MinImage(xp.Z - xq.Z, n);
-          }
-        }
-        const uint64_t mq = __ballot(hq), mp = __ballot(hp);
-        if (lane == 0) {
-          L.slot[0] = mq ? __builtin_ctzll(mq) : -1;
-          L.slot[1] = mp ? __builtin_ctzll(mp) : -1;
-          s_p = p;
-          s_q = q;
-          s_xp = xp;
-          s_xq = xq;
-          s_ep = occ[p];
-          s_eq = occ[q];
-        }
-      } else if (lane == 0) {
+      occ[T.p] = static_cast<uint8_t>(T.eq);
+      occ[T.q] = static_cast<uint8_t>(T.ep);
+      energy += de;
+      abse += de;
+      ++acc;
+    }
+    if (c.anneal) AnnealUpdate(A, sched, accept, step, energy);
+    return accept;
+  };
+  bool fresh = true;  // waves 0 and 1 hold the attempts of steps and steps + 1
+  uint64_t it = 0;
+  while (it < ntrials) {
+    if (s_done) break;
+    if (wave < 2) {
+      if (!fresh) cand = DrawAttempt(c, r, steps + wave, static_cast<uint32_t>(lane), N, n);
+      FindTrial(c, r, steps + wave, occ, N, n, t.S, lane, ox, oy, oz, cand, L, 2 * wave,
+                s_tr[wave]);
+      if (wave == 0 && lane == 0 && s_tr[0].win == -2) {
         atomicOr(c.error, 2);
         s_done = 1;
       }
@@ -404,56 +487,65 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     __syncthreads();
     if (s_done) break;
     {
-      const int ep = s_ep, eq = s_eq;
-      double v = ClusterDelta(L, cl, 0, ep, eq, L.slot[0]) + ClusterDelta(L, cl, 1, eq, ep, L.slot[1]);
-      v = WaveSum(v);
-      if (lane == 0) L.part[wave] = v;
+      const TrialLds &TA = s_tr[0], &TB = s_tr[1];
+      const bool hasB = TB.win != -2;
+      double va = ClusterDelta(L, cl, 0, TA.ep, TA.eq, L.slot[0]) +
+                  ClusterDelta(L, cl, 1, TA.eq, TA.ep, L.slot[1]);
+      double vb = hasB ? ClusterDelta(L, cl, 2, TB.ep, TB.eq, L.slot[2]) +
+                             ClusterDelta(L, cl, 3, TB.eq, TB.ep, L.slot[3])
+                       : 0.0;
+      va = WaveSum(va);
+      vb = WaveSum(vb);
+      if (lane == 0) {
+        L.part[wave] = va;
+        L.part2[wave] = vb;
+      }
+      if (c.anneal && tid == 128) {  // both betas of the round, off the Metropolis lane's path
+        const double T = s_T, Tb = T * decay;
+        s_bA = 1.0 / kBoltzmann / fmax(T, 1e-12);
+        s_bB = 1.0 / kBoltzmann / fmax(Tb, 1e-12);
+      }
+      if (wave == 1) {  // does A's swap touch B's draw or environments?
+        bool hit = false;
+        if (TB.win >= 0 && lane <= TB.win)
+          hit = cand.a == TA.p || cand.a == TA.q || cand.b == TA.p || cand.b == TA.q;
+        if (hasB && lane < kSiteSlots) {
+          const int64_t s1 = SiteAt(TB.xp, n, ox, oy, oz), s2 = SiteAt(TB.xq, n, ox, oy, oz);
+          hit = hit || s1 == TA.p || s1 == TA.q || s2 == TA.p || s2 == TA.q;
+        }
+        const uint64_t mh = __ballot(hit);
+        if (lane == 0) s_conf = (!hasB || TB.win < 0 || mh) ? 1 : 0;
+      }
     }
     __syncthreads();
     if (tid == kAcceptThread) {
       double de = 0.0;
       for (int w = 0; w < kThreads / 64; ++w) de += L.part[w];
-      const int64_t p = s_p, q = s_q;
-      double beta;
-      if (c.anneal) {
-        if (energy < A.lowest - 1e-4) A.lowest = energy;  // Dump (SimulatedAnnealing.cpp:80-97)
-        beta = 1.0 / kBoltzmann / fmax(A.T, 1e-12);
-      } else {
-        const double T = c.t0[r] + (c.t1[r] - c.t0[r]) *
-                                       (static_cast<double>(steps) / static_cast<double>(maxs));
-        beta = 1.0 / kBoltzmann / T;
+      const double T = c.anneal ? s_T : 0.0;
+      const bool acc_a = metropolis(steps, trace_base + it, s_tr[0], de, T, c.anneal ? s_bA : 0.0);
+      int adv = 1;
+      const bool b_ok = s_tr[1].win != -2 && !(acc_a && s_conf);
+      if (steps + 1 <= maxs && it + 1 < ntrials && b_ok) {
+        double deb = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) deb += L.part2[w];
+        (void)metropolis(steps + 1, trace_base + it + 1, s_tr[1], deb, T * decay,
+                         c.anneal ? s_bB : 0.0);
+        adv = 2;
       }
-      const bool accept = de < 0 || s_u[steps & 1] < exp(-de * beta);
-      const uint64_t ti = trace_base + it;
-      if (ti < c.trace_cap) {
-        const uint64_t o = static_cast<uint64_t>(r) * c.trace_cap + ti;
-        c.tr_p[o] = p;
-        c.tr_q[o] = q;
-        c.tr_de[o] = de;
-        c.tr_acc[o] = accept ? 1 : 0;
-      }
-      if (accept) {
-        if (c.atom) {
-          const XYZ a = s_xp, b = s_xq;
-          const int64_t o = c.off[r];
-          AtomJump(c.atom + o, c.shift + 3 * o, p, q, ImageChange(b.X - a.X, n),
-                   ImageChange(b.Y - a.Y, n), ImageChange(b.Z - a.Z, n));
-        }
-        occ[p] = static_cast<uint8_t>(s_eq);
-        occ[q] = static_cast<uint8_t>(s_ep);
-        energy += de;
-        abse += de;
-        ++acc;
-      }
-      if (c.anneal) AnnealUpdate(A, sched, accept, steps, energy);
-      if (steps + 1 > maxs) s_done = 1;
-    } else if (wave == 0) {
-      cand = DrawAttempt(c, r, steps + 1, static_cast<uint32_t>(lane), N, n);
-    } else if (tid == 64) {
-      s_u[(steps + 1) & 1] = TrialUniform(c, r, steps + 1);
+      s_adv = adv;
+      if (c.anneal) s_T = A.T;
+      if (steps + adv > maxs) s_done = 1;
+    } else if (wave < 2) {
+      cand = DrawAttempt(c, r, steps + 2 + wave, static_cast<uint32_t>(lane), N, n);
+    } else if (tid == 128) {
+      draw_ahead(steps + 2);
+      draw_ahead(steps + 3);
     }
-    ++steps;
     __syncthreads();
+    const int adv = s_adv;
+    fresh = adv == 2;
+    steps += adv;
+    it += adv;
   }
   if constexpr (kLdsOcc) {
     __syncthreads();
