@@ -22,6 +22,7 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <limits>
@@ -231,6 +232,111 @@ class Config {
     std::swap(atoms_[l], atoms_[r]);
     dirty_ = true;
   }
+  // ---- host-side queries of the reference's Config (Config.cpp:87-420), on lattice and atom ids
+  [[nodiscard]] std::string GetElementAtAtomId(size_t atom_id) const {
+    return GetElementAtLatticeId(GetLatticeIdFromAtomId(atom_id));
+  }
+  [[nodiscard]] std::vector<int64_t> GetAtomToLatticeVector() const {
+    std::vector<int64_t> v(atoms_.size());
+    for (size_t s = 0; s < atoms_.size(); ++s) v[static_cast<size_t>(atoms_[s])] = static_cast<int64_t>(s);
+    return v;
+  }
+  [[nodiscard]] std::map<std::string, size_t> GetElementCountMap() const {
+    std::map<std::string, size_t> m;
+    for (uint8_t c : codes_) ++m[species_[c]];
+    return m;
+  }
+  // atom ids by element, ascending (Config.cpp:204-210)
+  [[nodiscard]] std::map<std::string, std::vector<size_t>> GetElementAtomIdVectorMap() const {
+    std::map<std::string, std::vector<size_t>> m;
+    const auto a2l = GetAtomToLatticeVector();
+    for (size_t a = 0; a < a2l.size(); ++a) m[species_[codes_[static_cast<size_t>(a2l[a])]]].push_back(a);
+    return m;
+  }
+  [[nodiscard]] size_t GetVacancyAtomId() const { return GetAtomIdFromLatticeId(GetVacancyLatticeId()); }
+  [[nodiscard]] double GetVacancyConcentration() const {
+    const auto m = GetElementCountMap();
+    const auto it = m.find("X");
+    return static_cast<double>(it == m.end() ? 0 : it->second) / static_cast<double>(GetNumAtoms());
+  }
+  [[nodiscard]] double GetSoluteConcentration(const std::string &solvent_element) const {
+    size_t k = 0;
+    for (const auto &[e, c] : GetElementCountMap())
+      if (e != solvent_element && e != "X") k += c;
+    return static_cast<double>(k) / static_cast<double>(GetNumAtoms());
+  }
+  // the first element (string order) with the largest count (Config.cpp:411-421)
+  [[nodiscard]] std::string GetSolventElement() const {
+    std::string best = "X";
+    size_t mx = 0;
+    for (const auto &[e, c] : GetElementCountMap())
+      if (c > mx) {
+        mx = c;
+        best = e;
+      }
+    return best;
+  }
+  // 1, 2 or 3 for neighbours in that shell, else -1 (Config.cpp:354-371)
+  [[nodiscard]] int FindDistanceLabelBetweenLattice(size_t lattice_id1, size_t lattice_id2) const {
+    for (int shell = 1; shell <= 3; ++shell) {
+      const auto nb = GetNeighborLatticeIdVector(lattice_id1, shell);
+      if (std::find(nb.begin(), nb.end(), lattice_id2) != nb.end()) return shell;
+    }
+    return -1;
+  }
+  [[nodiscard]] std::vector<size_t> GetFirstNeighborsAtomIdVectorOfAtom(size_t atom_id) const {
+    return NeighbourAtoms(atom_id, 1);
+  }
+  [[nodiscard]] std::vector<size_t> GetSecondNeighborsAtomIdVectorOfAtom(size_t atom_id) const {
+    return NeighbourAtoms(atom_id, 2);
+  }
+  [[nodiscard]] std::vector<size_t> GetThirdNeighborsAtomIdVectorOfAtom(size_t atom_id) const {
+    return NeighbourAtoms(atom_id, 3);
+  }
+  // element counts of a shell around a site, every element of the set listed (Config.cpp:393-410)
+  [[nodiscard]] std::map<std::string, size_t> GetLocalInfoOfLatticeId(size_t lattice_id, int shell) const {
+    std::map<std::string, size_t> m;
+    for (const auto &e : GetElementSetWithoutVacancy()) m.emplace(e, 0);
+    for (size_t nb : GetNeighborLatticeIdVector(lattice_id, shell)) ++m[GetElementAtLatticeId(nb)];
+    return m;
+  }
+  // Element masses summed over the atoms that are not the solvent (Config.cpp:148-160)
+  [[nodiscard]] double GetTotalSoluteMass() const {
+    const std::string solvent = GetSolventElement();
+    double m = 0.0;
+    for (uint8_t c : codes_)
+      if (species_[c] != solvent) m += Mass(species_[c]);
+    return m;
+  }
+  // (relative position + map_shift) times the basis (Config.cpp:266-291)
+  [[nodiscard]] std::array<double, 3> GetUnwrappedCartesianPositionOfLattice(size_t lattice_id) const {
+    const auto rel = RelativePosition(lattice_id);
+    const int64_t a = atoms_.at(lattice_id);
+    double u[3];
+    for (int d = 0; d < 3; ++d) u[d] = rel[d] + static_cast<double>(shifts_[3 * a + d]);
+    return Times(u);
+  }
+  [[nodiscard]] std::array<double, 3> GetUnwrappedCartesianPositionOfAtom(size_t atom_id) const {
+    return GetUnwrappedCartesianPositionOfLattice(GetLatticeIdFromAtomId(atom_id));
+  }
+  // Config::AtomJump / SetAtomElementTypeAtAtom on atom ids.
+  void AtomJump(const std::pair<size_t, size_t> &atom_id_jump_pair) {
+    LatticeJump({GetLatticeIdFromAtomId(atom_id_jump_pair.first),
+                 GetLatticeIdFromAtomId(atom_id_jump_pair.second)});
+  }
+  void SetAtomElementTypeAtAtom(size_t atom_id, const std::string &element) {
+    SetAtomElementTypeAtLattice(GetLatticeIdFromAtomId(atom_id), element);
+  }
+  // Relative position of a site: X / 2n, Y / 2n, Z / 2n in a/2 units.
+  [[nodiscard]] std::array<double, 3> RelativePosition(size_t lattice_id) const {
+    const int64_t plane = 2ll * n_ * n_;
+    const int64_t id = static_cast<int64_t>(lattice_id);
+    const int64_t X = id / plane, r = id - X * plane, Y = r / n_;
+    const int64_t Z = 2 * (r - Y * n_) + ((X + Y) & 1);
+    const double m = 2.0 * n_;
+    return {static_cast<double>(X) / m, static_cast<double>(Y) / m, static_cast<double>(Z) / m};
+  }
+
   [[nodiscard]] size_t GetVacancyLatticeId() const {
     const uint8_t x = CodeOf("X");
     for (size_t i = 0; i < codes_.size(); ++i)
@@ -278,6 +384,27 @@ class Config {
 
  private:
   Config() = default;
+  std::vector<size_t> NeighbourAtoms(size_t atom_id, int shell) const {
+    std::vector<size_t> out;
+    for (size_t nb : GetNeighborLatticeIdVector(GetLatticeIdFromAtomId(atom_id), shell))
+      out.push_back(GetAtomIdFromLatticeId(nb));
+    return out;
+  }
+  // Vector_d * Matrix_d (VectorMatrix.hpp:275-280)
+  std::array<double, 3> Times(const double *u) const {
+    std::array<double, 3> o{};
+    for (int i = 0; i < 3; ++i) o[i] = u[0] * basis_[i] + u[1] * basis_[3 + i] + u[2] * basis_[6 + i];
+    return o;
+  }
+  static double Mass(const std::string &e) {  // Element::GetMass (Element.hpp:71-88)
+    const std::string b = e.size() > 1 && e[0] == 'p' ? e.substr(1) : e;
+    if (b == "Al") return 26.98;
+    if (b == "Mg") return 24.31;
+    if (b == "Zn") return 65.38;
+    if (b == "Cu") return 63.55;
+    if (b == "Sn") return 118.71;
+    return 0.0;
+  }
   void SetSpecies(std::set<std::string> element_set_without_vacancy) {
     element_set_without_vacancy.erase("X");
     species_.assign(element_set_without_vacancy.begin(), element_set_without_vacancy.end());

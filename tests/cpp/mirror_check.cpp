@@ -3,6 +3,7 @@
 //
 //   mirror_check cpu <json> <dir>                             host-only checks, prints "ok"
 //   mirror_check gpu <n> <occ.bin> <e0.json> <pairsite.json> <dir>  one JSON line per result
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iterator>
@@ -77,6 +78,32 @@ int CpuChecks(const std::string &json, const std::string &dir) {
     return Fail("missing cfg accepted");
   } catch (const std::runtime_error &e) {
     if (std::string(e.what()).find("Cannot open") == std::string::npos) return Fail("cfg io message");
+  }
+  {  // host queries of the reference's Config
+    cfg::Config q = cfg::GenerateFCC(4, els, "Al");
+    q.SetAtomElementTypeAtLattice(5, "X");
+    q.SetAtomElementTypeAtLattice(9, "Mg");
+    q.SetAtomElementTypeAtLattice(10, "Zn");
+    const auto cnt = q.GetElementCountMap();
+    if (cnt.at("Al") != 253 || cnt.at("X") != 1 || q.GetSolventElement() != "Al") return Fail("counts");
+    if (q.GetVacancyConcentration() != 1.0 / 256 || q.GetSoluteConcentration("Al") != 2.0 / 256)
+      return Fail("concentrations");
+    if (q.GetTotalSoluteMass() != 24.31 + 65.38) return Fail("solute mass");
+    const size_t nb = q.GetNeighborLatticeIdVector(5, 2)[0];
+    if (q.FindDistanceLabelBetweenLattice(5, nb) != 2 || q.FindDistanceLabelBetweenLattice(5, 5) != -1)
+      return Fail("distance label");
+    if (q.GetElementAtAtomId(q.GetVacancyAtomId()) != "X") return Fail("vacancy atom");
+    const auto m = q.GetElementAtomIdVectorMap();
+    if (m.at("Mg").size() != 1 || q.GetLatticeIdFromAtomId(m.at("Mg")[0]) != 9) return Fail("atom map");
+    const auto local = q.GetLocalInfoOfLatticeId(5, 1);
+    if (local.size() != 3) return Fail("local info");
+    const auto u0 = q.GetUnwrappedCartesianPositionOfLattice(5);
+    const size_t v5 = q.GetVacancyAtomId();
+    q.AtomJump({v5, q.GetAtomIdFromLatticeId(q.GetNeighborLatticeIdVector(5, 1)[0])});
+    const auto u1 = q.GetUnwrappedCartesianPositionOfAtom(v5);
+    const double d2 = (u1[0] - u0[0]) * (u1[0] - u0[0]) + (u1[1] - u0[1]) * (u1[1] - u0[1]) +
+                      (u1[2] - u0[2]) * (u1[2] - u0[2]);
+    if (std::fabs(d2 - 4.046 * 4.046 / 2) > 1e-9) return Fail("unwrapped jump length");
   }
   std::printf("ok\n");
   return 0;
