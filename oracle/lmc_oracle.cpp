@@ -28,6 +28,7 @@
 #include "lmc_oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -619,11 +620,18 @@ struct Model {
     for (auto &s : SortedSymOfPair(*lat, a, b, true)) t_mm2[f].push_back(s.id);
   }
   const std::vector<size_t> &Site43(size_t s) const {
+    if (sites_ready) return t_site[s];  // every table built: read without the lock
     std::lock_guard<std::mutex> g(mu);
     if (t_site[s].empty())
       for (auto &x : SortedStateOfSite(*lat, s)) t_site[s].push_back(x.id);
     return t_site[s];
   }
+  // Builds every site table up front (before concurrent runs).
+  void PrepareSites() const {
+    for (size_t s = 0; s < lat->N; ++s) (void)Site43(s);
+    sites_ready = true;
+  }
+  mutable std::atomic<bool> sites_ready{false};
 };
 
 static double DotSeq(const std::vector<double> &a, const std::vector<double> &b) {
@@ -1710,6 +1718,39 @@ double orc_kmc_baseline(const void *bm, const void *em, const void *lat, const u
   uint64_t hits = 0;
   for (auto &k : ks) hits += k->lru ? k->lru->hits : 0;
   if (lru_hits) *lru_hits = hits;
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// The CanonicalMc / SimulatedAnnealing trials of `replicas` independent replicas of occ, one
+// thread each, `trials` trials per replica (maximum_steps for the ramp and the schedule);
+// returns the wall seconds of the timed loop (tables built before it).
+double orc_cmc_baseline(const void *pm, const void *lat, const uint8_t *occ, double t_start,
+                        uint64_t seed, int replicas, uint64_t trials, uint64_t maximum_steps,
+                        int anneal) {
+  const Model &m = *static_cast<const Model *>(pm);
+  m.PrepareSites();
+  std::vector<void *> cs;
+  for (int r = 0; r < replicas; ++r) {
+    orc_cmc_params p{};
+    p.t_start = t_start;
+    p.t_end = t_start;
+    p.seed = seed;
+    p.replica = static_cast<uint32_t>(r);
+    p.anneal = anneal;
+    void *c = orc_cmc_new(pm, nullptr, lat, occ, &p, nullptr, 0);
+    if (!c) {
+      for (void *d : cs) orc_cmc_free(d);
+      return -1.0;
+    }
+    cs.push_back(c);
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for num_threads(replicas) schedule(static, 1) if (replicas > 1)
+  for (int r = 0; r < replicas; ++r)
+    (void)orc_cmc_run(cs[static_cast<size_t>(r)], trials, maximum_steps, nullptr, nullptr, nullptr,
+                      nullptr);
+  const auto t1 = std::chrono::steady_clock::now();
+  for (void *c : cs) orc_cmc_free(c);
   return std::chrono::duration<double>(t1 - t0).count();
 }
 

@@ -129,6 +129,13 @@ void orc_cmc_state(const void *cmc, uint64_t *steps, double *energy, double *abs
                    uint64_t *accepted);
 void orc_cmc_occupancy(const void *cmc, uint8_t *occ);
 
+// Wall-clock timing helper for the CanonicalMc / SimulatedAnnealing CPU baseline: runs
+// `replicas` independent replicas of occ concurrently, one thread each, `trials` trials each;
+// returns seconds of the trial loops (site tables and setup excluded).
+double orc_cmc_baseline(const void *pairsite_model, const void *lat, const uint8_t *occ,
+                        double t_start, uint64_t seed, int replicas, uint64_t trials,
+                        uint64_t maximum_steps, int anneal);
+
 // Wall-clock timing helper for the CPU baseline: runs `replicas` independent KMC trajectories
 // (each with `threads_per_replica` OpenMP threads on the 12-event loop) concurrently for
 // `steps` steps each, returns seconds of the step loops (setup excluded).

@@ -88,6 +88,8 @@ def lib():
             "orc_cmc_state": (None, [vp, C.POINTER(C.c_uint64), _f64p, _f64p,
                                      C.POINTER(C.c_uint64)]),
             "orc_cmc_occupancy": (None, [vp, _u8p]),
+            "orc_cmc_baseline": (C.c_double, [vp, vp, _u8p, C.c_double, C.c_uint64, C.c_int,
+                                              C.c_uint64, C.c_uint64, C.c_int]),
             "orc_kmc_baseline": (C.c_double, [vp, vp, vp, _u8p, C.c_double, C.c_uint64,
                                               C.c_int, C.c_int, C.c_int, C.c_uint64,
                                               C.POINTER(C.c_uint64), C.c_int]),
@@ -351,6 +353,18 @@ class Cmc:
         out = np.zeros(self.num_sites, np.uint8)
         lib().orc_cmc_occupancy(self.h, _ptr(out, _u8p))
         return out
+
+
+def cmc_baseline(pairsite_model, lattice, occ, t_start, seed, replicas, trials, maximum_steps,
+                 anneal):
+    """Seconds for `replicas` concurrent CanonicalMc (anneal: SimulatedAnnealing) replicas of occ,
+    one thread each, `trials` trials each."""
+    occ = np.ascontiguousarray(occ, np.uint8)
+    secs = lib().orc_cmc_baseline(pairsite_model.h, lattice.h, _ptr(occ, _u8p), t_start, seed,
+                                  replicas, trials, maximum_steps, int(anneal))
+    if secs < 0:
+        raise RuntimeError("orc_cmc_baseline: invalid model or occupancy")
+    return secs
 
 
 def kmc_baseline(barrier_model, energy_model, lattice, occ, temperature, seed, replicas,
