@@ -781,6 +781,7 @@ struct ChainLds {
   uint16_t soff[12];
   uint8_t sm[12];
   double barrier[12], de[12], rate[12], back[12], total_i[12];
+  double b[12];     // p_ki (1 - p_ik) of the selection
   int64_t prev;     // the site the vacancy came from
 };
 
@@ -840,7 +841,8 @@ __device__ void ChainSelectAndApply(const KTables &t, const KmcDev &k, const Jum
   double total = 0.0;
   for (int i = 0; i < 12; ++i) total += C.rate[i];
   const double t1 = 1 / total / kPrefactor;
-  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, b[12];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double *b = C.b;
   int jprev = -1;
   for (int i = 0; i < 12; ++i) {
     const double p_ki = C.rate[i] / total;
