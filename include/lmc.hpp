@@ -904,6 +904,18 @@ class CanonicalMcBatch {
   void SetLdsOccupancy(bool on) { lmc::Check(lmc_cmc_set_lds_occupancy(c_.get(), on ? 1 : 0)); }
   // SimulatedAnnealing's adaptive schedule from t_start (see lmc_cmc_set_annealing).
   void SetAnnealing(bool on) { lmc::Check(lmc_cmc_set_annealing(c_.get(), on ? 1 : 0)); }
+  // cmc_log.txt records (CanonicalMcAbstract::Dump) with ThermodynamicAveraging over the last
+  // thermodynamic_averaging_steps energies; with annealing on, the SimulatedAnnealing::Dump log.
+  void SetLog(uint64_t log_dump_steps, uint64_t thermodynamic_averaging_steps = 0,
+              uint64_t capacity = 100000) {
+    lmc::Check(lmc_cmc_set_log(c_.get(), log_dump_steps, capacity, thermodynamic_averaging_steps));
+  }
+  void WriteLog(const std::vector<std::string> &filenames) {
+    if (filenames.size() != R_) throw std::invalid_argument("one log file per replica");
+    std::vector<const char *> f;
+    for (const auto &s : filenames) f.push_back(s.c_str());
+    lmc::Check(lmc_cmc_write_log(c_.get(), f.data()));
+  }
   [[nodiscard]] std::vector<double> GetAnnealingTemperature() const {
     std::vector<double> T(R_);
     lmc::Check(lmc_cmc_annealing_state(c_.get(), T.data(), nullptr, nullptr));

@@ -367,6 +367,29 @@ int lmc_cmc_trace(lmc_cmc *c, int64_t *p, int64_t *q, double *de, int8_t *accept
  * Config::LatticeJump({p, q}) (CanonicalMcAbstract.cpp:86-100). */
 int lmc_cmc_set_atoms(lmc_cmc *c, const int64_t *lattice_to_atom, const int32_t *map_shift);
 int lmc_cmc_atoms(lmc_cmc *c, int64_t *lattice_to_atom, int32_t *map_shift);
+/* cmc_log.txt, replacing CanonicalMcAbstract::Dump's log (CanonicalMcAbstract.cpp:53-84), and
+ * with annealing on the log of SimulatedAnnealing::Dump (SimulatedAnnealing.cpp:75-97). With
+ * log_dump_steps > 0 the kernel keeps a record of the step about to be tried, up to `capacity`
+ * per replica between writes: steps, temperature, energy, the fourth column and the absolute
+ * energy.
+ * - canonical MC: on KineticMc's cadence (every step below step 99, then every
+ *   10^(floor(log10(steps + 1)) - 1) steps up to log_dump_steps, every log_dump_steps steps past
+ *   10 log_dump_steps); the fourth column is ThermodynamicAveraging's average over the last
+ *   thermodynamic_averaging_steps energies (ThermodynamicAveraging.cpp:6-40; 0 when that is 0),
+ *   counted from this call. The averages are formed after each launch from a ring of
+ *   (thermodynamic_averaging_steps + 65536) doubles per replica, so runs are split into launches
+ *   of 65536 trials while it is on.
+ * - annealing: every log_dump_steps steps and at every step whose energy is below the lowest so
+ *   far by more than 1e-4 eV; the fourth column is the lowest energy.
+ * log_dump_steps = 0 turns it off. Synchronous. */
+int lmc_cmc_set_log(lmc_cmc *c, uint64_t log_dump_steps, uint64_t capacity,
+                    uint64_t thermodynamic_averaging_steps);
+/* Appends the records taken since the last call to filenames[r] (null: skip and drop that
+ * replica's records) in the reference's text format: tab-separated columns, precision 16 in the
+ * default float format, the header line and a truncated file for a run from step 0, an appended
+ * file for a restarted run (canonical MC also without its first dump). LMC_ERR_ARG after
+ * writing if records were dropped for lack of capacity. */
+int lmc_cmc_write_log(lmc_cmc *c, const char *const *filenames);
 
 /* ---- Learned surrogate of the jump evaluation ------------------------------------------
  * No reference counterpart: an ensemble of BF16 MLPs (one-hot jump environment -> 128 -> 128

@@ -512,6 +512,24 @@ class CanonicalMc:
         """(per-replica lattice_to_atom, per-replica map_shift [N, 3])."""
         return _atoms(lib().lmc_cmc_atoms, self.h, self.ncell)
 
+    def set_log(self, log_dump_steps, capacity=100000, thermodynamic_averaging_steps=0):
+        """Keep cmc_log.txt records at the reference's dump cadence (0 turns it off), with the
+        thermodynamic average over the last thermodynamic_averaging_steps energies (canonical
+        MC); with annealing, the simulated-annealing log (every log_dump_steps steps and at each
+        new lowest energy)."""
+        check(lib().lmc_cmc_set_log(self.h, int(log_dump_steps), int(capacity),
+                                    int(thermodynamic_averaging_steps)))
+
+    def write_log(self, filenames):
+        """Append the records taken since the last call to one log file per replica (None skips
+        a replica)."""
+        if isinstance(filenames, (str, bytes)) or hasattr(filenames, "__fspath__"):
+            filenames = [filenames]
+        if len(filenames) != self.R:
+            raise ValueError(f"need {self.R} file names")
+        arr = (C.c_char_p * self.R)(*[None if f is None else str(f).encode() for f in filenames])
+        check(lib().lmc_cmc_write_log(self.h, arr))
+
     def occupancy(self):
         out = np.zeros(self.total_sites, np.uint8)
         check(lib().lmc_cmc_occupancy(self.h, ptr(out, capi._u8p)))

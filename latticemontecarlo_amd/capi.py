@@ -115,6 +115,8 @@ SIGNATURES = {
     "lmc_kmc_atoms": (C.c_int, [_vp, _i64p, _i32p]),
     "lmc_kmc_set_log": (C.c_int, [_vp, _u64, _u64, _f64p]),
     "lmc_kmc_write_log": (C.c_int, [_vp, C.POINTER(C.c_char_p)]),
+    "lmc_cmc_set_log": (C.c_int, [_vp, _u64, _u64, _u64]),
+    "lmc_cmc_write_log": (C.c_int, [_vp, C.POINTER(C.c_char_p)]),
     "lmc_cmc_set_atoms": (C.c_int, [_vp, _i64p, _i32p]),
     "lmc_cmc_atoms": (C.c_int, [_vp, _i64p, _i32p]),
     "lmc_cmc_set_lds_occupancy": (C.c_int, [_vp, _i32]),

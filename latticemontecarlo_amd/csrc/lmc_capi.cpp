@@ -630,7 +630,18 @@ struct lmc_cmc {
   const lmc_model *pm = nullptr;
   lmc::CmcDev d;
   uint64_t trace_pos = 0;
+  std::vector<uint64_t> start_steps;         // restart_steps of each replica
+  std::vector<char> log_begun, log_skip_first;
+  void FreeLog() {
+    DevFree(d.log_n);
+    DevFree(d.log_lo);
+    DevFree(d.log);
+    DevFree(d.ring);
+    DevFree(d.ring_n);
+    d.log_every = d.log_cap = d.avg_m = d.ring_cap = 0;
+  }
   ~lmc_cmc() {
+    FreeLog();
     DevFree(d.anneal);
     DevFree(d.atom);
     DevFree(d.shift);
@@ -1549,6 +1560,10 @@ int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
 }
 
 namespace {
+constexpr uint64_t kMaxLogEvery = 100000000000000000ull;  // 1e17: 10 L stays in 64 bits
+constexpr uint64_t kMaxAveragingSteps = 1ull << 31;
+constexpr uint64_t kAveragingChunk = 1ull << 16;  // trials per launch with the averaging ring on
+
 // A double as McAbstract's log stream prints it: precision(16), the default float format until
 // the first vector has set std::fixed (VectorMatrix.hpp:41-51).
 void PutDouble(std::string &o, double v, bool fixed) {
@@ -1684,6 +1699,7 @@ int lmc_cmc_create(const lmc_model *pairsite, const lmc_model *energy, int32_t R
     d.energy = DevCopy(en);
     d.abs_energy = DevCopy(std::vector<double>(R, 0.0));
     d.error = DevCopy(std::vector<int32_t>(1, 0));
+    c->start_steps = steps;
     lmc::Upload(*pairsite->m);
     if (energy) {
       RequireModel(energy);
@@ -1787,8 +1803,98 @@ int lmc_cmc_run(lmc_cmc *c, uint64_t trials, void *stream) {
   return Guard([&] {
     if (!c) throw std::invalid_argument("null driver");
     const auto &t = lmc::Upload(*c->pm->m);
-    Check(lmc::LaunchCmc(t, c->d, trials, c->trace_pos, S(stream)), "CMC kernel");
-    c->trace_pos += trials;
+    // with the averaging ring on, launches of at most ring_cap - avg_m trials, each followed by
+    // the averages of its records (the ring still holds every record's window then)
+    const uint64_t chunk = c->d.avg_m ? c->d.ring_cap - c->d.avg_m : trials;
+    for (uint64_t done = 0; done < trials;) {
+      const uint64_t k = std::min(chunk, trials - done);
+      Check(lmc::LaunchCmc(t, c->d, k, c->trace_pos, S(stream)), "CMC kernel");
+      Check(lmc::LaunchCmcAverage(c->d, S(stream)), "CMC averaging kernel");
+      c->trace_pos += k;
+      done += k;
+    }
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_set_log(lmc_cmc *c, uint64_t log_dump_steps, uint64_t capacity,
+                    uint64_t thermodynamic_averaging_steps) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    c->FreeLog();
+    if (!log_dump_steps) return LMC_OK;
+    if (!capacity) throw std::invalid_argument("log capacity must be positive");
+    if (log_dump_steps > kMaxLogEvery) throw std::invalid_argument("log_dump_steps is too large");
+    const int R = c->d.R;
+    auto &d = c->d;
+    d.log_n = DevCopy(std::vector<uint64_t>(R, 0));
+    d.log_lo = DevCopy(std::vector<uint64_t>(R, 0));
+    d.log = DevAlloc<lmc::CmcLogRecord>(static_cast<size_t>(capacity) * R);
+    if (thermodynamic_averaging_steps) {
+      if (thermodynamic_averaging_steps > kMaxAveragingSteps)
+        throw std::invalid_argument("thermodynamic_averaging_steps is too large");
+      d.ring_cap = thermodynamic_averaging_steps + kAveragingChunk;
+      d.ring = DevAlloc<double>(static_cast<size_t>(d.ring_cap) * R);
+      d.ring_n = DevCopy(std::vector<uint64_t>(R, 0));
+      d.avg_m = thermodynamic_averaging_steps;
+    }
+    d.log_every = log_dump_steps;
+    d.log_cap = capacity;
+    c->log_begun.assign(R, 0);
+    // a restarted canonical run skips its first dump (CanonicalMcAbstract.cpp:54-57)
+    c->log_skip_first.resize(R);
+    for (int r = 0; r < R; ++r) c->log_skip_first[r] = c->start_steps[r] > 0 ? 1 : 0;
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_write_log(lmc_cmc *c, const char *const *filenames) {
+  return Guard([&] {
+    if (!c || !filenames) throw std::invalid_argument("null argument");
+    if (!c->d.log_every) throw std::invalid_argument("logging is off");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const int R = c->d.R;
+    const bool anneal = c->d.anneal != nullptr;
+    std::vector<uint64_t> cnt(R);
+    ToHost(cnt.data(), c->d.log_n, cnt.size());
+    std::vector<lmc::CmcLogRecord> rec(static_cast<size_t>(c->d.log_cap) * R);
+    ToHost(rec.data(), c->d.log, rec.size());
+    Check(hipMemset(c->d.log_n, 0, sizeof(uint64_t) * R), "hipMemset");
+    uint64_t lost = 0;
+    for (int r = 0; r < R; ++r) {
+      if (!filenames[r]) continue;
+      std::string o;
+      const bool fresh = !c->log_begun[r];
+      if (fresh && c->start_steps[r] == 0)  // header of a run from step 0
+        o += anneal ? "steps\ttemperature\tenergy\tlowest_energy\tabsolute_energy\n"
+                    : "steps\ttemperature\tenergy\taverage_energy\tabsolute_energy\n";
+      const uint64_t n = std::min(cnt[r], c->d.log_cap);
+      lost += cnt[r] - n;
+      for (uint64_t i = 0; i < n; ++i) {
+        const lmc::CmcLogRecord &L = rec[static_cast<size_t>(r) * c->d.log_cap + i];
+        if (c->log_skip_first[r]) {
+          c->log_skip_first[r] = 0;
+          if (!anneal && L.steps == c->start_steps[r]) continue;
+        }
+        o += std::to_string(L.steps);
+        for (double v : {L.temperature, L.energy, L.second, L.abs_energy}) {
+          o += '\t';
+          PutDouble(o, v, false);  // the stream never switches to std::fixed here
+        }
+        o += '\n';
+      }
+      // a run from step 0 truncates (std::ofstream::out); a restarted one appends
+      std::FILE *f = std::fopen(filenames[r], fresh && c->start_steps[r] == 0 ? "wb" : "ab");
+      if (!f) throw std::runtime_error(std::string("Cannot open ") + filenames[r]);
+      const bool ok = std::fwrite(o.data(), 1, o.size(), f) == o.size();
+      if (std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("Cannot write ") + filenames[r]);
+      c->log_begun[r] = 1;
+    }
+    if (lost)
+      throw std::invalid_argument(std::to_string(lost) +
+                                  " log records past the capacity were lost; write the log more "
+                                  "often or raise its capacity");
     return LMC_OK;
   });
 }

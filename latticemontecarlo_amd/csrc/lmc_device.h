@@ -134,6 +134,17 @@ struct CmcAnneal {
   uint32_t window_trials, window_accepts, reheats, pad;
 };
 
+// One line of cmc_log.txt (CanonicalMcAbstract::Dump, CanonicalMcAbstract.cpp:53-84) or of the
+// simulated-annealing log (SimulatedAnnealing::Dump, SimulatedAnnealing.cpp:75-97): the step, the
+// temperature, the energy before the step's trial, `second` (canonical MC: the thermodynamic
+// average over the averaging window, filled in after the launch; annealing: the lowest energy)
+// and the absolute energy. `count`: energies in the averaging ring up to this step.
+struct CmcLogRecord {
+  uint64_t steps;
+  double temperature, energy, second, abs_energy;
+  uint64_t count;
+};
+
 struct CmcDev {
   int32_t R = 0;
   int64_t max_sites = 0;   // largest replica, in sites
@@ -151,6 +162,16 @@ struct CmcDev {
   int64_t *tr_p = nullptr, *tr_q = nullptr;
   double *tr_de = nullptr;
   int8_t *tr_acc = nullptr;
+  // optional log: records on the Dump cadence (log_every > 0), log_n counts past log_cap;
+  // log_lo: the replica's record count when the current launch began
+  uint64_t log_every = 0, log_cap = 0;
+  uint64_t *log_n = nullptr, *log_lo = nullptr;
+  CmcLogRecord *log = nullptr;
+  // ThermodynamicAveraging over the last avg_m energies: ring [R][ring_cap] of the energy at
+  // every step (ring_cap >= avg_m + trials per launch), ring_n energies written per replica
+  uint64_t avg_m = 0, ring_cap = 0;
+  double *ring = nullptr;
+  uint64_t *ring_n = nullptr;
 };
 
 // env_out (optional): [12 R][64] environment rows of the events in rank order; ekra_out
@@ -179,6 +200,9 @@ hipError_t LaunchSwapDe(const KTables &t, const uint8_t *occ, const int64_t *off
 hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off,
                         const int32_t *ncell, const int64_t *sites, const uint8_t *from,
                         const uint8_t *codes, int32_t R, double *de, hipStream_t s);
+// The thermodynamic averages of the canonical-MC log records of the last launch
+// (ThermodynamicAveraging::GetThermodynamicAverage, ThermodynamicAveraging.cpp:25-40).
+hipError_t LaunchCmcAverage(const CmcDev &c, hipStream_t s);
 hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_t trace_base,
                      hipStream_t s);
 // GetEncode of every replica: encode[R][n_eci] and energy[R] (either may be null).

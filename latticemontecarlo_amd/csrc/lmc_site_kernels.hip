@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "lmc_dev_common.h"
 #include "lmc_device.h"
@@ -359,6 +360,34 @@ __device__ void FindTrial(const CmcDev &c, int r, uint64_t step, const uint8_t *
   }
 }
 
+// The dump cadence, evaluated for consecutive steps with one division per change of cadence.
+// Canonical MC (CanonicalMcAbstract.cpp:69-81): steps % every == 0 with every =
+// 10^(floor(log10(steps + 1)) - 1), at least 1 and at most L, while steps <= 10 L, and L past
+// that. Annealing (`plain`, SimulatedAnnealing.cpp:90): steps % L == 0.
+struct LogClock {
+  uint64_t L = 0, every = 1, rem = 0, change = 0;  // change: the first step of another cadence
+  bool plain = false;
+  __device__ void Reset(uint64_t s) {
+    every = L;
+    change = ~0ull;
+    if (!plain && s <= 10 * L) {
+      uint64_t p = 1;
+      for (uint64_t t = s + 1; t >= 100; t /= 10) p *= 10;
+      every = p < L ? p : L;
+      const uint64_t next = p <= (~0ull) / 100 ? 100 * p - 1 : ~0ull;  // s + 1 reaches 100 p
+      change = next < 10 * L + 1 ? next : 10 * L + 1;
+    }
+    rem = s % every;
+  }
+  // Whether step s is due; s is the step Reset saw, then each following step once, in order.
+  __device__ bool Due(uint64_t s) {
+    if (s >= change) Reset(s);
+    const bool due = rem == 0;
+    if (++rem == every) rem = 0;
+    return due;
+  }
+};
+
 // CanonicalMcSerial (CanonicalMcSerial.cpp:40-51, CanonicalMcAbstract.cpp:43-101) with an
 // optional linear temperature ramp t0 -> t1 over maximum_steps, or the adaptive annealing
 // schedule. One workgroup per replica. kLdsOcc: the replica's occupancy (N bytes, N a multiple
@@ -374,7 +403,8 @@ __device__ void FindTrial(const CmcDev &c, int r, uint64_t step, const uint8_t *
 //      then redone as the next round's A, so the trajectory is the sequential one); meanwhile
 //      waves 0 and 1 draw the attempts of steps s + 2 and s + 3 and lane 0 of wave 2 their
 //      Metropolis uniforms.
-template <bool kLdsOcc>
+// kLog: the log records and the averaging ring are compiled in (taken when c.log_every is set).
+template <bool kLdsOcc, bool kLog>
 __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint64_t ntrials,
                                                       uint64_t trace_base) {
   __shared__ SiteLds L;
@@ -402,6 +432,8 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
   double energy = 0, abse = 0;
   CmcAnneal A{};
   AnnealSchedule sched{};
+  LogClock clk;
+  uint64_t log_i = 0, ring_i = 0, ring_at = 0;  // records, energies added, ring slot
   if (tid == kAcceptThread) {
     acc = c.accepted[r];
     energy = c.energy[r];
@@ -410,6 +442,17 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
       A = c.anneal[r];
       sched = MakeAnnealSchedule(maxs);
       s_T = A.T;
+    }
+    if constexpr (kLog) {
+      clk.L = c.log_every;
+      clk.plain = c.anneal != nullptr;
+      clk.Reset(steps);
+      log_i = c.log_n[r];
+      c.log_lo[r] = log_i;
+      if (c.avg_m) {
+        ring_i = c.ring_n[r];
+        ring_at = ring_i % c.ring_cap;
+      }
     }
   }
   const double decay = c.anneal ? MakeAnnealSchedule(maxs).decay : 1.0;
@@ -420,11 +463,23 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     cand = DrawAttempt(c, r, steps + wave, static_cast<uint32_t>(lane), N, n);
     if (lane < kSiteSlots) Unpack(t.site_off[lane], ox, oy, oz);
   }
-  // the ramp's beta of a step (CanonicalMcAbstract: beta = 1 / kB / T)
-  auto ramp_beta = [&](uint64_t step) {
-    const double T0 = c.t0[r] + (c.t1[r] - c.t0[r]) *
-                                    (static_cast<double>(step) / static_cast<double>(maxs));
-    return 1.0 / kBoltzmann / T0;
+  // the ramp's temperature and beta of a step (CanonicalMcAbstract: beta = 1 / kB / T)
+  auto ramp_T = [&](uint64_t step) {
+    return c.t0[r] + (c.t1[r] - c.t0[r]) * (static_cast<double>(step) / static_cast<double>(maxs));
+  };
+  auto ramp_beta = [&](uint64_t step) { return 1.0 / kBoltzmann / ramp_T(step); };
+  // a log record of the step about to be tried (accept thread)
+  auto put_log = [&](uint64_t step, double T, double second) {
+    if (log_i < c.log_cap) {
+      CmcLogRecord &rec = c.log[static_cast<uint64_t>(r) * c.log_cap + log_i];
+      rec.steps = step;
+      rec.temperature = T;
+      rec.energy = energy;
+      rec.second = second;
+      rec.abs_energy = abse;
+      rec.count = ring_i;
+    }
+    ++log_i;  // counts past the capacity too, so the host can report the loss
   };
   // lane 0 of wave 2: the Metropolis uniforms (and ramp betas) of steps s and s + 1
   auto draw_ahead = [&](uint64_t step) {
@@ -441,12 +496,27 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
   // pre_T, pre_beta: a temperature and its beta computed ahead (used when A.T equals pre_T).
   auto metropolis = [&](uint64_t step, uint64_t trace_index, const TrialLds &T, double de,
                         double pre_T, double pre_beta) {
+    if constexpr (kLog) {
+      if (c.avg_m) {  // ThermodynamicAveraging::AddEnergy before Dump (CanonicalMcSerial.cpp:45)
+        c.ring[static_cast<uint64_t>(r) * c.ring_cap + ring_at] = energy;
+        if (++ring_at == c.ring_cap) ring_at = 0;
+        ++ring_i;
+      }
+    }
     double beta;
     if (c.anneal) {
-      if (energy < A.lowest - 1e-4) A.lowest = energy;  // Dump (SimulatedAnnealing.cpp:80-97)
+      const bool lower = energy < A.lowest - 1e-4;  // Dump (SimulatedAnnealing.cpp:80-97)
+      if (lower) A.lowest = energy;
+      if constexpr (kLog) {
+        if (lower) put_log(step, A.T, A.lowest);
+        if (clk.Due(step)) put_log(step, A.T, A.lowest);
+      }
       beta = A.T == pre_T ? pre_beta : 1.0 / kBoltzmann / fmax(A.T, 1e-12);
     } else {
       beta = s_beta[step & 3];
+      if constexpr (kLog) {
+        if (clk.Due(step)) put_log(step, ramp_T(step), 0.0);
+      }
     }
     const bool accept = de < 0 || s_u[step & 3] < exp(-de * beta);
     if (trace_index < c.trace_cap) {
@@ -558,6 +628,56 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
     c.energy[r] = energy;
     c.abs_energy[r] = abse;
     if (c.anneal) c.anneal[r] = A;
+    if constexpr (kLog) {
+      c.log_n[r] = log_i;
+      if (c.avg_m) c.ring_n[r] = ring_i;
+    }
+  }
+}
+
+// ThermodynamicAveraging::GetThermodynamicAverage (ThermodynamicAveraging.cpp:25-40) of the log
+// records of the last launch: over the last min(avg_m, count) energies of the ring, the mean m
+// and sum((e - m) exp(-(e - m) beta)) / sum(exp(-(e - m) beta)) + m, beta = 1 / kB / T of the
+// record. Grid (G, R): workgroup x takes records lo + x, lo + x + G, ... of replica y.
+__device__ __forceinline__ double BlockSum256(double v, double *red) {
+  v = WaveSum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double s = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(kThreads) void CmcAverageKernel(CmcDev c) {
+  static_assert(kThreads == 256, "BlockSum256 reduces four waves");
+  __shared__ double red[4];
+  const int r = blockIdx.y;
+  const uint64_t lo = c.log_lo[r];
+  const uint64_t hi = c.log_n[r] < c.log_cap ? c.log_n[r] : c.log_cap;
+  const double *ring = c.ring + static_cast<uint64_t>(r) * c.ring_cap;
+  for (uint64_t i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+    CmcLogRecord &rec = c.log[static_cast<uint64_t>(r) * c.log_cap + i];
+    const uint64_t cnt = rec.count;
+    const uint64_t w = cnt < c.avg_m ? cnt : c.avg_m;  // >= 1: the record's own energy
+    const uint64_t first = (cnt - w) % c.ring_cap;
+    double sum = 0.0;
+    for (uint64_t j = threadIdx.x; j < w; j += kThreads) {
+      const uint64_t x = first + j;
+      sum += ring[x < c.ring_cap ? x : x - c.ring_cap];
+    }
+    const double mean = BlockSum256(sum, red) / static_cast<double>(w);
+    const double beta = 1.0 / kBoltzmann / rec.temperature;
+    double num = 0.0, den = 0.0;
+    for (uint64_t j = threadIdx.x; j < w; j += kThreads) {
+      const uint64_t x = first + j;
+      const double e = ring[x < c.ring_cap ? x : x - c.ring_cap] - mean;
+      const double ex = exp(-e * beta);
+      num += e * ex;
+      den += ex;
+    }
+    num = BlockSum256(num, red);
+    den = BlockSum256(den, red);
+    if (threadIdx.x == 0) rec.second = num / den + mean;
   }
 }
 
@@ -754,26 +874,40 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
     int dev = 0;
     hipDeviceProp_t p;
     hipFuncAttributes fa;
+    hipFuncAttributes fb;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
-        hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(CmcKernel<true>)) != hipSuccess)
+        hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(CmcKernel<true, false>)) !=
+            hipSuccess ||
+        hipFuncGetAttributes(&fb, reinterpret_cast<const void *>(CmcKernel<true, true>)) !=
+            hipSuccess)
       return hipErrorInvalidDevice;
     lds_block = p.sharedMemPerBlock;
-    stat = fa.sharedSizeBytes;
+    stat = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
   }
   const size_t dyn = static_cast<size_t>(c.max_sites);
-  if (c.lds_occupancy && dyn % 4 == 0 && stat + dyn <= lds_block) {
-    if (dyn > 65536 - stat) {
-      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(CmcKernel<true>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               static_cast<int>(dyn));
-      if (e != hipSuccess) return e;
+  auto launch = [&](auto log) {
+    constexpr bool kL = decltype(log)::value;
+    if (c.lds_occupancy && dyn % 4 == 0 && stat + dyn <= lds_block) {
+      if (dyn > 65536 - stat) {
+        const hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void *>(CmcKernel<true, kL>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dyn));
+        if (e != hipSuccess) return e;
+      }
+      hipLaunchKernelGGL((CmcKernel<true, kL>), dim3(c.R), dim3(kThreads), dyn, s, t, c, trials,
+                         trace_base);
+    } else {
+      hipLaunchKernelGGL((CmcKernel<false, kL>), dim3(c.R), dim3(kThreads), 0, s, t, c, trials,
+                         trace_base);
     }
-    hipLaunchKernelGGL(CmcKernel<true>, dim3(c.R), dim3(kThreads), dyn, s, t, c, trials,
-                       trace_base);
-  } else {
-    hipLaunchKernelGGL(CmcKernel<false>, dim3(c.R), dim3(kThreads), 0, s, t, c, trials,
-                       trace_base);
-  }
+    return hipGetLastError();
+  };
+  return c.log_every ? launch(std::true_type{}) : launch(std::false_type{});
+}
+
+hipError_t LaunchCmcAverage(const CmcDev &c, hipStream_t s) {
+  if (c.R <= 0 || !c.log_every || !c.avg_m) return hipSuccess;
+  hipLaunchKernelGGL(CmcAverageKernel, dim3(64, c.R), dim3(kThreads), 0, s, c);
   return hipGetLastError();
 }
 

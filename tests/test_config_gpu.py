@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import cfg_oracle as CO
+import log_oracle as LO
 from conftest import random_alloc
 
 pytestmark = pytest.mark.gpu
@@ -157,19 +158,6 @@ def test_kmc_resume_from_cfg_is_bit_exact(torch, coeffs, tmp_path):
     assert np.array_equal(ra, fa) and np.array_equal(rs, fs)
 
 
-def _reference_due(steps, L):
-    """KineticMcFirstAbstract::Dump's cadence restated with its floating-point log10
-    (KineticMcAbstract.cpp:83-98); a negative exponent casts to 0 and is raised to 1."""
-    import math
-    if steps > 10 * L:
-        every = L
-    else:
-        q = int(math.log10(steps + 1) - 1)  # truncation toward zero, as the cast
-        every = int(10.0 ** q) if q >= 0 else 0
-        every = min(max(every, 1), L)
-    return steps % every == 0
-
-
 def _expected_log(n, occ, l2a, sh, tr, steps0, e0, t0, T, L, sp, first_fixed, skip_first):
     """kmc_log.txt lines of a trajectory from its trace (without the COM columns)."""
     o = CO.generate_fcc(n, "Al")
@@ -184,7 +172,7 @@ def _expected_log(n, occ, l2a, sh, tr, steps0, e0, t0, T, L, sp, first_fixed, sk
     energy, time, lines = e0, t0, []
     for i, (s, ea, de, tnew) in enumerate(zip(tr["sel"], tr["ea"], tr["de"], tr["time"])):
         step = steps0 + i
-        if _reference_due(step, L) and not (skip_first and i == 0):
+        if LO.reference_due(step, L) and not (skip_first and i == 0):
             l2a_now = o.lattice_to_atom
             X = vac // (2 * n * n)
             Y = (vac - X * 2 * n * n) // n
