@@ -310,7 +310,7 @@ int lmc_kmc_atoms(lmc_kmc *k, int64_t *lattice_to_atom, int32_t *map_shift);
  * time, temperature, energy, Ea, dE, the migrating atom (-1 without lmc_kmc_set_atoms), the
  * vacancy's unwrapped Cartesian position (box_edge[r] A, null: 4.046 n, times the relative
  * position plus the vacancy atom's map_shift) and, with solute_disp, the solute centre of mass.
- * log_dump_steps = 0 turns it off. Synchronous. */
+ * log_dump_steps = 0 turns it off; at most 1e17. Synchronous. */
 int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
                     const double *box_edge);
 /* Appends the records taken since the last call to filenames[r] (null: skip and drop that
@@ -381,7 +381,8 @@ int lmc_cmc_atoms(lmc_cmc *c, int64_t *lattice_to_atom, int32_t *map_shift);
  *   of 65536 trials while it is on.
  * - annealing: every log_dump_steps steps and at every step whose energy is below the lowest so
  *   far by more than 1e-4 eV; the fourth column is the lowest energy.
- * log_dump_steps = 0 turns it off. Synchronous. */
+ * log_dump_steps = 0 turns it off; at most 1e17 (thermodynamic_averaging_steps at most 2^31).
+ * Synchronous. */
 int lmc_cmc_set_log(lmc_cmc *c, uint64_t log_dump_steps, uint64_t capacity,
                     uint64_t thermodynamic_averaging_steps);
 /* Appends the records taken since the last call to filenames[r] (null: skip and drop that

@@ -1524,6 +1524,16 @@ int lmc_kmc_atoms(lmc_kmc *k, int64_t *lattice_to_atom, int32_t *map_shift) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+constexpr uint64_t kMaxLogEvery = 100000000000000000ull;  // 1e17: 10 L stays in 64 bits
+constexpr uint64_t kMaxAveragingSteps = 1ull << 31;
+constexpr uint64_t kAveragingChunk = 1ull << 16;  // trials per launch with the averaging ring on
+}  // namespace
+
+extern "C" {
+
 int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
                     const double *box_edge) {
   return Guard([&] {
@@ -1538,6 +1548,7 @@ int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
     k->DropGraph();
     if (!log_dump_steps) return LMC_OK;
     if (!capacity) throw std::invalid_argument("log capacity must be positive");
+    if (log_dump_steps > kMaxLogEvery) throw std::invalid_argument("log_dump_steps is too large");
     std::vector<int32_t> nc(R);
     ToHost(nc.data(), k->d.ncell, nc.size());
     std::vector<double> edge(R);
@@ -1560,10 +1571,6 @@ int lmc_kmc_set_log(lmc_kmc *k, uint64_t log_dump_steps, uint64_t capacity,
 }
 
 namespace {
-constexpr uint64_t kMaxLogEvery = 100000000000000000ull;  // 1e17: 10 L stays in 64 bits
-constexpr uint64_t kMaxAveragingSteps = 1ull << 31;
-constexpr uint64_t kAveragingChunk = 1ull << 16;  // trials per launch with the averaging ring on
-
 // A double as McAbstract's log stream prints it: precision(16), the default float format until
 // the first vector has set std::fixed (VectorMatrix.hpp:41-51).
 void PutDouble(std::string &o, double v, bool fixed) {
