@@ -46,34 +46,31 @@ struct SurLds {
   double dt[kG];
 };
 
-// First-layer weight fragments of wave (m, h) for one workgroup step (4 column tiles x KS
-// K-steps), loaded before the gather so that their latency overlaps it.
+// First-layer weight fragments of wave (m, h) for one of its 4 column tiles (KS K-steps). The
+// first tile is loaded before the gather so that its latency overlaps it; each further tile is
+// loaded while the previous one is multiplied (two tiles live instead of four).
 template <int C>
-struct WaveWeights {
+struct TileWeights {
   static constexpr int KS = (kEnvUsed * C + 31) / 32;
-  u32x4 b1[4][KS];
+  u32x4 b1[KS];
 };
 
 template <int C>
-__device__ __forceinline__ void LoadWaveWeights(const SurDev &sd, WaveWeights<C> &ww) {
+__device__ __forceinline__ void LoadTileWeights(const SurDev &sd, int nt, TileWeights<C> &tw) {
   constexpr int Kp = (kEnvUsed * C + 31) / 32 * 32, KS = Kp / 32;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15, grp = lane >> 4;
   const int m = wv >> 1, q = wv & 1;
-  const MlpDev &w = sd.w;
-  const uint16_t *W1t = w.w1t + static_cast<size_t>(m) * kHidden * Kp;
+  const uint16_t *W1t = sd.w.w1t + static_cast<size_t>(m) * kHidden * Kp;
+  const int n = 64 * q + 16 * nt + col;
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const int n = 64 * q + 16 * nt + col;
-#pragma unroll
-    for (int s = 0; s < KS; ++s) ww.b1[nt][s] = Load8(W1t + static_cast<size_t>(n) * Kp + 32 * s + 8 * grp);
-  }
+  for (int s = 0; s < KS; ++s) tw.b1[s] = Load8(W1t + static_cast<size_t>(n) * Kp + 32 * s + 8 * grp);
 }
 
 // Phase 2: ensemble outputs (normalised) of the 48 rows into S.yp (all 8 waves).
 template <int C>
-__device__ void EnsembleForward(const SurDev &sd, const WaveWeights<C> &ww, const JumpLds *Ls,
+__device__ void EnsembleForward(const SurDev &sd, const TileWeights<C> &first, const JumpLds *Ls,
                                 const bool *act, SurLds &S) {
-  constexpr int KS = WaveWeights<C>::KS;
+  constexpr int KS = TileWeights<C>::KS;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15, grp = lane >> 4;
   const int m = wv >> 1, q = wv & 1;
   const MlpDev &w = sd.w;
@@ -89,16 +86,19 @@ __device__ void EnsembleForward(const SurDev &sd, const WaveWeights<C> &ww, cons
 #pragma unroll
     for (int s = 0; s < KS; ++s) a[mt][s] = OneHotFragment<C>(e, 32 * s + 8 * grp, chan);
   }
+  TileWeights<C> cur = first, next;
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) {
     const int n = 64 * q + 16 * nt + col;
+    if (nt < 3) LoadTileWeights<C>(sd, nt + 1, next);
     f32x4 acc[kMT];
 #pragma unroll
     for (int mt = 0; mt < kMT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < KS; ++s)
 #pragma unroll
-      for (int mt = 0; mt < kMT; ++mt) acc[mt] = Mfma(a[mt][s], ww.b1[nt][s], acc[mt]);
+      for (int mt = 0; mt < kMT; ++mt) acc[mt] = Mfma(a[mt][s], cur.b1[s], acc[mt]);
+    if (nt < 3) cur = next;
     const float bias = par[w.lay.ob1 + n];
 #pragma unroll
     for (int mt = 0; mt < kMT; ++mt)
@@ -336,8 +336,8 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
     const bool on = act[g];
     Gathered<kSub> gat;
     if (on) GatherIssue<kSub>(L, occ, n, VacancyXYZ(Q), lt, gat);
-    WaveWeights<C> ww;  // issued after the occupancy loads, consumed after the gather
-    LoadWaveWeights<C>(sd, ww);
+    TileWeights<C> ww;  // issued after the occupancy loads, consumed after the gather
+    LoadTileWeights<C>(sd, 0, ww);
     if (kKmc && on && lt == kSub - 1) DrawStep(L, Q);
     if (on) GatherStore<kSub>(t, L, k.error, lt, gat);
     __syncthreads();
