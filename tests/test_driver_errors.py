@@ -72,3 +72,16 @@ def test_kmc_needs_a_barrier_model(models, coeffs):
     occ = random_alloc(4, ge.species, 0.05, seed=3)
     with pytest.raises(capi.LmcError, match="E0 or Quartic"):
         _kmc(ge, [4], [occ])
+
+
+def test_log_entry_points_reject_null_handles():
+    """The log entry points of both drivers check their handles before any HIP call."""
+    lib = capi.lib()
+    names = (C.c_char_p * 1)(b"log.txt")
+    calls = [lambda: lib.lmc_cmc_set_log(None, 10, 10, 0),
+             lambda: lib.lmc_cmc_write_log(None, names),
+             lambda: lib.lmc_kmc_set_log(None, 10, 10, None),
+             lambda: lib.lmc_kmc_write_log(None, names)]
+    for call in calls:
+        with pytest.raises(capi.LmcError, match="null (driver|argument)"):
+            capi.check(call())
