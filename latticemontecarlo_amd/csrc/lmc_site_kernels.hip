@@ -638,7 +638,7 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
 // ThermodynamicAveraging::GetThermodynamicAverage (ThermodynamicAveraging.cpp:25-40) of the log
 // records of the last launch: over the last min(avg_m, count) energies of the ring, the mean m
 // and sum((e - m) exp(-(e - m) beta)) / sum(exp(-(e - m) beta)) + m, beta = 1 / kB / T of the
-// record. Grid (G, R): workgroup x takes records lo + x, lo + x + G, ... of replica y.
+// record. Grid (R, G): workgroup (x, y) takes records lo + y, lo + y + G, ... of replica x.
 __device__ __forceinline__ double BlockSum256(double v, double *red) {
   v = WaveSum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -651,11 +651,11 @@ __device__ __forceinline__ double BlockSum256(double v, double *red) {
 __global__ __launch_bounds__(kThreads) void CmcAverageKernel(CmcDev c) {
   static_assert(kThreads == 256, "BlockSum256 reduces four waves");
   __shared__ double red[4];
-  const int r = blockIdx.y;
+  const int r = blockIdx.x;
   const uint64_t lo = c.log_lo[r];
   const uint64_t hi = c.log_n[r] < c.log_cap ? c.log_n[r] : c.log_cap;
   const double *ring = c.ring + static_cast<uint64_t>(r) * c.ring_cap;
-  for (uint64_t i = lo + blockIdx.x; i < hi; i += gridDim.x) {
+  for (uint64_t i = lo + blockIdx.y; i < hi; i += gridDim.y) {
     CmcLogRecord &rec = c.log[static_cast<uint64_t>(r) * c.log_cap + i];
     const uint64_t cnt = rec.count;
     const uint64_t w = cnt < c.avg_m ? cnt : c.avg_m;  // >= 1: the record's own energy
@@ -706,15 +706,15 @@ __global__ __launch_bounds__(kThreads) void TotalCountsKernel(KTables t, const u
                                                               const int32_t *ncell,
                                                               unsigned long long *counts) {
   __shared__ unsigned int hist[512];
-  const int r = blockIdx.y;
+  const int r = blockIdx.x;  // grid (R, 64): x the replica, y a slice of its sites
   const int n = ncell[r];
   const int64_t N = 4ll * n * n * n;
   const uint8_t *o = occ + off[r];
   for (int i = threadIdx.x; i < t.n_eci; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const int S = t.S;
-  for (int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < N;
-       s += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  for (int64_t s = static_cast<int64_t>(blockIdx.y) * blockDim.x + threadIdx.x; s < N;
+       s += static_cast<int64_t>(gridDim.y) * blockDim.x) {
     const XYZ c = ToXYZ(s, n);
     const int e1 = Code(o[s], S);
     atomicAdd(&hist[t.eci1[e1]], 1u);
@@ -907,7 +907,7 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
 
 hipError_t LaunchCmcAverage(const CmcDev &c, hipStream_t s) {
   if (c.R <= 0 || !c.log_every || !c.avg_m) return hipSuccess;
-  hipLaunchKernelGGL(CmcAverageKernel, dim3(64, c.R), dim3(kThreads), 0, s, c);
+  hipLaunchKernelGGL(CmcAverageKernel, dim3(c.R, 64), dim3(kThreads), 0, s, c);
   return hipGetLastError();
 }
 
@@ -919,7 +919,7 @@ hipError_t LaunchTotalEncode(const KTables &t, const uint8_t *occ, const int64_t
   hipLaunchKernelGGL(ZeroKernel, dim3((nc + 255) / 256), dim3(256), 0, s, counts, nc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(TotalCountsKernel, dim3(64, R), dim3(kThreads), 0, s, t, occ, off, ncell,
+  hipLaunchKernelGGL(TotalCountsKernel, dim3(R, 64), dim3(kThreads), 0, s, t, occ, off, ncell,
                      counts);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -951,7 +951,7 @@ hipError_t LaunchTotalEnergy(const KTables &t, const uint8_t *occ, const int64_t
   hipLaunchKernelGGL(ZeroKernel, dim3((nc + 255) / 256), dim3(256), 0, s, counts, nc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(TotalCountsKernel, dim3(64, R), dim3(kThreads), 0, s, t, occ, off, ncell,
+  hipLaunchKernelGGL(TotalCountsKernel, dim3(R, 64), dim3(kThreads), 0, s, t, occ, off, ncell,
                      counts);
   e = hipGetLastError();
   if (e != hipSuccess) return e;

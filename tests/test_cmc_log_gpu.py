@@ -116,3 +116,31 @@ def test_annealing_log_matches_reference(torch, coeffs, tmp_path):
         assert len(want) > N // Ld + 10  # improvement rows besides the periodic ones
         _check(got[:-1], want, None)
         assert s.T == pytest.approx(float(an["temperature"][r]), rel=1e-12)
+
+
+def test_cmc_log_argument_errors(torch, coeffs, tmp_path):
+    """lmc_cmc_set_log / lmc_cmc_write_log reject a zero capacity, intervals and windows past
+    their limits and a write with the log off; records past the capacity are reported after the
+    ones that fit are written."""
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import capi
+    path, els = coeffs["e0_3n"]
+    gp = L.Model(path, els, L.PAIRSITE)
+    occ = random_alloc(4, gp.species, 0.1, seed=5, vacancy=False)
+    drv = L.CanonicalMc(gp, None, [4], [occ], 800.0, 800.0, 1, maximum_steps=1000)
+    f = tmp_path / "cmc_log.txt"
+    with pytest.raises(capi.LmcError, match="logging is off"):
+        drv.write_log([f])
+    with pytest.raises(capi.LmcError, match="log capacity must be positive"):
+        drv.set_log(10, capacity=0)
+    with pytest.raises(capi.LmcError, match="log_dump_steps is too large"):
+        drv.set_log(10 ** 18)
+    with pytest.raises(capi.LmcError, match="thermodynamic_averaging_steps is too large"):
+        drv.set_log(10, thermodynamic_averaging_steps=2 ** 31 + 1)
+    drv.set_log(10, capacity=5)
+    drv.run(200)  # due steps 0..98 and every 10 after: far more than 5
+    with pytest.raises(capi.LmcError, match="log records past the capacity were lost"):
+        drv.write_log([f])
+    lines = f.read_text().split("\n")
+    assert lines[0].startswith("steps\t") and [ln.split("\t")[0] for ln in lines[1:6]] == \
+        ["0", "1", "2", "3", "4"]
