@@ -385,6 +385,99 @@ def anneal_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def exittime_workload(args):
+    """ExitTime's per-site jump evaluations (ExitTime::GetBarrierListAndExitTime and
+    GetJumpEnergetics, lmc/ansys/src/ExitTime.cpp:216-311): a vacancy placed at every site of one
+    --ncell^3-cell frame (Al with 3 at.% Mg and Zn each, E0 model), the frame's own vacancy
+    holding Al, 12 jumps per site, in one lmc_vacancy_scan launch over device buffers. CPU
+    baseline: the oracle's single-jump evaluation over a sample of sites. Prints one JSON line."""
+    import torch
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import capi, fcc_neighbors, synth
+    n = args.ncell
+    N = 4 * n ** 3
+    els = ["Al", "Mg", "Zn"]
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0.json")
+    synth.write_coefficients(coeff, els, "E0", seed=args.seed)
+    bm = L.Model(coeff, els, L.E0)
+    sp = bm.species
+    occ = synth.random_occupancy(n, sp, {"Mg": round(0.03 * N), "Zn": round(0.03 * N)},
+                                 seed=args.seed)
+    vac = int(np.flatnonzero(occ == sp.index("X"))[0])
+    dev = torch.device("cuda")
+    d_occ = torch.from_numpy(occ).to(dev)
+    sites = torch.arange(N, dtype=torch.int64, device=dev)
+    nb = torch.empty((N, 12), dtype=torch.int64, device=dev)
+    ea = torch.empty((N, 12), dtype=torch.float64, device=dev)
+    de = torch.empty((N, 12), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sp_ptr = capi.stream_ptr(stream)
+
+    def scan():
+        capi.check(capi.lib().lmc_vacancy_scan(bm.h, capi.dptr(d_occ), n, vac, sp.index("Al"),
+                                               capi.dptr(sites), N, capi.dptr(nb), capi.dptr(ea),
+                                               capi.dptr(de), sp_ptr))
+    for _ in range(max(1, args.warmup // 500)):
+        scan()
+    reps = max(1, args.steps // 10000)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        scan()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    secs = e0.elapsed_time(e1) / 1e3 / reps
+    bm.check_errors()
+    value = 12 * N / secs
+    log(f"exit-time scan: {value:.4g} jump evaluations/s ({N} sites, {secs * 1e3:.3f} ms per scan)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = O.Model(O.Lattice(n), coeff, els, O.E0)
+        base = occ.copy()
+        base[vac] = sp.index("Al")
+        sample = range(0, N, max(1, N // 400))
+        nbs = {s: [int(y) for y in fcc_neighbors(n, s, 1)] for s in sample}
+
+        def cpu_pass():
+            k = 0
+            for s in sample:
+                keep = base[s]
+                base[s] = sp.index("X")
+                for y in nbs[s]:
+                    om.barrier_de(base, s, y)
+                    k += 1
+                base[s] = keep
+            return k
+        cpu_pass()  # builds the oracle's pair tables, as the reference does at construction
+        t0 = time.perf_counter()
+        k = cpu_pass()
+        cv = k / (time.perf_counter() - t0)
+        cpu = {"value": round(cv, 1), "unit": "jump evaluations/s", "cores": 1, "kind": "port",
+               "sample": f"oracle GetBarrierAndDiffFromLatticeIdPair (E0, no memo) of the 12 jumps "
+                         f"of {len(sample)} sites of the same frame, one thread, tables built "
+                         f"before timing"}
+        log(f"cpu baseline (exit-time scan): {cv:.1f} jump evaluations/s")
+    line = {"metric": "ExitTime jump evaluations/s (vacancy at every site of a frame)",
+            "value": round(value, 1), "unit": "jump evaluations/s", "n_gpus": 1, "steps": reps,
+            "warmup": max(1, args.warmup // 500), "ms_per_step": round(secs * 1e3, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"ExitTime barrier scan, one {n}^3-cell frame ({N} sites, "
+                                   f"{12 * N} jumps per scan), Al with 3 at.% Mg and Zn each, "
+                                   "E0 barrier model"},
+            # 300 algorithmic bytes per jump: its 60 environment sites x (1 B species + 4 B index)
+            "roofline": {"bound": "hbm", "achieved": round(300 * value / 1e9, 3), "peak": 8000.0,
+                         "unit": "GB/s", "frac": round(300 * value / 1e9 / 8000.0, 6),
+                         "traffic": None},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def uneven_workload(args):
     """Uneven-size batched KMC alongside canonical MC: 256 KMC replicas of n^3 cells with n drawn
     from 8..40 (2048-256000 sites; Mg, Zn 1-5 at.% each, T 400-700 K) on one stream and 64
@@ -468,7 +561,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch eagerly instead of a HIP graph")
-    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain", "anneal"], default="batched")
+    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain", "anneal", "exittime"],
+                    default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
     args = ap.parse_args()
@@ -494,6 +588,9 @@ def main():
         return
     if args.workload == "anneal":
         anneal_workload(args)
+        return
+    if args.workload == "exittime":
+        exittime_workload(args)
         return
     R, n = args.replicas, args.ncell
     tmp = tempfile.mkdtemp()
