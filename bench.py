@@ -433,6 +433,29 @@ def exittime_workload(args):
     bm.check_errors()
     value = 12 * N / secs
     log(f"exit-time scan: {value:.4g} jump evaluations/s ({N} sites, {secs * 1e3:.3f} ms per scan)")
+    # GetProfileEnergy / GetBindingEnergy's per-site energy changes (ExitTime.cpp:401-512): every
+    # site of the frame from a vacancy to Mg, one batched lmc_site_change_de launch
+    ps = L.Model(coeff, els, L.PAIRSITE)
+    z_off = torch.zeros(N, dtype=torch.int64, device=dev)
+    ncs = torch.full((N,), n, dtype=torch.int32, device=dev)
+    frm = torch.full((N,), sp.index("X"), dtype=torch.uint8, device=dev)
+    to = torch.full((N,), sp.index("Mg"), dtype=torch.uint8, device=dev)
+    sde = torch.empty(N, dtype=torch.float64, device=dev)
+
+    def site_changes():
+        capi.check(capi.lib().lmc_site_change_de(ps.h, capi.dptr(d_occ), capi.dptr(z_off),
+                                                 capi.dptr(ncs), capi.dptr(sites), capi.dptr(frm),
+                                                 capi.dptr(to), N, capi.dptr(sde), sp_ptr))
+    site_changes()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        site_changes()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ssecs = e0.elapsed_time(e1) / 1e3 / reps
+    ps.check_errors()
+    log(f"exit-time site energy changes: {N / ssecs:.4g} sites/s ({ssecs * 1e3:.3f} ms per frame)")
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -469,7 +492,8 @@ def exittime_workload(args):
             "data": "synthetic",
             "config": {"workload": f"ExitTime barrier scan, one {n}^3-cell frame ({N} sites, "
                                    f"{12 * N} jumps per scan), Al with 3 at.% Mg and Zn each, "
-                                   "E0 barrier model"},
+                                   "E0 barrier model",
+                       "site_energy_changes_per_s": round(N / ssecs, 1)},
             # 300 algorithmic bytes per jump: its 60 environment sites x (1 B species + 4 B index)
             "roofline": {"bound": "hbm", "achieved": round(300 * value / 1e9, 3), "peak": 8000.0,
                          "unit": "GB/s", "frac": round(300 * value / 1e9 / 8000.0, 6),
