@@ -10,6 +10,10 @@ Prints one JSON line: KMC steps/s summed over all replicas, the roofline of the 
 (algorithmic bytes: 677 B per replica-step, measured with HIP events on the launch stream) and
 the CPU baseline (the oracle's KineticMcFirst restatement on this host's cores, bounded sample
 of the single-replica 10^3-cell workload).
+
+Other workloads (--workload): uneven (uneven-size KMC beside canonical MC), large (64 replicas of
+50^3 cells with the quaternary surrogate), chain (second-order KMC), anneal (simulated
+annealing), exittime (ExitTime's per-site jump scan and energy changes). Each prints one JSON line.
 """
 import argparse
 import json
