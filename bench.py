@@ -83,20 +83,20 @@ def make_workload(R, n, seed):
     return occs, np.array(temps)
 
 
-def cpu_baseline(coeff_path, budget_s=12.0):
-    """Oracle KineticMcFirst on the host cores: single replica (n=10, 4000 sites, T=500 K)
-    with min(12, cores) OpenMP threads on the 12-event loop, and `cores` independent
-    single-thread replicas; returns the larger aggregate steps/s."""
+def cpu_baseline(coeff_path, budget_s=12.0, els=("Al", "Mg", "Zn")):
+    """Oracle KineticMcFirst on the host cores: single replica (n=10, 4000 sites, 80 atoms of
+    each solute, T=500 K) with min(12, cores) OpenMP threads on the 12-event loop, and `cores`
+    independent single-thread replicas; returns the larger aggregate steps/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from latticemontecarlo_amd import synth
     cores = min(16, os.cpu_count() or 1)
     n = 10
-    els = ["Al", "Mg", "Zn"]
+    els = list(els)
     lat = O.Lattice(n)
     bm = O.Model(lat, coeff_path, els, O.E0)
     em = O.Model(lat, coeff_path, els, O.ENERGY)
-    occ = synth.random_occupancy(n, ["Al", "Mg", "X", "Zn"], {"Mg": 80, "Zn": 80}, seed=1)
+    occ = synth.random_occupancy(n, sorted(els + ["X"]), {e: 80 for e in els if e != "Al"}, seed=1)
     # calibration (also builds every pair table once)
     secs, _ = O.kmc_baseline(bm, em, lat, occ, 500.0, 7, 1, 1, 100000, 200)
     per_step = max(secs / 200, 1e-6)
@@ -110,7 +110,7 @@ def cpu_baseline(coeff_path, budget_s=12.0):
         f"throughput {thru:.1f} steps/s ({cores} replicas)")
     best = max(single, thru)
     return {"value": round(best, 2), "unit": "KMC steps/s", "cores": cores, "kind": "port",
-            "sample": (f"oracle KineticMcFirst (E0, LRU 100000), 10^3 cells x 4000 sites, "
+            "sample": (f"oracle KineticMcFirst (E0 {'/'.join(els)}, LRU 100000), 10^3 cells x 4000 sites, "
                        f"T=500 K: 1 replica x {steps} steps with {min(12, cores)} threads "
                        f"({single:.1f} steps/s) and {cores} replicas x {steps_t} steps "
                        f"single-threaded ({thru:.1f} steps/s); value = larger")}
@@ -275,6 +275,8 @@ def large_workload(args):
                        "formulation": FORMS[tune["formulation"]],
                        "event_evaluations_per_s": round(12 * value, 1)},
             "surrogate": s}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(coeff, els=els)
     print(json.dumps(line), flush=True)
 
 
@@ -574,6 +576,22 @@ def uneven_workload(args):
                        "cmc_replicas": Rc, "cmc_sites": Nc, "cmc_trials_per_replica": trials,
                        "cmc_trials_per_s": round(cval, 1),
                        "cmc_acceptance": round(float(cst["accepted"].sum()) / max(1, cdone), 4)}}
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(coeff)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        lat = O.Lattice(nc)
+        op = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.PAIRSITE)
+        cores = min(16, os.cpu_count() or 1)
+        kc = 4000
+        per = O.cmc_baseline(op, lat, cocc[0], 800.0, args.seed, 1, kc, trials, False) / kc
+        kc = int(min(trials, max(2000, 10.0 / max(per, 1e-7))))
+        secs_c = O.cmc_baseline(op, lat, cocc[0], 800.0, args.seed, cores, kc, trials, False)
+        cpu["cmc"] = {"value": round(cores * kc / secs_c, 1), "unit": "trials/s", "cores": cores,
+                      "sample": f"oracle CanonicalMc trials, {cores} replicas of {nc}^3 cells run "
+                                f"concurrently, one thread each, {kc} trials each at 800 K"}
+        log(f"cpu baseline (canonical MC): {cores * kc / secs_c:.1f} trials/s")
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 
