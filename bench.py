@@ -7,7 +7,7 @@ model), FP64 energies, rates, prefix sums and clock. A bench "step" is one KMC s
 replica; the persistent kernel advances all replicas `--chunk` steps per launch.
 
 Prints one JSON line: KMC steps/s summed over all replicas, the roofline of the dominant kernel
-(algorithmic bytes: 677 B per replica-step, measured with HIP events on the launch stream) and
+(algorithmic bytes: 137 B per replica-step, measured with HIP events on the launch stream) and
 the CPU baseline (the oracle's KineticMcFirst restatement on this host's cores, bounded sample
 of the single-replica 10^3-cell workload).
 
@@ -27,7 +27,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-ALGO_BYTES_PER_STEP = 677  # 135 unique sites x (1 B species + 4 B index) + 2 B written
+# Algorithmic HBM bytes of one replica-step: the species bytes of the 135 distinct sites within
+# 3NN of the vacancy's 12 jump pairs, read once, plus the 2 bytes the jump writes. Site ids are
+# closed forms of the coordinates here, so no neighbour-index bytes are read.
+ALGO_BYTES_PER_STEP = 137
+ALGO_BYTES_NOTE = ("137 B per replica-step = 135 distinct occupancy bytes read + 2 written; site ids "
+                   "are closed forms, so the 4 B neighbour index per site of a table-driven design "
+                   "(677 B per step) is not read. The kernel's working set is cache/LDS resident "
+                   "(traffic is measured HBM bytes per launch), so HBM is not its bound: the step is "
+                   "VALU/LDS-issue and latency bound (issue)")
 FORMS = ["dense", "solvent-baseline", "dense-wide"]  # kernel shapes of the autotuner
 METRIC = "KMC steps/s over all replicas, surrogate vs plain GPU at \u22645 meV barrier MAE"
 HBM_PEAK_GBS = 8000.0
@@ -37,11 +45,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(R, n, chunk):
+def measured_traffic(R, n, chunk, model="e0"):
     """HBM bytes per launch of the KMC kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (copy-calibrated), scaled to `chunk` steps per launch; None when no
     summary exists for this replica count and lattice size."""
-    path = os.path.join(ROOT, "profiles", f"kmc_e0_{R}x{4 * n ** 3}_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"kmc_{model}_{R}x{4 * n ** 3}_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -51,10 +59,10 @@ def measured_traffic(R, n, chunk):
     return None
 
 
-def measured_issue(R, n):
+def measured_issue(R, n, model="e0"):
     """VALU issue utilisation and instructions per replica-step of the KMC kernel from the
     committed rocprofv3 SQ-counter summary of this workload; None when none exists."""
-    path = os.path.join(ROOT, "profiles", f"kmc_e0_{R}x{4 * n ** 3}_sq_counters.txt")
+    path = os.path.join(ROOT, "profiles", f"kmc_{model}_{R}x{4 * n ** 3}_sq_counters.txt")
     if not os.path.exists(path):
         return None
     out = {"source": os.path.relpath(path, ROOT)}
@@ -67,6 +75,62 @@ def measured_issue(R, n):
             elif line.startswith("VALU issue:"):
                 out["valu_issue_frac"] = float(line.split("=")[-1].split()[0])
     return out
+
+
+def launch_plan(steps, chunk):
+    """Launch sizes that add up to exactly `steps` KMC steps: steps // chunk launches of `chunk`
+    steps (chunk clamped to steps), then one shorter launch for the remainder. Returns a list of
+    (steps per launch, launches)."""
+    if steps <= 0 or chunk <= 0:
+        raise ValueError("steps and chunk must be positive")
+    chunk = min(chunk, steps)
+    full, rest = divmod(steps, chunk)
+    plan = [(chunk, full)]
+    if rest:
+        plan.append((rest, 1))
+    return plan
+
+
+def run_plan(drv, plan, stream, eager=False):
+    """Issues the launches of `plan` on `stream`: the full launches from the captured HIP graph
+    (built beforehand by prepare_plan), the remainder launch eagerly."""
+    (chunk, full), rest = plan[0], plan[1:]
+    if eager:
+        for _ in range(full):
+            drv.run(chunk, stream)
+    else:
+        drv.run_graph(chunk, full, stream)
+    for c, _ in rest:
+        drv.run(c, stream)
+
+
+def prepare_plan(drv, plan, stream, eager=False):
+    """Captures the graph of the plan's full launches (outside any timed region) and sets up the
+    remainder's kernel shape with a zero-cost check of the plan."""
+    assert sum(c * k for c, k in plan) > 0
+    if not eager:
+        drv.build_graph(plan[0][0], plan[0][1])
+
+
+def host_cores():
+    """Threads the CPU baselines use, and the host facts behind the figure. The count is the
+    process's affinity mask, capped by OMP_NUM_THREADS when that is set (the thread allotment of
+    a host that is shared between GPUs)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cores = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, {"nproc": os.cpu_count(), "affinity_cpus": aff,
+                   "omp_num_threads": omp or None, "cpu_model": model,
+                   "threads_used_rule": "affinity mask, capped by OMP_NUM_THREADS when set"}
 
 
 def make_workload(R, n, seed):
@@ -83,18 +147,19 @@ def make_workload(R, n, seed):
     return occs, np.array(temps)
 
 
-def cpu_baseline(coeff_path, budget_s=12.0, els=("Al", "Mg", "Zn")):
+def cpu_baseline(coeff_path, budget_s=12.0, els=("Al", "Mg", "Zn"), kind="E0"):
     """Oracle KineticMcFirst on the host cores: single replica (n=10, 4000 sites, 80 atoms of
     each solute, T=500 K) with min(12, cores) OpenMP threads on the 12-event loop, and `cores`
-    independent single-thread replicas; returns the larger aggregate steps/s."""
+    independent single-thread replicas; returns the larger aggregate steps/s. `kind` is the
+    barrier model, E0 or Quartic (the reference's KMC driver holds the Quartic LRU predictor)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from latticemontecarlo_amd import synth
-    cores = min(16, os.cpu_count() or 1)
+    cores, host = host_cores()
     n = 10
     els = list(els)
     lat = O.Lattice(n)
-    bm = O.Model(lat, coeff_path, els, O.E0)
+    bm = O.Model(lat, coeff_path, els, O.QUARTIC if kind == "Quartic" else O.E0)
     em = O.Model(lat, coeff_path, els, O.ENERGY)
     occ = synth.random_occupancy(n, sorted(els + ["X"]), {e: 80 for e in els if e != "Al"}, seed=1)
     # calibration (also builds every pair table once)
@@ -110,7 +175,9 @@ def cpu_baseline(coeff_path, budget_s=12.0, els=("Al", "Mg", "Zn")):
         f"throughput {thru:.1f} steps/s ({cores} replicas)")
     best = max(single, thru)
     return {"value": round(best, 2), "unit": "KMC steps/s", "cores": cores, "kind": "port",
-            "sample": (f"oracle KineticMcFirst (E0 {'/'.join(els)}, LRU 100000), 10^3 cells x 4000 sites, "
+            "host": host, "single_replica_steps_per_s": round(single, 2),
+            "throughput_steps_per_s": round(thru, 2),
+            "sample": (f"oracle KineticMcFirst ({kind} {'/'.join(els)}, LRU 100000), 10^3 cells x 4000 sites, "
                        f"T=500 K: 1 replica x {steps} steps with {min(12, cores)} threads "
                        f"({single:.1f} steps/s) and {cores} replicas x {steps_t} steps "
                        f"single-threaded ({thru:.1f} steps/s); value = larger")}
@@ -206,8 +273,9 @@ def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_va
 
 
 def timed_plain(L, drv, R, steps, warmup, chunk):
-    """Warm-up launches, then `steps` KMC steps of every replica replayed from one HIP graph of
-    steps / chunk launches; returns the seconds measured with HIP events on the stream."""
+    """Warm-up launches, then exactly `steps` KMC steps of every replica (a HIP graph of the
+    full `chunk`-step launches, then a shorter launch for the remainder); returns the seconds
+    measured with HIP events on the stream."""
     import torch
     stream = torch.cuda.current_stream()
     left = warmup
@@ -215,14 +283,13 @@ def timed_plain(L, drv, R, steps, warmup, chunk):
         c = min(chunk, left)
         drv.run(c, stream)
         left -= c
-    n_launch = max(1, steps // chunk)
-    assert n_launch * chunk == steps, "--steps must be a multiple of --chunk"
-    drv.build_graph(chunk, n_launch)
+    plan = launch_plan(steps, chunk)
+    prepare_plan(drv, plan, stream)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     start.record(stream)
-    drv.run_graph(chunk, n_launch, stream)
+    run_plan(drv, plan, stream)
     end.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(end) / 1e3
@@ -304,7 +371,7 @@ def chain_workload(args):
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        cores, host = host_cores()
         lat = O.Lattice(10)
         om = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.E0)
         occ = synth.random_occupancy(10, ["Al", "Mg", "X", "Zn"], {"Mg": 80, "Zn": 80}, seed=1)
@@ -313,6 +380,7 @@ def chain_workload(args):
         thru = cores * 200 / secs
         log(f"cpu baseline (chain): {thru:.1f} steps/s ({cores} single-thread replicas)")
         cpu = {"value": round(thru, 2), "unit": "KMC steps/s", "cores": cores, "kind": "port",
+               "host": host,
                "sample": f"oracle KineticMcChainOmpi steps (E0, LRU 100000), 10^3 cells x 4000 "
                          f"sites, T=500 K: {cores} single-thread replicas x 200 steps"}
     line = {"metric": "KMC steps/s over all replicas (second-order chain steps)",
@@ -367,13 +435,14 @@ def anneal_workload(args):
         import oracle as O
         lat = O.Lattice(n)
         om = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.PAIRSITE)
-        cores = min(16, os.cpu_count() or 1)
+        cores, host = host_cores()
         k = 4000  # calibration, then a sample of about 15 s of concurrent work
         per = O.cmc_baseline(om, lat, occs[0], 800.0, args.seed, 1, k, trials, True) / k
         k = int(min(trials, max(2000, 15.0 / max(per, 1e-7))))
         secs_c = O.cmc_baseline(om, lat, occs[0], 800.0, args.seed, cores, k, trials, True)
         thru = cores * k / secs_c
         cpu = {"value": round(thru, 1), "unit": "trials/s", "cores": cores, "kind": "port",
+               "host": host,
                "sample": f"oracle SimulatedAnnealing trials, {cores} replicas of 30^3 cells run "
                          f"concurrently, one thread each, {k} trials each ({secs_c:.1f} s)"}
         log(f"cpu baseline (annealing): {thru:.1f} trials/s ({cores} concurrent replicas)")
@@ -547,8 +616,9 @@ def uneven_workload(args):
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     ev[0].record(sk)
     ev[2].record(sc)
-    for _ in range(max(1, ksteps // args.chunk)):
-        kmc.run(args.chunk, sk)
+    for c, k in launch_plan(ksteps, args.chunk):
+        for _ in range(k):
+            kmc.run(c, sk)
     cchunk = max(1, trials // 20)
     for _ in range(max(1, trials // cchunk)):
         cmc.run(cchunk, sc)
@@ -582,7 +652,7 @@ def uneven_workload(args):
         import oracle as O
         lat = O.Lattice(nc)
         op = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.PAIRSITE)
-        cores = min(16, os.cpu_count() or 1)
+        cores, host = host_cores()
         kc = 4000
         per = O.cmc_baseline(op, lat, cocc[0], 800.0, args.seed, 1, kc, trials, False) / kc
         kc = int(min(trials, max(2000, 10.0 / max(per, 1e-7))))
@@ -611,6 +681,8 @@ def main():
                     default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
+    ap.add_argument("--model", choices=["e0", "quartic"], default="e0",
+                    help="barrier model of the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); replicas are "
@@ -639,62 +711,59 @@ def main():
         exittime_workload(args)
         return
     R, n = args.replicas, args.ncell
+    quartic = args.model == "quartic"
     tmp = tempfile.mkdtemp()
-    coeff = os.path.join(tmp, "coeff_e0.json")
-    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "E0", seed=args.seed)
-    log(f"setup: {R} replicas x {4 * n ** 3} sites")
+    coeff = os.path.join(tmp, f"coeff_{args.model}.json")
+    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "Quartic" if quartic else "E0",
+                             seed=args.seed)
+    log(f"setup: {R} replicas x {4 * n ** 3} sites, {args.model} barrier model")
     occs, temps = make_workload(R, n, args.seed)
-    bm = L.Model(coeff, ["Al", "Mg", "Zn"], L.E0)
+    bm = L.Model(coeff, ["Al", "Mg", "Zn"], L.QUARTIC if quartic else L.E0)
     em = L.Model(coeff, ["Al", "Mg", "Zn"], L.ENERGY)
-    total_steps = args.warmup + args.steps
-    drv = L.KineticMcFirst(bm, em, [n] * R, occs, temps, seed=args.seed,
-                           maximum_steps=total_steps + 10)
+    start_steps = 0
+    # maximum_steps is far beyond any run, so the timed steps never meet it
+    drv = L.KineticMcFirst(bm, em, [n] * R, occs, temps, seed=args.seed, maximum_steps=10 ** 12)
     stream = torch.cuda.current_stream()
     tune = drv.autotune(steps=300)
     log(f"autotune: dense {tune['ms_dense']:.4f} ms, solvent baseline {tune['ms_solvent']:.4f} ms, "
         f"dense wide {tune['ms_wide']:.4f} ms per KMC step of the batch -> "
         f"{FORMS[tune['formulation']]}")
     t0 = time.time()
-    left = args.warmup
-    while left > 0:
-        c = min(args.chunk, left)
-        drv.run(c, stream)
-        left -= c
+    for c, k in launch_plan(args.warmup, args.chunk) if args.warmup > 0 else []:
+        for _ in range(k):
+            drv.run(c, stream)
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} steps: {time.time() - t0:.2f} s")
+    steps0 = int(drv.state()["steps"].min())
 
-    # timed region: args.steps / chunk launches of `chunk` steps, captured once in a HIP graph
-    # (outside the timed region) and replayed
-    n_launch = max(1, args.steps // args.chunk)
-    assert n_launch * args.chunk == args.steps, "--steps must be a multiple of --chunk"
-    if not args.eager:
-        drv.build_graph(args.chunk, n_launch)
+    # timed region: exactly args.steps KMC steps of every replica -- the full launches of
+    # args.chunk steps replayed from a HIP graph captured beforehand, then one shorter launch
+    plan = launch_plan(args.steps, args.chunk)
+    prepare_plan(drv, plan, stream, args.eager)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     start.record(stream)
-    if args.eager:
-        for _ in range(n_launch):
-            drv.run(args.chunk, stream)
-    else:
-        drv.run_graph(args.chunk, n_launch, stream)
+    run_plan(drv, plan, stream, args.eager)
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     elapsed = start.elapsed_time(end) / 1e3
-    log(f"{n_launch} launches of {args.chunk} steps ({'eager' if args.eager else 'HIP graph'}): "
-        f"{elapsed * 1e3:.1f} ms, {elapsed / n_launch * 1e3:.2f} ms per launch")
+    n_launch = sum(k for _, k in plan)
+    log(f"{n_launch} launches ({plan}, {'eager' if args.eager else 'HIP graph'}): "
+        f"{elapsed * 1e3:.3f} ms, {elapsed / n_launch * 1e3:.3f} ms per launch")
     st = drv.state()
-    assert int(st["steps"].min()) == total_steps, "every replica must have run every step"
+    assert int(st["steps"].min()) == steps0 + args.steps, "every replica must run every timed step"
+    assert int(st["steps"].max()) == steps0 + args.steps, "no replica may run past the timed steps"
     bm.check_errors()
 
     replica_steps = R * args.steps
     value = replica_steps / elapsed
-    # algorithmic bytes of one launch over its average duration (the launches run back to back
-    # on one stream, so the region's time over the launch count is the average launch duration)
-    achieved = ALGO_BYTES_PER_STEP * R * args.chunk / (elapsed / n_launch) / 1e9
-    traffic = measured_traffic(R, n, args.chunk)
+    # algorithmic bytes over the launches' average duration (the launches run back to back on
+    # one stream, so the region's time is the sum of the launch durations)
+    achieved = ALGO_BYTES_PER_STEP * replica_steps / elapsed / 1e9
+    traffic = measured_traffic(R, n, args.steps / n_launch, args.model)
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -710,30 +779,40 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain batched KMC, {R} replicas x {n}^3 FCC cells "
                                f"({4 * n ** 3} sites), Al with 1-5 at.% Mg and Zn each, "
-                               "T 400-700 K, E0 barrier model, one vacancy per replica",
+                               f"T 400-700 K, {'Quartic' if quartic else 'E0'} barrier model, "
+                               "one vacancy per replica",
                    "value_path": "plain GPU path (exact); the surrogate path is reported under "
                                  "surrogate",
                    "replicas": R, "sites_per_replica": 4 * n ** 3,
-                   "steps_per_replica": args.steps, "steps_per_launch": args.chunk,
-                   "launch": "eager" if args.eager else "hip graph",
+                   "steps_per_replica": args.steps, "launches": [list(x) for x in plan],
+                   "launch": "eager" if args.eager else "hip graph (+ eager remainder)",
                    "formulation": FORMS[tune["formulation"]],
+                   "autotune_ms_per_batch_step": {FORMS[f]: (None if tune[k] != tune[k] else
+                                                             round(tune[k], 6))
+                                                  for f, k in enumerate(["ms_dense", "ms_solvent",
+                                                                         "ms_wide"])},
                    "event_evaluations_per_s": round(12 * value, 1),
-                   "wall_s": round(wall, 3)},
+                   "wall_s": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                      "traffic": traffic,
-                     "note": "the kernel does not stream: its working set is cache/LDS resident "
-                             "and the step is latency- and VALU/LDS-issue bound (issue)",
-                     "issue": measured_issue(R, n)},
+                     "algorithmic_bytes_per_replica_step": ALGO_BYTES_PER_STEP,
+                     "note": ALGO_BYTES_NOTE,
+                     "issue": measured_issue(R, n, args.model)},
     }
     log(f"GPU: {value:.4g} KMC steps/s ({12 * value:.4g} event evaluations/s), "
-        f"{elapsed:.3f} s for {replica_steps} replica-steps")
-    if not args.no_surrogate:
+        f"{elapsed:.4f} s for {replica_steps} replica-steps")
+    if not args.no_surrogate and not quartic:
         line["surrogate"] = surrogate_leg(L, bm, R, n, occs, temps, args.seed,
-                                          args.surrogate_steps, min(args.warmup, 1000),
+                                          args.surrogate_steps, min(max(args.warmup, 100), 1000),
                                           args.chunk, value)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(coeff)
+        line["cpu_baseline"] = cpu_baseline(coeff, kind="Quartic" if quartic else "E0")
+        if not quartic:
+            # the reference's KMC driver holds the Quartic LRU predictor; timed beside E0
+            qcoeff = os.path.join(tmp, "coeff_quartic.json")
+            synth.write_coefficients(qcoeff, ["Al", "Mg", "Zn"], "Quartic", seed=args.seed)
+            line["cpu_baseline"]["quartic"] = cpu_baseline(qcoeff, budget_s=8.0, kind="Quartic")
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -1305,6 +1305,17 @@ int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *
         clone(s.disp, k->d.disp, 3 * R);
         clone(s.com, k->d.com, 3 * R);
         clone(s.error, k->d.error, 1);
+        // the timing runs ignore maximum_steps and the escape stop, so every replica runs all
+        // `steps` steps whatever the driver's own limits are
+        s.max_steps = DevCopy(std::vector<uint64_t>(R, std::numeric_limits<uint64_t>::max()));
+        owned.push_back(s.max_steps);
+        {
+          std::vector<int32_t> fl(R);
+          ToHost(fl.data(), k->d.flags, R);
+          for (auto &f : fl) f &= ~2;
+          s.flags = DevCopy(fl);
+          owned.push_back(s.flags);
+        }
         if (k->d.atom) {  // tracked atoms are written by every jump
           clone(s.atom, k->d.atom, static_cast<size_t>(k->total_sites));
           clone(s.shift, k->d.shift, 3 * static_cast<size_t>(k->total_sites));
@@ -1315,6 +1326,14 @@ int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *
         Check(hipEventRecord(e1, nullptr), "hipEventRecord");
         Check(hipEventSynchronize(e1), "hipEventSynchronize");
         Check(hipEventElapsedTime(&ms[f], e0, e1), "hipEventElapsedTime");
+        // divide by the steps every replica actually ran
+        std::vector<uint64_t> a(R), b(R);
+        ToHost(a.data(), s.steps, R);
+        ToHost(b.data(), k->d.steps, R);
+        uint64_t ran = static_cast<uint64_t>(steps);
+        for (size_t r = 0; r < R; ++r) ran = std::min<uint64_t>(ran, a[r] - b[r]);
+        if (ran == 0) throw std::runtime_error("autotune: a timing run advanced no step");
+        ms[f] /= static_cast<float>(ran);
       }
     } catch (...) {
       for (void *p : owned) (void)hipFree(p);
@@ -1332,7 +1351,7 @@ int lmc_kmc_autotune_shapes(lmc_kmc *k, int32_t steps, int32_t *chosen, double *
     if (chosen) *chosen = k->formulation;
     if (ms_per_step)
       for (int f = 0; f < 3; ++f)
-        ms_per_step[f] = f < forms ? ms[f] / steps : std::numeric_limits<double>::quiet_NaN();
+        ms_per_step[f] = f < forms ? ms[f] : std::numeric_limits<double>::quiet_NaN();
     return LMC_OK;
   });
 }
@@ -1351,6 +1370,9 @@ int lmc_kmc_set_schedule(lmc_kmc *k, int32_t n_points, const double *time, const
       tv.push_back(p.first);
       Tv.push_back(p.second);
     }
+    // a captured graph holds the old schedule's pointers and point count
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    k->DropGraph();
     DevFree(k->d.tt_time);
     DevFree(k->d.tt_temp);
     k->d.n_tt = n_points;
@@ -2329,6 +2351,9 @@ int lmc_kmc_set_surrogate(lmc_kmc *k, const lmc_surrogate *s, double threshold, 
     if (s && s->M != 4) throw std::invalid_argument("the KMC surrogate path needs 4 members");
     if (s && s->S != k->bm->m->S) throw std::invalid_argument("surrogate built for another model");
     if (s && k->formulation == 3) throw std::invalid_argument("chain steps take no surrogate");
+    // a captured graph holds the surrogate's parameters by value
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    k->DropGraph();
     k->sur = s;
     k->sur_threshold = threshold;
     k->sur_window = window;

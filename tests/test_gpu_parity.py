@@ -110,17 +110,24 @@ def test_total_energy_matches_oracle(torch, coeffs, name):
         assert abs(e[i] - om.total_energy(occs[i])) <= TOL_E * max(1.0, abs(e[i]))
 
 
-def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, **opts):
+def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, maximum_steps=10 ** 9,
+                  occs=None, **opts):
+    """The GPU driver and the oracle's KMC on the same replicas. `n` is one box size for all
+    replicas or a list of R sizes (an uneven-size batch); `occs` overrides the random alloys."""
     import latticemontecarlo_amd as L
     import oracle as O
     path, els = coeffs[name]
     gb = L.Model(path, els, kind)
     ge = L.Model(path, els, L.ENERGY)
     species = gb.species
-    occs = [random_alloc(n, species, 0.05 + 0.03 * r, seed=500 + r) for r in range(R)]
+    sizes = list(n) if np.ndim(n) else [n] * R
+    assert len(sizes) == R
+    if occs is None:
+        occs = [random_alloc(sizes[r], species, 0.05 + 0.03 * (r % 4), seed=500 + r)
+                for r in range(R)]
     seed = 987654321
     tt = opts.pop("tt", None)
-    drv = L.KineticMcFirst(gb, ge, [n] * R, occs, temps, seed, maximum_steps=10 ** 9, **opts)
+    drv = L.KineticMcFirst(gb, ge, sizes, occs, temps, seed, maximum_steps=maximum_steps, **opts)
     drv.set_formulation(form)
     if tt is not None:
         drv.set_schedule(*tt)
@@ -129,14 +136,17 @@ def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, **opts):
     gst = drv.state()
     gtr = drv.trace()
     gocc = drv.occupancy()
-    lat = O.Lattice(n)
-    ob = O.Model(lat, path, els, kind)
-    oe = O.Model(lat, path, els, 0)
+    models = {}
     refs = []
     for r in range(R):
+        nr = sizes[r]
+        if nr not in models:
+            lat = O.Lattice(nr)
+            models[nr] = (lat, O.Model(lat, path, els, kind), O.Model(lat, path, els, 0))
+        lat, ob, oe = models[nr]
         k = O.Kmc(ob, oe, lat, occs[r], float(np.broadcast_to(temps, (R,))[r]), seed, replica=r,
                   tt=tt, chain=form == 3, **opts)
-        tr = k.run(steps, trace=True)
+        tr = k.run(steps, maximum_steps=maximum_steps, trace=True)
         refs.append((k.state(), tr, k.occupancy()))
     return gst, gtr, gocc, refs
 
@@ -316,5 +326,49 @@ def test_kmc_graph_replay_matches_eager(torch, coeffs, form):
     sa, sb = a.state(), b.state()
     assert np.array_equal(sa["steps"], sb["steps"]) and np.array_equal(sa["vacancy"], sb["vacancy"])
     assert np.array_equal(sa["time"], sb["time"]) and np.array_equal(sa["energy"], sb["energy"])
+    for x, y in zip(a.occupancy(), b.occupancy()):
+        assert np.array_equal(x, y)
+
+
+def test_kmc_autotune_is_not_truncated_by_maximum_steps(torch, coeffs):
+    """The autotuner's timing runs ignore the driver's maximum_steps: a driver limited to 5 steps
+    times the shapes like one without a limit, and its own replicas do not move."""
+    import latticemontecarlo_amd as L
+    path, els = coeffs["e0_3"]
+    gb = L.Model(path, els, L.E0)
+    n, R = 8, 64
+    occs = [random_alloc(n, gb.species, 0.03, seed=170 + r) for r in range(R)]
+    short = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=5)
+    free = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 3, maximum_steps=10 ** 12)
+    a = short.autotune(steps=300)
+    b = free.autotune(steps=300)
+    print(f"autotune limited {a} unlimited {b}", flush=True)
+    for key in ("ms_dense", "ms_solvent", "ms_wide"):
+        assert 1 / 3 < a[key] / b[key] < 3, key
+    st = short.state()
+    assert np.array_equal(st["steps"], np.zeros(R, np.uint64))
+
+
+def test_kmc_graph_follows_a_new_schedule(torch, coeffs):
+    """A captured graph is dropped when the temperature schedule changes: build_graph, then
+    set_schedule, then run_graph does what eager launches under the new schedule do."""
+    import latticemontecarlo_amd as L
+    path, els = coeffs["e0_3n"]
+    gb = L.Model(path, els, L.E0)
+    n, R = 6, 4
+    occs = [random_alloc(n, gb.species, 0.06, seed=330 + r) for r in range(R)]
+    tt = (np.array([0.0, 1e-9, 1e-6]), np.array([900.0, 600.0, 400.0]))
+    a = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 21, maximum_steps=10 ** 6)
+    b = L.KineticMcFirst(gb, None, [n] * R, occs, 500.0, 21, maximum_steps=10 ** 6)
+    a.set_schedule(*tt)
+    for _ in range(4):
+        a.run(50)
+    b.build_graph(50, 2)
+    b.set_schedule(*tt)
+    b.run_graph(50, 2)
+    b.run_graph(50, 2)
+    sa, sb = a.state(), b.state()
+    assert np.array_equal(sa["vacancy"], sb["vacancy"]) and np.array_equal(sa["time"], sb["time"])
+    assert np.array_equal(sa["temperature"], sb["temperature"])
     for x, y in zip(a.occupancy(), b.occupancy()):
         assert np.array_equal(x, y)
