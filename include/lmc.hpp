@@ -23,8 +23,11 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
+#include <iostream>
 #include <limits>
 #include <map>
 #include <memory>
@@ -37,7 +40,44 @@
 
 #include "lmc_capi.h"
 
+// Element (lmc/cfg/include/Element.hpp:7-100): the reference's element names, ordered by name
+// (std::set<Element> is string-sorted); an unknown string is the vacancy X.
+class Element {
+ public:
+  Element() = default;
+  explicit Element(const std::string &element_string)
+      : name_(Known(element_string) ? element_string : "X") {}
+  [[nodiscard]] std::string GetString() const { return name_; }
+  [[nodiscard]] double GetMass() const {
+    const std::string b = name_.size() > 1 && name_[0] == 'p' ? name_.substr(1) : name_;
+    if (b == "Al") return 26.98;
+    if (b == "Mg") return 24.31;
+    if (b == "Zn") return 65.38;
+    if (b == "Cu") return 63.55;
+    if (b == "Sn") return 118.71;
+    return 0.0;
+  }
+  bool operator==(const Element &rhs) const { return name_ == rhs.name_; }
+  bool operator!=(const Element &rhs) const { return name_ != rhs.name_; }
+  bool operator<(const Element &rhs) const { return name_ < rhs.name_; }
+
+ private:
+  static bool Known(const std::string &s) {
+    for (const char *k : {"X", "Al", "Mg", "Zn", "Cu", "Sn", "pAl", "pMg", "pZn", "pCu", "pSn"})
+      if (s == k) return true;
+    return false;
+  }
+  std::string name_ = "X";
+};
+
 namespace lmc {
+
+// Element names of a reference-style element set.
+inline std::set<std::string> ElementNames(const std::set<Element> &element_set) {
+  std::set<std::string> out;
+  for (const auto &e : element_set) out.insert(e.GetString());
+  return out;
+}
 
 inline void Check(int rc) {
   if (rc == LMC_OK) return;
@@ -149,6 +189,22 @@ class Config {
   }
   Config(Config &&) = default;
   Config &operator=(Config &&) = default;
+
+  // An empty configuration, to be assigned from ReadConfig / ReadMap (api/src/Home.cpp:163-169).
+  Config() = default;
+  // Config::ReassignLatticeVector (Config.cpp:466-552): the lattice ids of every configuration
+  // here are already the x-major ones it assigns (ReadConfig and ReadMap sort on reading), so
+  // this keeps the configuration as it is.
+  void ReassignLatticeVector() {}
+  // The same configuration with its species codes taken from sorted(element_set + "X"): the
+  // code order of the models built on that element set. Elements outside the set become X.
+  [[nodiscard]] Config WithElementSet(std::set<std::string> element_set) const {
+    Config out(*this);
+    out.SetSpecies(std::move(element_set));
+    for (size_t i = 0; i < codes_.size(); ++i) out.codes_[i] = out.CodeOf(species_.at(codes_[i]));
+    out.dirty_ = true;
+    return out;
+  }
 
   // Config::ReadConfig(filename) + ReassignLatticeVector (Config.cpp:554-661; gzip by the .gz
   // extension). The species list is sorted(element_set + "X"); an empty element_set takes the
@@ -383,7 +439,6 @@ class Config {
   }
 
  private:
-  Config() = default;
   std::vector<size_t> NeighbourAtoms(size_t atom_id, int shell) const {
     std::vector<size_t> out;
     for (size_t nb : GetNeighborLatticeIdVector(GetLatticeIdFromAtomId(atom_id), shell))
@@ -486,6 +541,8 @@ class EnergyPredictor {
  public:
   EnergyPredictor(const std::string &predictor_filename, const std::set<std::string> &element_set)
       : m_(predictor_filename, element_set, LMC_MODEL_ENERGY) {}
+  EnergyPredictor(const std::string &predictor_filename, const std::set<Element> &element_set)
+      : EnergyPredictor(predictor_filename, lmc::ElementNames(element_set)) {}
   [[nodiscard]] double GetEnergy(const cfg::Config &config) const {
     detail::One one(config.GetNumCells());
     lmc::Check(lmc_total_energy(m_.get(), config.DeviceOccupancy(), one.off.as<int64_t>(),
@@ -574,6 +631,10 @@ class VacancyMigrationPredictorE0 : public VacancyMigrationPredictor {
   VacancyMigrationPredictorE0(const std::string &predictor_filename, const cfg::Config &reference_config,
                               const std::set<std::string> &element_set)
       : VacancyMigrationPredictor(predictor_filename, reference_config, element_set, LMC_MODEL_E0) {}
+  VacancyMigrationPredictorE0(const std::string &predictor_filename, const cfg::Config &reference_config,
+                              const std::set<Element> &element_set)
+      : VacancyMigrationPredictorE0(predictor_filename, reference_config,
+                                    lmc::ElementNames(element_set)) {}
 };
 
 class VacancyMigrationPredictorQuartic : public VacancyMigrationPredictor {
@@ -583,6 +644,11 @@ class VacancyMigrationPredictorQuartic : public VacancyMigrationPredictor {
                                    const std::set<std::string> &element_set)
       : VacancyMigrationPredictor(predictor_filename, reference_config, element_set,
                                   LMC_MODEL_QUARTIC) {}
+  VacancyMigrationPredictorQuartic(const std::string &predictor_filename,
+                                   const cfg::Config &reference_config,
+                                   const std::set<Element> &element_set)
+      : VacancyMigrationPredictorQuartic(predictor_filename, reference_config,
+                                         lmc::ElementNames(element_set)) {}
 };
 
 // The ...Lru predictors (VacancyMigrationPredictorE0Lru.h, VacancyMigrationPredictorQuarticLru.h)
@@ -592,6 +658,10 @@ class VacancyMigrationPredictorE0Lru : public VacancyMigrationPredictorE0 {
   VacancyMigrationPredictorE0Lru(const std::string &predictor_filename, const cfg::Config &reference_config,
                                  const std::set<std::string> &element_set, size_t /*cache_size*/)
       : VacancyMigrationPredictorE0(predictor_filename, reference_config, element_set) {}
+  VacancyMigrationPredictorE0Lru(const std::string &predictor_filename, const cfg::Config &reference_config,
+                                 const std::set<Element> &element_set, size_t cache_size)
+      : VacancyMigrationPredictorE0Lru(predictor_filename, reference_config,
+                                       lmc::ElementNames(element_set), cache_size) {}
 };
 class VacancyMigrationPredictorQuarticLru : public VacancyMigrationPredictorQuartic {
  public:
@@ -599,6 +669,11 @@ class VacancyMigrationPredictorQuarticLru : public VacancyMigrationPredictorQuar
                                       const cfg::Config &reference_config,
                                       const std::set<std::string> &element_set, size_t /*cache_size*/)
       : VacancyMigrationPredictorQuartic(predictor_filename, reference_config, element_set) {}
+  VacancyMigrationPredictorQuarticLru(const std::string &predictor_filename,
+                                      const cfg::Config &reference_config,
+                                      const std::set<Element> &element_set, size_t cache_size)
+      : VacancyMigrationPredictorQuarticLru(predictor_filename, reference_config,
+                                            lmc::ElementNames(element_set), cache_size) {}
 };
 
 class EnergyChangePredictorPairSite {
@@ -607,6 +682,11 @@ class EnergyChangePredictorPairSite {
                                 const cfg::Config & /*reference_config*/,
                                 const std::set<std::string> &element_set)
       : m_(predictor_filename, element_set, LMC_MODEL_PAIRSITE) {}
+  EnergyChangePredictorPairSite(const std::string &predictor_filename,
+                                const cfg::Config &reference_config,
+                                const std::set<Element> &element_set)
+      : EnergyChangePredictorPairSite(predictor_filename, reference_config,
+                                      lmc::ElementNames(element_set)) {}
   [[nodiscard]] double GetDeFromLatticeIdPair(const cfg::Config &config,
                                               const std::pair<size_t, size_t> &lattice_id_jump_pair) const {
     detail::One one(config.GetNumCells());
@@ -835,6 +915,7 @@ class KineticMcFirstBatch {
     lmc::Check(lmc_kmc_occupancy(k_.get(), occ.data()));
     return occ;
   }
+  [[nodiscard]] lmc_kmc *handle() const { return k_.get(); }
 
  private:
   struct Free {
@@ -931,6 +1012,7 @@ class CanonicalMcBatch {
     lmc::Check(lmc_cmc_state(c_.get(), nullptr, nullptr, nullptr, a.data()));
     return a;
   }
+  [[nodiscard]] lmc_cmc *handle() const { return c_.get(); }
 
  private:
   struct Free {
@@ -952,6 +1034,278 @@ class SimulatedAnnealingBatch : public CanonicalMcBatch {
       : CanonicalMcBatch(configs, params, pairsite, energy) {
     SetAnnealing(true);
   }
+};
+
+// ---- Drop-in drivers: the reference's constructors and Simulate() for one configuration ----
+namespace detail {
+// TimeTemperatureInterpolator(filename) (lmc/pred/src/TimeTemperatureInterpolator.cpp:10-26):
+// the lines before the first one that starts with '0' are skipped, then "time temperature"
+// pairs are read to the end; the points are sorted by time (SortPoints, :32-43). An empty name
+// gives no schedule.
+inline void ReadTimeTemperature(const std::string &filename, std::vector<double> &time,
+                                std::vector<double> &temperature) {
+  time.clear();
+  temperature.clear();
+  if (filename.empty()) return;
+  std::ifstream ifs(filename);
+  if (!ifs.is_open()) throw std::runtime_error("Cannot open " + filename);
+  std::string line;
+  std::streampos at = ifs.tellg();
+  bool found = false;
+  while (std::getline(ifs, line)) {  // the reference loops on peek() != '0'
+    if (!line.empty() && line[0] == '0') {
+      found = true;
+      break;
+    }
+    at = ifs.tellg();
+  }
+  if (!found) throw std::runtime_error("no line starting with 0 in " + filename);
+  ifs.clear();
+  ifs.seekg(at);
+  std::vector<std::pair<double, double>> pts;
+  double t = 0, T = 0;
+  while (ifs >> t >> T) {
+    pts.emplace_back(t, T);
+    ifs.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+  }
+  std::sort(pts.begin(), pts.end());
+  for (const auto &p : pts) {
+    time.push_back(p.first);
+    temperature.push_back(p.second);
+  }
+}
+// McAbstract seeds its generator from the clock (lmc/mc/src/McAbstract.cpp:32).
+inline uint64_t ClockSeed() {
+  return static_cast<uint64_t>(std::chrono::system_clock::now().time_since_epoch().count());
+}
+// The steps at which McAbstract-derived drivers write a configuration before the step's trial:
+// d.cfg.gz where d % config_dump_steps == 0, end.cfg.gz at maximum_steps
+// (KineticMcAbstract.cpp:59-77, CanonicalMcAbstract.cpp:53-68). The first step of a restarted
+// run writes nothing (is_restarted_). Returns the first such step after `s`, or max + 1.
+inline uint64_t NextDump(uint64_t s, uint64_t config_dump_steps, uint64_t maximum_steps) {
+  uint64_t next = maximum_steps + 1;
+  if (s < maximum_steps) next = maximum_steps;
+  if (config_dump_steps) {
+    const uint64_t d = (s / config_dump_steps + 1) * config_dump_steps;
+    next = std::min(next, d);
+  }
+  return next;
+}
+inline void WriteDumps(const cfg::Config &config, uint64_t step, uint64_t config_dump_steps,
+                       uint64_t maximum_steps) {
+  if (config_dump_steps && step % config_dump_steps == 0)
+    config.WriteConfig(std::to_string(step) + ".cfg.gz");
+  if (step == maximum_steps) config.WriteConfig("end.cfg.gz");
+}
+constexpr uint64_t kDriverChunk = 100000;  // steps per launch of the drop-in drivers
+}  // namespace detail
+
+// KineticMcFirstOmp (lmc/mc/include/KineticMcFirstOmp.h:11-25): the reference's constructor and
+// Simulate() (KineticMcAbstract.cpp:140-190) on one configuration, its 12 events per step
+// evaluated on the GPU. As in the reference, the barriers come from the JSON file's Quartic
+// model (KineticMcAbstract.h:50) and the absolute energy from its EnergyPredictor
+// (McAbstract.cpp:26). Simulate() writes kmc_log.txt, d.cfg.gz every config_dump_steps steps
+// and end.cfg.gz in the working directory, and with is_early_stop escaped.cfg.gz plus the
+// "t_exit: " and "steps: " lines on std::cout. thermodynamic_averaging_steps is accepted and
+// unused, as the first-order driver never prints its average.
+class KineticMcFirstOmp {
+ public:
+  KineticMcFirstOmp(cfg::Config config, unsigned long long int log_dump_steps,
+                    unsigned long long int config_dump_steps, unsigned long long int maximum_steps,
+                    unsigned long long int thermodynamic_averaging_steps,
+                    unsigned long long int restart_steps, double restart_energy,
+                    double restart_time, double temperature, const std::set<Element> &element_set,
+                    const std::string &json_coefficients_filename,
+                    const std::string &time_temperature_filename, bool is_rate_corrector,
+                    bool is_early_stop, bool is_solute_disp)
+      : config_(config.WithElementSet(lmc::ElementNames(element_set))),
+        barrier_(json_coefficients_filename, config_, element_set, 100000),
+        energy_(json_coefficients_filename, element_set),
+        log_dump_steps_(log_dump_steps),
+        config_dump_steps_(config_dump_steps),
+        maximum_steps_(maximum_steps),
+        seed_(detail::ClockSeed()) {
+    (void)thermodynamic_averaging_steps;
+    detail::ReadTimeTemperature(time_temperature_filename, tt_time_, tt_temp_);
+    p_.temperature = temperature;
+    p_.replica_id = 0;
+    p_.restart_steps = restart_steps;
+    p_.restart_energy = restart_energy;
+    p_.restart_time = restart_time;
+    p_.maximum_steps = maximum_steps;
+    p_.rate_corrector = is_rate_corrector ? 1 : 0;
+    p_.early_stop = is_early_stop ? 1 : 0;
+    p_.solute_disp = is_solute_disp ? 1 : 0;
+  }
+  virtual ~KineticMcFirstOmp() = default;
+  KineticMcFirstOmp(KineticMcFirstOmp &&) = default;
+
+  void Simulate() {
+    p_.seed = seed_;
+    KineticMcFirstBatch b({config_}, {p_}, barrier_, &energy_);
+    b.SetFormulation(formulation_);
+    b.TrackAtoms({config_});
+    if (!tt_time_.empty()) b.SetTimeTemperature(tt_time_, tt_temp_);
+    if (log_dump_steps_) b.SetLog(log_dump_steps_, detail::kDriverChunk + 64);
+    const uint64_t s0 = p_.restart_steps, mx = maximum_steps_;
+    std::vector<cfg::Config> cfgs{config_};
+    uint64_t s = s0;
+    if (s0 == 0 && s <= mx) detail::WriteDumps(config_, s, config_dump_steps_, mx);
+    while (s <= mx) {
+      const uint64_t target = detail::NextDump(s, config_dump_steps_, mx);
+      while (s < target) {
+        b.Run(std::min<uint64_t>(target - s, detail::kDriverChunk));
+        if (log_dump_steps_) b.WriteLog({"kmc_log.txt"});
+        const auto st = b.GetState();
+        const uint64_t before = s;
+        s = st.steps[0];
+        if (Escaped(b)) {
+          b.UpdateConfigs(cfgs);
+          config_ = cfgs[0];
+          OnEscape(b, st.vacancy[0]);
+          return;
+        }
+        if (s == before) throw std::runtime_error("KMC run advanced no step");
+      }
+      if (s <= mx) {
+        b.UpdateConfigs(cfgs);
+        config_ = cfgs[0];
+        detail::WriteDumps(config_, s, config_dump_steps_, mx);
+      }
+    }
+    b.UpdateConfigs(cfgs);
+    config_ = cfgs[0];
+    state_ = b.GetState();
+  }
+  // No reference counterpart: a fixed seed for the Philox stream in place of the clock, so that
+  // a run can be repeated.
+  void SetSeed(uint64_t seed) { seed_ = seed; }
+  // The configuration after Simulate() (before it: the initial one).
+  [[nodiscard]] const cfg::Config &GetConfig() const { return config_; }
+  // steps, time, energy, absolute energy, temperature and vacancy after Simulate().
+  [[nodiscard]] const KineticMcFirstBatch::State &GetState() const { return state_; }
+
+ protected:
+  int formulation_ = 2;  // one replica: the wide shape, a CU per replica
+
+ private:
+  static bool Escaped(KineticMcFirstBatch &b) {
+    int32_t esc = 0;
+    lmc::Check(lmc_kmc_state(b.handle(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, &esc));
+    return esc != 0;
+  }
+  // IsEscaped (KineticMcAbstract.cpp:192-223): the configuration the check saw (before the
+  // step's jump) goes to escaped.cfg.gz, and to (maximum_steps + 1).cfg.gz when Dump's cadence
+  // names that step; the final configuration keeps the jump.
+  void OnEscape(KineticMcFirstBatch &b, int64_t vacancy) {
+    double t_exit = 0;
+    uint64_t step = 0;
+    int64_t before = -1;
+    lmc::Check(lmc_kmc_escape_info(b.handle(), &t_exit, &step, &before));
+    state_ = b.GetState();
+    cfg::Config at_check = config_;
+    at_check.LatticeJump({static_cast<size_t>(vacancy), static_cast<size_t>(before)});
+    at_check.WriteConfig("escaped.cfg.gz");
+    std::cout << "t_exit: " << t_exit << std::endl;
+    std::cout << "steps: " << step << std::endl;
+    if (config_dump_steps_ && (maximum_steps_ + 1) % config_dump_steps_ == 0)
+      at_check.WriteConfig(std::to_string(maximum_steps_ + 1) + ".cfg.gz");
+  }
+
+  cfg::Config config_;
+  pred::VacancyMigrationPredictorQuarticLru barrier_;
+  pred::EnergyPredictor energy_;
+  uint64_t log_dump_steps_, config_dump_steps_, maximum_steps_, seed_;
+  lmc_kmc_replica p_{};
+  std::vector<double> tt_time_, tt_temp_;
+  KineticMcFirstBatch::State state_;
+};
+
+// KineticMcChainOmpi (lmc/mc/include/KineticMcChainOmpi.h): the same constructor and outputs,
+// second-order residence-time steps (KineticMcChainOmpi.cpp:56-152).
+class KineticMcChainOmpi : public KineticMcFirstOmp {
+ public:
+  template <typename... A>
+  explicit KineticMcChainOmpi(A &&...args) : KineticMcFirstOmp(std::forward<A>(args)...) {
+    formulation_ = 3;
+  }
+};
+
+// CanonicalMcSerial (lmc/mc/include/CanonicalMcSerial.h:11-20) and Simulate()
+// (CanonicalMcSerial.cpp:40-51) on one configuration at constant temperature: cmc_log.txt with
+// the thermodynamic average over the last thermodynamic_averaging_steps energies, d.cfg.gz every
+// config_dump_steps trials and end.cfg.gz, in the working directory.
+class CanonicalMcSerial {
+ public:
+  CanonicalMcSerial(cfg::Config config, unsigned long long int log_dump_steps,
+                    unsigned long long int config_dump_steps, unsigned long long int maximum_steps,
+                    unsigned long long int thermodynamic_averaging_steps,
+                    unsigned long long int restart_steps, double restart_energy,
+                    double temperature, const std::set<Element> &element_set,
+                    const std::string &json_coefficients_filename)
+      : config_(config.WithElementSet(lmc::ElementNames(element_set))),
+        pairsite_(json_coefficients_filename, config_, element_set),
+        energy_(json_coefficients_filename, element_set),
+        log_dump_steps_(log_dump_steps),
+        config_dump_steps_(config_dump_steps),
+        maximum_steps_(maximum_steps),
+        averaging_(thermodynamic_averaging_steps),
+        seed_(detail::ClockSeed()) {
+    p_.t_start = p_.t_end = temperature;
+    p_.replica_id = 0;
+    p_.restart_steps = restart_steps;
+    p_.restart_energy = restart_energy;
+    p_.maximum_steps = maximum_steps;
+  }
+  virtual ~CanonicalMcSerial() = default;
+  CanonicalMcSerial(CanonicalMcSerial &&) = default;
+
+  void Simulate() {
+    p_.seed = seed_;
+    CanonicalMcBatch b({config_}, {p_}, pairsite_, &energy_);
+    Configure(b);
+    b.TrackAtoms({config_});
+    if (log_dump_steps_) b.SetLog(log_dump_steps_, averaging_, detail::kDriverChunk + 64);
+    const uint64_t s0 = p_.restart_steps, mx = maximum_steps_;
+    std::vector<cfg::Config> cfgs{config_};
+    uint64_t s = s0;
+    if (s0 == 0 && s <= mx) detail::WriteDumps(config_, s, config_dump_steps_, mx);
+    while (s <= mx) {
+      const uint64_t target = detail::NextDump(s, config_dump_steps_, mx);
+      while (s < target) {
+        b.Run(std::min<uint64_t>(target - s, detail::kDriverChunk));
+        if (log_dump_steps_) b.WriteLog({"cmc_log.txt"});
+        const uint64_t before = s;
+        lmc::Check(lmc_cmc_state(b.handle(), &s, nullptr, nullptr, nullptr));
+        if (s == before) throw std::runtime_error("CMC run advanced no trial");
+      }
+      if (s <= mx) {
+        b.UpdateConfigs(cfgs);
+        config_ = cfgs[0];
+        detail::WriteDumps(config_, s, config_dump_steps_, mx);
+      }
+    }
+    b.UpdateConfigs(cfgs);
+    config_ = cfgs[0];
+    energies_ = b.GetEnergy();
+  }
+  // No reference counterpart: a fixed seed for the Philox stream in place of the clock.
+  void SetSeed(uint64_t seed) { seed_ = seed; }
+  [[nodiscard]] const cfg::Config &GetConfig() const { return config_; }
+  // The energy relative to the start (restart_energy included) after Simulate().
+  [[nodiscard]] double GetEnergy() const { return energies_.empty() ? p_.restart_energy : energies_[0]; }
+
+ protected:
+  virtual void Configure(CanonicalMcBatch &) {}
+  cfg::Config config_;
+
+ private:
+  pred::EnergyChangePredictorPairSite pairsite_;
+  pred::EnergyPredictor energy_;
+  uint64_t log_dump_steps_, config_dump_steps_, maximum_steps_, averaging_, seed_;
+  lmc_cmc_replica p_{};
+  std::vector<double> energies_;
 };
 
 }  // namespace mc

@@ -289,6 +289,13 @@ int lmc_kmc_build_graph(lmc_kmc *k, uint64_t steps_per_launch, int32_t launches)
 int lmc_kmc_state(lmc_kmc *k, uint64_t *steps, double *time, double *energy, double *abs_energy,
                   double *temperature, int64_t *vacancy, int64_t *disp, double *com,
                   int32_t *escaped);
+/* The escape of an early-stop replica (KineticMcFirstAbstract::IsEscaped, KineticMcAbstract.cpp:
+ * 192-223), arrays of length R, any may be NULL: t_exit, the time the check found every 1-3NN
+ * site of the vacancy solvent (the reference prints it as "t_exit: "), the step it ran in
+ * ("steps: "), and the vacancy's site before that step's jump (-1: no escape), so that the
+ * configuration the reference writes to escaped.cfg.gz is the final one with that last jump
+ * undone (LatticeJump of the final vacancy site and site_before). Synchronous. */
+int lmc_kmc_escape_info(lmc_kmc *k, double *t_exit, uint64_t *step, int64_t *site_before);
 int lmc_kmc_occupancy(lmc_kmc *k, uint8_t *occ);
 /* Trace arrays [R][capacity]. */
 int lmc_kmc_trace(lmc_kmc *k, int64_t *sel, double *ea, double *de, double *time);

@@ -358,6 +358,17 @@ class KineticMcFirst:
                                   ptr(out["com"], capi._f64p), ptr(out["escaped"], capi._i32p)))
         return out
 
+    def escape_info(self):
+        """Per replica: the time and step of the escape check that stopped it, and the vacancy's
+        site before that step's jump (-1 where no escape happened)."""
+        R = self.R
+        out = dict(t_exit=np.zeros(R), step=np.zeros(R, np.uint64),
+                   site_before=np.zeros(R, np.int64))
+        check(lib().lmc_kmc_escape_info(self.h, ptr(out["t_exit"], capi._f64p),
+                                        ptr(out["step"], capi._u64p),
+                                        ptr(out["site_before"], capi._i64p)))
+        return out
+
     def set_atoms(self, lattice_to_atom, map_shift=None):
         """Track atom ids and map_shift through the jumps (Config::LatticeJump); per-replica
         lists as cfg.Config.atoms() returns them. None turns tracking off."""

@@ -1,32 +1,58 @@
-"""Build recipe of the engine library (hipcc, gfx950) -> latticemontecarlo_amd/liblmc.so."""
+"""Build recipe of the engine library (hipcc, gfx950) -> latticemontecarlo_amd/liblmc.so.
+
+Each source is compiled to an object of its own (in parallel, under latticemontecarlo_amd/_build),
+then the objects are linked into the shared library. Every translation unit launches only its own
+kernels, so no relocatable device code is needed.
+"""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblmc.so")
 
 SOURCES = ["lmc_tables.cpp", "lmc_model.cpp", "lmc_capi.cpp", "lmc_config_io.cpp", "lmc_jump_kernels.hip",
            "lmc_site_kernels.hip", "lmc_surrogate.hip", "lmc_surrogate_kmc.hip"]
 FLAGS = [
-    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
 
-def build(verbose=False):
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *FLAGS, "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-           *[os.path.join(CSRC, s) for s in SOURCES], "-lz", "-o", LIB + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + err)
+    return r.returncode, err
+
+
+def build(verbose=False, jobs=None):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJ, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    objs, cmds = [], []
+    for src in SOURCES:
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([hipcc, *FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or min(len(cmds), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(_run, cmds))
+    for cmd, (rc, err) in zip(cmds, results):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        if rc != 0:
+            raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + err)
+    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lz", "-o", LIB + ".tmp"]
+    if verbose:
+        print(" ".join(link), flush=True)
+    rc, err = _run(link)
+    if rc != 0:
+        raise RuntimeError("link failed:\n" + err)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -101,6 +101,7 @@ SIGNATURES = {
     "lmc_kmc_run": (C.c_int, [_vp, _u64, _vp]),
     "lmc_kmc_state": (C.c_int, [_vp, _u64p, _f64p, _f64p, _f64p, _f64p, _i64p, _i64p, _f64p,
                                 _i32p]),
+    "lmc_kmc_escape_info": (C.c_int, [_vp, _f64p, _u64p, _i64p]),
     "lmc_kmc_occupancy": (C.c_int, [_vp, _u8p]),
     "lmc_kmc_trace": (C.c_int, [_vp, _i64p, _f64p, _f64p, _f64p]),
     "lmc_kmc_device_views": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp),

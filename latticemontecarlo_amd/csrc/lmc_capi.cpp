@@ -608,6 +608,9 @@ struct lmc_kmc {
     DevFree(d.flags);
     DevFree(d.solvent);
     DevFree(d.escaped);
+    DevFree(d.esc_time);
+    DevFree(d.esc_step);
+    DevFree(d.esc_site);
     DevFree(d.disp);
     DevFree(d.com);
     DevFree(d.error);
@@ -1205,6 +1208,9 @@ int lmc_kmc_create(const lmc_model *barrier, const lmc_model *energy, int32_t R,
     d.flags = DevCopy(flags);
     d.solvent = DevCopy(solvent);
     d.escaped = DevCopy(std::vector<int32_t>(R, 0));
+    d.esc_time = DevCopy(std::vector<double>(R, 0.0));
+    d.esc_step = DevCopy(std::vector<uint64_t>(R, 0));
+    d.esc_site = DevCopy(std::vector<int64_t>(R, -1));
     d.disp = DevCopy(std::vector<int64_t>(3 * R, 0));
     d.com = DevCopy(std::vector<double>(3 * R, 0.0));
     d.error = DevCopy(std::vector<int32_t>(1, 0));
@@ -1498,6 +1504,18 @@ int lmc_kmc_state(lmc_kmc *k, uint64_t *steps, double *time, double *energy, dou
     ToHost(com, k->d.com, 3 * R);
     ToHost(escaped, k->d.escaped, R);
     return KmcErrors(k);
+  });
+}
+
+int lmc_kmc_escape_info(lmc_kmc *k, double *t_exit, uint64_t *step, int64_t *site_before) {
+  return Guard([&] {
+    if (!k) throw std::invalid_argument("null driver");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t R = static_cast<size_t>(k->d.R);
+    ToHost(t_exit, k->d.esc_time, R);
+    ToHost(step, k->d.esc_step, R);
+    ToHost(site_before, k->d.esc_site, R);
+    return LMC_OK;
   });
 }
 

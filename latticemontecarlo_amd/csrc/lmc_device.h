@@ -104,6 +104,11 @@ struct KmcDev {
   int32_t *flags = nullptr;     // bit0 rate corrector, bit1 early stop, bit2 solute disp
   int32_t *solvent = nullptr;   // species code of the solvent
   int32_t *escaped = nullptr;
+  // [R] the escape check's moment (KineticMcAbstract.cpp:210-218): the time and step it found
+  // the vacancy's 1-3NN shells all solvent, and the vacancy's site before that step's jump
+  double *esc_time = nullptr;
+  uint64_t *esc_step = nullptr;
+  int64_t *esc_site = nullptr;
   int64_t *disp = nullptr;      // [R][3] vacancy displacement, a/2 units
   double *com = nullptr;        // [R][3] solute centre-of-mass trajectory
   int32_t *error = nullptr;     // [1] first error code

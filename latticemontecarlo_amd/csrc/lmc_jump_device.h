@@ -694,6 +694,11 @@ __device__ void ApplyJump(const KTables &t, const KmcDev &k, ReplicaLds &Q, uint
                           uint64_t trace_index, int64_t jump, int dx, int dy, int dz, int mig,
                           double ea, double de, double dt, bool escaped) {
   if (escaped) {
+    if (k.esc_time) {
+      k.esc_time[r] = Q.time;
+      k.esc_step[r] = Q.steps;
+      k.esc_site[r] = Q.vac;
+    }
     Q.esc = 1;
     Q.steps = Q.max_steps + 1;
   }

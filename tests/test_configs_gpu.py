@@ -244,9 +244,10 @@ def test_early_stop_matches_oracle(torch, coeffs, form):
     n, R, steps, maxs = 6, 5, 400, 10 ** 6
     occs = [_escape_start(n, species, 40 + r) for r in range(R)]
     print(f"early stop form {form}", flush=True)
-    gst, gtr, gocc, refs = _run_kmc_pair(coeffs, "e0_3n", 1, n, R, steps,
-                                         np.linspace(450.0, 650.0, R), form=form,
-                                         maximum_steps=maxs, occs=occs, early_stop=True)
+    gst, gtr, gocc, refs, esc = _run_kmc_pair(coeffs, "e0_3n", 1, n, R, steps,
+                                              np.linspace(450.0, 650.0, R), form=form,
+                                              maximum_steps=maxs, occs=occs, early_stop=True,
+                                              escape_info=True)
     escaped = 0
     for r, (st, tr, occ) in enumerate(refs):
         assert int(gst["escaped"][r]) == int(st["escaped"]), f"replica {r}"
@@ -265,4 +266,12 @@ def test_early_stop_matches_oracle(torch, coeffs, form):
         assert int(gst["vacancy"][r]) == st["vacancy"]
         np.testing.assert_allclose(gst["time"][r], st["time"], rtol=1e-12)
         assert abs(gst["energy"][r] - st["energy"]) <= TOL_E * steps
+        if st["escaped"]:  # the escape check's moment: before the last step's jump
+            v0 = int(np.flatnonzero(occs[r] == species.index("X"))[0])
+            assert int(esc["step"][r]) == ran - 1
+            assert int(esc["site_before"][r]) == (int(tr["sel"][ran - 2]) if ran > 1 else v0)
+            t_before = tr["time"][ran - 2] if ran > 1 else 0.0
+            np.testing.assert_allclose(esc["t_exit"][r], t_before, rtol=1e-12)
+        else:
+            assert int(esc["site_before"][r]) == -1
     assert escaped >= 3, "the start configurations are built to escape within the run"

@@ -1,0 +1,226 @@
+"""The drop-in drivers of the C++ mirror: mc::KineticMcFirstOmp, mc::KineticMcChainOmpi and
+mc::CanonicalMcSerial built with the reference's constructors by a host program written like
+lmc/api/src/Home.cpp:159-189, 240-292 (tests/cpp/home_drivers.cpp), run in a scratch directory,
+and their files checked against the oracle's trajectory of the same configuration:
+
+* d.cfg.gz every config_dump_steps and end.cfg.gz hold the configuration at the start of step d
+  (KineticMcAbstract.cpp:59-77, CanonicalMcAbstract.cpp:53-68), none at a restart's first step;
+* kmc_log.txt / cmc_log.txt take a line at the reference's cadence;
+* a time-temperature file (TimeTemperatureInterpolator.cpp:10-26) drives T(t);
+* an early stop writes escaped.cfg.gz with the configuration the escape check saw, and prints
+  "t_exit: " and "steps: " (KineticMcAbstract.cpp:192-223).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, TERNARY, random_alloc
+
+LIBDIR = os.path.join(ROOT, "latticemontecarlo_amd")
+SPECIES = sorted(TERNARY + ["X"])
+
+
+@pytest.fixture(scope="module")
+def exe(tmp_path_factory):
+    out = tmp_path_factory.mktemp("cpp") / "home_drivers"
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "home_drivers.cpp"), "-L", LIBDIR, "-llmc",
+           f"-Wl,-rpath,{LIBDIR}", "-o", str(out)]
+    subprocess.run(cmd, check=True)
+    return str(out)
+
+
+def test_home_program_compiles_and_checks_arguments(exe):
+    """The Home.cpp-style program builds against lmc.hpp (fixture) and rejects bad arguments."""
+    r = subprocess.run([exe, "nothing"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+
+
+def test_missing_time_temperature_file_is_the_reference_error(exe, coeffs, tmp_path):
+    from latticemontecarlo_amd import cfg
+    path, _ = coeffs["q_3n"]
+    c = cfg.generate_fcc(4, "Al")
+    occ = c.codes(SPECIES)
+    occ[0] = SPECIES.index("X")
+    c.update(SPECIES, occ)
+    c.write(str(tmp_path / "start.cfg"))
+    r = subprocess.run([exe, "kmc", "start.cfg", path, "1", "10", "5", "0", "0", "0", "500",
+                        "missing.txt", "0", "0", "0", "1", "x"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Cannot open missing.txt" in r.stderr
+
+
+def _write_start(tmp_path, n, occ):
+    from latticemontecarlo_amd import cfg
+    c = cfg.generate_fcc(n, "Al")
+    c.update(SPECIES, occ)
+    c.write(str(tmp_path / "start.cfg.gz"))
+
+
+def _read_codes(path):
+    from latticemontecarlo_amd import cfg
+    return cfg.read_config(str(path)).codes(SPECIES)
+
+
+def _due(steps, L):
+    import log_oracle
+    return log_oracle.reference_due(steps, L)
+
+
+def _kmc_snapshots(path, kind, occ, n, T, seed, steps, tt=None, **opts):
+    """The oracle's states at the start of every step 0..steps (occupancy, time, energy)."""
+    import oracle as O
+    lat = O.Lattice(n)
+    ob, oe = O.Model(lat, path, TERNARY, kind), O.Model(lat, path, TERNARY, O.ENERGY)
+    k = O.Kmc(ob, oe, lat, occ, T, seed, replica=0, tt=tt, **opts)
+    snaps = []
+    for _ in range(steps + 1):
+        st = k.state()
+        snaps.append((k.occupancy(), st))
+        k.run(1)
+    return snaps, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["kmc", "chain"])
+def test_kinetic_driver_files_match_oracle(exe, coeffs, tmp_path, mode):
+    import oracle as O
+    path, _ = coeffs["q_3n"]
+    n, T, seed, mx, cds, L = 6, 600.0, 99, 150, 40, 7
+    occ = random_alloc(n, SPECIES, 0.08, seed=21)
+    _write_start(tmp_path, n, occ)
+    print(f"drop-in {mode} n={n} max={mx}", flush=True)
+    r = subprocess.run([exe, mode, "start.cfg.gz", path, str(L), str(cds), str(mx), "0", "0", "0",
+                        str(T), "-", "0", "0", "1", str(seed), "x"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    snaps, k = _kmc_snapshots(path, O.QUARTIC, occ, n, T, seed, mx, solute_disp=True,
+                              chain=mode == "chain")
+    for d in range(0, mx + 1, cds):
+        assert np.array_equal(_read_codes(tmp_path / f"{d}.cfg.gz"), snaps[d][0]), d
+    assert not (tmp_path / f"{mx + 1}.cfg.gz").exists()
+    assert np.array_equal(_read_codes(tmp_path / "end.cfg.gz"), snaps[mx][0])
+    # the final state: maximum_steps + 1 steps (steps_ <= maximum_steps)
+    k.run(0)
+    fin = dict(x.split("=") for x in r.stdout.split()[1:])
+    st = k.state()
+    assert int(fin["steps"]) == st["steps"] == mx + 1
+    assert int(fin["vacancy"]) == st["vacancy"]
+    assert float(fin["time"]) == pytest.approx(st["time"], rel=1e-12)
+    assert abs(float(fin["energy"]) - st["energy"]) < 1e-9 * mx
+    # kmc_log.txt: header, then one line per due step with that step's time and energy
+    lines = (tmp_path / "kmc_log.txt").read_text().splitlines()
+    assert lines[0].split("\t")[:7] == ["steps", "time", "temperature", "energy", "Ea", "dE",
+                                        "selected"]
+    assert lines[0].endswith("solute_com3")
+    rows = [l.split("\t") for l in lines[1:]]
+    assert [int(x[0]) for x in rows] == [s for s in range(mx + 1) if _due(s, L)]
+    for x in rows:
+        s = int(x[0])
+        assert float(x[1]) == pytest.approx(snaps[s][1]["time"], rel=1e-12, abs=0)
+        assert abs(float(x[3]) - snaps[s][1]["energy"]) < 1e-9 * (s + 1)
+
+
+@pytest.mark.gpu
+def test_kinetic_driver_restart_and_schedule(exe, coeffs, tmp_path):
+    """restart_steps = 100: no dump or log line at step 100, the log appended; a time-temperature
+    file whose first lines are a header."""
+    import oracle as O
+    path, _ = coeffs["q_3n"]
+    n, T, seed, s0, mx, cds = 5, 500.0, 7, 100, 220, 50
+    occ = random_alloc(n, SPECIES, 0.1, seed=5)
+    _write_start(tmp_path, n, occ)
+    (tmp_path / "tt.txt").write_text("time temperature\n# ramp\n0 900\n1e-7 450\n5e-7 400\n")
+    (tmp_path / "kmc_log.txt").write_text("earlier run\n")
+    print("drop-in kmc restart + schedule", flush=True)
+    r = subprocess.run([exe, "kmc", "start.cfg.gz", path, "10", str(cds), str(mx), str(s0), "0.25",
+                        "1e-8", str(T), "tt.txt", "1", "0", "0", str(seed), "x"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lat = O.Lattice(n)
+    ob, oe = O.Model(lat, path, TERNARY, O.QUARTIC), O.Model(lat, path, TERNARY, O.ENERGY)
+    k = O.Kmc(ob, oe, lat, occ, T, seed, replica=0, restart_steps=s0, restart_energy=0.25,
+              restart_time=1e-8, rate_corrector=True,
+              tt=(np.array([0.0, 1e-7, 5e-7]), np.array([900.0, 450.0, 400.0])))
+    assert not (tmp_path / "100.cfg.gz").exists()
+    k.run(150 - s0)
+    assert np.array_equal(_read_codes(tmp_path / "150.cfg.gz"), k.occupancy())
+    k.run(200 - 150)
+    assert np.array_equal(_read_codes(tmp_path / "200.cfg.gz"), k.occupancy())
+    k.run(mx - 200)
+    assert np.array_equal(_read_codes(tmp_path / "end.cfg.gz"), k.occupancy())
+    k.run(1)
+    st = k.state()
+    fin = dict(x.split("=") for x in r.stdout.split()[1:])
+    assert int(fin["steps"]) == st["steps"] == mx + 1
+    assert float(fin["time"]) == pytest.approx(st["time"], rel=1e-12)
+    assert abs(float(fin["energy"]) - st["energy"]) < 1e-9 * mx
+    lines = (tmp_path / "kmc_log.txt").read_text().splitlines()
+    assert lines[0] == "earlier run" and not lines[1].startswith("steps")
+    assert int(lines[1].split("\t")[0]) == 110  # step 100 skipped, then every 10
+
+
+@pytest.mark.gpu
+def test_kinetic_driver_escape(exe, coeffs, tmp_path):
+    import oracle as O
+    path, _ = coeffs["q_3n"]
+    n, T, seed, mx = 6, 550.0, 3, 10 ** 6
+    occ = random_alloc(n, SPECIES, 0.02, seed=8)
+    x = SPECIES.index("X")
+    v = int(np.flatnonzero(occ == x)[0])
+    lat = O.Lattice(n)
+    for s in lat.neighbors(v, 1)[:3]:
+        occ[s] = SPECIES.index("Mg")
+    _write_start(tmp_path, n, occ)
+    print("drop-in kmc escape", flush=True)
+    r = subprocess.run([exe, "kmc", "start.cfg.gz", path, "1000", "1000000", str(mx), "0", "0", "0",
+                        str(T), "-", "0", "1", "0", str(seed), "x"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ob, oe = O.Model(lat, path, TERNARY, O.QUARTIC), O.Model(lat, path, TERNARY, O.ENERGY)
+    k = O.Kmc(ob, oe, lat, occ, T, seed, replica=0, early_stop=True)
+    before = None
+    for i in range(5000):
+        prev_occ, prev_t = k.occupancy(), k.state()["time"]
+        k.run(1, maximum_steps=mx)
+        if k.state()["escaped"]:
+            before = (i, prev_occ, prev_t)
+            break
+    assert before is not None, "the start configuration escapes within the run"
+    i, prev_occ, prev_t = before
+    assert np.array_equal(_read_codes(tmp_path / "escaped.cfg.gz"), prev_occ)
+    out = dict(l.split(": ") for l in r.stdout.splitlines() if ": " in l)
+    assert int(out["steps"]) == i
+    assert float(out["t_exit"]) == pytest.approx(prev_t, rel=1e-5)  # printed with precision 6
+    fin = dict(x.split("=") for x in r.stdout.splitlines()[-1].split()[1:])
+    assert int(fin["steps"]) == mx + 2 and int(fin["vacancy"]) == k.state()["vacancy"]
+
+
+@pytest.mark.gpu
+def test_canonical_driver_files_match_oracle(exe, coeffs, tmp_path):
+    import oracle as O
+    path, _ = coeffs["e0_3n"]
+    n, T, seed, mx, cds = 5, 700.0, 5, 1000, 300
+    occ = random_alloc(n, SPECIES, 0.08, seed=12, vacancy=False)
+    _write_start(tmp_path, n, occ)
+    print("drop-in cmc", flush=True)
+    r = subprocess.run([exe, "cmc", "start.cfg.gz", path, "10", str(cds), str(mx), "0", "0", "0",
+                        str(T), str(seed)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lat = O.Lattice(n)
+    op, oe = O.Model(lat, path, TERNARY, O.PAIRSITE), O.Model(lat, path, TERNARY, O.ENERGY)
+    c = O.Cmc(op, oe, lat, occ, T, T, seed, replica=0)
+    done = 0
+    for d in list(range(0, mx + 1, cds)) + [mx]:
+        c.run(d - done, mx)
+        done = d
+        assert np.array_equal(_read_codes(tmp_path / ("end.cfg.gz" if d == mx else f"{d}.cfg.gz")),
+                              c.occupancy()), d
+    c.run(1, mx)
+    fin = float(r.stdout.split("=")[1])
+    assert abs(fin - c.state()["energy"]) < 1e-9 * mx
+    lines = (tmp_path / "cmc_log.txt").read_text().splitlines()
+    assert lines[0] == "steps\ttemperature\tenergy\taverage_energy\tabsolute_energy"
+    assert [int(l.split("\t")[0]) for l in lines[1:]] == [s for s in range(mx + 1) if _due(s, 10)]

@@ -111,7 +111,7 @@ def test_total_energy_matches_oracle(torch, coeffs, name):
 
 
 def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, maximum_steps=10 ** 9,
-                  occs=None, **opts):
+                  occs=None, escape_info=False, **opts):
     """The GPU driver and the oracle's KMC on the same replicas. `n` is one box size for all
     replicas or a list of R sizes (an uneven-size batch); `occs` overrides the random alloys."""
     import latticemontecarlo_amd as L
@@ -136,6 +136,7 @@ def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, maximum_steps=
     gst = drv.state()
     gtr = drv.trace()
     gocc = drv.occupancy()
+    esc = drv.escape_info() if escape_info else None
     models = {}
     refs = []
     for r in range(R):
@@ -148,6 +149,8 @@ def _run_kmc_pair(coeffs, name, kind, n, R, steps, temps, form=1, maximum_steps=
                   tt=tt, chain=form == 3, **opts)
         tr = k.run(steps, maximum_steps=maximum_steps, trace=True)
         refs.append((k.state(), tr, k.occupancy()))
+    if escape_info:
+        return gst, gtr, gocc, refs, esc
     return gst, gtr, gocc, refs
 
 
