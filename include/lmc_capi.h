@@ -208,6 +208,19 @@ int lmc_site_change_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *
  * (lmc/pred/src/EnergyPredictor.cpp:173-177). */
 int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
                      const int32_t *d_ncell, int32_t n_replicas, double *d_energy, void *stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): lmc_total_energy
+ * with the same kernels and results, its cluster-count scratch a fixed per-device array of
+ * 2^19 counts (4 MiB, held until the process ends) instead of a hipMalloc'd buffer. It allocates
+ * nothing, does not synchronise, and rejects with LMC_ERR_ARG
+ * - n_replicas * n_eci above 2^19 (n_eci = 95 for Al/Mg/Zn: at most 5518 replicas; 167 for
+ *   Al/Cu/Mg/Zn: 3139), and
+ * - a model whose tables are not yet on the current device (any other compute entry point with
+ *   that model uploads them).
+ * Every call on one device shares that device's array: calls that overlap on one device race. */
+int lmc_total_energy_noalloc(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                             const int32_t *d_ncell, int32_t n_replicas, double *d_energy,
+                             void *stream);
 /* EnergyPredictor::GetEncode (lmc/pred/src/EnergyPredictor.cpp:40-96): per replica the ordered
  * cluster counts divided by the total-energy normalisers, d_encode[R][n_eci] (n_eci from
  * lmc_model_layout, in ECI key order); theta . encode is lmc_total_energy's value. */

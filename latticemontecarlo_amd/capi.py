@@ -89,6 +89,7 @@ SIGNATURES = {
     "lmc_swap_de": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lmc_site_de": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lmc_total_energy": (C.c_int, [_vp, _vp, _vp, _vp, _i32, _vp, _vp]),
+    "lmc_total_energy_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _i32, _vp, _vp]),
     "lmc_check_errors": (C.c_int, [_vp]),
     "lmc_kmc_create": (C.c_int, [_vp, _vp, _i32, _i32p, _u8p, C.POINTER(KmcReplica),
                                  C.POINTER(_vp)]),

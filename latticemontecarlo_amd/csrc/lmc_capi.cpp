@@ -1023,6 +1023,38 @@ int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_
   });
 }
 
+int lmc_total_energy_noalloc(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
+                             const int32_t *d_ncell, int32_t n_replicas, double *d_energy,
+                             void *stream) {
+  return Guard([&] {
+    RequireModel(m);
+    if (n_replicas < 0 || (n_replicas && (!d_occ || !d_offset || !d_ncell || !d_energy)))
+      throw std::invalid_argument("lmc_total_energy_noalloc: bad arguments");
+    // the model's tables must already be on this device: uploading them would allocate
+    if (!m->m->dev || m->m->dev->device != CurrentDevice())
+      throw std::invalid_argument(
+          "lmc_total_energy_noalloc: the model's tables are not on this device; run any compute "
+          "entry point with this model on it first");
+    const lmc::KTables &t = m->m->dev->k;
+    const size_t need = static_cast<size_t>(n_replicas) * static_cast<size_t>(t.n_eci);
+    const size_t cap = lmc::TotalEnergyScratchCapacity();
+    if (need > cap)
+      throw std::invalid_argument("lmc_total_energy_noalloc: " + std::to_string(n_replicas) +
+                                  " replicas need " + std::to_string(need) +
+                                  " cluster counts, more than the scratch's " +
+                                  std::to_string(cap) + " (at most " +
+                                  std::to_string(cap / static_cast<size_t>(t.n_eci)) +
+                                  " replicas for this model)");
+    if (n_replicas == 0) return LMC_OK;
+    unsigned long long *counts = nullptr;
+    Check(lmc::TotalEnergyScratch(&counts), "scratch address");
+    Check(lmc::LaunchTotalEnergy(t, d_occ, d_offset, d_ncell, n_replicas, counts, d_energy,
+                                 S(stream)),
+          "total energy kernel");
+    return LMC_OK;
+  });
+}
+
 int lmc_total_encode(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offset,
                      const int32_t *d_ncell, int32_t n_replicas, double *d_encode, void *stream) {
   return Guard([&] {

@@ -221,5 +221,9 @@ hipError_t LaunchClusterEncode(const KTables &t, const uint8_t *occ, int32_t n, 
 hipError_t LaunchTotalEnergy(const KTables &t, const uint8_t *occ, const int64_t *off,
                              const int32_t *ncell, int32_t R, unsigned long long *counts,
                              double *energy, hipStream_t s);
+// The per-device address of the fixed cluster-count scratch array of lmc_total_energy_noalloc
+// (looked up once per device) and its capacity in counts.
+hipError_t TotalEnergyScratch(unsigned long long **ptr);
+size_t TotalEnergyScratchCapacity();
 
 }  // namespace lmc

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "lmc_dev_common.h"
@@ -941,6 +942,34 @@ hipError_t LaunchClusterEncode(const KTables &t, const uint8_t *occ, int32_t n, 
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(EncodeKernel, dim3(1), dim3(64), 0, s, t, counts, 1, encode, energy);
   return hipGetLastError();
+}
+
+// The cluster-count scratch of lmc_total_energy_noalloc: one copy per device, held until the
+// process ends. Host code never passes `scratch` itself (its host-side name is not a device
+// address); it passes the address TotalEnergyScratch looked up on the current device.
+constexpr size_t kScratchCounts = size_t{1} << 19;  // 4 MiB of 64-bit counts
+static __device__ __align__(256) unsigned long long scratch[kScratchCounts];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_ptr[kMaxScratchDevices] = {};
+
+size_t TotalEnergyScratchCapacity() { return kScratchCounts; }
+
+hipError_t TotalEnergyScratch(unsigned long long **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {  // once per device, with that device current
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<unsigned long long *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return hipSuccess;
 }
 
 hipError_t LaunchTotalEnergy(const KTables &t, const uint8_t *occ, const int64_t *off,
