@@ -378,6 +378,20 @@ int lmc_cmc_annealing_state(lmc_cmc *c, double *temperature, double *lowest, uin
  * kernel's tables (about 150 000 sites in a 160 KB workgroup); 0 keeps it in HBM and leaves the
  * CU's LDS to kernels that run beside it. Results are the same either way. */
 int lmc_cmc_set_lds_occupancy(lmc_cmc *c, int32_t enable);
+/* CanonicalMcOmp (lmc/mc/include/CanonicalMcOmp.h:9-29, CanonicalMcOmp.cpp:40-92) in place of
+ * the serial trials: rounds of up to trials_per_round (1..64; the reference's OpenMP thread
+ * count) swap pairs, each event's pair the first draw with neither site on or within 3NN of an
+ * earlier event's sites, the 50th draw of an event ending the round; every dE on the round's
+ * starting occupancy, then the Metropolis steps in trial order. A new round starts only while
+ * steps <= maximum_steps; a started round runs to its end, across runs. The attempts of the
+ * round starting at step s0 are numbered through the round: attempt a is Philox counter (s0,
+ * replica_id, 2a+1); the Metropolis uniform of step s stays (s, replica_id, 0xFFFFFFFE), so one
+ * trial per round gives the serial trajectory. 0 restores the serial trials. No annealing with
+ * rounds. Synchronous. */
+int lmc_cmc_set_parallel_trials(lmc_cmc *c, int32_t trials_per_round);
+/* Per replica, the trials of the current CanonicalMcOmp round not yet run (0 for serial trials);
+ * left[R]. Synchronous. */
+int lmc_cmc_round_left(lmc_cmc *c, int32_t *left);
 int lmc_cmc_run(lmc_cmc *c, uint64_t trials, void *stream);
 int lmc_cmc_state(lmc_cmc *c, uint64_t *steps, double *energy, double *abs_energy,
                   uint64_t *accepted);

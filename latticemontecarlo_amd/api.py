@@ -501,6 +501,18 @@ class CanonicalMc:
         HBM (leaves the CU's LDS to kernels running beside it). Results are identical."""
         check(lib().lmc_cmc_set_lds_occupancy(self.h, 1 if enable else 0))
 
+    def set_parallel_trials(self, trials_per_round):
+        """CanonicalMcOmp (CanonicalMcOmp.cpp:40-92): rounds of up to trials_per_round (1..64)
+        swap pairs under the 3NN exclusion set, their dE on the round's starting occupancy, then
+        the Metropolis steps in order; 0 restores the serial trials."""
+        check(lib().lmc_cmc_set_parallel_trials(self.h, int(trials_per_round)))
+
+    def round_left(self):
+        """Per replica, the trials of the current CanonicalMcOmp round not yet run."""
+        out = np.zeros(self.R, np.int32)
+        check(lib().lmc_cmc_round_left(self.h, ptr(out, capi._i32p)))
+        return out
+
     def run(self, trials, stream=None):
         s = capi.stream_ptr(stream) if stream is not None else None
         check(lib().lmc_cmc_run(self.h, trials, s))

@@ -111,6 +111,8 @@ SIGNATURES = {
                                  C.POINTER(_vp)]),
     "lmc_cmc_free": (None, [_vp]),
     "lmc_cmc_set_trace": (C.c_int, [_vp, _u64]),
+    "lmc_cmc_set_parallel_trials": (C.c_int, [_vp, _i32]),
+    "lmc_cmc_round_left": (C.c_int, [_vp, _i32p]),
     "lmc_cmc_run": (C.c_int, [_vp, _u64, _vp]),
     "lmc_cmc_set_annealing": (C.c_int, [_vp, _i32]),
     "lmc_kmc_set_atoms": (C.c_int, [_vp, _i64p, _i32p]),

@@ -645,6 +645,11 @@ struct lmc_cmc {
   }
   ~lmc_cmc() {
     FreeLog();
+    DevFree(d.omp_p);
+    DevFree(d.omp_q);
+    DevFree(d.omp_de);
+    DevFree(d.omp_len);
+    DevFree(d.omp_pos);
     DevFree(d.anneal);
     DevFree(d.atom);
     DevFree(d.shift);
@@ -1837,9 +1842,53 @@ int lmc_cmc_set_lds_occupancy(lmc_cmc *c, int32_t enable) {
   });
 }
 
+int lmc_cmc_set_parallel_trials(lmc_cmc *c, int32_t trials_per_round) {
+  return Guard([&] {
+    if (!c) throw std::invalid_argument("null driver");
+    if (trials_per_round < 0 || trials_per_round > 64)
+      throw std::invalid_argument("trials per round must be in [0, 64]");
+    if (trials_per_round && c->d.anneal)
+      throw std::invalid_argument("CanonicalMcOmp rounds take no annealing schedule");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    DevFree(c->d.omp_p);
+    DevFree(c->d.omp_q);
+    DevFree(c->d.omp_de);
+    DevFree(c->d.omp_len);
+    DevFree(c->d.omp_pos);
+    c->d.omp_k = trials_per_round;
+    if (!trials_per_round) return LMC_OK;
+    const size_t R = static_cast<size_t>(c->d.R), n = R * static_cast<size_t>(trials_per_round);
+    c->d.omp_p = DevAlloc<int64_t>(n);
+    c->d.omp_q = DevAlloc<int64_t>(n);
+    c->d.omp_de = DevAlloc<double>(n);
+    c->d.omp_len = DevCopy(std::vector<int32_t>(R, 0));
+    c->d.omp_pos = DevCopy(std::vector<int32_t>(R, 0));
+    return LMC_OK;
+  });
+}
+
+int lmc_cmc_round_left(lmc_cmc *c, int32_t *left) {
+  return Guard([&] {
+    if (!c || !left) throw std::invalid_argument("null argument");
+    Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    const size_t R = static_cast<size_t>(c->d.R);
+    if (!c->d.omp_k) {
+      std::fill(left, left + R, 0);
+      return LMC_OK;
+    }
+    std::vector<int32_t> len(R), pos(R);
+    ToHost(len.data(), c->d.omp_len, R);
+    ToHost(pos.data(), c->d.omp_pos, R);
+    for (size_t r = 0; r < R; ++r) left[r] = len[r] - pos[r];
+    return LMC_OK;
+  });
+}
+
 int lmc_cmc_set_annealing(lmc_cmc *c, int32_t enable) {
   return Guard([&] {
     if (!c) throw std::invalid_argument("null driver");
+    if (enable && c->d.omp_k)
+      throw std::invalid_argument("CanonicalMcOmp rounds take no annealing schedule");
     Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     DevFree(c->d.anneal);
     c->d.anneal = nullptr;

@@ -177,6 +177,12 @@ struct CmcDev {
   uint64_t avg_m = 0, ring_cap = 0;
   double *ring = nullptr;
   uint64_t *ring_n = nullptr;
+  // CanonicalMcOmp rounds (omp_k > 0 trials per round; 0: serial trials): per replica the
+  // round's pairs and dE [R][omp_k], its length and the next event to try
+  int32_t omp_k = 0;
+  int64_t *omp_p = nullptr, *omp_q = nullptr;
+  double *omp_de = nullptr;
+  int32_t *omp_len = nullptr, *omp_pos = nullptr;
 };
 
 // env_out (optional): [12 R][64] environment rows of the events in rank order; ekra_out

@@ -636,6 +636,355 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// CanonicalMcOmp (lmc/mc/src/CanonicalMcOmp.cpp:40-92): rounds of up to K swap trials. A round
+// draws its pairs through one attempt stream (attempt a of the round starting at step s0 is
+// Philox counter (s0, replica, 2a + 1), redrawn while the species agree); each event takes the
+// first pair with neither site within 3NN of (or on) an earlier event's sites, and the 50th
+// draw of an event ends the round. All dE are taken on the round's starting occupancy (the
+// exclusion keeps them exact), the Metropolis tests of a round are independent of each other
+// (uniform of step s: counter (s, replica, 0xFFFFFFFE)), and the accepted swaps touch disjoint
+// sites. A new round starts only while steps <= maximum_steps; a started round runs to its end,
+// across launches if need be (the round is kept in c.omp_*).
+//
+// One 256-thread workgroup per replica. Per round:
+//   draw   : passes of 256 attempts, one per thread; the species check and the check against the
+//            round's earlier events in parallel; the (at most 64) valid attempts of a pass ranked
+//            by ballots, their pairwise conflicts as bit masks, then one lane walks them in order
+//            (the reference's do/while with its draw count) -- a few passes per round;
+//   dE     : both 43-site environments of every event gathered at once; every thread holds one
+//            site cluster and adds its change for every event (DPP wave sums, waves in order);
+//   trials : event lanes test their Metropolis steps in parallel and apply the accepted swaps;
+//            one lane adds the energies in trial order (and takes the log records).
+constexpr int kOmpMaxK = 64;
+constexpr int kOmpPass = kThreads;  // attempts per draw pass
+constexpr int kOmpMaxValid = 64;    // valid attempts resolved per pass
+
+struct OmpLds {
+  uint8_t env[2 * kOmpMaxK][48];
+  int slot[2 * kOmpMaxK];
+  XYZ ch[2 * kOmpMaxK];  // sites of the round's events so far
+  int64_t p[kOmpMaxK], q[kOmpMaxK];
+  XYZ xp[kOmpMaxK], xq[kOmpMaxK];
+  int ep[kOmpMaxK], eq[kOmpMaxK];
+  double de[kOmpMaxK], part[kThreads / 64][kOmpMaxK];
+  int8_t acc[kOmpMaxK];
+  // one draw pass: its valid attempts in attempt order
+  int64_t va[kOmpMaxValid], vb[kOmpMaxValid];
+  XYZ vxa[kOmpMaxValid], vxb[kOmpMaxValid];
+  uint64_t conf[kOmpMaxValid];  // bit j: conflicts with valid attempt j of the pass
+  int vidx[kOmpMaxValid], excl[kOmpMaxValid];
+  int wave_valid[kThreads / 64];
+  // round control
+  int len, pos, nch, ct, finished, error;
+  uint32_t att0;
+};
+
+// Whether site b lies on site a or in its 1st-3rd neighbour shells (squared minimum-image
+// distance at most 6 in (a/2)^2; FCC distances are even).
+__device__ __forceinline__ bool Near(const XYZ &a, const XYZ &b, int n) {
+  const int dx = MinImage(b.X - a.X, n), dy = MinImage(b.Y - a.Y, n), dz = MinImage(b.Z - a.Z, n);
+  return dx * dx + dy * dy + dz * dz <= 6;
+}
+
+// w(x -> en) - w(x -> eo) of cluster q in environment e, 0 when it contains slot skip.
+__device__ __forceinline__ double ClusterDeltaAt(const SiteLds &L, const SiteCluster &q,
+                                                 const uint8_t *e, int eo, int en, int skip) {
+  if (!q.live || (skip >= 0 && (q.r1 == skip || q.r2 == skip))) return 0.0;
+  const int rest = q.base + e[q.o1] * q.k1 + e[q.o2] * q.k2;
+  return L.w[rest + en * q.kx] - L.w[rest + eo * q.kx];
+}
+
+// The round starting at step s0: pairs, environments, partner slots and dE into O (all threads).
+__device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, OmpLds &O,
+                              const SiteCluster &cl, int r, uint64_t s0, const uint8_t *occ,
+                              uint64_t N, int n, int K) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    O.len = 0;
+    O.nch = 0;
+    O.ct = 0;
+    O.finished = 0;
+    O.att0 = 0;
+  }
+  __syncthreads();
+  while (!O.finished) {
+    const uint32_t att = O.att0 + static_cast<uint32_t>(tid);
+    const PairDraw d = DrawAttempt(c, r, s0, att, N, n);
+    const bool valid = occ[d.a] != occ[d.b];
+    bool ex = false;
+    if (valid)
+      for (int j = 0; j < O.nch; ++j) ex = ex || Near(O.ch[j], d.xa, n) || Near(O.ch[j], d.xb, n);
+    const uint64_t m = __ballot(valid);
+    if (lane == 0) O.wave_valid[wave] = __builtin_popcountll(m);
+    __syncthreads();
+    int u = __builtin_popcountll(m & ((1ull << lane) - 1));
+    for (int w = 0; w < wave; ++w) u += O.wave_valid[w];
+    int total = 0;
+    for (int w = 0; w < kThreads / 64; ++w) total += O.wave_valid[w];
+    const int nv = total < kOmpMaxValid ? total : kOmpMaxValid;
+    if (valid && u < kOmpMaxValid) {
+      O.va[u] = d.a;
+      O.vb[u] = d.b;
+      O.vxa[u] = d.xa;
+      O.vxb[u] = d.xb;
+      O.vidx[u] = tid;
+      O.excl[u] = ex ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid < nv) {  // conflicts with the pass's earlier valid attempts
+      uint64_t cm = 0;
+      const XYZ a = O.vxa[tid], b = O.vxb[tid];
+      for (int j = 0; j < tid; ++j) {
+        const bool hit = Near(O.vxa[j], a, n) || Near(O.vxa[j], b, n) || Near(O.vxb[j], a, n) ||
+                         Near(O.vxb[j], b, n);
+        cm |= hit ? (1ull << j) : 0ull;
+      }
+      O.conf[tid] = cm;
+    }
+    __syncthreads();
+    if (tid == 0) {  // the reference's draw loop over the pass's valid attempts, in order
+      uint64_t taken = 0;
+      int ct = O.ct, len = O.len, nch = O.nch;
+      bool fin = false;
+      for (int v = 0; v < nv && !fin; ++v) {
+        if (++ct == 50) {
+          fin = true;
+          break;
+        }
+        if (O.excl[v] || (O.conf[v] & taken)) continue;
+        taken |= 1ull << v;
+        O.p[len] = O.va[v];
+        O.q[len] = O.vb[v];
+        O.xp[len] = O.vxa[v];
+        O.xq[len] = O.vxb[v];
+        O.ch[nch++] = O.vxa[v];
+        O.ch[nch++] = O.vxb[v];
+        ++len;
+        ct = 0;
+        if (len == K) fin = true;
+      }
+      O.ct = ct;
+      O.len = len;
+      O.nch = nch;
+      // the next pass starts after the last attempt looked at
+      const uint32_t adv = total > kOmpMaxValid ? static_cast<uint32_t>(O.vidx[nv - 1] + 1)
+                                                : static_cast<uint32_t>(kOmpPass);
+      O.att0 += adv;
+      if (!fin && O.att0 >= kMaxPairAttempts) {
+        O.error = 1;
+        fin = true;
+      }
+      O.finished = fin ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  const int len = O.len;
+  // species, both environments and partner slots of every event
+  for (int e = tid; e < len; e += kThreads) {
+    O.ep[e] = occ[O.p[e]];
+    O.eq[e] = occ[O.q[e]];
+  }
+  for (int i = tid; i < 2 * len * 64; i += kThreads) {
+    const int e2 = i >> 6, j = i & 63;
+    if (j < kSiteSlots) {
+      int dx, dy, dz;
+      Unpack(L.off[j], dx, dy, dz);
+      const XYZ &x = (e2 & 1) ? O.xq[e2 >> 1] : O.xp[e2 >> 1];
+      const uint8_t v = occ[SiteAt(x, n, dx, dy, dz)];
+      O.env[e2][j] = v < t.S ? v : 0;  // codes are validated at creation
+    }
+  }
+  for (int e2 = tid; e2 < 2 * len; e2 += kThreads) {
+    const XYZ &a = (e2 & 1) ? O.xq[e2 >> 1] : O.xp[e2 >> 1];
+    const XYZ &b = (e2 & 1) ? O.xp[e2 >> 1] : O.xq[e2 >> 1];
+    const int dx = MinImage(b.X - a.X, n), dy = MinImage(b.Y - a.Y, n), dz = MinImage(b.Z - a.Z, n);
+    int s = -1;
+    for (int j = 1; j < kSiteSlots && s < 0; ++j) {
+      int x, y, z;
+      Unpack(L.off[j], x, y, z);
+      if (x == dx && y == dy && z == dz) s = j;
+    }
+    O.slot[e2] = s;
+  }
+  __syncthreads();
+  // dE of every event: the thread's cluster for both swapped sites, DPP wave sums
+  for (int e = 0; e < len; ++e) {
+    const int ep = O.ep[e], eq = O.eq[e];
+    double v = ClusterDeltaAt(L, cl, O.env[2 * e], ep, eq, O.slot[2 * e]) +
+               ClusterDeltaAt(L, cl, O.env[2 * e + 1], eq, ep, O.slot[2 * e + 1]);
+    v = WaveSum(v);
+    if (lane == 0) O.part[wave][e] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < len; e += kThreads) {
+    double s = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) s += O.part[w][e];
+    O.de[e] = s;
+    const int64_t o = static_cast<int64_t>(r) * c.omp_k + e;
+    c.omp_p[o] = O.p[e];
+    c.omp_q[o] = O.q[e];
+    c.omp_de[o] = s;
+  }
+  if (tid == 0) O.pos = 0;
+  __syncthreads();
+}
+
+template <bool kLdsOcc, bool kLog>
+__global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, uint64_t ntrials,
+                                                         uint64_t trace_base) {
+  __shared__ SiteLds L;
+  __shared__ OmpLds O;
+  __shared__ uint64_t s_steps;
+  extern __shared__ uint32_t occ_words[];
+  const int r = blockIdx.x;
+  const int tid = threadIdx.x;
+  LoadSiteTables(t, L);
+  const int n = c.ncell[r];
+  const uint64_t N = 4ull * n * n * n;
+  const int K = c.omp_k;
+  uint8_t *occ = c.occ + c.off[r];
+  if constexpr (kLdsOcc) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(occ);
+    for (uint64_t i = tid; i < N / 4; i += kThreads) occ_words[i] = src[i];
+    occ = reinterpret_cast<uint8_t *>(occ_words);
+  }
+  const uint64_t maxs = c.max_steps[r];
+  uint64_t acc = 0, log_i = 0, ring_i = 0, ring_at = 0;
+  double energy = 0, abse = 0;
+  LogClock clk;
+  if (tid == 0) {
+    s_steps = c.steps[r];
+    acc = c.accepted[r];
+    energy = c.energy[r];
+    abse = c.abs_energy[r];
+    O.len = c.omp_len[r];
+    O.pos = c.omp_pos[r];
+    O.error = 0;
+    if constexpr (kLog) {
+      clk.L = c.log_every;
+      clk.Reset(s_steps);
+      log_i = c.log_n[r];
+      c.log_lo[r] = log_i;
+      if (c.avg_m) {
+        ring_i = c.ring_n[r];
+        ring_at = ring_i % c.ring_cap;
+      }
+    }
+  }
+  __syncthreads();
+  const SiteCluster cl = DecodeSiteCluster(t, L, tid);
+  // a round left by the previous launch: its pairs and dE, species and coordinates again (its
+  // untried sites are untouched by the swaps tried so far)
+  if (O.pos < O.len) {
+    for (int e = tid; e < O.len; e += kThreads) {
+      const int64_t o = static_cast<int64_t>(r) * K + e;
+      O.p[e] = c.omp_p[o];
+      O.q[e] = c.omp_q[o];
+      O.de[e] = c.omp_de[o];
+      O.xp[e] = ToXYZ(O.p[e], n);
+      O.xq[e] = ToXYZ(O.q[e], n);
+      O.ep[e] = occ[O.p[e]];
+      O.eq[e] = occ[O.q[e]];
+    }
+    __syncthreads();
+  }
+  auto ramp_T = [&](uint64_t step) {
+    return c.t0[r] + (c.t1[r] - c.t0[r]) * (static_cast<double>(step) / static_cast<double>(maxs));
+  };
+  uint64_t it = 0;
+  while (it < ntrials) {
+    if (O.pos == O.len) {
+      if (s_steps > maxs) break;
+      BuildOmpRound(t, c, L, O, cl, r, s_steps, occ, N, n, K);
+      if (O.error) break;
+    }
+    const int pos = O.pos;
+    const uint64_t left = ntrials - it;
+    const int m = static_cast<uint64_t>(O.len - pos) < left ? O.len - pos : static_cast<int>(left);
+    const uint64_t s0 = s_steps;
+    // Metropolis tests of the round's next m events, in parallel
+    for (int e = pos + tid; e < pos + m; e += kThreads) {
+      const uint64_t step = s0 + static_cast<uint64_t>(e - pos);
+      const double de = O.de[e];
+      bool a = de < 0;
+      if (!a) a = TrialUniform(c, r, step) < exp(-de * (1.0 / kBoltzmann / ramp_T(step)));
+      O.acc[e] = a ? 1 : 0;
+      const uint64_t ti = trace_base + it + static_cast<uint64_t>(e - pos);
+      if (ti < c.trace_cap) {
+        const uint64_t o = static_cast<uint64_t>(r) * c.trace_cap + ti;
+        c.tr_p[o] = O.p[e];
+        c.tr_q[o] = O.q[e];
+        c.tr_de[o] = de;
+        c.tr_acc[o] = a ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {  // energies in trial order, with the averaging ring and log records
+      for (int e = pos; e < pos + m; ++e) {
+        const uint64_t step = s0 + static_cast<uint64_t>(e - pos);
+        if constexpr (kLog) {
+          if (c.avg_m) {
+            c.ring[static_cast<uint64_t>(r) * c.ring_cap + ring_at] = energy;
+            if (++ring_at == c.ring_cap) ring_at = 0;
+            ++ring_i;
+          }
+          if (clk.Due(step)) {
+            if (log_i < c.log_cap) {
+              CmcLogRecord &rec = c.log[static_cast<uint64_t>(r) * c.log_cap + log_i];
+              rec.steps = step;
+              rec.temperature = ramp_T(step);
+              rec.energy = energy;
+              rec.second = 0.0;
+              rec.abs_energy = abse;
+              rec.count = ring_i;
+            }
+            ++log_i;
+          }
+        }
+        if (O.acc[e]) {
+          energy += O.de[e];
+          abse += O.de[e];
+          ++acc;
+        }
+      }
+      s_steps = s0 + static_cast<uint64_t>(m);
+      O.pos = pos + m;
+    }
+    // the accepted swaps (disjoint sites)
+    for (int e = pos + tid; e < pos + m; e += kThreads) {
+      if (!O.acc[e]) continue;
+      if (c.atom) {
+        const int64_t o = c.off[r];
+        AtomJump(c.atom + o, c.shift + 3 * o, O.p[e], O.q[e], ImageChange(O.xq[e].X - O.xp[e].X, n),
+                 ImageChange(O.xq[e].Y - O.xp[e].Y, n), ImageChange(O.xq[e].Z - O.xp[e].Z, n));
+      }
+      occ[O.p[e]] = static_cast<uint8_t>(O.eq[e]);
+      occ[O.q[e]] = static_cast<uint8_t>(O.ep[e]);
+    }
+    it += static_cast<uint64_t>(m);
+    __syncthreads();
+  }
+  if constexpr (kLdsOcc) {
+    __syncthreads();
+    uint32_t *dst = reinterpret_cast<uint32_t *>(c.occ + c.off[r]);
+    for (uint64_t i = tid; i < N / 4; i += kThreads) dst[i] = occ_words[i];
+  }
+  if (tid == 0) {
+    if (O.error) atomicOr(c.error, 2);
+    c.steps[r] = s_steps;
+    c.accepted[r] = acc;
+    c.energy[r] = energy;
+    c.abs_energy[r] = abse;
+    c.omp_len[r] = O.len;
+    c.omp_pos[r] = O.pos;
+    if constexpr (kLog) {
+      c.log_n[r] = log_i;
+      if (c.avg_m) c.ring_n[r] = ring_i;
+    }
+  }
+}
+
 // ThermodynamicAveraging::GetThermodynamicAverage (ThermodynamicAveraging.cpp:25-40) of the log
 // records of the last launch: over the last min(avg_m, count) energies of the ring, the mean m
 // and sum((e - m) exp(-(e - m) beta)) / sum(exp(-(e - m) beta)) + m, beta = 1 / kB / T of the
@@ -870,40 +1219,45 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
                      hipStream_t s) {
   if (c.R <= 0 || trials == 0) return hipSuccess;
   static size_t lds_block = 0;
-  static size_t stat = 0;
+  static size_t stat = 0, stat_omp = 0;
   if (!lds_block) {
     int dev = 0;
     hipDeviceProp_t p;
-    hipFuncAttributes fa;
-    hipFuncAttributes fb;
+    hipFuncAttributes fa, fb, fc, fd;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
         hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(CmcKernel<true, false>)) !=
             hipSuccess ||
         hipFuncGetAttributes(&fb, reinterpret_cast<const void *>(CmcKernel<true, true>)) !=
+            hipSuccess ||
+        hipFuncGetAttributes(&fc, reinterpret_cast<const void *>(CmcOmpKernel<true, false>)) !=
+            hipSuccess ||
+        hipFuncGetAttributes(&fd, reinterpret_cast<const void *>(CmcOmpKernel<true, true>)) !=
             hipSuccess)
       return hipErrorInvalidDevice;
     lds_block = p.sharedMemPerBlock;
     stat = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
+    stat_omp = std::max(fc.sharedSizeBytes, fd.sharedSizeBytes);
   }
   const size_t dyn = static_cast<size_t>(c.max_sites);
-  auto launch = [&](auto log) {
-    constexpr bool kL = decltype(log)::value;
-    if (c.lds_occupancy && dyn % 4 == 0 && stat + dyn <= lds_block) {
-      if (dyn > 65536 - stat) {
-        const hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(CmcKernel<true, kL>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dyn));
+  auto launch = [&](auto kernel_lds, auto kernel_hbm, size_t st) {
+    if (c.lds_occupancy && dyn % 4 == 0 && st + dyn <= lds_block) {
+      if (dyn > 65536 - st) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel_lds),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 static_cast<int>(dyn));
         if (e != hipSuccess) return e;
       }
-      hipLaunchKernelGGL((CmcKernel<true, kL>), dim3(c.R), dim3(kThreads), dyn, s, t, c, trials,
-                         trace_base);
+      hipLaunchKernelGGL(kernel_lds, dim3(c.R), dim3(kThreads), dyn, s, t, c, trials, trace_base);
     } else {
-      hipLaunchKernelGGL((CmcKernel<false, kL>), dim3(c.R), dim3(kThreads), 0, s, t, c, trials,
-                         trace_base);
+      hipLaunchKernelGGL(kernel_hbm, dim3(c.R), dim3(kThreads), 0, s, t, c, trials, trace_base);
     }
     return hipGetLastError();
   };
-  return c.log_every ? launch(std::true_type{}) : launch(std::false_type{});
+  if (c.omp_k > 0)
+    return c.log_every ? launch(CmcOmpKernel<true, true>, CmcOmpKernel<false, true>, stat_omp)
+                       : launch(CmcOmpKernel<true, false>, CmcOmpKernel<false, false>, stat_omp);
+  return c.log_every ? launch(CmcKernel<true, true>, CmcKernel<false, true>, stat)
+                     : launch(CmcKernel<true, false>, CmcKernel<false, false>, stat);
 }
 
 hipError_t LaunchCmcAverage(const CmcDev &c, hipStream_t s) {

@@ -1291,6 +1291,11 @@ struct Cmc {
   double t0, t1, energy, abs_energy;
   uint64_t steps, seed, accepted = 0;
   uint32_t replica;
+  // CanonicalMcOmp rounds (CanonicalMcOmp.cpp:40-92): the pairs and dE of the current round,
+  // all evaluated on the occupancy at the round's start, and the next one to try
+  std::vector<std::pair<size_t, size_t>> round;
+  std::vector<double> round_de;
+  size_t round_pos = 0;
   // SimulatedAnnealing schedule (lmc/mc/include/SimulatedAnnealing.h:31-71)
   bool anneal = false;
   double T = 0, lowest = 0, recent_best = 0;
@@ -1674,6 +1679,99 @@ int orc_cmc_run(void *cv, uint64_t trials, uint64_t maximum_steps, int32_t *tr_p
     return -1;
   }
 }
+// CanonicalMcOmp::BuildEventVector (CanonicalMcOmp.cpp:40-78) for the round starting at the
+// current step s0, with up to K events: the attempts are numbered through the round, attempt a
+// giving the pair of Philox counter (s0, replica, 2a + 1), redrawn while the species agree
+// (GenerateLatticeIdJumpPair, CanonicalMcAbstract.cpp:43-51). Each event takes the first drawn
+// pair with neither site in the unavailable set; the 50th draw of an event ends the round
+// (the `ct == 50` breaks). An accepted pair adds both sites and their 1st-3rd neighbour shells
+// to the set. Every dE is taken on the occupancy at the round's start.
+static void BuildCmcRound(Cmc &c, int K) {
+  const Model &m = *c.pm;
+  const uint64_t N = c.lat->N;
+  std::vector<char> unavailable(N, 0);
+  c.round.clear();
+  c.round_de.clear();
+  c.round_pos = 0;
+  uint32_t att = 0;
+  for (int i = 0; i < K; ++i) {
+    int ct = 0;
+    size_t p = 0, q = 0;
+    do {
+      for (;; ++att) {  // GenerateLatticeIdJumpPair
+        if (att >= (1u << 20)) throw std::runtime_error("no distinct pair");
+        uint32_t r[4];
+        Draw(c.seed, c.steps, c.replica, 2u * att + 1u, r);
+        p = Below(Bits64(r[0], r[1]), N);
+        q = Below(Bits64(r[2], r[3]), N);
+        if (c.occ[p] != c.occ[q]) {
+          ++att;
+          break;
+        }
+      }
+      ++ct;
+      if (ct == 50) break;
+    } while (unavailable[p] || unavailable[q]);
+    if (ct == 50) break;
+    for (size_t site : {p, q}) {
+      unavailable[site] = 1;
+      for (int sh = 0; sh < 3; ++sh)
+        for (size_t nb : c.lat->nn[sh][site]) unavailable[nb] = 1;
+    }
+    c.round.emplace_back(p, q);
+  }
+  for (const auto &pq : c.round) c.round_de.push_back(SwapDe(m, c.occ.data(), pq.first, pq.second));
+}
+
+// CanonicalMcOmp::Simulate (CanonicalMcOmp.cpp:79-92): a new round starts only while steps <=
+// maximum_steps, and a started round runs to its end; each event is the Metropolis step of
+// CanonicalMcAbstract::SelectEvent (:86-101) at its own step, uniform of Philox counter (step,
+// replica, 0xFFFFFFFE), temperature of the linear ramp. Rounds carry over between calls.
+int orc_cmc_omp_run(void *cv, uint64_t trials, uint64_t maximum_steps, int K, int32_t *tr_p,
+                    int32_t *tr_q, double *tr_de, int8_t *tr_acc) {
+  Cmc &c = *static_cast<Cmc *>(cv);
+  if (K < 1 || c.anneal) return -1;
+  try {
+    for (uint64_t i = 0; i < trials; ++i) {
+      if (c.round_pos == c.round.size()) {
+        if (c.steps > maximum_steps) break;
+        BuildCmcRound(c, K);
+      }
+      const auto [p, q] = c.round[c.round_pos];
+      const double de = c.round_de[c.round_pos];
+      ++c.round_pos;
+      const double T = c.t0 + (c.t1 - c.t0) * (static_cast<double>(c.steps) /
+                                               static_cast<double>(maximum_steps));
+      const double beta = 1.0 / kBoltzmann / T;
+      bool acc = de < 0;
+      if (!acc) {
+        uint32_t r[4];
+        Draw(c.seed, c.steps, c.replica, 0xFFFFFFFEu, r);
+        acc = UnitOpenRight(Bits64(r[0], r[1])) < std::exp(-de * beta);
+      }
+      if (acc) {
+        std::swap(c.occ[p], c.occ[q]);
+        c.energy += de;
+        c.abs_energy += de;
+        ++c.accepted;
+      }
+      if (tr_p) tr_p[i] = static_cast<int32_t>(p);
+      if (tr_q) tr_q[i] = static_cast<int32_t>(q);
+      if (tr_de) tr_de[i] = de;
+      if (tr_acc) tr_acc[i] = acc ? 1 : 0;
+      ++c.steps;
+    }
+    return 0;
+  } catch (const std::exception &) {
+    return -1;
+  }
+}
+// Events of the current round not yet tried.
+int orc_cmc_round_left(const void *cv) {
+  const Cmc &c = *static_cast<const Cmc *>(cv);
+  return static_cast<int>(c.round.size() - c.round_pos);
+}
+
 void orc_cmc_state(const void *cv, uint64_t *steps, double *energy, double *abs_energy,
                    uint64_t *accepted) {
   const Cmc &c = *static_cast<const Cmc *>(cv);

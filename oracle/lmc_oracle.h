@@ -125,6 +125,11 @@ void orc_cmc_free(void *cmc);
 // Optional per-trial trace: site pair, dE, accepted flag.
 int orc_cmc_run(void *cmc, uint64_t trials, uint64_t maximum_steps, int32_t *tr_p, int32_t *tr_q,
                 double *tr_de, int8_t *tr_acc);
+/* CanonicalMcOmp trials: rounds of up to K swap pairs under the 3NN exclusion set, all dE on the
+ * round's starting occupancy, then the Metropolis steps in order; rounds carry over calls. */
+int orc_cmc_omp_run(void *cmc, uint64_t trials, uint64_t maximum_steps, int K, int32_t *tr_p,
+                    int32_t *tr_q, double *tr_de, int8_t *tr_acc);
+int orc_cmc_round_left(const void *cmc);
 void orc_cmc_state(const void *cmc, uint64_t *steps, double *energy, double *abs_energy,
                    uint64_t *accepted);
 void orc_cmc_occupancy(const void *cmc, uint8_t *occ);

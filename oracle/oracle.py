@@ -85,6 +85,9 @@ def lib():
             "orc_cmc_free": (None, [vp]),
             "orc_cmc_run": (C.c_int, [vp, C.c_uint64, C.c_uint64, _i32p, _i32p, _f64p,
                                       C.POINTER(C.c_int8)]),
+            "orc_cmc_omp_run": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_int, _i32p, _i32p,
+                                          _f64p, C.POINTER(C.c_int8)]),
+            "orc_cmc_round_left": (C.c_int, [vp]),
             "orc_cmc_state": (None, [vp, C.POINTER(C.c_uint64), _f64p, _f64p,
                                      C.POINTER(C.c_uint64)]),
             "orc_cmc_occupancy": (None, [vp, _u8p]),
@@ -338,6 +341,23 @@ class Cmc:
         if rc:
             raise RuntimeError("oracle CMC failed")
         return tr
+
+    def run_omp(self, trials, maximum_steps, K, trace=False):
+        """CanonicalMcOmp trials: rounds of up to K pairs (see orc_cmc_omp_run)."""
+        tr = None
+        if trace:
+            tr = dict(p=np.zeros(trials, np.int32), q=np.zeros(trials, np.int32),
+                      de=np.zeros(trials), acc=np.zeros(trials, np.int8))
+        rc = lib().orc_cmc_omp_run(self.h, trials, maximum_steps, K,
+                                   _opt(tr and tr["p"], _i32p), _opt(tr and tr["q"], _i32p),
+                                   _opt(tr and tr["de"], _f64p),
+                                   _opt(tr and tr["acc"], C.POINTER(C.c_int8)))
+        if rc:
+            raise RuntimeError("oracle CanonicalMcOmp failed")
+        return tr
+
+    def round_left(self):
+        return lib().orc_cmc_round_left(self.h)
 
     def anneal_state(self):
         T, low, rh = C.c_double(), C.c_double(), C.c_uint32()
