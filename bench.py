@@ -13,7 +13,8 @@ of the single-replica 10^3-cell workload).
 
 Other workloads (--workload): uneven (uneven-size KMC beside canonical MC), large (64 replicas of
 50^3 cells with the quaternary surrogate), chain (second-order KMC), anneal (simulated
-annealing), exittime (ExitTime's per-site jump scan and energy changes). Each prints one JSON line.
+annealing), exittime (ExitTime's per-site jump scan and energy changes), cmc (CanonicalMcSerial and
+CanonicalMcOmp trials). Each prints one JSON line.
 """
 import argparse
 import json
@@ -91,24 +92,29 @@ def launch_plan(steps, chunk):
     return plan
 
 
+def uses_graph(plan, eager=False):
+    """Full launches come from a captured HIP graph when there are at least two of them (a
+    one-launch graph saves nothing) and --eager is not set."""
+    return not eager and plan[0][1] >= 2
+
+
 def run_plan(drv, plan, stream, eager=False):
-    """Issues the launches of `plan` on `stream`: the full launches from the captured HIP graph
-    (built beforehand by prepare_plan), the remainder launch eagerly."""
+    """Issues the launches of `plan` on `stream`: the full launches (replayed from the graph
+    prepare_plan captured, or eagerly), then the remainder launch eagerly."""
     (chunk, full), rest = plan[0], plan[1:]
-    if eager:
+    if uses_graph(plan, eager):
+        drv.run_graph(chunk, full, stream)
+    else:
         for _ in range(full):
             drv.run(chunk, stream)
-    else:
-        drv.run_graph(chunk, full, stream)
     for c, _ in rest:
         drv.run(c, stream)
 
 
 def prepare_plan(drv, plan, stream, eager=False):
-    """Captures the graph of the plan's full launches (outside any timed region) and sets up the
-    remainder's kernel shape with a zero-cost check of the plan."""
+    """Captures the HIP graph of the plan's full launches, outside any timed region."""
     assert sum(c * k for c, k in plan) > 0
-    if not eager:
+    if uses_graph(plan, eager):
         drv.build_graph(plan[0][0], plan[0][1])
 
 
@@ -395,6 +401,80 @@ def chain_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def cmc_workload(args):
+    """Canonical MC on the canonical workload's replicas: 64 replicas of 30^3 cells (108000 sites,
+    Al/Mg/Zn 90/5/5), T linear 800 -> 300 K over --cmc-trials trials each, pair-site swap dE, as
+    CanonicalMcSerial trials and as CanonicalMcOmp rounds of 64 trials (lmc_cmc_set_parallel_trials).
+    CPU baseline: the oracle's CanonicalMc trials, one single-thread replica per host core. Prints
+    one JSON line; value = the CanonicalMcOmp rate."""
+    import torch
+    import latticemontecarlo_amd as L
+    from latticemontecarlo_amd import synth
+    tmp = tempfile.mkdtemp()
+    coeff = os.path.join(tmp, "coeff_e0.json")
+    synth.write_coefficients(coeff, ["Al", "Mg", "Zn"], "E0", seed=args.seed)
+    ps = L.Model(coeff, ["Al", "Mg", "Zn"], L.PAIRSITE)
+    species = ["Al", "Mg", "X", "Zn"]
+    R, n, trials, K = 64, 30, args.cmc_trials, 64
+    N = 4 * n ** 3
+    occs = [synth.random_occupancy(n, species, {"Mg": round(0.05 * N), "Zn": round(0.05 * N)},
+                                   seed=args.seed * 11 + r, vacancy=False) for r in range(R)]
+    stream = torch.cuda.current_stream()
+    rates = {}
+    for mode in ("serial", "omp"):
+        drv = L.CanonicalMc(ps, None, [n] * R, occs, 800.0, 300.0, args.seed, maximum_steps=trials)
+        if mode == "omp":
+            drv.set_parallel_trials(K)
+        drv.run(min(2000, trials), stream)  # warm-up
+        torch.cuda.synchronize()
+        done0 = int(drv.state()["steps"].sum())
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        drv.run(trials, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        secs = e0.elapsed_time(e1) / 1e3
+        st = drv.state()
+        ps.check_errors()
+        done = int(st["steps"].sum()) - done0
+        rates[mode] = (done / secs, secs, float(st["accepted"].sum()) / max(1, int(st["steps"].sum())))
+        log(f"canonical MC {mode}: {done / secs:.4g} trials/s over {R} replicas ({secs:.3f} s)")
+        del drv
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        lat = O.Lattice(n)
+        om = O.Model(lat, coeff, ["Al", "Mg", "Zn"], O.PAIRSITE)
+        cores, host = host_cores()
+        k = 4000
+        per = O.cmc_baseline(om, lat, occs[0], 800.0, args.seed, 1, k, trials, False) / k
+        k = int(min(trials, max(2000, 12.0 / max(per, 1e-7))))
+        secs_c = O.cmc_baseline(om, lat, occs[0], 800.0, args.seed, cores, k, trials, False)
+        thru = cores * k / secs_c
+        cpu = {"value": round(thru, 1), "unit": "trials/s", "cores": cores, "kind": "port",
+               "host": host,
+               "sample": f"oracle CanonicalMc trials, {cores} replicas of 30^3 cells run "
+                         f"concurrently, one thread each, {k} trials each ({secs_c:.1f} s)"}
+        log(f"cpu baseline (canonical MC): {thru:.1f} trials/s ({cores} concurrent replicas)")
+    value, secs, accr = rates["omp"]
+    line = {"metric": "canonical MC trials/s over all replicas (CanonicalMcOmp rounds)",
+            "value": round(value, 1), "unit": "trials/s", "n_gpus": 1, "steps": trials,
+            "warmup": min(2000, trials), "ms_per_step": round(secs / trials * 1e3, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"canonical MC, {R} replicas x {n}^3 FCC cells ({N} sites), "
+                                   "Al/Mg/Zn 90/5/5, T linear 800 -> 300 K, pair-site swap dE",
+                       "trials_per_round": K,
+                       "serial_trials_per_s": round(rates["serial"][0], 1),
+                       "omp_over_serial": round(value / rates["serial"][0], 3),
+                       "acceptance_omp": round(accr, 4),
+                       "acceptance_serial": round(rates["serial"][2], 4)},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
 def anneal_workload(args):
     """Simulated annealing (SimulatedAnnealing.cpp:99-185): 64 replicas of 30^3 cells (108000
     sites, Al/Mg/Zn 90/5/5) from T = 800 K over --cmc-trials trials each, pair-site swap dE.
@@ -677,7 +757,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch eagerly instead of a HIP graph")
-    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain", "anneal", "exittime"],
+    ap.add_argument("--workload", choices=["batched", "uneven", "large", "chain", "anneal", "exittime",
+                                           "cmc"],
                     default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
@@ -706,6 +787,9 @@ def main():
         return
     if args.workload == "anneal":
         anneal_workload(args)
+        return
+    if args.workload == "cmc":
+        cmc_workload(args)
         return
     if args.workload == "exittime":
         exittime_workload(args)
@@ -751,7 +835,7 @@ def main():
     wall = time.perf_counter() - wall0
     elapsed = start.elapsed_time(end) / 1e3
     n_launch = sum(k for _, k in plan)
-    log(f"{n_launch} launches ({plan}, {'eager' if args.eager else 'HIP graph'}): "
+    log(f"{n_launch} launches ({plan}, {'HIP graph' if uses_graph(plan, args.eager) else 'eager'}): "
         f"{elapsed * 1e3:.3f} ms, {elapsed / n_launch * 1e3:.3f} ms per launch")
     st = drv.state()
     assert int(st["steps"].min()) == steps0 + args.steps, "every replica must run every timed step"
@@ -785,7 +869,8 @@ def main():
                                  "surrogate",
                    "replicas": R, "sites_per_replica": 4 * n ** 3,
                    "steps_per_replica": args.steps, "launches": [list(x) for x in plan],
-                   "launch": "eager" if args.eager else "hip graph (+ eager remainder)",
+                   "launch": "hip graph (+ eager remainder)" if uses_graph(plan, args.eager)
+                             else "eager",
                    "formulation": FORMS[tune["formulation"]],
                    "autotune_ms_per_batch_step": {FORMS[f]: (None if tune[k] != tune[k] else
                                                              round(tune[k], 6))

@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <limits>
@@ -1267,12 +1268,10 @@ class CanonicalMcSerial {
     Configure(b);
     b.TrackAtoms({config_});
     if (log_dump_steps_) b.SetLog(log_dump_steps_, averaging_, detail::kDriverChunk + 64);
-    const uint64_t s0 = p_.restart_steps, mx = maximum_steps_;
+    const uint64_t s0 = p_.restart_steps, mx = maximum_steps_, cds = config_dump_steps_;
     std::vector<cfg::Config> cfgs{config_};
     uint64_t s = s0;
-    if (s0 == 0 && s <= mx) detail::WriteDumps(config_, s, config_dump_steps_, mx);
-    while (s <= mx) {
-      const uint64_t target = detail::NextDump(s, config_dump_steps_, mx);
+    auto run_to = [&](uint64_t target) {
       while (s < target) {
         b.Run(std::min<uint64_t>(target - s, detail::kDriverChunk));
         if (log_dump_steps_) b.WriteLog({"cmc_log.txt"});
@@ -1280,11 +1279,22 @@ class CanonicalMcSerial {
         lmc::Check(lmc_cmc_state(b.handle(), &s, nullptr, nullptr, nullptr));
         if (s == before) throw std::runtime_error("CMC run advanced no trial");
       }
-      if (s <= mx) {
-        b.UpdateConfigs(cfgs);
-        config_ = cfgs[0];
-        detail::WriteDumps(config_, s, config_dump_steps_, mx);
-      }
+    };
+    auto dump = [&]() {
+      b.UpdateConfigs(cfgs);
+      config_ = cfgs[0];
+      detail::WriteDumps(config_, s, cds, mx);
+    };
+    if (s0 == 0 && s <= mx) detail::WriteDumps(config_, s, cds, mx);
+    while (s <= mx) {
+      run_to(detail::NextDump(s, cds, mx));
+      if (s <= mx) dump();
+    }
+    // a started CanonicalMcOmp round runs to its end past maximum_steps, dumping on the way
+    for (uint64_t left = RoundLeft(b); left > 0; left = RoundLeft(b)) {
+      const uint64_t next = cds ? (s / cds + 1) * cds : std::numeric_limits<uint64_t>::max();
+      run_to(std::min(s + left, next));
+      if (cds && s % cds == 0 && RoundLeft(b) > 0) dump();
     }
     b.UpdateConfigs(cfgs);
     config_ = cfgs[0];
@@ -1298,6 +1308,8 @@ class CanonicalMcSerial {
 
  protected:
   virtual void Configure(CanonicalMcBatch &) {}
+  // Trials of the current round still to run (serial trials: none).
+  virtual uint64_t RoundLeft(CanonicalMcBatch &) { return 0; }
   cfg::Config config_;
 
  private:
@@ -1306,6 +1318,39 @@ class CanonicalMcSerial {
   uint64_t log_dump_steps_, config_dump_steps_, maximum_steps_, averaging_, seed_;
   lmc_cmc_replica p_{};
   std::vector<double> energies_;
+};
+
+// CanonicalMcOmp (lmc/mc/include/CanonicalMcOmp.h:9-29): the same constructor and outputs,
+// trials in rounds of up to K pairs under the 3NN exclusion set (CanonicalMcOmp.cpp:40-92). The
+// reference takes K from its OpenMP thread count; here K is OMP_NUM_THREADS when that is set
+// (1..64), else 64, and SetParallelTrials changes it.
+class CanonicalMcOmp : public CanonicalMcSerial {
+ public:
+  template <typename... A>
+  explicit CanonicalMcOmp(A &&...args) : CanonicalMcSerial(std::forward<A>(args)...) {
+    if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+      const long v = std::strtol(e, nullptr, 10);
+      if (v >= 1) k_ = static_cast<int32_t>(std::min<long>(v, 64));
+    }
+  }
+  // No reference counterpart: the number of pairs per round (1..64).
+  void SetParallelTrials(int32_t k) {
+    if (k < 1 || k > 64) throw std::invalid_argument("trials per round must be in [1, 64]");
+    k_ = k;
+  }
+
+ protected:
+  void Configure(CanonicalMcBatch &b) override {
+    lmc::Check(lmc_cmc_set_parallel_trials(b.handle(), k_));
+  }
+  uint64_t RoundLeft(CanonicalMcBatch &b) override {
+    int32_t left = 0;
+    lmc::Check(lmc_cmc_round_left(b.handle(), &left));
+    return static_cast<uint64_t>(left);
+  }
+
+ private:
+  int32_t k_ = 64;
 };
 
 }  // namespace mc

@@ -7,7 +7,7 @@
 //   home_drivers kmc|chain <config> <json> <log> <cfgdump> <max> <restart_steps> <restart_energy>
 //                <restart_time> <T> <time_temperature_file|-> <rate_corrector> <early_stop>
 //                <solute_disp> <seed>
-//   home_drivers cmc <config> <json> <log> <cfgdump> <max> <thermo> <restart_steps>
+//   home_drivers cmc|cmc_omp <config> <json> <log> <cfgdump> <max> <thermo> <restart_steps>
 //                <restart_energy> <T> <seed>
 #include <cstdio>
 #include <cstdlib>
@@ -52,7 +52,8 @@ Driver BuildKineticMcFromParameter(const Parameter &parameter) {
                 parameter.solute_disp_};
 }
 
-mc::CanonicalMcSerial BuildCanonicalMcSerialFromParameter(const Parameter &parameter) {
+template <typename Driver>
+Driver BuildCanonicalMcFromParameter(const Parameter &parameter) {
   std::set<Element> element_set;
   for (const auto &element_string : parameter.element_set_) {
     element_set.insert(Element(element_string));
@@ -60,16 +61,16 @@ mc::CanonicalMcSerial BuildCanonicalMcSerialFromParameter(const Parameter &param
   cfg::Config config;
   config = cfg::Config::ReadConfig(parameter.config_filename_);
   config.ReassignLatticeVector();
-  return mc::CanonicalMcSerial{config,
-                               parameter.log_dump_steps_,
-                               parameter.config_dump_steps_,
-                               parameter.maximum_steps_,
-                               parameter.thermodynamic_averaging_steps_,
-                               parameter.restart_steps_,
-                               parameter.restart_energy_,
-                               parameter.temperature_,
-                               element_set,
-                               parameter.json_coefficients_filename_};
+  return Driver{config,
+                parameter.log_dump_steps_,
+                parameter.config_dump_steps_,
+                parameter.maximum_steps_,
+                parameter.thermodynamic_averaging_steps_,
+                parameter.restart_steps_,
+                parameter.restart_energy_,
+                parameter.temperature_,
+                element_set,
+                parameter.json_coefficients_filename_};
 }
 
 unsigned long long U(const char *s) { return std::strtoull(s, nullptr, 10); }
@@ -111,7 +112,7 @@ int main(int argc, char **argv) {
         run(BuildKineticMcFromParameter<mc::KineticMcChainOmpi>(p));
       return 0;
     }
-    if (mode == "cmc" && argc == 12) {
+    if ((mode == "cmc" || mode == "cmc_omp") && argc == 12) {
       p.config_filename_ = argv[2];
       p.json_coefficients_filename_ = argv[3];
       p.log_dump_steps_ = U(argv[4]);
@@ -121,10 +122,15 @@ int main(int argc, char **argv) {
       p.restart_steps_ = U(argv[8]);
       p.restart_energy_ = D(argv[9]);
       p.temperature_ = D(argv[10]);
-      auto driver = BuildCanonicalMcSerialFromParameter(p);
-      driver.SetSeed(U(argv[11]));
-      driver.Simulate();
-      std::printf("final energy=%.17g\n", driver.GetEnergy());
+      auto run = [&](auto &&driver) {
+        driver.SetSeed(U(argv[11]));
+        driver.Simulate();
+        std::printf("final energy=%.17g\n", driver.GetEnergy());
+      };
+      if (mode == "cmc")
+        run(BuildCanonicalMcFromParameter<mc::CanonicalMcSerial>(p));
+      else
+        run(BuildCanonicalMcFromParameter<mc::CanonicalMcOmp>(p));
       return 0;
     }
     std::cerr << "bad arguments\n";

@@ -49,3 +49,10 @@ def test_host_cores_reports_its_sources(bench, monkeypatch):
     monkeypatch.setenv("OMP_NUM_THREADS", "1")
     cores, host = bench.host_cores()
     assert cores == 1 and host["omp_num_threads"] == "1"
+
+
+def test_graph_only_for_repeated_launches(bench):
+    assert not bench.uses_graph(bench.launch_plan(20, 5000))
+    assert bench.uses_graph(bench.launch_plan(100000, 5000))
+    assert not bench.uses_graph(bench.launch_plan(100000, 5000), eager=True)
+    assert not bench.uses_graph(bench.launch_plan(7000, 5000))  # one full launch + remainder

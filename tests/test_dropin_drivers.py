@@ -225,3 +225,39 @@ def test_canonical_driver_files_match_oracle(exe, coeffs, tmp_path):
     lines = (tmp_path / "cmc_log.txt").read_text().splitlines()
     assert lines[0] == "steps\ttemperature\tenergy\taverage_energy\tabsolute_energy"
     assert [int(l.split("\t")[0]) for l in lines[1:]] == [s for s in range(mx + 1) if _due(s, 10)]
+
+
+@pytest.mark.gpu
+def test_canonical_omp_driver_files_match_oracle(exe, coeffs, tmp_path):
+    """mc::CanonicalMcOmp with OMP_NUM_THREADS = 16: rounds of 16 trials; the last round started
+    at a step <= maximum_steps runs to its end (CanonicalMcOmp.cpp:79-92), dumping on the way."""
+    import oracle as O
+    path, _ = coeffs["e0_3n"]
+    n, T, seed, mx, cds, K = 6, 700.0, 5, 1000, 7, 16
+    occ = random_alloc(n, SPECIES, 0.08, seed=13, vacancy=False)
+    _write_start(tmp_path, n, occ)
+    print("drop-in cmc omp", flush=True)
+    env = dict(os.environ, OMP_NUM_THREADS=str(K))
+    r = subprocess.run([exe, "cmc_omp", "start.cfg.gz", path, "10", str(cds), str(mx), "0", "0",
+                        "0", str(T), str(seed)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lat = O.Lattice(n)
+    op, oe = O.Model(lat, path, TERNARY, O.PAIRSITE), O.Model(lat, path, TERNARY, O.ENERGY)
+    c = O.Cmc(op, oe, lat, occ, T, T, seed, replica=0)
+    c.run_omp(mx + 1, mx, K)
+    tail = c.round_left()
+    end = mx + 1 + tail
+    c2 = O.Cmc(op, oe, lat, occ, T, T, seed, replica=0)
+    done = 0
+    for d in range(0, end, cds):
+        c2.run_omp(d - done, mx, K)
+        done = d
+        assert np.array_equal(_read_codes(tmp_path / f"{d}.cfg.gz"), c2.occupancy()), d
+    assert not (tmp_path / f"{(end // cds + 1) * cds}.cfg.gz").exists()
+    c2.run_omp(mx - done, mx, K)
+    assert np.array_equal(_read_codes(tmp_path / "end.cfg.gz"), c2.occupancy())
+    c.run_omp(tail, mx, K)
+    assert c.round_left() == 0 and c.state()["steps"] == end
+    fin = float(r.stdout.split("=")[1])
+    assert abs(fin - c.state()["energy"]) < 1e-9 * end
