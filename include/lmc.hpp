@@ -1292,9 +1292,9 @@ class CanonicalMcSerial {
     }
     // a started CanonicalMcOmp round runs to its end past maximum_steps, dumping on the way
     for (uint64_t left = RoundLeft(b); left > 0; left = RoundLeft(b)) {
+      if (cds && s % cds == 0) dump();  // the trial of step s runs
       const uint64_t next = cds ? (s / cds + 1) * cds : std::numeric_limits<uint64_t>::max();
       run_to(std::min(s + left, next));
-      if (cds && s % cds == 0 && RoundLeft(b) > 0) dump();
     }
     b.UpdateConfigs(cfgs);
     config_ = cfgs[0];
