@@ -647,18 +647,33 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
 // sites. A new round starts only while steps <= maximum_steps; a started round runs to its end,
 // across launches if need be (the round is kept in c.omp_*).
 //
-// One 256-thread workgroup per replica. Per round:
-//   draw   : passes of 256 attempts, one per thread; the species check and the check against the
-//            round's earlier events in parallel; the (at most 64) valid attempts of a pass ranked
-//            by ballots, their pairwise conflicts as bit masks, then one lane walks them in order
-//            (the reference's do/while with its draw count) -- a few passes per round;
-//   dE     : both 43-site environments of every event gathered at once; every thread holds one
-//            site cluster and adds its change for every event (DPP wave sums, waves in order);
+// One 1024-thread workgroup per replica (the canonical workloads have few replicas, so a CU
+// each; 16 waves hide the phases' latencies). Per round:
+//   draw   : passes of 1024 attempts, one per thread, species checked in parallel; the first 128
+//            valid attempts of a pass ranked by ballots; their exclusion by the round's earlier
+//            events and their pairwise conflicts checked by 8 threads each (LDS atomics); then
+//            one lane walks them in order (the reference's do/while with its draw count);
+//   dE     : both 43-site environments of every event gathered at once; thread (chunk, event)
+//            adds the changes of 16 site clusters for both swapped sites, and the 16 chunk sums
+//            of an event are added in chunk order;
 //   trials : event lanes test their Metropolis steps in parallel and apply the accepted swaps;
 //            one lane adds the energies in trial order (and takes the log records).
+constexpr int kOmpThreads = 1024;
 constexpr int kOmpMaxK = 64;
-constexpr int kOmpPass = kThreads;  // attempts per draw pass
-constexpr int kOmpMaxValid = 64;    // valid attempts resolved per pass
+constexpr int kOmpPass = kOmpThreads;          // attempts per draw pass
+constexpr int kOmpMaxValid = 128;              // valid attempts resolved per pass
+constexpr int kOmpGroups = kOmpThreads / kOmpMaxValid;  // checking threads per valid attempt
+constexpr int kOmpChunks = kOmpThreads / kOmpMaxK;      // cluster chunks per event
+constexpr int kOmpChunkLen = kMaxSiteClusters / kOmpChunks;
+static_assert(kOmpChunks * kOmpChunkLen == kMaxSiteClusters, "chunks cover the site clusters");
+
+struct OmpDraw {  // one draw pass: its valid attempts in attempt order
+  int64_t va[kOmpMaxValid], vb[kOmpMaxValid];
+  XYZ vxa[kOmpMaxValid], vxb[kOmpMaxValid];
+  uint32_t conf[kOmpMaxValid][kOmpMaxValid / 32];  // bit j: conflicts with valid attempt j
+  uint32_t excl[kOmpMaxValid];                     // meets an earlier pass's event
+  int vidx[kOmpMaxValid];
+};
 
 struct OmpLds {
   uint8_t env[2 * kOmpMaxK][48];
@@ -667,14 +682,13 @@ struct OmpLds {
   int64_t p[kOmpMaxK], q[kOmpMaxK];
   XYZ xp[kOmpMaxK], xq[kOmpMaxK];
   int ep[kOmpMaxK], eq[kOmpMaxK];
-  double de[kOmpMaxK], part[kThreads / 64][kOmpMaxK];
+  double de[kOmpMaxK];
   int8_t acc[kOmpMaxK];
-  // one draw pass: its valid attempts in attempt order
-  int64_t va[kOmpMaxValid], vb[kOmpMaxValid];
-  XYZ vxa[kOmpMaxValid], vxb[kOmpMaxValid];
-  uint64_t conf[kOmpMaxValid];  // bit j: conflicts with valid attempt j of the pass
-  int vidx[kOmpMaxValid], excl[kOmpMaxValid];
-  int wave_valid[kThreads / 64];
+  union {
+    OmpDraw d;                          // draw phase
+    double part[kOmpChunks][kOmpMaxK];  // dE phase
+  } u;
+  int wave_valid[kOmpThreads / 64];
   // round control
   int len, pos, nch, ct, finished, error;
   uint32_t att0;
@@ -696,10 +710,10 @@ __device__ __forceinline__ double ClusterDeltaAt(const SiteLds &L, const SiteClu
 }
 
 // The round starting at step s0: pairs, environments, partner slots and dE into O (all threads).
-__device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, OmpLds &O,
-                              const SiteCluster &cl, int r, uint64_t s0, const uint8_t *occ,
-                              uint64_t N, int n, int K) {
+__device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, OmpLds &O, int r,
+                              uint64_t s0, const uint8_t *occ, uint64_t N, int n, int K) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  OmpDraw &D = O.u.d;
   if (tid == 0) {
     O.len = 0;
     O.nch = 0;
@@ -709,42 +723,54 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
   }
   __syncthreads();
   while (!O.finished) {
-    const uint32_t att = O.att0 + static_cast<uint32_t>(tid);
-    const PairDraw d = DrawAttempt(c, r, s0, att, N, n);
+    const PairDraw d = DrawAttempt(c, r, s0, O.att0 + static_cast<uint32_t>(tid), N, n);
     const bool valid = occ[d.a] != occ[d.b];
-    bool ex = false;
-    if (valid)
-      for (int j = 0; j < O.nch; ++j) ex = ex || Near(O.ch[j], d.xa, n) || Near(O.ch[j], d.xb, n);
     const uint64_t m = __ballot(valid);
     if (lane == 0) O.wave_valid[wave] = __builtin_popcountll(m);
     __syncthreads();
     int u = __builtin_popcountll(m & ((1ull << lane) - 1));
     for (int w = 0; w < wave; ++w) u += O.wave_valid[w];
     int total = 0;
-    for (int w = 0; w < kThreads / 64; ++w) total += O.wave_valid[w];
+    for (int w = 0; w < kOmpThreads / 64; ++w) total += O.wave_valid[w];
     const int nv = total < kOmpMaxValid ? total : kOmpMaxValid;
     if (valid && u < kOmpMaxValid) {
-      O.va[u] = d.a;
-      O.vb[u] = d.b;
-      O.vxa[u] = d.xa;
-      O.vxb[u] = d.xb;
-      O.vidx[u] = tid;
-      O.excl[u] = ex ? 1 : 0;
+      D.va[u] = d.a;
+      D.vb[u] = d.b;
+      D.vxa[u] = d.xa;
+      D.vxb[u] = d.xb;
+      D.vidx[u] = tid;
+    }
+    if (tid < kOmpMaxValid) {
+      D.excl[tid] = 0;
+      for (int w = 0; w < kOmpMaxValid / 32; ++w) D.conf[tid][w] = 0;
     }
     __syncthreads();
-    if (tid < nv) {  // conflicts with the pass's earlier valid attempts
-      uint64_t cm = 0;
-      const XYZ a = O.vxa[tid], b = O.vxb[tid];
-      for (int j = 0; j < tid; ++j) {
-        const bool hit = Near(O.vxa[j], a, n) || Near(O.vxa[j], b, n) || Near(O.vxb[j], a, n) ||
-                         Near(O.vxb[j], b, n);
-        cm |= hit ? (1ull << j) : 0ull;
+    {  // thread (g, v): the g-th share of valid attempt v's checks
+      const int v = tid % kOmpMaxValid, g = tid / kOmpMaxValid;
+      if (v < nv) {
+        const XYZ a = D.vxa[v], b = D.vxb[v];
+        bool ex = false;
+        for (int j = g; j < O.nch; j += kOmpGroups) ex = ex || Near(O.ch[j], a, n) || Near(O.ch[j], b, n);
+        if (ex) atomicOr(&D.excl[v], 1u);
+        static_assert(kOmpMaxValid == 128, "conflict masks are two 64-bit words");
+        uint64_t lo = 0, hi = 0;  // registers, not a run-time indexed array
+        for (int j = g; j < v; j += kOmpGroups) {
+          const bool hit = Near(D.vxa[j], a, n) || Near(D.vxa[j], b, n) || Near(D.vxb[j], a, n) ||
+                           Near(D.vxb[j], b, n);
+          if (hit) {
+            if (j < 64) lo |= 1ull << j;
+            else hi |= 1ull << (j - 64);
+          }
+        }
+        if (lo & 0xFFFFFFFFull) atomicOr(&D.conf[v][0], static_cast<uint32_t>(lo));
+        if (lo >> 32) atomicOr(&D.conf[v][1], static_cast<uint32_t>(lo >> 32));
+        if (hi & 0xFFFFFFFFull) atomicOr(&D.conf[v][2], static_cast<uint32_t>(hi));
+        if (hi >> 32) atomicOr(&D.conf[v][3], static_cast<uint32_t>(hi >> 32));
       }
-      O.conf[tid] = cm;
     }
     __syncthreads();
     if (tid == 0) {  // the reference's draw loop over the pass's valid attempts, in order
-      uint64_t taken = 0;
+      uint64_t tlo = 0, thi = 0;  // the pass's attempts taken so far
       int ct = O.ct, len = O.len, nch = O.nch;
       bool fin = false;
       for (int v = 0; v < nv && !fin; ++v) {
@@ -752,14 +778,17 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
           fin = true;
           break;
         }
-        if (O.excl[v] || (O.conf[v] & taken)) continue;
-        taken |= 1ull << v;
-        O.p[len] = O.va[v];
-        O.q[len] = O.vb[v];
-        O.xp[len] = O.vxa[v];
-        O.xq[len] = O.vxb[v];
-        O.ch[nch++] = O.vxa[v];
-        O.ch[nch++] = O.vxb[v];
+        const uint64_t clo = D.conf[v][0] | (static_cast<uint64_t>(D.conf[v][1]) << 32);
+        const uint64_t chi = D.conf[v][2] | (static_cast<uint64_t>(D.conf[v][3]) << 32);
+        if (D.excl[v] || (clo & tlo) || (chi & thi)) continue;
+        if (v < 64) tlo |= 1ull << v;
+        else thi |= 1ull << (v - 64);
+        O.p[len] = D.va[v];
+        O.q[len] = D.vb[v];
+        O.xp[len] = D.vxa[v];
+        O.xq[len] = D.vxb[v];
+        O.ch[nch++] = D.vxa[v];
+        O.ch[nch++] = D.vxb[v];
         ++len;
         ct = 0;
         if (len == K) fin = true;
@@ -768,7 +797,7 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
       O.len = len;
       O.nch = nch;
       // the next pass starts after the last attempt looked at
-      const uint32_t adv = total > kOmpMaxValid ? static_cast<uint32_t>(O.vidx[nv - 1] + 1)
+      const uint32_t adv = total > kOmpMaxValid ? static_cast<uint32_t>(D.vidx[nv - 1] + 1)
                                                 : static_cast<uint32_t>(kOmpPass);
       O.att0 += adv;
       if (!fin && O.att0 >= kMaxPairAttempts) {
@@ -781,11 +810,11 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
   }
   const int len = O.len;
   // species, both environments and partner slots of every event
-  for (int e = tid; e < len; e += kThreads) {
+  for (int e = tid; e < len; e += kOmpThreads) {
     O.ep[e] = occ[O.p[e]];
     O.eq[e] = occ[O.q[e]];
   }
-  for (int i = tid; i < 2 * len * 64; i += kThreads) {
+  for (int i = tid; i < 2 * len * 64; i += kOmpThreads) {
     const int e2 = i >> 6, j = i & 63;
     if (j < kSiteSlots) {
       int dx, dy, dz;
@@ -795,44 +824,52 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
       O.env[e2][j] = v < t.S ? v : 0;  // codes are validated at creation
     }
   }
-  for (int e2 = tid; e2 < 2 * len; e2 += kThreads) {
+  for (int e2 = tid; e2 < 2 * len; e2 += kOmpThreads) {
     const XYZ &a = (e2 & 1) ? O.xq[e2 >> 1] : O.xp[e2 >> 1];
     const XYZ &b = (e2 & 1) ? O.xp[e2 >> 1] : O.xq[e2 >> 1];
     const int dx = MinImage(b.X - a.X, n), dy = MinImage(b.Y - a.Y, n), dz = MinImage(b.Z - a.Z, n);
-    int s = -1;
-    for (int j = 1; j < kSiteSlots && s < 0; ++j) {
+    int sl = -1;
+    for (int j = 1; j < kSiteSlots && sl < 0; ++j) {
       int x, y, z;
       Unpack(L.off[j], x, y, z);
-      if (x == dx && y == dy && z == dz) s = j;
+      if (x == dx && y == dy && z == dz) sl = j;
     }
-    O.slot[e2] = s;
+    O.slot[e2] = sl;
   }
   __syncthreads();
-  // dE of every event: the thread's cluster for both swapped sites, DPP wave sums
-  for (int e = 0; e < len; ++e) {
-    const int ep = O.ep[e], eq = O.eq[e];
-    double v = ClusterDeltaAt(L, cl, O.env[2 * e], ep, eq, O.slot[2 * e]) +
-               ClusterDeltaAt(L, cl, O.env[2 * e + 1], eq, ep, O.slot[2 * e + 1]);
-    v = WaveSum(v);
-    if (lane == 0) O.part[wave][e] = v;
+  {  // dE: thread (chunk, event) adds its chunk of site clusters for both swapped sites
+    const int e = tid % kOmpMaxK, ch = tid / kOmpMaxK;
+    if (e < len) {
+      const int ep = O.ep[e], eq = O.eq[e];
+      const uint8_t *ea = O.env[2 * e], *eb = O.env[2 * e + 1];
+      const int sa = O.slot[2 * e], sb = O.slot[2 * e + 1];
+      double v = 0.0;
+#pragma unroll 4
+      for (int i = 0; i < kOmpChunkLen; ++i) {
+        const SiteCluster q = DecodeSiteCluster(t, L, ch * kOmpChunkLen + i);
+        v += ClusterDeltaAt(L, q, ea, ep, eq, sa) + ClusterDeltaAt(L, q, eb, eq, ep, sb);
+      }
+      O.u.part[ch][e] = v;
+    }
   }
   __syncthreads();
-  for (int e = tid; e < len; e += kThreads) {
-    double s = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) s += O.part[w][e];
-    O.de[e] = s;
+  for (int e = tid; e < len; e += kOmpThreads) {
+    double sum = 0.0;
+    for (int ch = 0; ch < kOmpChunks; ++ch) sum += O.u.part[ch][e];
+    O.de[e] = sum;
     const int64_t o = static_cast<int64_t>(r) * c.omp_k + e;
     c.omp_p[o] = O.p[e];
     c.omp_q[o] = O.q[e];
-    c.omp_de[o] = s;
+    c.omp_de[o] = sum;
   }
   if (tid == 0) O.pos = 0;
   __syncthreads();
 }
 
 template <bool kLdsOcc, bool kLog>
-__global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, uint64_t ntrials,
-                                                         uint64_t trace_base) {
+__global__ __launch_bounds__(kOmpThreads) void CmcOmpKernel(KTables t, CmcDev c,
+                                                            uint64_t ntrials,
+                                                            uint64_t trace_base) {
   __shared__ SiteLds L;
   __shared__ OmpLds O;
   __shared__ uint64_t s_steps;
@@ -846,7 +883,7 @@ __global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, ui
   uint8_t *occ = c.occ + c.off[r];
   if constexpr (kLdsOcc) {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(occ);
-    for (uint64_t i = tid; i < N / 4; i += kThreads) occ_words[i] = src[i];
+    for (uint64_t i = tid; i < N / 4; i += kOmpThreads) occ_words[i] = src[i];
     occ = reinterpret_cast<uint8_t *>(occ_words);
   }
   const uint64_t maxs = c.max_steps[r];
@@ -873,11 +910,10 @@ __global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, ui
     }
   }
   __syncthreads();
-  const SiteCluster cl = DecodeSiteCluster(t, L, tid);
   // a round left by the previous launch: its pairs and dE, species and coordinates again (its
   // untried sites are untouched by the swaps tried so far)
   if (O.pos < O.len) {
-    for (int e = tid; e < O.len; e += kThreads) {
+    for (int e = tid; e < O.len; e += kOmpThreads) {
       const int64_t o = static_cast<int64_t>(r) * K + e;
       O.p[e] = c.omp_p[o];
       O.q[e] = c.omp_q[o];
@@ -894,9 +930,13 @@ __global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, ui
   };
   uint64_t it = 0;
   while (it < ntrials) {
-    if (O.pos == O.len) {
-      if (s_steps > maxs) break;
-      BuildOmpRound(t, c, L, O, cl, r, s_steps, occ, N, n, K);
+    // every thread reads the round state before BuildOmpRound's first lane rewrites it
+    const bool need = O.pos == O.len;
+    const bool stop = need && s_steps > maxs;
+    __syncthreads();
+    if (stop) break;
+    if (need) {
+      BuildOmpRound(t, c, L, O, r, s_steps, occ, N, n, K);
       if (O.error) break;
     }
     const int pos = O.pos;
@@ -904,7 +944,7 @@ __global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, ui
     const int m = static_cast<uint64_t>(O.len - pos) < left ? O.len - pos : static_cast<int>(left);
     const uint64_t s0 = s_steps;
     // Metropolis tests of the round's next m events, in parallel
-    for (int e = pos + tid; e < pos + m; e += kThreads) {
+    for (int e = pos + tid; e < pos + m; e += kOmpThreads) {
       const uint64_t step = s0 + static_cast<uint64_t>(e - pos);
       const double de = O.de[e];
       bool a = de < 0;
@@ -952,7 +992,7 @@ __global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, ui
       O.pos = pos + m;
     }
     // the accepted swaps (disjoint sites)
-    for (int e = pos + tid; e < pos + m; e += kThreads) {
+    for (int e = pos + tid; e < pos + m; e += kOmpThreads) {
       if (!O.acc[e]) continue;
       if (c.atom) {
         const int64_t o = c.off[r];
@@ -968,7 +1008,7 @@ __global__ __launch_bounds__(kThreads) void CmcOmpKernel(KTables t, CmcDev c, ui
   if constexpr (kLdsOcc) {
     __syncthreads();
     uint32_t *dst = reinterpret_cast<uint32_t *>(c.occ + c.off[r]);
-    for (uint64_t i = tid; i < N / 4; i += kThreads) dst[i] = occ_words[i];
+    for (uint64_t i = tid; i < N / 4; i += kOmpThreads) dst[i] = occ_words[i];
   }
   if (tid == 0) {
     if (O.error) atomicOr(c.error, 2);
@@ -1239,7 +1279,7 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
     stat_omp = std::max(fc.sharedSizeBytes, fd.sharedSizeBytes);
   }
   const size_t dyn = static_cast<size_t>(c.max_sites);
-  auto launch = [&](auto kernel_lds, auto kernel_hbm, size_t st) {
+  auto launch = [&](auto kernel_lds, auto kernel_hbm, size_t st, int threads) {
     if (c.lds_occupancy && dyn % 4 == 0 && st + dyn <= lds_block) {
       if (dyn > 65536 - st) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel_lds),
@@ -1247,17 +1287,18 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
                                                  static_cast<int>(dyn));
         if (e != hipSuccess) return e;
       }
-      hipLaunchKernelGGL(kernel_lds, dim3(c.R), dim3(kThreads), dyn, s, t, c, trials, trace_base);
+      hipLaunchKernelGGL(kernel_lds, dim3(c.R), dim3(threads), dyn, s, t, c, trials, trace_base);
     } else {
-      hipLaunchKernelGGL(kernel_hbm, dim3(c.R), dim3(kThreads), 0, s, t, c, trials, trace_base);
+      hipLaunchKernelGGL(kernel_hbm, dim3(c.R), dim3(threads), 0, s, t, c, trials, trace_base);
     }
     return hipGetLastError();
   };
   if (c.omp_k > 0)
-    return c.log_every ? launch(CmcOmpKernel<true, true>, CmcOmpKernel<false, true>, stat_omp)
-                       : launch(CmcOmpKernel<true, false>, CmcOmpKernel<false, false>, stat_omp);
-  return c.log_every ? launch(CmcKernel<true, true>, CmcKernel<false, true>, stat)
-                     : launch(CmcKernel<true, false>, CmcKernel<false, false>, stat);
+    return c.log_every
+               ? launch(CmcOmpKernel<true, true>, CmcOmpKernel<false, true>, stat_omp, kOmpThreads)
+               : launch(CmcOmpKernel<true, false>, CmcOmpKernel<false, false>, stat_omp, kOmpThreads);
+  return c.log_every ? launch(CmcKernel<true, true>, CmcKernel<false, true>, stat, kThreads)
+                     : launch(CmcKernel<true, false>, CmcKernel<false, false>, stat, kThreads);
 }
 
 hipError_t LaunchCmcAverage(const CmcDev &c, hipStream_t s) {
