@@ -689,6 +689,7 @@ struct OmpLds {
     double part[kOmpChunks][kOmpMaxK];  // dE phase
   } u;
   int wave_valid[kOmpThreads / 64];
+  int8_t slot_lut[125];  // env slot of offset (dx, dy, dz) in [-2, 2]^3, -1 beyond the 3rd shell
   // round control
   int len, pos, nch, ct, finished, error;
   uint32_t att0;
@@ -769,42 +770,100 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
       }
     }
     __syncthreads();
-    if (tid == 0) {  // the reference's draw loop over the pass's valid attempts, in order
-      uint64_t tlo = 0, thi = 0;  // the pass's attempts taken so far
-      int ct = O.ct, len = O.len, nch = O.nch;
+    if (wave == 0) {
+      // The reference's draw loop over the pass's valid attempts, in order: an attempt is taken
+      // unless it meets an earlier pass's event or conflicts with an attempt taken before it.
+      // Lane l decides attempts l and l + 64 as soon as every earlier attempt it conflicts with
+      // is decided (ballots; the chains of conflicts are short), then the draw count and K cut
+      // the taken set (a wave-uniform scan), and the taken attempts become the next events.
+      const int v0 = lane, v1 = lane + 64;
+      const uint64_t c0 = v0 < nv ? D.conf[v0][0] | (static_cast<uint64_t>(D.conf[v0][1]) << 32) : 0;
+      const uint64_t c1lo = v1 < nv ? D.conf[v1][0] | (static_cast<uint64_t>(D.conf[v1][1]) << 32) : 0;
+      const uint64_t c1hi = v1 < nv ? D.conf[v1][2] | (static_cast<uint64_t>(D.conf[v1][3]) << 32) : 0;
+      bool d0 = v0 >= nv || D.excl[v0] != 0, d1 = v1 >= nv || D.excl[v1] != 0;
+      bool a0 = false, a1 = false;
+      uint64_t dlo = __ballot(d0), dhi = __ballot(d1), alo = 0, ahi = 0;
+      while (~dlo | ~dhi) {
+        if (!d0 && !(c0 & ~dlo)) {
+          d0 = true;
+          a0 = !(c0 & alo);
+        }
+        if (!d1 && !(c1lo & ~dlo) && !(c1hi & ~dhi)) {
+          d1 = true;
+          a1 = !(c1lo & alo) && !(c1hi & ahi);
+        }
+        dlo = __ballot(d0);
+        dhi = __ballot(d1);
+        alo = __ballot(a0);
+        ahi = __ballot(a1);
+      }
+      // draw count and K (wave-uniform): draw v of the current event has count base + v - last
+      const int len0 = O.len, nch0 = O.nch;
+      int base = O.ct, last = -1, len = len0;
       bool fin = false;
-      for (int v = 0; v < nv && !fin; ++v) {
-        if (++ct == 50) {
+      uint64_t tlo = 0, thi = 0, rlo = alo, rhi = ahi;
+      while (true) {
+        const int p = rlo ? __builtin_ctzll(rlo) : (rhi ? 64 + __builtin_ctzll(rhi) : nv);
+        if (p >= nv) {  // no further taken attempt in this pass
+          if (base + (nv - 1 - last) >= 50) fin = true;
+          else base += nv - 1 - last;
+          break;
+        }
+        if (base + (p - last) >= 50) {  // the 50th draw of the event comes first
           fin = true;
           break;
         }
-        const uint64_t clo = D.conf[v][0] | (static_cast<uint64_t>(D.conf[v][1]) << 32);
-        const uint64_t chi = D.conf[v][2] | (static_cast<uint64_t>(D.conf[v][3]) << 32);
-        if (D.excl[v] || (clo & tlo) || (chi & thi)) continue;
-        if (v < 64) tlo |= 1ull << v;
-        else thi |= 1ull << (v - 64);
-        O.p[len] = D.va[v];
-        O.q[len] = D.vb[v];
-        O.xp[len] = D.vxa[v];
-        O.xq[len] = D.vxb[v];
-        O.ch[nch++] = D.vxa[v];
-        O.ch[nch++] = D.vxb[v];
+        if (p < 64) {
+          tlo |= 1ull << p;
+          rlo &= rlo - 1;
+        } else {
+          thi |= 1ull << (p - 64);
+          rhi &= rhi - 1;
+        }
         ++len;
-        ct = 0;
-        if (len == K) fin = true;
+        base = 0;
+        last = p;
+        if (len == K) {
+          fin = true;
+          break;
+        }
       }
-      O.ct = ct;
-      O.len = len;
-      O.nch = nch;
-      // the next pass starts after the last attempt looked at
-      const uint32_t adv = total > kOmpMaxValid ? static_cast<uint32_t>(D.vidx[nv - 1] + 1)
-                                                : static_cast<uint32_t>(kOmpPass);
-      O.att0 += adv;
-      if (!fin && O.att0 >= kMaxPairAttempts) {
-        O.error = 1;
-        fin = true;
+      const bool t0 = (tlo >> v0) & 1, t1 = (thi >> v0) & 1;
+      if (t0 || t1) {
+        const int v = t0 ? v0 : v1;
+        const int e = len0 + (t0 ? __builtin_popcountll(tlo & ((1ull << v0) - 1))
+                                 : __builtin_popcountll(tlo) +
+                                       __builtin_popcountll(thi & ((1ull << v0) - 1)));
+        O.p[e] = D.va[v];
+        O.q[e] = D.vb[v];
+        O.xp[e] = D.vxa[v];
+        O.xq[e] = D.vxb[v];
+        O.ch[nch0 + 2 * (e - len0)] = D.vxa[v];
+        O.ch[nch0 + 2 * (e - len0) + 1] = D.vxb[v];
       }
-      O.finished = fin ? 1 : 0;
+      if (t0 && t1) {  // the lane's second taken attempt
+        const int e = len0 + __builtin_popcountll(tlo) + __builtin_popcountll(thi & ((1ull << v0) - 1));
+        O.p[e] = D.va[v1];
+        O.q[e] = D.vb[v1];
+        O.xp[e] = D.vxa[v1];
+        O.xq[e] = D.vxb[v1];
+        O.ch[nch0 + 2 * (e - len0)] = D.vxa[v1];
+        O.ch[nch0 + 2 * (e - len0) + 1] = D.vxb[v1];
+      }
+      if (lane == 0) {
+        O.ct = base;
+        O.len = len;
+        O.nch = nch0 + 2 * (len - len0);
+        // the next pass starts after the last attempt looked at
+        const uint32_t adv = total > kOmpMaxValid ? static_cast<uint32_t>(D.vidx[nv - 1] + 1)
+                                                  : static_cast<uint32_t>(kOmpPass);
+        O.att0 += adv;
+        if (!fin && O.att0 >= kMaxPairAttempts) {
+          O.error = 1;
+          fin = true;
+        }
+        O.finished = fin ? 1 : 0;
+      }
     }
     __syncthreads();
   }
@@ -824,17 +883,12 @@ __device__ void BuildOmpRound(const KTables &t, const CmcDev &c, SiteLds &L, Omp
       O.env[e2][j] = v < t.S ? v : 0;  // codes are validated at creation
     }
   }
-  for (int e2 = tid; e2 < 2 * len; e2 += kOmpThreads) {
+  for (int e2 = tid; e2 < 2 * len; e2 += kOmpThreads) {  // partner slots
     const XYZ &a = (e2 & 1) ? O.xq[e2 >> 1] : O.xp[e2 >> 1];
     const XYZ &b = (e2 & 1) ? O.xp[e2 >> 1] : O.xq[e2 >> 1];
     const int dx = MinImage(b.X - a.X, n), dy = MinImage(b.Y - a.Y, n), dz = MinImage(b.Z - a.Z, n);
-    int sl = -1;
-    for (int j = 1; j < kSiteSlots && sl < 0; ++j) {
-      int x, y, z;
-      Unpack(L.off[j], x, y, z);
-      if (x == dx && y == dy && z == dz) sl = j;
-    }
-    O.slot[e2] = sl;
+    const bool in = dx >= -2 && dx <= 2 && dy >= -2 && dy <= 2 && dz >= -2 && dz <= 2;
+    O.slot[e2] = in ? O.slot_lut[(dx + 2) * 25 + (dy + 2) * 5 + (dz + 2)] : -1;
   }
   __syncthreads();
   {  // dE: thread (chunk, event) adds its chunk of site clusters for both swapped sites
@@ -890,6 +944,13 @@ __global__ __launch_bounds__(kOmpThreads) void CmcOmpKernel(KTables t, CmcDev c,
   uint64_t acc = 0, log_i = 0, ring_i = 0, ring_at = 0;
   double energy = 0, abse = 0;
   LogClock clk;
+  if (tid < 125) O.slot_lut[tid] = -1;
+  __syncthreads();
+  if (tid >= 1 && tid < kSiteSlots) {  // the first slot of each offset (slot 0: the site itself)
+    int x, y, z;
+    Unpack(L.off[tid], x, y, z);
+    O.slot_lut[(x + 2) * 25 + (y + 2) * 5 + (z + 2)] = static_cast<int8_t>(tid);
+  }
   if (tid == 0) {
     s_steps = c.steps[r];
     acc = c.accepted[r];
@@ -961,6 +1022,7 @@ __global__ __launch_bounds__(kOmpThreads) void CmcOmpKernel(KTables t, CmcDev c,
     }
     __syncthreads();
     if (tid == 0) {  // energies in trial order, with the averaging ring and log records
+#pragma unroll 4
       for (int e = pos; e < pos + m; ++e) {
         const uint64_t step = s0 + static_cast<uint64_t>(e - pos);
         if constexpr (kLog) {
