@@ -275,3 +275,42 @@ def test_early_stop_matches_oracle(torch, coeffs, form):
         else:
             assert int(esc["site_before"][r]) == -1
     assert escaped >= 3, "the start configurations are built to escape within the run"
+
+
+@pytest.mark.parametrize("name,kind,record", [("e0_3n", 1, False), ("q_3n", 2, True)])
+def test_occupancy_window_long_trajectories(torch, coeffs, name, kind, record):
+    """The dense kernel's LDS occupancy window (boxes of at least 8 cells): 1500 steps on boxes of
+    8 cells (the window spans the whole box along Z), 9 and 12 cells, with the window reloaded
+    many times; with atom tracking and the log compiled in (record) or not."""
+    import latticemontecarlo_amd as L
+    sizes = [8, 8, 9, 12]
+    steps = 1500
+    opts = {}
+    print(f"window {name} record={record}", flush=True)
+    if not record:
+        gst, gtr, gocc, refs = _run_kmc_pair(coeffs, name, kind, sizes, len(sizes), steps,
+                                             np.array([450.0, 700.0, 550.0, 600.0]), form=0)
+        _compare_kmc(gst, gtr, gocc, refs, steps)
+        return
+    import oracle as O
+    path, els = coeffs[name]
+    gb = L.Model(path, els, kind)
+    species = gb.species
+    from conftest import random_alloc
+    occs = [random_alloc(n, species, 0.06, seed=800 + r) for r, n in enumerate(sizes)]
+    drv = L.KineticMcFirst(gb, None, sizes, occs, 600.0, 77, maximum_steps=10 ** 9)
+    drv.set_formulation(0)
+    l2a = [np.arange(4 * n ** 3) for n in sizes]
+    drv.set_atoms(l2a)
+    drv.set_log(50, capacity=1000)
+    drv.set_trace(steps)
+    drv.run(steps)
+    gtr, gocc = drv.trace(), drv.occupancy()
+    for r, n in enumerate(sizes):
+        lat = O.Lattice(n)
+        ob = O.Model(lat, path, els, kind)
+        k = O.Kmc(ob, None, lat, occs[r], 600.0, 77, replica=r)
+        tr = k.run(steps, trace=True)
+        assert np.array_equal(gtr["sel"][r], tr["sel"].astype(np.int64)), f"replica {r}"
+        np.testing.assert_allclose(gtr["ea"][r], tr["ea"], rtol=0, atol=TOL_E)
+        assert np.array_equal(gocc[r], k.occupancy())
