@@ -2100,7 +2100,8 @@ struct lmc_surrogate {
   lmc::MlpLayout lay{};
   double mu[2] = {0.0, 0.0}, sigma[2] = {0.0, 0.0};
   float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
-  uint16_t *w1t = nullptr, *w2t = nullptr, *w2k = nullptr, *w3 = nullptr;
+  uint16_t *w1t = nullptr, *w2t = nullptr, *w2k = nullptr, *w3 = nullptr, *w1f = nullptr,
+           *w2f = nullptr;
   uint64_t adam_t = 0;
   ~lmc_surrogate() {
     DevFree(master);
@@ -2110,6 +2111,8 @@ struct lmc_surrogate {
     DevFree(w2t);
     DevFree(w2k);
     DevFree(w3);
+    DevFree(w1f);
+    DevFree(w2f);
   }
   lmc::MlpDev View() const {
     lmc::MlpDev v{};
@@ -2123,6 +2126,8 @@ struct lmc_surrogate {
     v.w2t = w2t;
     v.w2k = w2k;
     v.w3 = w3;
+    v.w1f = w1f;
+    v.w2f = w2f;
     return v;
   }
   lmc::MlpMut Mut() {
@@ -2138,6 +2143,8 @@ struct lmc_surrogate {
     v.w2t = w2t;
     v.w2k = w2k;
     v.w3 = w3;
+    v.w1f = w1f;
+    v.w2f = w2f;
     return v;
   }
   void RefreshCopies() {  // bf16 copies from the master parameters
@@ -2206,6 +2213,8 @@ int lmc_surrogate_create(const lmc_model *barrier, int32_t n_members, uint64_t s
     s->w2t = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kHidden);
     s->w2k = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kHidden);
     s->w3 = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kMlpOut);
+    s->w1f = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * s->Kp);
+    s->w2f = DevAlloc<uint16_t>(static_cast<size_t>(s->M) * lmc::kHidden * lmc::kHidden);
     s->RefreshCopies();
     *out = s.release();
     return LMC_OK;

@@ -61,6 +61,10 @@ struct MlpDev {
   const uint16_t *w2t;      // [M][128][128] bf16 of W2 as (n, k)
   const uint16_t *w2k;      // [M][128][128] bf16 of W2 as (k, n)
   const uint16_t *w3;       // [M][128][2] bf16
+  // the same W1 and W2 in the inference kernel's fragment order, one 16-byte fragment per lane
+  // and 1 KB per (unit tile, k-step): w1f [M][8][Kp/32][64][8], w2f [M][8][4][64][8] (W2 in
+  // the chained k order of the transposed first layer's accumulators)
+  const uint16_t *w1f, *w2f;
 };
 
 // Round to nearest even (finite inputs).

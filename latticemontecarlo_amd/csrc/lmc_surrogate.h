@@ -14,7 +14,7 @@ struct MlpMut {
   int M, C, Kp;
   MlpLayout lay;
   float *master, *adam_m, *adam_v;
-  uint16_t *w1t, *w2t, *w2k, *w3;
+  uint16_t *w1t, *w2t, *w2k, *w3, *w1f, *w2f;
 };
 
 struct AdamStep {

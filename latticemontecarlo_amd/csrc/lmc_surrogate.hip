@@ -325,10 +325,17 @@ __global__ void MlpReduceAdamKernel(MlpMut w, const float *grad_part, int nparts
   if (p < L.ob1) {
     const int k = p / kHidden, nn = p - k * kHidden;
     w.w1t[(static_cast<size_t>(m) * kHidden + nn) * L.Kp + k] = b;
+    // fragment order: unit tile nn / 16, k-step k / 32, lane (k % 32 / 8) * 16 + nn % 16
+    const int ks = k >> 5, kw = k & 31, lane = (kw >> 3) * 16 + (nn & 15);
+    w.w1f[((static_cast<size_t>(m) * 8 + (nn >> 4)) * (L.Kp >> 5) + ks) * 512 + lane * 8 + (kw & 7)] = b;
   } else if (p >= L.oW2 && p < L.ob2) {
     const int q = p - L.oW2, k = q / kHidden, nn = q - k * kHidden;
     w.w2t[(static_cast<size_t>(m) * kHidden + nn) * kHidden + k] = b;
     w.w2k[(static_cast<size_t>(m) * kHidden + k) * kHidden + nn] = b;
+    // chained order: input k = 32j + 4grp + e (e < 4) or 32j + 16 + 4grp + (e - 4)
+    const int j = k >> 5, kw = k & 31, hi = kw >> 4, grp = (kw & 15) >> 2;
+    const int e = 4 * hi + (kw & 3), lane = grp * 16 + (nn & 15);
+    w.w2f[((static_cast<size_t>(m) * 8 + (nn >> 4)) * 4 + j) * 512 + lane * 8 + e] = b;
   } else if (p >= L.oW3 && p < L.ob3) {
     w.w3[static_cast<size_t>(m) * kHidden * kMlpOut + (p - L.oW3)] = b;
   }

@@ -1,19 +1,23 @@
 // gfx950 persistent KineticMcFirst kernel with the surrogate ensemble standing in for the
 // jump evaluation, and its stateless counterpart (one evaluation per replica).
 //
-// Work mapping: one 512-thread workgroup holds kG = 4 replicas (a 128-thread block each; two
-// waves per SIMD leave 256 registers per lane for the ensemble). Per step:
+// Work mapping: one workgroup holds kG = 4 replicas, kSub threads each (a block). Per step:
 //   1. gather    : each block gathers its replica's 12 jump environments into LDS;
-//   2. ensemble  : the 8 waves evaluate the 4 members on the 48 event rows; wave (m, h) owns
-//                  member m's columns 64h .. 64h+63 of both hidden layers, so every weight
-//                  fragment is fetched once per workgroup step and used for 3 row tiles;
+//   2. ensemble  : wave w evaluates member w % 4 on the 16-row tiles (12 events + 4 unused rows)
+//                  of kG / (waves / 4) replicas. Layer 1 is computed transposed (hidden units as
+//                  rows, events as columns), so its accumulators, rounded to bf16, are already the
+//                  second layer's B fragments in a permuted k order; the second layer's W2
+//                  fragments are loaded in that order. Hidden activations never leave registers;
+//                  W1 and W2 fragments are prefetched one 16-unit tile ahead;
 //   3. events    : per event, the member mean of (E_kra, dE) gives Ea = max(0, E_kra + dE/2);
 //                  an event falls back to the cluster expansion when its environment is out of
-//                  distribution (a vacancy off the endpoints) or when the members disagree by
-//                  more than the threshold on an event whose rate is not negligible (lowest
-//                  member barrier within `window` kT of the replica's lowest barrier);
-//   4. fallback  : the flagged events of each replica are summed exactly, with all 128 lanes of
-//                  its block spread over them;
+//                  distribution (a vacancy off the endpoints, more solutes than any training row)
+//                  or when the members disagree by more than the threshold (with a relevance
+//                  window: only on events whose lowest member barrier is within `window` kT of
+//                  the replica's lowest barrier);
+//   4. fallback  : the flagged events of all kG replicas are spread over the whole workgroup,
+//                  an aligned power-of-two lane group per event; each group reduces its cluster
+//                  sums by lane exchanges and its first lane finishes the event;
 //   5. select    : ordered cumulative probabilities, Philox draw, residence time (as the plain
 //                  kernel);
 //   6. exact dE  : the selected jump's dE is recomputed from the state clusters when the
@@ -28,151 +32,200 @@
 namespace lmc {
 namespace {
 
-constexpr int kG = 4;                // replicas per workgroup
-constexpr int kSub = 128;            // threads per replica
-constexpr int kBlock = kG * kSub;
-constexpr int kRowsS = 12 * kG;      // event rows per workgroup step
-constexpr int kMT = kRowsS / 16;     // 16-row tiles
+constexpr int kG = 4;        // replicas per workgroup
 constexpr int kMembers = 4;
+constexpr int kTile = 16;    // rows of a replica's tile: its 12 events and 4 unused rows
 
-constexpr int kH1Stride = kHidden + 8;  // bf16 per h1 row: 272 B, so 16 rows hit distinct banks
+template <int kSub>
+struct Shape {
+  static constexpr int kBlock = kG * kSub;
+  static constexpr int kWaves = kBlock / 64;
+  static constexpr int kQ = kWaves / kMembers;  // waves per member
+  static constexpr int kTPW = kG / kQ;          // replica tiles per wave
+  static constexpr int kSubWaves = kSub / 64;
+  static_assert(kQ * kTPW == kG && kSubWaves * 64 == kSub, "whole tiles and waves");
+};
+
+template <int C>
+struct MlpDims {
+  static constexpr int Kp = (kEnvUsed * C + 31) / 32 * 32;
+  static constexpr int KS = Kp / 32;
+};
 
 struct SurLds {
-  uint16_t h1[kMembers][kRowsS][kH1Stride];
-  float yp[kMembers][2][kRowsS][kMlpOut];
-  uint8_t none[kEnvRow];  // input row of an absent replica (no channel matches)
-  double dsum[kG][2];     // exact dE: per-wave sums
+  float b1[kMembers][kHidden], b2[kMembers][kHidden];  // biases of the hidden layers
+  uint32_t w3[kMembers][kHidden];                      // bf16 W3 row pairs (out 0 | out 1 << 16)
+  float yp[kMembers][kG * kTile][kMlpOut];  // member outputs, normalised, without b3
+  uint8_t none[kEnvRow];                    // input row of an absent replica (no channel matches)
+  double dsum[kG][4];                       // exact dE: per-wave sums
   int fbmask[kG], sel[kG];
   double dt[kG];
 };
 
-// First-layer weight fragments of wave (m, h) for one of its 4 column tiles (KS K-steps). The
-// first tile is loaded before the gather so that its latency overlaps it; each further tile is
-// loaded while the previous one is multiplied (two tiles live instead of four).
-template <int C>
-struct TileWeights {
-  static constexpr int KS = (kEnvUsed * C + 31) / 32;
-  u32x4 b1[KS];
+// Per-lane element offset of the weight fragments (8 bf16 per lane), made opaque once per step
+// so that the compiler recomputes each fragment's address from a uniform base (cheap) instead of
+// keeping every address of the persistent loop live in registers.
+struct LaneOffsets {
+  uint32_t f;
 };
-
 template <int C>
-__device__ __forceinline__ void LoadTileWeights(const SurDev &sd, int nt, TileWeights<C> &tw) {
-  constexpr int Kp = (kEnvUsed * C + 31) / 32 * 32, KS = Kp / 32;
-  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15, grp = lane >> 4;
-  const int m = wv >> 1, q = wv & 1;
-  const uint16_t *W1t = sd.w.w1t + static_cast<size_t>(m) * kHidden * Kp;
-  const int n = 64 * q + 16 * nt + col;
-#pragma unroll
-  for (int s = 0; s < KS; ++s) tw.b1[s] = Load8(W1t + static_cast<size_t>(n) * Kp + 32 * s + 8 * grp);
+__device__ __forceinline__ LaneOffsets MakeLaneOffsets(int lane) {
+  LaneOffsets o{static_cast<uint32_t>(8 * lane)};
+  asm volatile("" : "+v"(o.f));
+  return o;
 }
 
-// Phase 2: ensemble outputs (normalised) of the 48 rows into S.yp (all 8 waves).
+// A fragments of W1 (member base W1m of the fragment-ordered copy) for hidden units 16h ..
+// 16h+15 (rows): one contiguous 1 KB load per k-step.
 template <int C>
-__device__ void EnsembleForward(const SurDev &sd, const TileWeights<C> &first, const JumpLds *Ls,
-                                const bool *act, SurLds &S) {
-  constexpr int KS = TileWeights<C>::KS;
-  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15, grp = lane >> 4;
-  const int m = wv >> 1, q = wv & 1;
+__device__ __forceinline__ void LoadW1(const uint16_t *W1m, int h, uint32_t lo,
+                                       u32x4 (&a)[MlpDims<C>::KS]) {
+  const uint16_t *p = W1m + h * MlpDims<C>::KS * 512;
+#pragma unroll
+  for (int s = 0; s < MlpDims<C>::KS; ++s) a[s] = Load8(p + 512 * s + lo);
+}
+
+// A fragments of W2^T (member base W2m of the fragment-ordered copy) for output units 16h ..
+// 16h+15 (rows), in the chained k order: at k-step j lane group grp holds the inputs
+// 32j + 4 grp + (0..3) and 32j + 16 + 4 grp + (0..3), which is where layer 1's accumulators of
+// unit tiles 2j and 2j + 1 sit.
+__device__ __forceinline__ void LoadW2(const uint16_t *W2m, int h, uint32_t lo, u32x4 (&a)[4]) {
+  const uint16_t *p = W2m + h * 4 * 512;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) a[j] = Load8(p + 512 * j + lo);
+}
+
+// Weight ring of a wave: kDepth slots of KS fragments. Slot h % kDepth holds W1 unit tile h
+// (h < 8), then W2 unit tile h - 8 in its first four fragments; a slot is refilled as soon as
+// its tile has been multiplied, so up to kDepth tiles of weights are in flight.
+template <int C>
+struct Ring {
+  static constexpr int KS = MlpDims<C>::KS;
+  static constexpr int kDepth = KS <= 4 ? 8 : 4;  // 128 / 96 registers for C = 2 / 3
+  u32x4 v[kDepth][KS];
+};
+
+// Issues weight tile i (0..7: W1, 8..15: W2) into its ring slot.
+template <int C>
+__device__ __forceinline__ void RingIssue(Ring<C> &R, int i, const uint16_t *W1m,
+                                          const uint16_t *W2m, const LaneOffsets &lo) {
+  constexpr int D = Ring<C>::kDepth;
+  if (i < 8) {
+    LoadW1<C>(W1m, i, lo.f, R.v[i % D]);
+  } else {
+    u32x4 a[4];
+    LoadW2(W2m, i - 8, lo.f, a);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) R.v[i % D][j] = a[j];
+  }
+}
+
+// Member of this wave (uniform). Workgroups that run on the same XCD (every eighth block index)
+// start on different members, so that their weight streams do not read the same lines at once.
+__device__ __forceinline__ int WaveMember() {
+  return (__builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6) +
+          static_cast<int>(blockIdx.x >> 3)) & 3;
+}
+
+// The replica of tile slot g of this workgroup takes part in the step (uniform: LDS state
+// written before the previous step's last barrier).
+__device__ __forceinline__ bool Active(const KmcDev &k, const ReplicaLds *Qs, int g) {
+  return static_cast<int>(blockIdx.x) * kG + g < k.R && !Qs[g].done;
+}
+
+// Phase 2: member outputs (normalised, without b3) of every replica's 12 events into S.yp.
+// The ring R holds the wave's first weight tiles, issued before the gather barrier.
+template <int C, int kSub>
+__device__ void EnsembleForward(const SurDev &sd, const KmcDev &k, const JumpLds *Ls,
+                                const ReplicaLds *Qs, const LaneOffsets &lo, Ring<C> &R,
+                                SurLds &S) {
+  using Sh = Shape<kSub>;
+  constexpr int KS = MlpDims<C>::KS, Kp = MlpDims<C>::Kp, TPW = Sh::kTPW, D = Ring<C>::kDepth;
+  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int m = WaveMember();
+  const int q = __builtin_amdgcn_readfirstlane(tid >> 6) >> 2;
   const MlpDev &w = sd.w;
-  const float *par = w.master + static_cast<size_t>(m) * w.lay.P;
-  const uint16_t *W2t = w.w2t + static_cast<size_t>(m) * kHidden * kHidden;
-  const uint16_t *W3 = w.w3 + static_cast<size_t>(m) * kHidden * kMlpOut;
+  const uint16_t *W1m = w.w1f + static_cast<size_t>(m) * kHidden * Kp;
+  const uint16_t *W2m = w.w2f + static_cast<size_t>(m) * kHidden * kHidden;
   const int chan[4] = {w.chan[0], w.chan[1], w.chan[2], w.chan[3]};
-  u32x4 a[kMT][KS];  // one-hot fragments of the wave's rows, built once per step
+  // B fragments of the one-hot inputs: column col = event col of the tile's replica
+  u32x4 x[TPW][KS];
 #pragma unroll
-  for (int mt = 0; mt < kMT; ++mt) {
-    const int row = 16 * mt + col, g = row / 12, k = row - 12 * g;
-    const uint8_t *e = act[g] ? Ls[g].env[k] : S.none;
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int g = q * TPW + tt;
+    const uint8_t *e = (col < 12 && Active(k, Qs, g)) ? Ls[g].env[col] : S.none;
 #pragma unroll
-    for (int s = 0; s < KS; ++s) a[mt][s] = OneHotFragment<C>(e, 32 * s + 8 * grp, chan);
+    for (int s = 0; s < KS; ++s) x[tt][s] = OneHotFragment<C>(e, 32 * s + 8 * grp, chan);
   }
-  TileWeights<C> cur = first, next;
+  // layer 1, transposed: z1^T = W1^T x^T; bf16(silu) packed as layer 2's B fragments
+  u32x4 hb[TPW][4];
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const int n = 64 * q + 16 * nt + col;
-    if (nt < 3) LoadTileWeights<C>(sd, nt + 1, next);
-    f32x4 acc[kMT];
+  for (int h = 0; h < 8; ++h) {
+    const f32x4 b = *reinterpret_cast<const f32x4 *>(&S.b1[m][16 * h + 4 * grp]);
 #pragma unroll
-    for (int mt = 0; mt < kMT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tt = 0; tt < TPW; ++tt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
-#pragma unroll
-      for (int mt = 0; mt < kMT; ++mt) acc[mt] = Mfma(a[mt][s], cur.b1[s], acc[mt]);
-    if (nt < 3) cur = next;
-    const float bias = par[w.lay.ob1 + n];
-#pragma unroll
-    for (int mt = 0; mt < kMT; ++mt)
+      for (int s = 0; s < KS; ++s) acc = Mfma(R.v[h % D][s], x[tt][s], acc);
+      uint32_t v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float z = acc[mt][i] + bias;
-        S.h1[m][16 * mt + 4 * grp + i][n] = static_cast<uint16_t>(Bf16Bits(z * Sigmoid(z)));
+        const float z = acc[i] + b[i];
+        v[i] = Bf16Bits(z * Sigmoid(z));
       }
+      hb[tt][h >> 1][(h & 1) * 2] = v[0] | (v[1] << 16);
+      hb[tt][h >> 1][(h & 1) * 2 + 1] = v[2] | (v[3] << 16);
+    }
+    RingIssue<C>(R, h + D, W1m, W2m, lo);
   }
-  u32x4 b2[4][4];  // second-layer fragments, in flight across the barrier
+  // layer 2 (z2^T = W2^T h1^T) and the output layer's partial sums over this lane's 4 units
+  float y0[TPW], y1[TPW];
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt)
+  for (int tt = 0; tt < TPW; ++tt) y0[tt] = y1[tt] = 0.f;
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      b2[nt][s] = Load8(W2t + static_cast<size_t>(64 * q + 16 * nt + col) * kHidden + 32 * s + 8 * grp);
-  __syncthreads();
-  // second layer and this wave's share of the output layer: per-lane partial sums over the
-  // wave's 4 column tiles, then one reduction over the 16 lanes of each row group (each level's
-  // 24 independent cross-lane moves are issued together)
-  float p[kMT * 4 * kMlpOut] = {};
+  for (int h = 0; h < 8; ++h) {
+    const f32x4 b = *reinterpret_cast<const f32x4 *>(&S.b2[m][16 * h + 4 * grp]);
+    const u32x4 w3 = *reinterpret_cast<const u32x4 *>(&S.w3[m][16 * h + 4 * grp]);
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    const int n = 64 * q + 16 * nt + col;
-    f32x4 acc[kMT];
+    for (int tt = 0; tt < TPW; ++tt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int mt = 0; mt < kMT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int mt = 0; mt < kMT; ++mt)
-        acc[mt] = Mfma(Load8(&S.h1[m][16 * mt + col][32 * s + 8 * grp]), b2[nt][s], acc[mt]);
-    const float bias = par[w.lay.ob2 + n];
-    const float w30 = Bf16Float(W3[n * kMlpOut]);
-    const float w31 = Bf16Float(W3[n * kMlpOut + 1]);
-#pragma unroll
-    for (int mt = 0; mt < kMT; ++mt)
+      for (int j = 0; j < 4; ++j) acc = Mfma(R.v[(h + 8) % D][j], hb[tt][j], acc);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float z = acc[mt][i] + bias;
-        const float h = Bf16Round(z * Sigmoid(z));
-        p[(mt * 4 + i) * kMlpOut] += h * w30;
-        p[(mt * 4 + i) * kMlpOut + 1] += h * w31;
+        const float z = acc[i] + b[i];
+        const float hh = Bf16Round(z * Sigmoid(z));
+        y0[tt] += hh * Bf16Float(w3[i] & 0xFFFFu);
+        y1[tt] += hh * Bf16Float(w3[i] >> 16);
       }
+    }
+    if (h + 8 + D < 16) RingIssue<C>(R, h + 8 + D, W1m, W2m, lo);
   }
 #pragma unroll
-  for (int off = 1; off < 16; off <<= 1) {
-    float o[kMT * 4 * kMlpOut];
-#pragma unroll
-    for (int j = 0; j < kMT * 4 * kMlpOut; ++j) o[j] = __shfl_xor(p[j], off);
-#pragma unroll
-    for (int j = 0; j < kMT * 4 * kMlpOut; ++j) p[j] += o[j];
+  for (int tt = 0; tt < TPW; ++tt) {
+    y0[tt] += __shfl_xor(y0[tt], 16);
+    y1[tt] += __shfl_xor(y1[tt], 16);
+    y0[tt] += __shfl_xor(y0[tt], 32);
+    y1[tt] += __shfl_xor(y1[tt], 32);
+    if (grp == 0) {
+      float *y = S.yp[m][kTile * (q * TPW + tt) + col];
+      y[0] = y0[tt];
+      y[1] = y1[tt];
+    }
   }
-  if (col == 0)
-#pragma unroll
-    for (int mt = 0; mt < kMT; ++mt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float *y = S.yp[m][q][16 * mt + 4 * grp + i];
-        y[0] = p[(mt * 4 + i) * kMlpOut];
-        y[1] = p[(mt * 4 + i) * kMlpOut + 1];
-      }
 }
 
 // Phase 3 (lanes k < 12 of block g): surrogate events, fallback flags. Returns the flag.
 __device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, const SurLds &S,
                               int g, int k, double beta) {
-  const int row = 12 * g + k;
+  const int row = kTile * g + k;
   double ek[kMembers], dv[kMembers];
   double ek_mean = 0.0, de_mean = 0.0;
 #pragma unroll
   for (int m = 0; m < kMembers; ++m) {
     const float *par = sd.w.master + static_cast<size_t>(m) * sd.w.lay.P + sd.w.lay.ob3;
-    const float y0 = par[0] + (S.yp[m][0][row][0] + S.yp[m][1][row][0]);
-    const float y1 = par[1] + (S.yp[m][0][row][1] + S.yp[m][1][row][1]);
+    const float y0 = par[0] + S.yp[m][row][0];
+    const float y1 = par[1] + S.yp[m][row][1];
     const double a = static_cast<double>(y0) * sd.sigma[0] + sd.mu[0];
     ek[m] = sd.log_ekra ? exp(a) : a;
     dv[m] = static_cast<double>(y1) * sd.sigma[1] + sd.mu[1];
@@ -191,9 +244,13 @@ __device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, co
     dlo = fmin(dlo, dv[m]);
     dhi = fmax(dhi, dv[m]);
   }
-  double emin = ea;  // lowest ensemble barrier of the replica
+  bool relevant = true;
+  if (sd.window > 0.0) {  // lowest ensemble barrier of the replica (lanes 0..11 of the block)
+    double emin = ea;
 #pragma unroll
-  for (int j = 0; j < 12; ++j) emin = fmin(emin, __shfl(ea, j));
+    for (int j = 0; j < 12; ++j) emin = fmin(emin, __shfl(ea, j));
+    relevant = lo < emin + sd.window / beta;
+  }
   int solutes = 0;  // environment slots holding neither the base species nor the vacancy
   {
     const uint32_t *words = reinterpret_cast<const uint32_t *>(L.env[k]);
@@ -208,80 +265,80 @@ __device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, co
     solutes -= (L.env[k][t.v_slot] != sd.base_code) ? 1 : 0;
   }
   const double spread = fmax(hi - lo, dhi - dlo);
-  const bool relevant = sd.window <= 0.0 || lo < emin + sd.window / beta;
   const bool flag = ((L.bad >> k) & 1) || solutes > sd.max_solutes || sd.threshold < 0.0 ||
                     !(spread == spread) || (relevant && spread > sd.threshold);
   if (!flag) StoreEvent(L, k, ea, de_mean, ek_mean, beta);
   return flag;
 }
 
-// Phase 4a (all lanes of block g): exact cluster sums of the flagged events, kSub / nf lanes
-// per event; partials in L.part.
-__device__ void FallbackClusters(const KTables &t, JumpLds &L, const Tabs &T, const SurLds &S,
-                                 int g, int lt) {
-  const int mask = S.fbmask[g];
-  const int nf = __popc(mask);
-  const int le = kSub / nf;
-  const int e = lt / le, lane = lt - e * le;
-  double de = 0.0, la = 0.0, lb = 0.0;
-  if (e < nf) {
-    int mm = mask;
-    for (int i = 0; i < e; ++i) mm &= mm - 1;
-    const int k = __ffs(mm) - 1;
-    const EventRanks &er = L.er[k];
-    const int M = L.env[k][t.n_slot];
-    de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, le);
-    if (t.kind == kE0 || t.kind == kQuartic)
-      la = OrbitRange(BRow(T.b1, t.F1, t, M), er, T.mmm, t.n_mmm, lane, le);
-    if (t.kind == kQuartic)
-      lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2, lane, le);
-  }
-  L.part[lt][0] = de;
-  L.part[lt][1] = la;
-  L.part[lt][2] = lb;
-}
-
-// Phase 4b (lane e < nf of block g): exact event e from its lanes' partials (in lane order).
-__device__ void FallbackFinish(const KTables &t, JumpLds &L, const SurLds &S, int g, int e,
-                               double beta, int32_t *err) {
-  const int mask = S.fbmask[g];
-  const int nf = __popc(mask);
-  if (e >= nf) return;
-  const int le = kSub / nf;
-  int mm = mask;
-  for (int i = 0; i < e; ++i) mm &= mm - 1;
-  const int k = __ffs(mm) - 1;
-  double de = 0.0, la = 0.0, lb = 0.0;
-  for (int i = e * le; i < (e + 1) * le; ++i) {
-    de += L.part[i][0];
-    la += L.part[i][1];
-    lb += L.part[i][2];
-  }
-  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1)) la = __builtin_nan("");
-  double ea, ekra;
-  Barrier(t, L, k, de, la, lb, ea, ekra);
-  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);
-  StoreEvent(L, k, ea, de, ekra, beta);
-}
-
-// Phase 6 (all lanes of block g): exact dE of direction k from the state clusters.
-__device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLds &S, int g, int lt,
-                        int k) {
-  const int M = L.env[k][t.n_slot];
-  double v = StateRange(WRow(t, T, M), L.er[k], T.state, 0, t.n_state, lt, kSub);
+// Phase 4 (all threads): exact cluster expansion of the flagged events of every replica of the
+// workgroup. The nf events get aligned lane groups of s threads (the largest power of two <= 64
+// with s nf <= the workgroup size); a group's partial sums meet by butterfly exchanges and its
+// first lane finishes the event.
+template <int kSub>
+__device__ void FallbackEvents(const KTables &t, JumpLds *Ls, const ReplicaLds *Qs, const Tabs &T,
+                               const SurLds &S, int32_t *err) {
+  constexpr int kBlock = Shape<kSub>::kBlock;
+  const int tid = threadIdx.x;
+  int cnt[kG], nf = 0;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off);
+  for (int g = 0; g < kG; ++g) {
+    cnt[g] = __popc(S.fbmask[g]);
+    nf += cnt[g];
+  }
+  if (nf == 0) return;  // uniform
+  int s = 64;
+  while (s * nf > kBlock) s >>= 1;
+  const int e = tid / s, lane = tid & (s - 1);
+  const bool mine = e < nf;
+  int g = 0, kk = 0;
+  double de = 0.0, la = 0.0, lb = 0.0;
+  if (mine) {
+    int rest = e;
+    while (rest >= cnt[g]) rest -= cnt[g++];
+    int mm = S.fbmask[g];
+    for (int i = 0; i < rest; ++i) mm &= mm - 1;
+    kk = __ffs(mm) - 1;
+    const JumpLds &L = Ls[g];
+    const EventRanks &er = L.er[kk];
+    const int M = L.env[kk][t.n_slot];
+    de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, s);
+    if (t.kind == kE0 || t.kind == kQuartic)
+      la = OrbitRange(BRow(T.b1, t.F1, t, M), er, T.mmm, t.n_mmm, lane, s);
+    if (t.kind == kQuartic) lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2, lane, s);
+  }
+  for (int off = s >> 1; off > 0; off >>= 1) {  // uniform trip count
+    de += __shfl_xor(de, off);
+    la += __shfl_xor(la, off);
+    lb += __shfl_xor(lb, off);
+  }
+  if (!mine || lane != 0) return;
+  JumpLds &L = Ls[g];
+  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> kk) & 1)) la = __builtin_nan("");
+  double ea, ekra;
+  Barrier(t, L, kk, de, la, lb, ea, ekra);
+  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);
+  StoreEvent(L, kk, ea, de, ekra, Qs[g].beta);
+}
+
+// Phase 6 (all lanes of block g): exact dE of direction k from the state clusters; per-wave sums
+// in S.dsum[g].
+template <int kSub>
+__device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLds &S, int g,
+                        int lt, int k) {
+  const int M = L.env[k][t.n_slot];
+  const double v = WaveSum(StateRange(WRow(t, T, M), L.er[k], T.state, 0, t.n_state, lt, kSub));
   if ((lt & 63) == 0) S.dsum[g][lt >> 6] = v;
 }
 
-template <int C, bool kKmc, bool kBL>
-__global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout lay, KmcDev k,
-                                                          SurDev sd, uint64_t nsteps,
-                                                          uint64_t trace_base, SurOut out) {
+template <int C, int kSub, bool kKmc, bool kBL>
+__global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayout lay, KmcDev k,
+                                                             SurDev sd, uint64_t nsteps,
+                                                             uint64_t trace_base, SurOut out) {
+  constexpr int kBlock = Shape<kSub>::kBlock, Kp = MlpDims<C>::Kp;
   __shared__ JumpLds Ls[kG];
   __shared__ ReplicaLds Qs[kG];
   __shared__ SurLds S;
-  __shared__ bool act[kG];
   extern __shared__ char dyn[];
   const int tid = threadIdx.x, g = tid / kSub, lt = tid - g * kSub;
   const int r = blockIdx.x * kG + g;
@@ -292,6 +349,14 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
   uint8_t *occ = live ? k.occ + k.off[r] : nullptr;
   const int n = live ? k.ncell[r] : 0;
   if (tid < kEnvRow) S.none[tid] = 0xFF;
+  for (int i = tid; i < kMembers * kHidden; i += kBlock) {  // constant through the launch
+    const int m = i / kHidden, u = i - m * kHidden;
+    const float *par = sd.w.master + static_cast<size_t>(m) * sd.w.lay.P;
+    S.b1[m][u] = par[sd.w.lay.ob1 + u];
+    S.b2[m][u] = par[sd.w.lay.ob2 + u];
+    const uint16_t *w3 = sd.w.w3 + (static_cast<size_t>(m) * kHidden + u) * kMlpOut;
+    S.w3[m][u] = static_cast<uint32_t>(w3[0]) | (static_cast<uint32_t>(w3[1]) << 16);
+  }
   if (lt == 0) {
     Q.done = 1;
     if (live && kKmc) {
@@ -324,38 +389,50 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
     }
   }
   __syncthreads();
+  const uint16_t *W1m = sd.w.w1f + static_cast<size_t>(WaveMember()) * kHidden * Kp;
+  const uint16_t *W2m = sd.w.w2f + static_cast<size_t>(WaveMember()) * kHidden * kHidden;
   uint64_t n_sur = 0, n_fb = 0;
   for (uint64_t it = 0; it < nsteps; ++it) {
-    if (!__syncthreads_or(live && !Q.done)) break;
-    if (lt == 0) act[g] = live && !Q.done;
-    if (kKmc && k.n_tt > 0 && lt == 0 && live && !Q.done) {
-      Q.T = ScheduleTemperature(k.tt_time, k.tt_temp, k.n_tt, Q.time);
-      Q.beta = 1.0 / kBoltzmann / Q.T;
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < kG; ++j) any = any || Active(k, Qs, j);
+    if (!any) break;  // uniform: every thread reads the same LDS state
+    const bool on = live && !Q.done;
+    if (kKmc && k.n_tt > 0) {  // before the draw lane reads T
+      if (lt == 0 && on) {
+        Q.T = ScheduleTemperature(k.tt_time, k.tt_temp, k.n_tt, Q.time);
+        Q.beta = 1.0 / kBoltzmann / Q.T;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    const bool on = act[g];
+    // lane index made opaque per step: the gather's per-lane offsets, slot indices and predicates
+    // are recomputed each step instead of being held in registers through the ensemble phase
+    int lg = lt;
+    asm volatile("" : "+v"(lg));
     Gathered<kSub> gat;
-    if (on) GatherIssue<kSub>(L, occ, n, VacancyXYZ(Q), lt, gat);
-    TileWeights<C> ww;  // issued after the occupancy loads, consumed after the gather
-    LoadTileWeights<C>(sd, 0, ww);
+    if (on) GatherIssue<kSub>(L, occ, n, VacancyXYZ(Q), lg, gat);
+    const LaneOffsets lo = MakeLaneOffsets<C>(tid & 63);
+    Ring<C> ring;  // the first weight tiles, issued after the occupancy loads
+#pragma unroll
+    for (int i = 0; i < Ring<C>::kDepth; ++i) RingIssue<C>(ring, i, W1m, W2m, lo);
     if (kKmc && on && lt == kSub - 1) DrawStep(L, Q);
-    if (on) GatherStore<kSub>(t, L, k.error, lt, gat);
+    if (on) GatherStore<kSub>(t, L, k.error, lg, gat);
     __syncthreads();
-    EnsembleForward<C>(sd, ww, Ls, act, S);
+    EnsembleForward<C, kSub>(sd, k, Ls, Qs, lo, ring, S);
     __syncthreads();
-    if (on && lt < 12) {
-      const bool f = EnsembleEvent(t, sd, L, S, g, lt, Q.beta);
+    if (lt < 12) {
+      const bool f = on && EnsembleEvent(t, sd, L, S, g, lt, Q.beta);
       const unsigned long long b = __ballot(f) & 0xFFFull;
       if (lt == 0) {
         S.fbmask[g] = static_cast<int>(b);
-        n_fb += __popcll(b);
-        n_sur += 12 - __popcll(b);
+        if (on) {
+          n_fb += __popcll(b);
+          n_sur += 12 - __popcll(b);
+        }
       }
     }
     __syncthreads();
-    if (on && S.fbmask[g]) FallbackClusters(t, L, T, S, g, lt);
-    __syncthreads();
-    if (on && S.fbmask[g] && lt < 12) FallbackFinish(t, L, S, g, lt, Q.beta, k.error);
+    FallbackEvents<kSub>(t, Ls, Qs, T, S, k.error);
     __syncthreads();
     if (!kKmc) {
       if (on && lt < 12) {
@@ -365,11 +442,10 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
         out.de[o] = L.de[lt];
         out.flag[o] = static_cast<int8_t>((S.fbmask[g] >> L.dir[lt]) & 1);
       }
-      if (lt == 0 && live) Q.done = 1;
-      continue;
+      break;  // one evaluation
     }
     if (on && lt < 12) {
-      if (lt == 0) L.bad = 0;
+      if (lt == 0) L.bad = 0;  // read by phases 3 and 4 only
       int sel;
       double dt;
       if (ChooseEvent(L, Q, lt, sel, dt)) {
@@ -380,10 +456,15 @@ __global__ __launch_bounds__(kBlock) void SurrogateKernel(KTables t, DynLayout l
     __syncthreads();
     const int sel = S.sel[g];
     const bool need = on && !((S.fbmask[g] >> L.dir[sel]) & 1);
-    if (need) ExactDe(t, L, T, S, g, lt, L.dir[sel]);
+    if (need) ExactDe<kSub>(t, L, T, S, g, lt, L.dir[sel]);
     __syncthreads();
     if (on && lt == 0) {
-      const double de = need ? S.dsum[g][0] + S.dsum[g][1] : L.de[sel];
+      double de = L.de[sel];
+      if (need) {
+        de = S.dsum[g][0];
+#pragma unroll
+        for (int j = 1; j < Shape<kSub>::kSubWaves; ++j) de += S.dsum[g][j];
+      }
       ApplyEvent(t, k, L, Q, occ, r, trace_base + it, sel, S.dt[g], de);
     }
     __syncthreads();
@@ -465,6 +546,9 @@ hipError_t AllowSurrogateLds(K kernel, size_t bytes) {
   return e;
 }
 
+// Threads per replica: 128 (512-thread workgroups).
+constexpr int kSubUsed = 128;
+
 template <bool kKmc>
 hipError_t LaunchSurrogate(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
                            uint64_t trace_base, const SurOut &out, hipStream_t s) {
@@ -475,16 +559,16 @@ hipError_t LaunchSurrogate(const KTables &t, const KmcDev &k, const SurDev &sd, 
   auto go = [&](auto kernel) {
     hipError_t e = AllowSurrogateLds(kernel, lay.bytes);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), lay.bytes, s, t, lay, k, sd, steps,
+    hipLaunchKernelGGL(kernel, grid, dim3(kG * kSubUsed), lay.bytes, s, t, lay, k, sd, steps,
                        trace_base, out);
     return hipGetLastError();
   };
   if (lay.b1_lds) {
-    if (sd.w.C == 2) return go(SurrogateKernel<2, kKmc, true>);
-    if (sd.w.C == 3) return go(SurrogateKernel<3, kKmc, true>);
+    if (sd.w.C == 2) return go(SurrogateKernel<2, kSubUsed, kKmc, true>);
+    if (sd.w.C == 3) return go(SurrogateKernel<3, kSubUsed, kKmc, true>);
   } else {
-    if (sd.w.C == 2) return go(SurrogateKernel<2, kKmc, false>);
-    if (sd.w.C == 3) return go(SurrogateKernel<3, kKmc, false>);
+    if (sd.w.C == 2) return go(SurrogateKernel<2, kSubUsed, kKmc, false>);
+    if (sd.w.C == 3) return go(SurrogateKernel<3, kSubUsed, kKmc, false>);
   }
   return hipErrorInvalidValue;
 }
@@ -494,11 +578,11 @@ hipError_t LaunchSurrogate(const KTables &t, const KmcDev &k, const SurDev &sd, 
 hipError_t PrepareSurrogateKmc(const KTables &t, int C) {
   const DynLayout lay = PlanSurrogateLds(t);
   if (lay.b1_lds) {
-    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, true, true>, lay.bytes);
-    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, true, true>, lay.bytes);
+    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, kSubUsed, true, true>, lay.bytes);
+    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, kSubUsed, true, true>, lay.bytes);
   } else {
-    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, true, false>, lay.bytes);
-    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, true, false>, lay.bytes);
+    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, kSubUsed, true, false>, lay.bytes);
+    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, kSubUsed, true, false>, lay.bytes);
   }
   return hipErrorInvalidValue;
 }
