@@ -873,7 +873,7 @@ class KineticMcFirstBatch {
   }
   void Simulate() { Run(max_steps_ + 1); }
   // Jump evaluation by `s` (nullptr: the cluster expansion) from the next run on.
-  void SetSurrogate(const pred::Surrogate *s, double threshold = 0.01, double window = 20.0) {
+  void SetSurrogate(const pred::Surrogate *s, double threshold = 0.01, double window = 0.0) {
     lmc::Check(lmc_kmc_set_surrogate(k_.get(), s ? s->handle() : nullptr, threshold, window));
   }
   // (events evaluated, events that fell back to the cluster expansion)

@@ -403,7 +403,7 @@ class KineticMcFirst:
         sizes = 4 * self.ncell.astype(np.int64) ** 3
         return np.split(out, np.cumsum(sizes)[:-1])
 
-    def set_surrogate(self, surrogate, threshold=0.01, window=20.0):
+    def set_surrogate(self, surrogate, threshold=0.01, window=0.0):
         """Evaluate jumps with `surrogate` (None: the cluster expansion) from the next run on."""
         check(lib().lmc_kmc_set_surrogate(self.h, surrogate.h if surrogate else None, threshold,
                                           window))
@@ -700,7 +700,7 @@ class Surrogate:
                                         epochs, batch, lr, seed, ptr(mse, capi._f64p)))
         return mse
 
-    def eval_jumps(self, model, batch, vac, temps, threshold=0.01, window=20.0, stream=None):
+    def eval_jumps(self, model, batch, vac, temps, threshold=0.01, window=0.0, stream=None):
         """The 12 jumps per replica as the surrogate KMC step evaluates them (event order):
         nb, Ea, dE [R, 12] and the fallback flags [R, 12] (1 = exact cluster expansion)."""
         torch = _torch()

@@ -141,7 +141,11 @@ __device__ void EnsembleForward(const SurDev &sd, const KmcDev &k, const JumpLds
                                 SurLds &S) {
   using Sh = Shape<kSub>;
   constexpr int KS = MlpDims<C>::KS, Kp = MlpDims<C>::Kp, TPW = Sh::kTPW, D = Ring<C>::kDepth;
-  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  // with three channels the one-hot slot and channel indices of a lane are divisions by 3: the
+  // lane comes from the per-step opaque offset, so that they are recomputed each step rather
+  // than held through the persistent loop
+  const int tid = threadIdx.x;
+  const int lane = C == 3 ? static_cast<int>(lo.f >> 3) : (tid & 63), col = lane & 15, grp = lane >> 4;
   const int m = WaveMember();
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6) >> 2;
   const MlpDev &w = sd.w;
@@ -271,6 +275,44 @@ __device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, co
   return flag;
 }
 
+// Sums of an event's state list (dE, row wm) and orbit list (barrier logit, row bm) over the
+// lanes lane, lane + kS, ...: four state and four orbit entries per lane at a time (eight
+// independent load chains); entries past a list's end are read clamped and not added.
+template <int kS>
+__device__ __forceinline__ void StateOrbitGroup(const char *wm, const char *bm, const EventRanks &er,
+                                                const uint32_t *sl, int ns, const uint32_t *ol,
+                                                int no, int lane, double &de, double &la) {
+  double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nmax = ns > no ? ns : no;
+  for (int c = lane; c < nmax; c += 4 * kS) {
+    uint32_t ds[4], dq[4];
+    bool vs[4], vq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = c + u * kS;
+      vs[u] = i < ns;
+      vq[u] = i < no;
+      ds[u] = sl[vs[u] ? i : 0];
+      dq[u] = ol[vq[u] ? i : 0];
+    }
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t x1 = R8At(er, ds[u] & 0xFF), x2 = Rn8At(er, (ds[u] >> 8) & 0xFF);
+      const uint32_t y0 = Rn8At(er, dq[u] & 0xFF), y1 = R8At(er, (dq[u] >> 8) & 0xFF);
+      v[u] = *reinterpret_cast<const double *>(wm + (x1 + x2 + (ds[u] >> 16)));
+      v[4 + u] = *reinterpret_cast<const double *>(bm + (y0 + y1 + (dq[u] >> 16)));
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] += vs[u] ? v[u] : 0.0;
+      b[u] += vq[u] ? v[4 + u] : 0.0;
+    }
+  }
+  de = (a[0] + a[1]) + (a[2] + a[3]);
+  la = (b[0] + b[1]) + (b[2] + b[3]);
+}
+
 // Phase 4 (all threads): exact cluster expansion of the flagged events of every replica of the
 // workgroup. The nf events get aligned lane groups of s threads (the largest power of two <= 64
 // with s nf <= the workgroup size); a group's partial sums meet by butterfly exchanges and its
@@ -302,9 +344,19 @@ __device__ void FallbackEvents(const KTables &t, JumpLds *Ls, const ReplicaLds *
     const JumpLds &L = Ls[g];
     const EventRanks &er = L.er[kk];
     const int M = L.env[kk][t.n_slot];
-    de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, s);
-    if (t.kind == kE0 || t.kind == kQuartic)
-      la = OrbitRange(BRow(T.b1, t.F1, t, M), er, T.mmm, t.n_mmm, lane, s);
+    if (t.kind == kE0 || t.kind == kQuartic) {
+      const char *wm = WRow(t, T, M), *bm = BRow(T.b1, t.F1, t, M);
+      if (s == 64)
+        StateOrbitGroup<64>(wm, bm, er, T.state, t.n_state, T.mmm, t.n_mmm, lane, de, la);
+      else if (s == 32)
+        StateOrbitGroup<32>(wm, bm, er, T.state, t.n_state, T.mmm, t.n_mmm, lane, de, la);
+      else if (s == 16)
+        StateOrbitGroup<16>(wm, bm, er, T.state, t.n_state, T.mmm, t.n_mmm, lane, de, la);
+      else
+        StateOrbitGroup<8>(wm, bm, er, T.state, t.n_state, T.mmm, t.n_mmm, lane, de, la);
+    } else {
+      de = StateRange(WRow(t, T, M), er, T.state, 0, t.n_state, lane, s);
+    }
     if (t.kind == kQuartic) lb = OrbitRange(BRow(T.b2, t.F2, t, M), er, T.mm2, t.n_mm2, lane, s);
   }
   for (int off = s >> 1; off > 0; off >>= 1) {  // uniform trip count

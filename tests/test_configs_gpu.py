@@ -7,6 +7,8 @@
 * uneven-size batches (8, 11, 13 and 16 cells in one batch) for the three first-order kernel
   shapes and the chain shape, and canonical MC on 30^3-cell replicas at Al/Mg/Zn 90/5/5 under
   the linear 800 -> 300 K ramp;
+* the surrogate's 12 x 1024-row inference batch: full fallback equals the plain path, energies
+  stay exact with the specified fallback rule;
 * 50^3-cell quaternary replicas (Al/Cu/Mg/Zn 94/2/2/2, T = 450 K);
 * the early stop of KineticMcFirstAbstract::IsEscaped (KineticMcAbstract.cpp:192-223).
 
@@ -97,6 +99,72 @@ def test_batched_1024_replicas_20_cells_graph(torch, bench_coeffs):
         np.testing.assert_allclose(st["time"][r], ost["time"], rtol=1e-12)
         assert abs(st["energy"][r] - ost["energy"]) <= TOL_E * 300
         assert abs(st["abs_energy"][r] - ost["abs_energy"]) <= TOL_E * (300 + abs(ost["abs_energy"]))
+
+
+def test_surrogate_12x1024_rows_full_fallback_and_exact_energies(torch, bench_coeffs):
+    """The surrogate inference batch of 12 x 1024 event rows per step (1024 replicas of 20^3
+    cells): with every event falling back, the trajectory is the plain path's (selections and
+    occupancies equal, barriers within 1e-9 eV); with the specified fallback rule, the tracked
+    energies stay those of the cluster expansion of the final configurations."""
+    import latticemontecarlo_amd as L
+    path, els = bench_coeffs["e0"]
+    gb = L.Model(path, els, L.E0)
+    ge = L.Model(path, els, L.ENERGY)
+    R, n, seed, steps = 1024, 20, 2025, 60
+    rng = np.random.default_rng(seed)
+    N = 4 * n ** 3
+    occs, temps = [], []
+    for r in range(R):
+        xm, xz = rng.uniform(0.01, 0.05, 2)
+        occs.append(_alloy(n, gb.species, {"Mg": round(xm * N), "Zn": round(xz * N)},
+                           seed=seed * 100003 + r))
+        temps.append(rng.uniform(400.0, 700.0))
+    temps = np.array(temps)
+    sur = L.Surrogate(gb, n_members=4, seed=4)
+    print("surrogate 1024 x 20^3: full fallback against plain", flush=True)
+    a = L.KineticMcFirst(gb, ge, [n] * R, occs, temps, seed, maximum_steps=10 ** 12)
+    b = L.KineticMcFirst(gb, ge, [n] * R, occs, temps, seed, maximum_steps=10 ** 12)
+    a.set_formulation(0)
+    b.set_surrogate(sur, threshold=-1.0)
+    a.set_trace(steps)
+    b.set_trace(steps)
+    a.run(steps)
+    b.run(steps)
+    gb.check_errors()
+    ta, tb = a.trace(), b.trace()
+    assert np.array_equal(ta["sel"], tb["sel"])
+    np.testing.assert_allclose(tb["ea"], ta["ea"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(tb["de"], ta["de"], rtol=0, atol=1e-9)
+    for x, y in zip(a.occupancy(), b.occupancy()):
+        assert np.array_equal(x, y)
+    st = b.surrogate_stats()
+    assert st["events"] == 12 * R * steps and st["fallbacks"] == st["events"]
+    del a, b
+    # trained surrogate, specified rule: energies tracked through served steps stay exact
+    print("surrogate 1024 x 20^3: trained, energies", flush=True)
+    s_drv = L.KineticMcFirst(gb, None, [n] * R, occs, temps, seed + 1, maximum_steps=10 ** 12)
+    E, K, D = [], [], []
+    for _ in range(6):
+        s_drv.run(8)
+        e, k, d = s_drv.samples()
+        E.append(e)
+        K.append(k)
+        D.append(d)
+    env, ek, de = L.augment(torch.cat(E), torch.cat(K), torch.cat(D))
+    del s_drv
+    sur.train(env, ek, de, epochs=8, batch=4096, lr=3e-3, seed=1)
+    c = L.KineticMcFirst(gb, ge, [n] * R, occs, temps, seed, maximum_steps=10 ** 12)
+    c.set_surrogate(sur)
+    c.run(steps)
+    gb.check_errors()
+    st = c.state()
+    final = c.occupancy()
+    e_final = L.total_energy(ge, L.ReplicaBatch([n] * R, final)).cpu().numpy()
+    np.testing.assert_allclose(st["abs_energy"], e_final, rtol=0,
+                               atol=TOL_E * steps + 1e-12 * np.abs(e_final).max())
+    s2 = c.surrogate_stats()
+    print(f"served {s2['events'] - s2['fallbacks']} of {s2['events']} events", flush=True)
+    assert s2["events"] == 12 * R * steps and s2["fallbacks"] < s2["events"]
 
 
 @pytest.mark.parametrize("form", [0, 1, 2, 3])

@@ -154,12 +154,42 @@ def test_surrogate_eval_matches_ensemble_and_fallback(torch, coeffs):
     np.testing.assert_allclose(ea[f], ea0[f], rtol=0, atol=1e-9)
     np.testing.assert_allclose(de[f], de0[f], rtol=0, atol=1e-9)
     spread = np.maximum(np.ptp(np.maximum(0.0, pred[..., 0] + pred[..., 1] / 2), -1), np.ptp(pred[..., 1], -1))
-    assert not np.any(~f & (spread > 0.01 + 1e-6) & (ea_s <= ea_s.min(1, keepdims=True)))
+    # the specified rule (window 0): no served event has members more than 10 meV apart
+    assert not np.any(~f & (spread > 0.01 + 1e-6))
+    # with a relevance window, disagreeing events above the window may be served
+    _, ea_w, _, fl_w = [x.cpu().numpy() for x in sur.eval_jumps(gm, batch, vacs, temps, window=20.0)]
+    f_w = fl_w.astype(bool)
+    assert np.all(f_w <= f)  # the window only removes fallbacks
+    assert not np.any(~f_w & (spread > 0.01 + 1e-6) & (ea_s <= ea_s.min(1, keepdims=True)))
+    np.testing.assert_allclose(ea_w[~f_w], ea_s[~f_w], rtol=2e-6, atol=1e-7)
     # every event falls back with a negative threshold
     _, ea2, de2, fl2 = [x.cpu().numpy() for x in sur.eval_jumps(gm, batch, vacs, temps, threshold=-1.0)]
     assert fl2.all()
     np.testing.assert_allclose(ea2, ea0, rtol=0, atol=1e-9)
     np.testing.assert_allclose(de2, de0, rtol=0, atol=1e-9)
+
+
+def test_surrogate_serves_events_within_the_error_target(torch, coeffs):
+    """What the trained network gains over the method without it: on held-out states, the
+    specified fallback rule serves a share of the events from the ensemble (their cluster
+    expansion is not evaluated) while the barriers of all events, served and exact, stay within
+    the 5 meV MAE target of the exact evaluation."""
+    import latticemontecarlo_amd as L
+    path, els = coeffs["e0_3"]
+    gm = L.Model(path, els, L.E0)
+    sur = _trained(L, gm, epochs=60)
+    n = 8
+    batch, occs, vacs = _batch(L, gm.species, n, list(np.linspace(0.015, 0.045, 60)), seed=7000)
+    temps = np.linspace(400.0, 700.0, 60)
+    _, ea, _, fl = [x.cpu().numpy() for x in sur.eval_jumps(gm, batch, vacs, temps)]
+    _, ea0, _ = [x.cpu().numpy() for x in L.eval_jumps(gm, batch, vacs)]
+    served = 1.0 - fl.mean()
+    mae_all = np.abs(ea - ea0).mean()
+    mae_served = np.abs(ea - ea0)[fl == 0].mean()
+    print(f"served {served * 100:.1f}% of {fl.size} events; Ea MAE {mae_all * 1e3:.3f} meV over "
+          f"all events, {mae_served * 1e3:.3f} meV over served events")
+    assert mae_all <= 0.005
+    assert served >= 0.15
 
 
 def test_surrogate_kmc_with_full_fallback_is_the_plain_trajectory(torch, coeffs):
