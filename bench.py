@@ -296,12 +296,13 @@ def time_surrogate(L, bm, sur, R, n, occs, temps, seed, steps, warmup, chunk, pl
     return out
 
 
-def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_value):
+def surrogate_leg(L, bm, R, n, occs, temps, seed, steps, warmup, chunk, plain_value,
+                  epochs=SURROGATE_EPOCHS):
     """Learned surrogate on the bench workload: trained on trajectories of 1024 replicas of
     10^3 cells of the same composition range, then timed on the bench replicas with the
     specified fallback rule, and once more with the relevance window."""
     s_occs, s_temps = make_workload(1024, 10, seed + 1)
-    sur, info = train_surrogate(L, bm, s_occs, s_temps, 10, seed)
+    sur, info = train_surrogate(L, bm, s_occs, s_temps, 10, seed, epochs)
     out = time_surrogate(L, bm, sur, R, n, occs, temps, seed, steps, warmup, chunk, plain_value)
     out.update(info)
     win = time_surrogate(L, bm, sur, R, n, occs, temps, seed, steps, warmup, chunk, plain_value,
@@ -367,7 +368,7 @@ def large_workload(args):
     log(f"plain: {value:.4g} KMC steps/s over {R} replicas ({elapsed:.2f} s)")
     del drv
     sur, info = train_surrogate(L, bm, alloy(1024, 10, args.seed + 1), np.full(1024, 450.0), 10,
-                                args.seed)
+                                args.seed, args.surrogate_epochs)
     s = time_surrogate(L, bm, sur, R, n, occs, temps, args.seed, args.steps, args.warmup,
                        args.chunk, value)
     s.update(info)
@@ -797,6 +798,8 @@ def main():
                     default="batched")
     ap.add_argument("--cmc-trials", type=int, default=100000)
     ap.add_argument("--surrogate-steps", type=int, default=20000)
+    ap.add_argument("--surrogate-epochs", type=int, default=SURROGATE_EPOCHS,
+                    help="training epochs of the surrogate (fewer for a profiling run)")
     ap.add_argument("--model", choices=["e0", "quartic"], default="e0",
                     help="barrier model of the batched workload")
     args = ap.parse_args()
@@ -925,7 +928,8 @@ def main():
     if not args.no_surrogate and not quartic:
         line["surrogate"] = surrogate_leg(L, bm, R, n, occs, temps, args.seed,
                                           args.surrogate_steps, min(max(args.warmup, 100), 1000),
-                                          args.chunk, value)
+                                          args.chunk, value,
+                                          args.surrogate_epochs)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(coeff, kind="Quartic" if quartic else "E0")
         if not quartic:
