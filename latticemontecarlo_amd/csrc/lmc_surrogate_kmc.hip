@@ -1,10 +1,12 @@
 // gfx950 persistent KineticMcFirst kernel with the surrogate ensemble standing in for the
 // jump evaluation, and its stateless counterpart (one evaluation per replica).
 //
-// Work mapping: one workgroup holds kG = 4 replicas, kSub threads each (a block). Per step:
+// Work mapping: one workgroup holds G replicas, kSub threads each (a block): G = 4 with 128
+// threads each when the batch fills the chip, G = 1 with 256 threads when there are no more
+// replicas than CUs. Per step:
 //   1. gather    : each block gathers its replica's 12 jump environments into LDS;
 //   2. ensemble  : wave w evaluates member w % 4 on the 16-row tiles (12 events + 4 unused rows)
-//                  of kG / (waves / 4) replicas. Layer 1 is computed transposed (hidden units as
+//                  of G / (waves / 4) replicas. Layer 1 is computed transposed (hidden units as
 //                  rows, events as columns), so its accumulators, rounded to bf16, are already the
 //                  second layer's B fragments in a permuted k order; the second layer's W2
 //                  fragments are loaded in that order. Hidden activations never leave registers;
@@ -15,7 +17,7 @@
 //                  or when the members disagree by more than the threshold (with a relevance
 //                  window: only on events whose lowest member barrier is within `window` kT of
 //                  the replica's lowest barrier);
-//   4. fallback  : the flagged events of all kG replicas are spread over the whole workgroup,
+//   4. fallback  : the flagged events of all G replicas are spread over the whole workgroup,
 //                  an aligned power-of-two lane group per event; each group reduces its cluster
 //                  sums by lane exchanges and its first lane finishes the event;
 //   5. select    : ordered cumulative probabilities, Philox draw, residence time (as the plain
@@ -25,6 +27,9 @@
 //   7. apply.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "lmc_jump_device.h"
 #include "lmc_mlp.h"
 #include "lmc_surrogate.h"
@@ -32,18 +37,17 @@
 namespace lmc {
 namespace {
 
-constexpr int kG = 4;        // replicas per workgroup
 constexpr int kMembers = 4;
 constexpr int kTile = 16;    // rows of a replica's tile: its 12 events and 4 unused rows
 
-template <int kSub>
+template <int G, int kSub>
 struct Shape {
-  static constexpr int kBlock = kG * kSub;
+  static constexpr int kBlock = G * kSub;
   static constexpr int kWaves = kBlock / 64;
   static constexpr int kQ = kWaves / kMembers;  // waves per member
-  static constexpr int kTPW = kG / kQ;          // replica tiles per wave
+  static constexpr int kTPW = G / kQ;           // replica tiles per wave
   static constexpr int kSubWaves = kSub / 64;
-  static_assert(kQ * kTPW == kG && kSubWaves * 64 == kSub, "whole tiles and waves");
+  static_assert(kQ * kTPW == G && kSubWaves * 64 == kSub && kSubWaves <= 4, "whole tiles and waves");
 };
 
 template <int C>
@@ -52,14 +56,15 @@ struct MlpDims {
   static constexpr int KS = Kp / 32;
 };
 
+template <int G>
 struct SurLds {
   float b1[kMembers][kHidden], b2[kMembers][kHidden];  // biases of the hidden layers
   uint32_t w3[kMembers][kHidden];                      // bf16 W3 row pairs (out 0 | out 1 << 16)
-  float yp[kMembers][kG * kTile][kMlpOut];  // member outputs, normalised, without b3
+  float yp[kMembers][G * kTile][kMlpOut];   // member outputs, normalised, without b3
   uint8_t none[kEnvRow];                    // input row of an absent replica (no channel matches)
-  double dsum[kG][4];                       // exact dE: per-wave sums
-  int fbmask[kG], sel[kG];
-  double dt[kG];
+  double dsum[G][4];                        // exact dE: per-wave sums
+  int fbmask[G], sel[G];
+  double dt[G];
 };
 
 // Per-lane element offset of the weight fragments (8 bf16 per lane), made opaque once per step
@@ -129,17 +134,18 @@ __device__ __forceinline__ int WaveMember() {
 
 // The replica of tile slot g of this workgroup takes part in the step (uniform: LDS state
 // written before the previous step's last barrier).
+template <int G>
 __device__ __forceinline__ bool Active(const KmcDev &k, const ReplicaLds *Qs, int g) {
-  return static_cast<int>(blockIdx.x) * kG + g < k.R && !Qs[g].done;
+  return static_cast<int>(blockIdx.x) * G + g < k.R && !Qs[g].done;
 }
 
 // Phase 2: member outputs (normalised, without b3) of every replica's 12 events into S.yp.
 // The ring R holds the wave's first weight tiles, issued before the gather barrier.
-template <int C, int kSub>
+template <int C, int G, int kSub>
 __device__ void EnsembleForward(const SurDev &sd, const KmcDev &k, const JumpLds *Ls,
                                 const ReplicaLds *Qs, const LaneOffsets &lo, Ring<C> &R,
-                                SurLds &S) {
-  using Sh = Shape<kSub>;
+                                SurLds<G> &S) {
+  using Sh = Shape<G, kSub>;
   constexpr int KS = MlpDims<C>::KS, Kp = MlpDims<C>::Kp, TPW = Sh::kTPW, D = Ring<C>::kDepth;
   // with three channels the one-hot slot and channel indices of a lane are divisions by 3: the
   // lane comes from the per-step opaque offset, so that they are recomputed each step rather
@@ -157,7 +163,7 @@ __device__ void EnsembleForward(const SurDev &sd, const KmcDev &k, const JumpLds
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt) {
     const int g = q * TPW + tt;
-    const uint8_t *e = (col < 12 && Active(k, Qs, g)) ? Ls[g].env[col] : S.none;
+    const uint8_t *e = (col < 12 && Active<G>(k, Qs, g)) ? Ls[g].env[col] : S.none;
 #pragma unroll
     for (int s = 0; s < KS; ++s) x[tt][s] = OneHotFragment<C>(e, 32 * s + 8 * grp, chan);
   }
@@ -220,7 +226,8 @@ __device__ void EnsembleForward(const SurDev &sd, const KmcDev &k, const JumpLds
 }
 
 // Phase 3 (lanes k < 12 of block g): surrogate events, fallback flags. Returns the flag.
-__device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, const SurLds &S,
+template <int G>
+__device__ bool EnsembleEvent(const KTables &t, const SurDev &sd, JumpLds &L, const SurLds<G> &S,
                               int g, int k, double beta) {
   const int row = kTile * g + k;
   double ek[kMembers], dv[kMembers];
@@ -317,14 +324,14 @@ __device__ __forceinline__ void StateOrbitGroup(const char *wm, const char *bm, 
 // workgroup. The nf events get aligned lane groups of s threads (the largest power of two <= 64
 // with s nf <= the workgroup size); a group's partial sums meet by butterfly exchanges and its
 // first lane finishes the event.
-template <int kSub>
+template <int G, int kSub>
 __device__ void FallbackEvents(const KTables &t, JumpLds *Ls, const ReplicaLds *Qs, const Tabs &T,
-                               const SurLds &S, int32_t *err) {
-  constexpr int kBlock = Shape<kSub>::kBlock;
+                               const SurLds<G> &S, int32_t *err) {
+  constexpr int kBlock = Shape<G, kSub>::kBlock;
   const int tid = threadIdx.x;
-  int cnt[kG], nf = 0;
+  int cnt[G], nf = 0;
 #pragma unroll
-  for (int g = 0; g < kG; ++g) {
+  for (int g = 0; g < G; ++g) {
     cnt[g] = __popc(S.fbmask[g]);
     nf += cnt[g];
   }
@@ -375,25 +382,25 @@ __device__ void FallbackEvents(const KTables &t, JumpLds *Ls, const ReplicaLds *
 
 // Phase 6 (all lanes of block g): exact dE of direction k from the state clusters; per-wave sums
 // in S.dsum[g].
-template <int kSub>
-__device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLds &S, int g,
+template <int G, int kSub>
+__device__ void ExactDe(const KTables &t, const JumpLds &L, const Tabs &T, SurLds<G> &S, int g,
                         int lt, int k) {
   const int M = L.env[k][t.n_slot];
   const double v = WaveSum(StateRange(WRow(t, T, M), L.er[k], T.state, 0, t.n_state, lt, kSub));
   if ((lt & 63) == 0) S.dsum[g][lt >> 6] = v;
 }
 
-template <int C, int kSub, bool kKmc, bool kBL>
-__global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayout lay, KmcDev k,
+template <int C, int G, int kSub, bool kKmc, bool kBL>
+__global__ __launch_bounds__(G * kSub) void SurrogateKernel(KTables t, DynLayout lay, KmcDev k,
                                                              SurDev sd, uint64_t nsteps,
                                                              uint64_t trace_base, SurOut out) {
-  constexpr int kBlock = Shape<kSub>::kBlock, Kp = MlpDims<C>::Kp;
-  __shared__ JumpLds Ls[kG];
-  __shared__ ReplicaLds Qs[kG];
-  __shared__ SurLds S;
+  constexpr int kBlock = Shape<G, kSub>::kBlock, Kp = MlpDims<C>::Kp;
+  __shared__ JumpLds Ls[G];
+  __shared__ ReplicaLds Qs[G];
+  __shared__ SurLds<G> S;
   extern __shared__ char dyn[];
   const int tid = threadIdx.x, g = tid / kSub, lt = tid - g * kSub;
-  const int r = blockIdx.x * kG + g;
+  const int r = blockIdx.x * G + g;
   const bool live = r < k.R;
   JumpLds &L = Ls[g];
   ReplicaLds &Q = Qs[g];
@@ -447,7 +454,7 @@ __global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayou
   for (uint64_t it = 0; it < nsteps; ++it) {
     bool any = false;
 #pragma unroll
-    for (int j = 0; j < kG; ++j) any = any || Active(k, Qs, j);
+    for (int j = 0; j < G; ++j) any = any || Active<G>(k, Qs, j);
     if (!any) break;  // uniform: every thread reads the same LDS state
     const bool on = live && !Q.done;
     if (kKmc && k.n_tt > 0) {  // before the draw lane reads T
@@ -470,7 +477,7 @@ __global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayou
     if (kKmc && on && lt == kSub - 1) DrawStep(L, Q);
     if (on) GatherStore<kSub>(t, L, k.error, lg, gat);
     __syncthreads();
-    EnsembleForward<C, kSub>(sd, k, Ls, Qs, lo, ring, S);
+    EnsembleForward<C, G, kSub>(sd, k, Ls, Qs, lo, ring, S);
     __syncthreads();
     if (lt < 12) {
       const bool f = on && EnsembleEvent(t, sd, L, S, g, lt, Q.beta);
@@ -484,7 +491,7 @@ __global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayou
       }
     }
     __syncthreads();
-    FallbackEvents<kSub>(t, Ls, Qs, T, S, k.error);
+    FallbackEvents<G, kSub>(t, Ls, Qs, T, S, k.error);
     __syncthreads();
     if (!kKmc) {
       if (on && lt < 12) {
@@ -508,14 +515,14 @@ __global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayou
     __syncthreads();
     const int sel = S.sel[g];
     const bool need = on && !((S.fbmask[g] >> L.dir[sel]) & 1);
-    if (need) ExactDe<kSub>(t, L, T, S, g, lt, L.dir[sel]);
+    if (need) ExactDe<G, kSub>(t, L, T, S, g, lt, L.dir[sel]);
     __syncthreads();
     if (on && lt == 0) {
       double de = L.de[sel];
       if (need) {
         de = S.dsum[g][0];
 #pragma unroll
-        for (int j = 1; j < Shape<kSub>::kSubWaves; ++j) de += S.dsum[g][j];
+        for (int j = 1; j < Shape<G, kSub>::kSubWaves; ++j) de += S.dsum[g][j];
       }
       ApplyEvent(t, k, L, Q, occ, r, trace_base + it, sel, S.dt[g], de);
     }
@@ -544,7 +551,7 @@ __global__ __launch_bounds__(kG * kSub) void SurrogateKernel(KTables t, DynLayou
 
 // LDS plan of the surrogate kernel: one workgroup per CU; the fallback's tables (dE weights,
 // cluster lists) always in LDS, the barrier weights when they fit.
-DynLayout PlanSurrogateLds(const KTables &t) {
+DynLayout PlanSurrogateLds(const KTables &t, int G) {
   static size_t lds_cu = 0, lds_block = 0;
   if (!lds_cu) {
     int dev = 0;
@@ -557,7 +564,8 @@ DynLayout PlanSurrogateLds(const KTables &t) {
       lds_cu = lds_block = 65536;
     }
   }
-  const size_t stat = kG * (sizeof(JumpLds) + sizeof(ReplicaLds)) + sizeof(SurLds) + 256;
+  const size_t stat = G * (sizeof(JumpLds) + sizeof(ReplicaLds)) +
+                      (G == 1 ? sizeof(SurLds<1>) : sizeof(SurLds<4>)) + 256;
   const size_t share = std::min(lds_block, lds_cu);
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
   DynLayout lay{};
@@ -587,40 +595,75 @@ DynLayout PlanSurrogateLds(const KTables &t) {
   return lay;
 }
 
+// Raises a kernel's dynamic LDS limit to `bytes` once (the largest size allowed so far is kept
+// per kernel function; every instance of the surrogate kernel has the same pointer type).
 template <typename K>
 hipError_t AllowSurrogateLds(K kernel, size_t bytes) {
-  static int allowed = -1;  // per kernel instance: the largest size already allowed
-  if (static_cast<int>(bytes) <= allowed) return hipSuccess;
-  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+  static std::mutex mu;
+  static std::map<const void *, int> allowed;
+  const void *f = reinterpret_cast<const void *>(kernel);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = allowed.find(f);
+  if (it != allowed.end() && static_cast<int>(bytes) <= it->second) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            static_cast<int>(bytes));
-  if (e == hipSuccess) allowed = static_cast<int>(bytes);
+  if (e == hipSuccess) allowed[f] = static_cast<int>(bytes);
   return e;
 }
 
-// Threads per replica: 128 (512-thread workgroups).
-constexpr int kSubUsed = 128;
+// Workgroup shapes: 4 replicas x 128 threads when the batch has more replicas than the device
+// has CUs, else 1 replica x 256 threads (every replica gets a CU of its own).
+int SurrogateGroup(int32_t R) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    cus = hipGetDeviceProperties(&p, dev) == hipSuccess ? p.multiProcessorCount : 256;
+  }
+  return R > cus ? 4 : 1;
+}
+
+template <bool kKmc, int G, int kSub>
+hipError_t LaunchSurrogateG(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
+                            uint64_t trace_base, const SurOut &out, hipStream_t s) {
+  const DynLayout lay = PlanSurrogateLds(t, G);
+  const dim3 grid((k.R + G - 1) / G);
+  auto go = [&](auto kernel) {
+    hipError_t e = AllowSurrogateLds(kernel, lay.bytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kernel, grid, dim3(G * kSub), lay.bytes, s, t, lay, k, sd, steps,
+                       trace_base, out);
+    return hipGetLastError();
+  };
+  if (lay.b1_lds) {
+    if (sd.w.C == 2) return go(SurrogateKernel<2, G, kSub, kKmc, true>);
+    if (sd.w.C == 3) return go(SurrogateKernel<3, G, kSub, kKmc, true>);
+  } else {
+    if (sd.w.C == 2) return go(SurrogateKernel<2, G, kSub, kKmc, false>);
+    if (sd.w.C == 3) return go(SurrogateKernel<3, G, kSub, kKmc, false>);
+  }
+  return hipErrorInvalidValue;
+}
 
 template <bool kKmc>
 hipError_t LaunchSurrogate(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
                            uint64_t trace_base, const SurOut &out, hipStream_t s) {
   if (k.R <= 0 || steps == 0) return hipSuccess;
   if (sd.w.M != kMembers) return hipErrorInvalidValue;
-  const DynLayout lay = PlanSurrogateLds(t);
-  const dim3 grid((k.R + kG - 1) / kG);
-  auto go = [&](auto kernel) {
-    hipError_t e = AllowSurrogateLds(kernel, lay.bytes);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kernel, grid, dim3(kG * kSubUsed), lay.bytes, s, t, lay, k, sd, steps,
-                       trace_base, out);
-    return hipGetLastError();
-  };
+  if (SurrogateGroup(k.R) == 4) return LaunchSurrogateG<kKmc, 4, 128>(t, k, sd, steps, trace_base, out, s);
+  return LaunchSurrogateG<kKmc, 1, 256>(t, k, sd, steps, trace_base, out, s);
+}
+
+template <int G, int kSub>
+hipError_t PrepareSurrogateG(const KTables &t, int C) {
+  const DynLayout lay = PlanSurrogateLds(t, G);
   if (lay.b1_lds) {
-    if (sd.w.C == 2) return go(SurrogateKernel<2, kSubUsed, kKmc, true>);
-    if (sd.w.C == 3) return go(SurrogateKernel<3, kSubUsed, kKmc, true>);
+    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, G, kSub, true, true>, lay.bytes);
+    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, G, kSub, true, true>, lay.bytes);
   } else {
-    if (sd.w.C == 2) return go(SurrogateKernel<2, kSubUsed, kKmc, false>);
-    if (sd.w.C == 3) return go(SurrogateKernel<3, kSubUsed, kKmc, false>);
+    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, G, kSub, true, false>, lay.bytes);
+    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, G, kSub, true, false>, lay.bytes);
   }
   return hipErrorInvalidValue;
 }
@@ -628,15 +671,8 @@ hipError_t LaunchSurrogate(const KTables &t, const KmcDev &k, const SurDev &sd, 
 }  // namespace
 
 hipError_t PrepareSurrogateKmc(const KTables &t, int C) {
-  const DynLayout lay = PlanSurrogateLds(t);
-  if (lay.b1_lds) {
-    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, kSubUsed, true, true>, lay.bytes);
-    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, kSubUsed, true, true>, lay.bytes);
-  } else {
-    if (C == 2) return AllowSurrogateLds(SurrogateKernel<2, kSubUsed, true, false>, lay.bytes);
-    if (C == 3) return AllowSurrogateLds(SurrogateKernel<3, kSubUsed, true, false>, lay.bytes);
-  }
-  return hipErrorInvalidValue;
+  const hipError_t e = PrepareSurrogateG<4, 128>(t, C);
+  return e != hipSuccess ? e : PrepareSurrogateG<1, 256>(t, C);
 }
 
 hipError_t LaunchSurrogateKmc(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
