@@ -471,11 +471,11 @@ __global__ __launch_bounds__(G * kSub) void SurrogateKernel(KTables t, DynLayout
     Gathered<kSub> gat;
     if (on) GatherIssue<kSub>(L, occ, n, VacancyXYZ(Q), lg, gat);
     const LaneOffsets lo = MakeLaneOffsets<C>(tid & 63);
-    Ring<C> ring;  // the first weight tiles, issued after the occupancy loads
-#pragma unroll
-    for (int i = 0; i < Ring<C>::kDepth; ++i) RingIssue<C>(ring, i, W1m, W2m, lo);
     if (kKmc && on && lt == kSub - 1) DrawStep(L, Q);
     if (on) GatherStore<kSub>(t, L, k.error, lg, gat);
+    Ring<C> ring;  // the first weight tiles, issued once the occupancy has arrived
+#pragma unroll
+    for (int i = 0; i < Ring<C>::kDepth; ++i) RingIssue<C>(ring, i, W1m, W2m, lo);
     __syncthreads();
     EnsembleForward<C, G, kSub>(sd, k, Ls, Qs, lo, ring, S);
     __syncthreads();
