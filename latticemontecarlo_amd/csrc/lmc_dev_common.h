@@ -46,6 +46,21 @@ __device__ __forceinline__ void Unpack16(uint32_t v, int &x, int &y, int &z) {
   y = static_cast<int>((v >> 4) & 15) - 8;
   z = static_cast<int>((v >> 8) & 15) - 8;
 }
+// Wave priority of step `it` of the calling workgroup (s_setprio takes an immediate). The SIMD
+// arbiter favours the oldest waves, so among the workgroups resident on one CU the first
+// dispatched would run fastest and the last would set the end of a launch (measured: 1.5x
+// between them with four per CU). Co-resident workgroups are 256 block indices apart on a
+// 256-CU GPU, so (block / 256 + it) mod 4 gives them distinct priorities that rotate step by
+// step, and each gets the same share of the CU over a launch.
+__device__ __forceinline__ void RotatePriority(uint64_t it) {
+  switch (static_cast<unsigned>((blockIdx.x >> 8) + it) & 3u) {
+    case 0: __builtin_amdgcn_s_setprio(0); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    default: __builtin_amdgcn_s_setprio(3); break;
+  }
+}
+
 // A replica's lattice size and a site id are usable (uniform over a block).
 __device__ __forceinline__ bool ValidSite(int n, int64_t site) {
   return n >= 4 && site >= 0 && site < 4ll * n * n * n;

@@ -140,6 +140,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
   const int sv = Q.solvent;
   for (uint64_t it = 0; it < nsteps; ++it) {
     if (Q.done) break;
+    RotatePriority(it);
     UpdateTemperature(k, Q, tid);
     EvaluateJumps<kSolvent>(t, L, T, occ, n, VacancyXYZ(Q), sv, Q.beta, k.error, tid, &Q);
     SelectAndApply<kRecord>(t, k, L, Q, occ, r, trace_base + it, tid);
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(kThreads, 4) void KmcChainKernel(KTables t, DynLayo
   const int sv = Q.solvent;
   for (uint64_t it = 0; it < nsteps; ++it) {
     if (Q.done) break;
+    RotatePriority(it);
     UpdateTemperature(k, Q, tid);
     ChainNeighbours(t, L, C, Q, occ, n, tid);
     if (tid == kThreads - 1) DrawStep(L, Q);
