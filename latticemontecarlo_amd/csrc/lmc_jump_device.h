@@ -67,6 +67,21 @@ struct Tabs {  // resolved table pointers (LDS or global)
   const uint16_t *inc_off;
 };
 
+// Copies n elements from global memory to LDS with threads gtid of gsize, eight loads per thread
+// in flight before their stores (the copy at a launch's start is latency-bound otherwise).
+template <typename E>
+__device__ __forceinline__ void CopyToLds(E *dst, const E *src, int n, int gtid, int gsize) {
+  int i = gtid;
+  for (; i + 7 * gsize < n; i += 8 * gsize) {
+    E v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[i + u * gsize];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) dst[i + u * gsize] = v[u];
+  }
+  for (; i < n; i += gsize) dst[i] = src[i];
+}
+
 // Per-replica LDS tables (threads tid of this replica's block) and the workgroup's shared
 // dynamic tables (threads gtid of gsize); returns the resolved table pointers.
 // kBL: the barrier weight tables live in LDS (lay.b1_lds, and lay.b2_lds for the quartic model),
@@ -89,15 +104,15 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
   if (tid == 0) L.bad = 0;
   Tabs T{};
   double *w = reinterpret_cast<double *>(dyn + lay.w);
-  for (int i = gtid; i < t.nwd; i += gsize) w[i] = t.wd[i];
+  CopyToLds(w, t.wd, t.nwd, gtid, gsize);
   T.w = w;
   if (kBL) {
     double *p = reinterpret_cast<double *>(dyn + lay.b1);
-    for (int i = gtid; i < t.ne * t.F1; i += gsize) p[i] = t.b1[i];
+    CopyToLds(p, t.b1, t.ne * t.F1, gtid, gsize);
     T.b1 = p;
     double *q = reinterpret_cast<double *>(dyn + lay.b2);
     if (t.kind == kQuartic)
-      for (int i = gtid; i < t.ne * t.F2; i += gsize) q[i] = t.b2[i];
+      CopyToLds(q, t.b2, t.ne * t.F2, gtid, gsize);
     T.b2 = q;
   } else {
     T.b1 = t.b1;
@@ -108,7 +123,7 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     for (int i = gtid; i < 3 * t.S * t.S; i += gsize) bs[i] = t.base[i];
     T.base = bs;
     uint32_t *in = reinterpret_cast<uint32_t *>(dyn + lay.lists);
-    for (int i = gtid; i < t.n_inc; i += gsize) in[i] = t.inc[i];
+    CopyToLds(in, t.inc, t.n_inc, gtid, gsize);
     T.inc = in;
     uint16_t *io = reinterpret_cast<uint16_t *>(dyn + lay.offs);
     for (int i = gtid; i < 3 * (kEnvSlots + 1); i += gsize) io[i] = t.inc_off[i];
@@ -120,14 +135,14 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     const uint32_t *gs = kWide ? t.state_w : t.state, *go = kWide ? t.mmm_w : t.mmm;
     const uint32_t *gq = kWide ? t.mm2_w : t.mm2;
     uint32_t *p = reinterpret_cast<uint32_t *>(dyn + lay.lists);
-    for (int i = gtid; i < ns; i += gsize) p[i] = gs[i];
+    CopyToLds(p, gs, ns, gtid, gsize);
     T.state = p;
     p += ns;
-    for (int i = gtid; i < no; i += gsize) p[i] = go[i];
+    CopyToLds(p, go, no, gtid, gsize);
     T.mmm = p;
     p += no;
     if (t.kind == kQuartic)
-      for (int i = gtid; i < nq; i += gsize) p[i] = gq[i];
+      CopyToLds(p, gq, nq, gtid, gsize);
     T.mm2 = p;
   }
   return T;
