@@ -225,6 +225,25 @@ def train_surrogate(L, bm, s_occs, s_temps, n, seed, epochs=SURROGATE_EPOCHS):
                  "train_epochs": epochs, "train_s": round(train_s, 2)}
 
 
+def measured_mfma(R, n):
+    """MFMA rate of the surrogate kernel from the committed rocprofv3 counter summary of this
+    workload (512 x SQ_INSTS_VALU_MFMA_MOPS_BF16 per launch over its duration); None when none
+    exists."""
+    path = os.path.join(ROOT, "profiles", f"surrogate_kmc_{R}x{4 * n ** 3}_kernel_trace.md")
+    if not os.path.exists(path):
+        return None
+    rates = []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("| dispatch"):
+                rates.append(float(line.split("|")[3]))
+    if not rates:
+        return None
+    return {"source": os.path.relpath(path, ROOT), "counter": "SQ_INSTS_VALU_MFMA_MOPS_BF16",
+            "tflop_per_s": rates, "frac_of_dense_bf16_peak": round(max(rates) / 2500.0, 4),
+            "note": "counted FLOP include the 4 unused rows of each 16-row replica tile"}
+
+
 def mlp_flops_per_replica_step(sur):
     """Multiply-adds x 2 of the ensemble on the 12 event rows of one replica-step (the rows the
     network sees; the kernel's 16-row tiles carry 4 unused rows per replica)."""
@@ -289,6 +308,7 @@ def time_surrogate(L, bm, sur, R, n, occs, temps, seed, steps, warmup, chunk, pl
            "mlp_flop_per_replica_step": flops,
            "mlp_tflop_per_s": round(flops * value / 1e12, 2),
            "mfma_frac_of_dense_bf16_peak": round(flops * value / 2.5e15, 4),
+           "mfma_counters": measured_mfma(R, n),
            "dtype": "bf16 weights and activations, fp32 accumulation"}
     log(f"surrogate KMC (window {window}): {value:.4g} steps/s ({value / plain_value:.2f}x "
         f"plain), fallback {out['fallback_rate'] * 100:.1f}%, MAE Ea "
