@@ -559,6 +559,37 @@ __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const
   StoreEvent(L, k, ea, de, ekra, beta);
 }
 
+// Phase 3, dense: wave 0 finishes all 12 events (after a workgroup barrier), so that the two FP64
+// exponentials, the rank search and the partial sums of the 12 events are issued once, in 12
+// lanes of one wave, instead of in 3 lanes of each of four waves. Lane 3e + c first adds the
+// kLanes partials of component c of event e (in lane order, as before); lane e then finishes
+// event e.
+__device__ __forceinline__ void FinishDense(const KTables &t, JumpLds &L, double beta,
+                                            int32_t *err, int tid) {
+  if (tid >= 64) return;
+  const int l = tid;
+  if (l < 36) {
+    const int e = l / 3, c = l - 3 * e;
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kLanes; ++i) s += L.part[e * kLanes + i][c];
+    L.sums[e][c] = s;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (l >= 12) return;
+  const int k = l;
+  const double de = L.sums[k][0], lb = L.sums[k][2];
+  double la = L.sums[k][1];
+  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
+    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
+  double ea, ekra;
+  Barrier(t, L, k, de, la, lb, ea, ekra);
+  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);  // unknown cluster type
+  StoreEvent(L, k, ea, de, ekra, beta);
+}
+
 // Wide shape (kWideThreads threads per replica): wave k evaluates event k with its 64 lanes (the
 // lists padded to kClusterPadWide, 4 entries of each per lane and batch), sums the lane partials
 // by a butterfly of shuffles, and lane 0 finishes the event. No workgroup barrier separates the
@@ -613,9 +644,14 @@ __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, cons
   if (draw && tid == kThreads - 1) DrawStep(L, *draw);
   GatherStore<kThreads, kScan>(t, L, err, tid, G, sub);
   __syncthreads();
-  if (kSolvent) SolventClusters(t, L, T, sv, tid);
-  else DenseClusters(t, L, T, tid);
-  FinishEvents<kSolvent>(t, L, T, sv, beta, err, tid);
+  if (kSolvent) {
+    SolventClusters(t, L, T, sv, tid);
+    FinishEvents<true>(t, L, T, sv, beta, err, tid);
+  } else {
+    DenseClusters(t, L, T, tid);
+    __syncthreads();
+    FinishDense(t, L, beta, err, tid);
+  }
   __syncthreads();
 }
 
