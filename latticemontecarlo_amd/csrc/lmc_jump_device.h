@@ -559,6 +559,28 @@ __device__ __forceinline__ void FinishEvents(const KTables &t, JumpLds &L, const
   StoreEvent(L, k, ea, de, ekra, beta);
 }
 
+// Orders the LDS traffic of one wave around a point (its lanes then see each other's writes).
+__device__ __forceinline__ void WaveFence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane k < 12 of wave 0: event k from its cluster sums in L.sums[k] (dE, MMM logit, MM2 logit):
+// Ea, dE and the rate, stored at the jump's rank among the 12 neighbour ids.
+__device__ __forceinline__ void FinishFromSums(const KTables &t, JumpLds &L, double beta,
+                                               int32_t *err, int k) {
+  if (k >= 12) return;
+  const double de = L.sums[k][0], lb = L.sums[k][2];
+  double la = L.sums[k][1];
+  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
+    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
+  double ea, ekra;
+  Barrier(t, L, k, de, la, lb, ea, ekra);
+  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);  // unknown cluster type
+  StoreEvent(L, k, ea, de, ekra, beta);
+}
+
 // Phase 3, dense: wave 0 finishes all 12 events (after a workgroup barrier), so that the two FP64
 // exponentials, the rank search and the partial sums of the 12 events are issued once, in 12
 // lanes of one wave, instead of in 3 lanes of each of four waves. Lane 3e + c first adds the
@@ -575,30 +597,18 @@ __device__ __forceinline__ void FinishDense(const KTables &t, JumpLds &L, double
     for (int i = 0; i < kLanes; ++i) s += L.part[e * kLanes + i][c];
     L.sums[e][c] = s;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (l >= 12) return;
-  const int k = l;
-  const double de = L.sums[k][0], lb = L.sums[k][2];
-  double la = L.sums[k][1];
-  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
-    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
-  double ea, ekra;
-  Barrier(t, L, k, de, la, lb, ea, ekra);
-  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);  // unknown cluster type
-  StoreEvent(L, k, ea, de, ekra, beta);
+  WaveFence();
+  FinishFromSums(t, L, beta, err, l);
 }
 
 // Wide shape (kWideThreads threads per replica): wave k evaluates event k with its 64 lanes (the
 // lists padded to kClusterPadWide, 4 entries of each per lane and batch), sums the lane partials
-// by a butterfly of shuffles, and lane 0 finishes the event. No workgroup barrier separates the
-// cluster sums from the finish.
+// by DPP wave sums, and lane 0 writes the event's sums to L.sums[k]; after a workgroup barrier,
+// wave 0 finishes all 12 events (FinishFromSums) in the same wave that selects the jump.
 constexpr int kWideThreads = 12 * 64;
 static_assert(kClusterPadWide == 4 * 64, "wide lists are padded to whole batches of 64 lanes");
 
-__device__ void WideEvent(const KTables &t, JumpLds &L, const Tabs &T, double beta, int32_t *err,
-                          int tid) {
+__device__ void WideSums(const KTables &t, JumpLds &L, const Tabs &T, int tid) {
   const int k = tid >> 6, lane = tid & 63;
   const EventRanks &er = L.er[k];
   const int M = L.env[k][t.n_slot];
@@ -613,12 +623,9 @@ __device__ void WideEvent(const KTables &t, JumpLds &L, const Tabs &T, double be
   la = WaveSum(la);
   lb = WaveSum(lb);
   if (lane != 0) return;
-  if ((t.kind == kE0 || t.kind == kQuartic) && ((L.bad >> k) & 1))
-    la = __builtin_nan("");  // no one-hot type for a vacancy (the reference throws)
-  double ea, ekra;
-  Barrier(t, L, k, de, la, lb, ea, ekra);
-  if (!(ea == ea) || !(de == de) || !(la == la)) atomicOr(err, 1);  // unknown cluster type
-  StoreEvent(L, k, ea, de, ekra, beta);
+  L.sums[k][0] = de;
+  L.sums[k][1] = la;
+  L.sums[k][2] = lb;
 }
 
 // Random numbers of step Q.steps (one lane, overlapping the gather): Philox u1, u2, log(u1) and
@@ -633,7 +640,7 @@ __device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
 
 // `draw`: the replica whose next random numbers the last thread draws while the environment loads
 // are in flight (null: none). kScan: the gather applies `sub` (see GatherSub).
-template <bool kSolvent, bool kScan = false>
+template <bool kSolvent, bool kScan = false, bool kTailSync = true>
 __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, const Tabs &T,
                                               const uint8_t *occ, int n, const XYZ &vac, int sv,
                                               double beta, int32_t *err, int tid,
@@ -652,7 +659,10 @@ __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, cons
     __syncthreads();
     FinishDense(t, L, beta, err, tid);
   }
-  __syncthreads();
+  // without kTailSync (dense only), the caller's next reader of the events must be wave 0,
+  // which finished them
+  if (kTailSync || kSolvent) __syncthreads();
+  else if (tid < 64) WaveFence();
 }
 
 // TimeTemperatureInterpolator::GetTemperature (TimeTemperatureInterpolator.cpp:44-65)

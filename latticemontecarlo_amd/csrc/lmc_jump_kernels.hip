@@ -142,7 +142,9 @@ __global__ __launch_bounds__(kThreads, 4) void KmcKernel(KTables t, DynLayout la
     if (Q.done) break;
     RotatePriority(it);
     UpdateTemperature(k, Q, tid);
-    EvaluateJumps<kSolvent>(t, L, T, occ, n, VacancyXYZ(Q), sv, Q.beta, k.error, tid, &Q);
+    // the selection reads the events in wave 0, which finished them (dense shape)
+    EvaluateJumps<kSolvent, false, false>(t, L, T, occ, n, VacancyXYZ(Q), sv, Q.beta, k.error,
+                                          tid, &Q);
     SelectAndApply<kRecord>(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }
@@ -176,8 +178,12 @@ __global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynL
       GatherStore<kWideThreads>(t, L, k.error, tid, G);
     }
     __syncthreads();
-    WideEvent(t, L, T, Q.beta, k.error, tid);
+    WideSums(t, L, T, tid);
     __syncthreads();
+    if (tid < 64) {  // wave 0 finishes the 12 events, then selects
+      FinishFromSums(t, L, Q.beta, k.error, tid);
+      WaveFence();
+    }
     SelectAndApply<kRecord>(t, k, L, Q, occ, r, trace_base + it, tid);
     __syncthreads();
   }
