@@ -34,6 +34,11 @@ struct JumpLds {
   int dir[12];
   // random numbers of the current step (drawn by an idle lane during the gather)
   double u2, log_u1, corr;
+  // Philox u2 and log u1 of steps rnd_base .. rnd_base + kDrawAhead - 1, drawn together by the
+  // lanes of one wave (rnd_ok: the buffer holds them)
+  double rnd_u2[64], rnd_lu1[64];
+  uint64_t rnd_base;
+  int rnd_ok;
   // selection probabilities by rank; per-event sums (dE, MMM logit, MM2 logit)
   double prob[12], sums[12][3];
   // events whose environment holds a vacancy off the jump endpoints (no one-hot type)
@@ -101,7 +106,10 @@ __device__ Tabs LoadJumpTables(const KTables &t, JumpLds &L, char *dyn, const Dy
     L.er[k].r8[j] = 0;
     L.er[k].rn8[j] = 0;
   }
-  if (tid == 0) L.bad = 0;
+  if (tid == 0) {
+    L.bad = 0;
+    L.rnd_ok = 0;  // the random-number buffer is refilled at the first step
+  }
   Tabs T{};
   double *w = reinterpret_cast<double *>(dyn + lay.w);
   CopyToLds(w, t.wd, t.nwd, gtid, gsize);
@@ -638,7 +646,40 @@ __device__ void DrawStep(JumpLds &L, const ReplicaLds &Q) {
                          : 1.0;
 }
 
-// `draw`: the replica whose next random numbers the last thread draws while the environment loads
+// Random numbers of step Q.steps from a buffer of kDrawAhead steps that the last wave of the block
+// (threads kBT - 64 .. kBT - 1) refills when the step leaves it, one step per lane: Philox, the
+// logarithm and their issue slots are spent once per 64 steps instead of in one lane per step.
+// Same values as DrawStep. The buffer's state (L.rnd_ok, L.rnd_base) is read and written by
+// that wave only; L.rnd_ok must be 0 at the start of a launch.
+constexpr int kDrawAhead = 64;
+template <int kBT>
+__device__ void DrawSteps(JumpLds &L, const ReplicaLds &Q, int tid) {
+  if (tid < kBT - 64) return;
+  const int lane = tid - (kBT - 64);
+  if (!L.rnd_ok || Q.steps - L.rnd_base >= static_cast<uint64_t>(kDrawAhead)) {  // uniform
+    const U4 u = Philox4x32_10(Q.seed, Q.steps + lane, Q.rid, 0u);
+    L.rnd_u2[lane] = UnitOpenRight(Join64(u.z, u.w));
+    L.rnd_lu1[lane] = log(UnitOpenLeft(Join64(u.x, u.y)));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 63) {
+      L.rnd_base = Q.steps;
+      L.rnd_ok = 1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane != 63) return;
+  const int i = static_cast<int>(Q.steps - L.rnd_base);
+  L.u2 = L.rnd_u2[i];
+  L.log_u1 = L.rnd_lu1[i];
+  L.corr = (Q.flags & 1) ? Q.cv / (1.64 * exp(-(0.66 / kBoltzmann / Q.T - 0.7))) / (1 - 13 * Q.cs)
+                         : 1.0;
+}
+
+// `draw`: the replica whose next random numbers the last wave draws while the environment loads
 // are in flight (null: none). kScan: the gather applies `sub` (see GatherSub).
 template <bool kSolvent, bool kScan = false, bool kTailSync = true>
 __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, const Tabs &T,
@@ -648,7 +689,7 @@ __device__ __forceinline__ void EvaluateJumps(const KTables &t, JumpLds &L, cons
                                               const GatherSub &sub = {-1, 0}) {
   Gathered<> G;
   GatherIssue(L, occ, n, vac, tid, G);
-  if (draw && tid == kThreads - 1) DrawStep(L, *draw);
+  if (draw) DrawSteps<kThreads>(L, *draw, tid);
   GatherStore<kThreads, kScan>(t, L, err, tid, G, sub);
   __syncthreads();
   if (kSolvent) {

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kWideThreads, 1) void KmcWideKernel(KTables t, DynL
     {
       Gathered<kWideThreads> G;
       GatherIssue<kWideThreads>(L, occ, n, VacancyXYZ(Q), tid, G);
-      if (tid == kWideThreads - 1) DrawStep(L, Q);
+      DrawSteps<kWideThreads>(L, Q, tid);
       GatherStore<kWideThreads>(t, L, k.error, tid, G);
     }
     __syncthreads();
