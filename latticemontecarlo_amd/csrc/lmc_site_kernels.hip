@@ -546,7 +546,6 @@ __global__ __launch_bounds__(kThreads) void CmcKernel(KTables t, CmcDev c, uint6
   uint64_t it = 0;
   while (it < ntrials) {
     if (s_done) break;
-    RotatePriority(it);
     if (wave < 2) {
       if (!fresh) cand = DrawAttempt(c, r, steps + wave, static_cast<uint32_t>(lane), N, n);
       FindTrial(c, r, steps + wave, occ, N, n, t.S, lane, ox, oy, oz, cand, L, 2 * wave,
