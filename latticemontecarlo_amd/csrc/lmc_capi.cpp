@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -262,8 +263,26 @@ static void BuildSolventTables(const Model &m, const JumpTables &jt, const std::
   }
 }
 
+// Models are shared by host threads (the reference's predictors are const objects called from
+// OpenMP threads), so the lazy upload and every read of m.dev take this lock.
+static std::mutex &UploadMutex() {
+  static std::mutex mu;
+  return mu;
+}
+
+bool IsUploaded(const Model &m) {
+  std::lock_guard<std::mutex> lock(UploadMutex());
+  return m.dev != nullptr;
+}
+
+const KTables *UploadedOn(const Model &m, int device) {
+  std::lock_guard<std::mutex> lock(UploadMutex());
+  return m.dev && m.dev->device == device ? &m.dev->k : nullptr;
+}
+
 const KTables &Upload(Model &m) {
   const int device = CurrentDevice();
+  std::lock_guard<std::mutex> lock(UploadMutex());
   if (m.dev) {
     if (m.dev->device != device)
       throw std::invalid_argument("model was uploaded to another device");
@@ -926,7 +945,7 @@ int lmc_vacancy_scan(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, in
 int lmc_check_errors(const lmc_model *m) {
   return Guard([&] {
     RequireModel(m);
-    if (!m->m->dev) return LMC_OK;
+    if (!lmc::IsUploaded(*m->m)) return LMC_OK;
     Check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     int32_t e = 0;
     Check(hipMemcpy(&e, m->m->dev_err, sizeof(e), hipMemcpyDeviceToHost), "hipMemcpy");
@@ -1036,11 +1055,12 @@ int lmc_total_energy_noalloc(const lmc_model *m, const uint8_t *d_occ, const int
     if (n_replicas < 0 || (n_replicas && (!d_occ || !d_offset || !d_ncell || !d_energy)))
       throw std::invalid_argument("lmc_total_energy_noalloc: bad arguments");
     // the model's tables must already be on this device: uploading them would allocate
-    if (!m->m->dev || m->m->dev->device != CurrentDevice())
+    const lmc::KTables *tp = lmc::UploadedOn(*m->m, CurrentDevice());
+    if (!tp)
       throw std::invalid_argument(
           "lmc_total_energy_noalloc: the model's tables are not on this device; run any compute "
           "entry point with this model on it first");
-    const lmc::KTables &t = m->m->dev->k;
+    const lmc::KTables &t = *tp;
     const size_t need = static_cast<size_t>(n_replicas) * static_cast<size_t>(t.n_eci);
     const size_t cap = lmc::TotalEnergyScratchCapacity();
     if (need > cap)

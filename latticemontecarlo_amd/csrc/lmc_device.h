@@ -75,8 +75,12 @@ struct DeviceTables {
   KTables k{};
 };
 
-// Uploads (once) and returns the kernel view of a model on the current device.
+// Uploads (once) and returns the kernel view of a model on the current device. Thread-safe, as
+// are the two queries below.
 const KTables &Upload(Model &m);
+bool IsUploaded(const Model &m);
+// The kernel view when the model is already on `device`, else null (never uploads).
+const KTables *UploadedOn(const Model &m, int device);
 
 // Per-replica KMC state, structure of arrays in device memory.
 // One line of kmc_log.txt (KineticMcFirstAbstract::Dump, KineticMcAbstract.cpp:59-104), taken
@@ -231,5 +235,14 @@ hipError_t LaunchTotalEnergy(const KTables &t, const uint8_t *occ, const int64_t
 // (looked up once per device) and its capacity in counts.
 hipError_t TotalEnergyScratch(unsigned long long **ptr);
 size_t TotalEnergyScratchCapacity();
+
+// Limits of the current device that the launch plans read: compute units, LDS per CU and LDS
+// one workgroup may use (bytes). Queried once per device under a lock, so that launches from
+// several host threads or on several devices each see their own device's values.
+struct DeviceLimits {
+  int cus;
+  size_t lds_cu, lds_block;
+};
+const DeviceLimits &CurrentDeviceLimits();
 
 }  // namespace lmc

@@ -23,6 +23,8 @@
 // fit the workgroup's share of the CU's LDS, otherwise they are read from global memory.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "lmc_jump_device.h"
@@ -243,21 +245,9 @@ __global__ __launch_bounds__(kThreads, 4) void KmcChainKernel(KTables t, DynLayo
 // form: 0 dense, 1 solvent baseline, 2 dense wide shape (one workgroup per CU), 3 chain steps.
 DynLayout PlanLds(const KTables &t, int32_t R, int form) {
   const bool solvent = form == 1;
-  static int cus = 0;
-  static size_t lds_cu = 0, lds_block = 0;
-  if (!cus) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
-      cus = p.multiProcessorCount;
-      lds_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : 65536;
-      lds_block = p.sharedMemPerBlock;
-    } else {
-      cus = 256;
-      lds_cu = lds_block = 65536;
-    }
-  }
+  const DeviceLimits &dl = CurrentDeviceLimits();
+  const int cus = dl.cus;
+  const size_t lds_cu = dl.lds_cu, lds_block = dl.lds_block;
   const int per_cu = form == 2 ? 1 : std::min(4, std::max(1, (R + cus - 1) / cus));
   const size_t stat = sizeof(JumpLds) + sizeof(ReplicaLds) + (form == 3 ? sizeof(ChainLds) : 0) + 256;
   const size_t share = std::min(lds_block, lds_cu / per_cu);
@@ -382,6 +372,24 @@ hipError_t LaunchKmc(const KTables &t, const KmcDev &k, uint64_t steps, uint64_t
                        trace_base);
     return hipGetLastError();
   });
+}
+
+const DeviceLimits &CurrentDeviceLimits() {
+  static std::mutex mu;
+  static std::map<int, DeviceLimits> known;  // entries are never erased: references stay valid
+  static const DeviceLimits fallback{256, 65536, 65536};
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess) return fallback;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto it = known.find(dev);
+  if (it != known.end()) return it->second;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return fallback;
+  DeviceLimits d;
+  d.cus = p.multiProcessorCount;
+  d.lds_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : 65536;
+  d.lds_block = p.sharedMemPerBlock;
+  return known.emplace(dev, d).first->second;
 }
 
 }  // namespace lmc

@@ -1320,26 +1320,30 @@ hipError_t LaunchSiteDe(const KTables &t, const uint8_t *occ, const int64_t *off
 hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_t trace_base,
                      hipStream_t s) {
   if (c.R <= 0 || trials == 0) return hipSuccess;
-  static size_t lds_block = 0;
-  static size_t stat = 0, stat_omp = 0;
-  if (!lds_block) {
-    int dev = 0;
-    hipDeviceProp_t p;
-    hipFuncAttributes fa, fb, fc, fd;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess ||
-        hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(CmcKernel<true, false>)) !=
-            hipSuccess ||
-        hipFuncGetAttributes(&fb, reinterpret_cast<const void *>(CmcKernel<true, true>)) !=
-            hipSuccess ||
-        hipFuncGetAttributes(&fc, reinterpret_cast<const void *>(CmcOmpKernel<true, false>)) !=
-            hipSuccess ||
-        hipFuncGetAttributes(&fd, reinterpret_cast<const void *>(CmcOmpKernel<true, true>)) !=
-            hipSuccess)
-      return hipErrorInvalidDevice;
-    lds_block = p.sharedMemPerBlock;
-    stat = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
-    stat_omp = std::max(fc.sharedSizeBytes, fd.sharedSizeBytes);
+  // static LDS of the occupancy-in-LDS kernels, the same on every device of this build
+  static std::mutex mu;
+  static size_t stat = 0, stat_omp = 0;  // 0 until a query succeeds
+  size_t st_serial, st_omp;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!stat) {
+      hipFuncAttributes fa, fb, fc, fd;
+      if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(CmcKernel<true, false>)) !=
+              hipSuccess ||
+          hipFuncGetAttributes(&fb, reinterpret_cast<const void *>(CmcKernel<true, true>)) !=
+              hipSuccess ||
+          hipFuncGetAttributes(&fc, reinterpret_cast<const void *>(CmcOmpKernel<true, false>)) !=
+              hipSuccess ||
+          hipFuncGetAttributes(&fd, reinterpret_cast<const void *>(CmcOmpKernel<true, true>)) !=
+              hipSuccess)
+        return hipErrorInvalidDevice;
+      stat_omp = std::max(fc.sharedSizeBytes, fd.sharedSizeBytes);
+      stat = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
+    }
+    st_serial = stat;
+    st_omp = stat_omp;
   }
+  const size_t lds_block = CurrentDeviceLimits().lds_block;
   const size_t dyn = static_cast<size_t>(c.max_sites);
   auto launch = [&](auto kernel_lds, auto kernel_hbm, size_t st, int threads) {
     if (c.lds_occupancy && dyn % 4 == 0 && st + dyn <= lds_block) {
@@ -1357,10 +1361,10 @@ hipError_t LaunchCmc(const KTables &t, const CmcDev &c, uint64_t trials, uint64_
   };
   if (c.omp_k > 0)
     return c.log_every
-               ? launch(CmcOmpKernel<true, true>, CmcOmpKernel<false, true>, stat_omp, kOmpThreads)
-               : launch(CmcOmpKernel<true, false>, CmcOmpKernel<false, false>, stat_omp, kOmpThreads);
-  return c.log_every ? launch(CmcKernel<true, true>, CmcKernel<false, true>, stat, kThreads)
-                     : launch(CmcKernel<true, false>, CmcKernel<false, false>, stat, kThreads);
+               ? launch(CmcOmpKernel<true, true>, CmcOmpKernel<false, true>, st_omp, kOmpThreads)
+               : launch(CmcOmpKernel<true, false>, CmcOmpKernel<false, false>, st_omp, kOmpThreads);
+  return c.log_every ? launch(CmcKernel<true, true>, CmcKernel<false, true>, st_serial, kThreads)
+                     : launch(CmcKernel<true, false>, CmcKernel<false, false>, st_serial, kThreads);
 }
 
 hipError_t LaunchCmcAverage(const CmcDev &c, hipStream_t s) {

@@ -552,18 +552,8 @@ __global__ __launch_bounds__(G * kSub) void SurrogateKernel(KTables t, DynLayout
 // LDS plan of the surrogate kernel: one workgroup per CU; the fallback's tables (dE weights,
 // cluster lists) always in LDS, the barrier weights when they fit.
 DynLayout PlanSurrogateLds(const KTables &t, int G) {
-  static size_t lds_cu = 0, lds_block = 0;
-  if (!lds_cu) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
-      lds_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : 65536;
-      lds_block = p.sharedMemPerBlock;
-    } else {
-      lds_cu = lds_block = 65536;
-    }
-  }
+  const DeviceLimits &dl = CurrentDeviceLimits();
+  const size_t lds_cu = dl.lds_cu, lds_block = dl.lds_block;
   const size_t stat = G * (sizeof(JumpLds) + sizeof(ReplicaLds)) +
                       (G == 1 ? sizeof(SurLds<1>) : sizeof(SurLds<4>)) + 256;
   const size_t share = std::min(lds_block, lds_cu);
@@ -613,16 +603,7 @@ hipError_t AllowSurrogateLds(K kernel, size_t bytes) {
 
 // Workgroup shapes: 4 replicas x 128 threads when the batch has more replicas than the device
 // has CUs, else 1 replica x 256 threads (every replica gets a CU of its own).
-int SurrogateGroup(int32_t R) {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipDeviceProp_t p;
-    cus = hipGetDeviceProperties(&p, dev) == hipSuccess ? p.multiProcessorCount : 256;
-  }
-  return R > cus ? 4 : 1;
-}
+int SurrogateGroup(int32_t R) { return R > CurrentDeviceLimits().cus ? 4 : 1; }
 
 template <bool kKmc, int G, int kSub>
 hipError_t LaunchSurrogateG(const KTables &t, const KmcDev &k, const SurDev &sd, uint64_t steps,
