@@ -3,6 +3,9 @@
  * Plain C types only (no torch, no STL). Every function returns LMC_OK (0) or a negative
  * status; lmc_last_error() then returns a thread-local message. Pointers named d_* are device
  * pointers (hipMalloc'ed or torch CUDA tensors); `stream` is a hipStream_t (NULL = default).
+ * Calls over a batch reject a negative count, and a null buffer when the count is positive,
+ * before touching the device; a call over zero items does nothing. A model may be shared by
+ * host threads (its first use uploads its tables once, under a lock); a driver may not.
  *
  * Occupancy format: one uint8 per lattice site, indexed by the reference's lattice id
  * (x-major rank of the fractional position, lmc/cfg/src/Config.cpp:466-470); the value is the

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -466,6 +467,15 @@ void RequireModel(const lmc_model *m) {
   if (!m || !m->m) throw std::invalid_argument("null model");
 }
 
+// The buffers of a call over n items: a negative count, or a null buffer when n > 0, is an
+// argument error (a null device pointer would otherwise reach a kernel).
+void RequireBuffers(const char *fn, int64_t n, std::initializer_list<const void *> buffers) {
+  if (n < 0) throw std::invalid_argument(std::string(fn) + ": negative count");
+  if (n == 0) return;
+  for (const void *b : buffers)
+    if (!b) throw std::invalid_argument(std::string(fn) + ": null buffer");
+}
+
 template <typename T>
 T *DevAlloc(size_t n) {
   void *p = nullptr;
@@ -910,7 +920,9 @@ int lmc_eval_jumps(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_of
     RequireModel(m);
     if (m->m->kind != LMC_MODEL_E0 && m->m->kind != LMC_MODEL_QUARTIC)
       throw std::invalid_argument("lmc_eval_jumps needs an E0 or Quartic model");
-    if (n_replicas < 0) throw std::invalid_argument("n_replicas < 0");
+    RequireBuffers("lmc_eval_jumps", n_replicas,
+                   {d_occ, d_offset, d_ncell, d_vac, d_nb, d_ea, d_de});
+    if (n_replicas == 0) return LMC_OK;
     const auto &t = lmc::Upload(*m->m);
     Check(lmc::LaunchEvalJumps(t, d_occ, d_offset, d_ncell, d_vac, n_replicas, d_nb, d_ea, d_de,
                                m->m->dev_err, S(stream), m->m->formulation == 1),
@@ -962,6 +974,9 @@ int lmc_eval_pair(const lmc_model *m, const uint8_t *d_occ, int32_t ncell, int64
                   double *ea, double *de) {
   return Guard([&] {
     RequireModel(m);
+    RequireBuffers("lmc_eval_pair", 1, {d_occ, ea, de});
+    if (ncell < 4) throw std::invalid_argument("ncell must be >= 4");
+    if (v < 0 || v >= 4ll * ncell * ncell * ncell) throw std::out_of_range("vacancy site out of range");
     const int k = FindDirection(ncell, v, nb);
     if (k < 0) throw std::out_of_range("Neighbor not found for lattice pair in GetPairFlatIndex");
     int64_t *buf = DevAlloc<int64_t>(2 + 12);
@@ -1004,6 +1019,8 @@ int lmc_swap_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offse
                 void *stream) {
   return Guard([&] {
     RequireModel(m);
+    RequireBuffers("lmc_swap_de", n_replicas, {d_occ, d_offset, d_ncell, d_pairs, d_de});
+    if (n_replicas == 0) return LMC_OK;
     const auto &t = lmc::Upload(*m->m);
     Check(lmc::LaunchSwapDe(t, d_occ, d_offset, d_ncell, d_pairs, n_replicas, d_de, S(stream)),
           "swap dE kernel");
@@ -1016,6 +1033,8 @@ int lmc_site_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_offse
                 int32_t n_replicas, double *d_de, void *stream) {
   return Guard([&] {
     RequireModel(m);
+    RequireBuffers("lmc_site_de", n_replicas, {d_occ, d_offset, d_ncell, d_sites, d_codes, d_de});
+    if (n_replicas == 0) return LMC_OK;
     const auto &t = lmc::Upload(*m->m);
     Check(lmc::LaunchSiteDe(t, d_occ, d_offset, d_ncell, d_sites, nullptr, d_codes, n_replicas,
                             d_de, S(stream)),
@@ -1029,7 +1048,9 @@ int lmc_site_change_de(const lmc_model *m, const uint8_t *d_occ, const int64_t *
                        const uint8_t *d_codes, int32_t n_replicas, double *d_de, void *stream) {
   return Guard([&] {
     RequireModel(m);
-    if (!d_from) throw std::invalid_argument("null argument");
+    RequireBuffers("lmc_site_change_de", n_replicas,
+                   {d_occ, d_offset, d_ncell, d_sites, d_from, d_codes, d_de});
+    if (n_replicas == 0) return LMC_OK;
     const auto &t = lmc::Upload(*m->m);
     Check(lmc::LaunchSiteDe(t, d_occ, d_offset, d_ncell, d_sites, d_from, d_codes, n_replicas,
                             d_de, S(stream)),
@@ -1042,6 +1063,8 @@ int lmc_total_energy(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_
                      const int32_t *d_ncell, int32_t n_replicas, double *d_energy, void *stream) {
   return Guard([&] {
     RequireModel(m);
+    RequireBuffers("lmc_total_energy", n_replicas, {d_occ, d_offset, d_ncell, d_energy});
+    if (n_replicas == 0) return LMC_OK;
     ComputeTotalEnergy(m, d_occ, d_offset, d_ncell, n_replicas, d_energy, S(stream));
     return LMC_OK;
   });
@@ -1084,7 +1107,8 @@ int lmc_total_encode(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_
                      const int32_t *d_ncell, int32_t n_replicas, double *d_encode, void *stream) {
   return Guard([&] {
     RequireModel(m);
-    if (n_replicas <= 0 || !d_encode) throw std::invalid_argument("bad argument");
+    if (n_replicas <= 0) throw std::invalid_argument("lmc_total_encode: n_replicas must be > 0");
+    RequireBuffers("lmc_total_encode", n_replicas, {d_occ, d_offset, d_ncell, d_encode});
     const auto &t = lmc::Upload(*m->m);
     unsigned long long *counts =
         DevAlloc<unsigned long long>(static_cast<size_t>(n_replicas) * t.n_eci);
@@ -2310,7 +2334,8 @@ int lmc_jump_samples(const lmc_model *m, const uint8_t *d_occ, const int64_t *d_
     RequireModel(m);
     if (m->m->kind != LMC_MODEL_E0 && m->m->kind != LMC_MODEL_QUARTIC)
       throw std::invalid_argument("lmc_jump_samples needs an E0 or Quartic model");
-    if (n_replicas < 0 || !d_env || !d_ekra || !d_de) throw std::invalid_argument("bad arguments");
+    if (!d_env || !d_ekra || !d_de) throw std::invalid_argument("lmc_jump_samples: null buffer");
+    RequireBuffers("lmc_jump_samples", n_replicas, {d_occ, d_offset, d_ncell, d_vac});
     if (n_replicas == 0) return LMC_OK;
     const auto &t = lmc::Upload(*m->m);
     int64_t *nb = DevAlloc<int64_t>(12 * static_cast<size_t>(n_replicas));

@@ -85,3 +85,38 @@ def test_log_entry_points_reject_null_handles():
     for call in calls:
         with pytest.raises(capi.LmcError, match="null (driver|argument)"):
             capi.check(call())
+
+
+def test_device_buffer_entry_points_reject_null_buffers(models):
+    """Null device buffers with a positive count, and negative counts, are argument errors
+    raised on the host, before any device call (a null pointer would otherwise reach a
+    kernel)."""
+    gb, ge = models
+    lib = capi.lib()
+    x = C.c_void_p(1)  # never dereferenced: every call below fails its argument check first
+    cases = [
+        ("lmc_eval_jumps", lambda: lib.lmc_eval_jumps(gb.h, None, x, x, x, 1, x, x, x, None)),
+        ("lmc_eval_jumps", lambda: lib.lmc_eval_jumps(gb.h, x, x, x, x, 1, x, x, None, None)),
+        ("lmc_swap_de", lambda: lib.lmc_swap_de(ge.h, x, x, x, None, 1, x, None)),
+        ("lmc_site_de", lambda: lib.lmc_site_de(ge.h, x, x, x, x, None, 1, x, None)),
+        ("lmc_site_change_de", lambda: lib.lmc_site_change_de(ge.h, x, x, x, x, None, x, 1, x,
+                                                              None)),
+        ("lmc_total_energy", lambda: lib.lmc_total_energy(ge.h, x, None, x, 2, x, None)),
+        ("lmc_total_encode", lambda: lib.lmc_total_encode(ge.h, None, x, x, 1, x, None)),
+        ("lmc_jump_samples", lambda: lib.lmc_jump_samples(gb.h, x, x, x, None, 1, x, x, x, None)),
+        ("lmc_eval_pair", lambda: lib.lmc_eval_pair(gb.h, None, 6, 0, 6, None, None)),
+    ]
+    for name, call in cases:
+        with pytest.raises(capi.LmcError, match=f"{name}: null buffer"):
+            capi.check(call())
+    with pytest.raises(capi.LmcError, match="lmc_swap_de: negative count"):
+        capi.check(lib.lmc_swap_de(ge.h, x, x, x, x, -1, x, None))
+    with pytest.raises(capi.LmcError, match="lmc_eval_jumps: negative count"):
+        capi.check(lib.lmc_eval_jumps(gb.h, x, x, x, x, -3, x, x, x, None))
+    ea, de = C.c_double(), C.c_double()
+    with pytest.raises(capi.LmcError, match="ncell must be >= 4"):
+        capi.check(lib.lmc_eval_pair(gb.h, x, 0, 0, 6, C.byref(ea), C.byref(de)))
+    with pytest.raises(capi.LmcError, match="vacancy site out of range"):
+        capi.check(lib.lmc_eval_pair(gb.h, x, 6, 10 ** 6, 6, C.byref(ea), C.byref(de)))
+    # an empty call is not an error, and launches nothing
+    capi.check(lib.lmc_swap_de(ge.h, None, None, None, None, 0, None, None))
