@@ -133,15 +133,49 @@ def augment(env, ekra, de):
     return torch.cat(rows), ekra.repeat(4), de.repeat(4)
 
 
+def _per_replica(values, R, what, dtype, width=None):
+    """values as a flat array of R (or R x width) entries; a ValueError otherwise, since the
+    kernels read exactly that many from the device copy."""
+    a = np.asarray(values, dtype)
+    want = R if width is None else R * width
+    if a.size != want:
+        shape = f"{R}" if width is None else f"{R} x {width}"
+        raise ValueError(f"{what}: expected {shape} entries for {R} replicas, got {a.size}")
+    return a.reshape(-1)
+
+
+def _device_rows(t, what, width, dtype):
+    """A contiguous device tensor of rows of `width` entries of torch dtype `dtype`."""
+    if t.dim() != 2 or t.shape[1] != width or t.dtype != dtype or not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous {dtype} tensor [n, {width}], got "
+                         f"{t.dtype} {list(t.shape)}")
+    return t
+
+
+def _concat_occupancies(ncell, occ_list):
+    """(ncell int32 [R], the occupancies back to back as contiguous uint8, per-replica sizes);
+    a ValueError unless replica r holds 4 ncell[r]^3 sites, since the engine reads exactly
+    that many bytes per replica."""
+    nc = np.asarray(ncell, np.int32).reshape(-1)
+    occ_list = list(occ_list)
+    if len(occ_list) != nc.size:
+        raise ValueError(f"{nc.size} box sizes for {len(occ_list)} occupancies")
+    sizes = [4 * int(n) ** 3 for n in nc]
+    arrays = [np.asarray(o, np.uint8).reshape(-1) for o in occ_list]
+    for r, (a, N) in enumerate(zip(arrays, sizes)):
+        if a.size != N:
+            raise ValueError(f"replica {r}: {a.size} sites, expected {N}")
+    occ = np.ascontiguousarray(np.concatenate(arrays)) if arrays else np.zeros(0, np.uint8)
+    return nc, occ, sizes
+
+
 class ReplicaBatch:
     """Occupancies of R replicas on the GPU (concatenated uint8 with offsets)."""
 
     def __init__(self, ncell, occ_list, device="cuda"):
         torch = _torch()
-        self.ncell_host = np.asarray(ncell, np.int32)
-        sizes = [4 * int(n) ** 3 for n in self.ncell_host]
+        self.ncell_host, host, sizes = _concat_occupancies(ncell, occ_list)
         self.offset_host = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-        host = np.concatenate([np.asarray(o, np.uint8) for o in occ_list])
         self.occ = torch.from_numpy(host).to(device)
         self.offset = torch.from_numpy(self.offset_host).to(device)
         self.ncell = torch.from_numpy(self.ncell_host).to(device)
@@ -152,7 +186,7 @@ def eval_jumps(model, batch, vac, stream=None):
     """12 jumps per replica in neighbour-id order: (nb[R,12], Ea[R,12], dE[R,12])."""
     torch = _torch()
     R = batch.R
-    vac_t = torch.as_tensor(np.asarray(vac, np.int64)).to(batch.occ.device)
+    vac_t = torch.as_tensor(_per_replica(vac, R, "vac", np.int64)).to(batch.occ.device)
     nb = torch.empty((R, 12), dtype=torch.int64, device=batch.occ.device)
     ea = torch.empty((R, 12), dtype=torch.float64, device=batch.occ.device)
     de = torch.empty((R, 12), dtype=torch.float64, device=batch.occ.device)
@@ -181,7 +215,7 @@ def vacancy_scan(model, occ, n, vacancy, solvent, sites, stream=None):
 
 def swap_de(model, batch, pairs, stream=None):
     torch = _torch()
-    p = torch.as_tensor(np.asarray(pairs, np.int64).reshape(-1)).to(batch.occ.device)
+    p = torch.as_tensor(_per_replica(pairs, batch.R, "pairs", np.int64, 2)).to(batch.occ.device)
     de = torch.empty(batch.R, dtype=torch.float64, device=batch.occ.device)
     check(lib().lmc_swap_de(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
                             dptr(p), batch.R, dptr(de), capi.stream_ptr(stream)))
@@ -190,8 +224,8 @@ def swap_de(model, batch, pairs, stream=None):
 
 def site_de(model, batch, sites, codes, stream=None):
     torch = _torch()
-    s = torch.as_tensor(np.asarray(sites, np.int64)).to(batch.occ.device)
-    c = torch.as_tensor(np.asarray(codes, np.uint8)).to(batch.occ.device)
+    s = torch.as_tensor(_per_replica(sites, batch.R, "sites", np.int64)).to(batch.occ.device)
+    c = torch.as_tensor(_per_replica(codes, batch.R, "codes", np.uint8)).to(batch.occ.device)
     de = torch.empty(batch.R, dtype=torch.float64, device=batch.occ.device)
     check(lib().lmc_site_de(model.h, dptr(batch.occ), dptr(batch.offset), dptr(batch.ncell),
                             dptr(s), dptr(c), batch.R, dptr(de), capi.stream_ptr(stream)))
@@ -294,7 +328,7 @@ class KineticMcFirst:
             p.rate_corrector = int(np.broadcast_to(rate_corrector, (R,))[r])
             p.early_stop = int(np.broadcast_to(early_stop, (R,))[r])
             p.solute_disp = int(np.broadcast_to(solute_disp, (R,))[r])
-        occ = np.ascontiguousarray(np.concatenate([np.asarray(o, np.uint8) for o in occ_list]))
+        _, occ, _ = _concat_occupancies(self.ncell, occ_list)
         self.total_sites = occ.size
         h = C.c_void_p()
         check(lib().lmc_kmc_create(barrier_model.h, energy_model.h if energy_model else None, R,
@@ -474,7 +508,7 @@ class CanonicalMc:
             p.restart_steps = int(np.broadcast_to(restart_steps, (R,))[r])
             p.restart_energy = float(np.broadcast_to(restart_energy, (R,))[r])
             p.maximum_steps = int(np.broadcast_to(maximum_steps, (R,))[r])
-        occ = np.ascontiguousarray(np.concatenate([np.asarray(o, np.uint8) for o in occ_list]))
+        _, occ, _ = _concat_occupancies(self.ncell, occ_list)
         self.total_sites = occ.size
         h = C.c_void_p()
         check(lib().lmc_cmc_create(pairsite_model.h, energy_model.h if energy_model else None, R,
@@ -615,7 +649,7 @@ def jump_samples(model, batch, vac, stream=None):
     torch = _torch()
     dev = batch.occ.device
     R = batch.R
-    vac_t = torch.as_tensor(np.asarray(vac, np.int64)).to(dev)
+    vac_t = torch.as_tensor(_per_replica(vac, R, "vac", np.int64)).to(dev)
     env = torch.empty((12 * R, 64), dtype=torch.uint8, device=dev)
     ekra = torch.empty(12 * R, dtype=torch.float64, device=dev)
     de = torch.empty(12 * R, dtype=torch.float64, device=dev)
@@ -675,6 +709,7 @@ class Surrogate:
     def forward(self, env, stream=None):
         """Normalised outputs [n, members, 2] for environment rows env (uint8 [n, 64])."""
         torch = _torch()
+        _device_rows(env, "env", 64, torch.uint8)
         n = env.shape[0]
         out = torch.empty((n, self.members, 2), dtype=torch.float32, device=env.device)
         check(lib().lmc_surrogate_forward(self.h, dptr(env), n, dptr(out),
@@ -685,16 +720,25 @@ class Surrogate:
         """Per-member gradients [members, n_params] of sum (y - t)^2 / (2n) and the sums of
         squared errors [members] for normalised targets (float32 [n, 2])."""
         torch = _torch()
+        _device_rows(env, "env", 64, torch.uint8)
         n = env.shape[0]
         g = torch.empty((self.members, self.n_params), dtype=torch.float32, device=env.device)
         loss = np.zeros(self.members)
-        t = target.to(torch.float32).contiguous()
+        t = _device_rows(target.to(torch.float32).contiguous(), "target", 2, torch.float32)
+        if t.shape[0] != n:
+            raise ValueError(f"target: {t.shape[0]} rows for {n} environments")
         check(lib().lmc_surrogate_gradients(self.h, dptr(env), dptr(t), n, dptr(g),
                                             ptr(loss, capi._f64p)))
         return g, loss
 
     def train(self, env, ekra, de, epochs, batch=1024, lr=2e-3, seed=0):
         """Adam training on samples; returns the last epoch's mean squared error per member."""
+        torch = _torch()
+        _device_rows(env, "env", 64, torch.uint8)
+        n = env.shape[0]
+        for name, v in (("ekra", ekra), ("de", de)):
+            if v.dtype != torch.float64 or v.numel() != n or not v.is_contiguous():
+                raise ValueError(f"{name}: expected a contiguous float64 tensor of {n} values")
         mse = np.zeros(self.members)
         check(lib().lmc_surrogate_train(self.h, dptr(env), dptr(ekra), dptr(de), env.shape[0],
                                         epochs, batch, lr, seed, ptr(mse, capi._f64p)))
@@ -706,7 +750,7 @@ class Surrogate:
         torch = _torch()
         dev = batch.occ.device
         R = batch.R
-        vac_t = torch.as_tensor(np.asarray(vac, np.int64)).to(dev)
+        vac_t = torch.as_tensor(_per_replica(vac, R, "vac", np.int64)).to(dev)
         T = torch.as_tensor(np.broadcast_to(np.asarray(temps, np.float64), (R,)).copy()).to(dev)
         nb = torch.empty((R, 12), dtype=torch.int64, device=dev)
         ea = torch.empty((R, 12), dtype=torch.float64, device=dev)

@@ -35,8 +35,14 @@ def test_kmc_rejects_bad_batches(models):
     with pytest.raises(capi.LmcError, match="n_replicas must be positive"):  # an empty batch
         capi.check(capi.lib().lmc_kmc_create(gb.h, None, 0, capi.ptr(nc, capi._i32p),
                                              capi.ptr(good, capi._u8p), params, C.byref(h)))
-    for n in (3, 1001):
+    params2 = (capi.KmcReplica * 2)()
+    for n in (3, 1001):  # the C ABI checks box sizes before it reads any occupancy
+        nc2 = np.array([4, n], np.int32)
         with pytest.raises(capi.LmcError, match=r"lattice size n must be in \[4, 1000\]"):
+            capi.check(capi.lib().lmc_kmc_create(gb.h, None, 2, capi.ptr(nc2, capi._i32p),
+                                                 capi.ptr(good, capi._u8p), params2,
+                                                 C.byref(h)))
+        with pytest.raises(ValueError, match="replica 1"):  # the Python mirror checks sizes
             _kmc(gb, [4, n], [good, good])
     bad = good.copy()
     bad[7] = len(sp)
@@ -120,3 +126,28 @@ def test_device_buffer_entry_points_reject_null_buffers(models):
         capi.check(lib.lmc_eval_pair(gb.h, x, 6, 10 ** 6, 6, C.byref(ea), C.byref(de)))
     # an empty call is not an error, and launches nothing
     capi.check(lib.lmc_swap_de(ge.h, None, None, None, None, 0, None, None))
+
+
+def test_python_wrappers_check_shapes_before_any_launch(models):
+    """The Python mirror refuses per-replica arrays of the wrong length and occupancies of the
+    wrong size before anything reaches the C ABI (the kernels read exactly R entries)."""
+    import types
+
+    import torch
+
+    gb, ge = models
+    good = random_alloc(4, gb.species, 0.05, seed=3)
+    with pytest.raises(ValueError, match="2 box sizes for 1 occupancies"):
+        L.ReplicaBatch([4, 4], [good])
+    with pytest.raises(ValueError, match="replica 1: 256 sites, expected 500"):
+        L.ReplicaBatch([4, 5], [good, good])
+    fake = types.SimpleNamespace(R=3, occ=torch.zeros(1, dtype=torch.uint8), offset=None,
+                                 ncell=None)
+    with pytest.raises(ValueError, match="vac: expected 3 entries for 3 replicas, got 2"):
+        L.eval_jumps(gb, fake, [1, 2])
+    with pytest.raises(ValueError, match="vac: expected 3"):
+        L.jump_samples(gb, fake, [1, 2, 3, 4])
+    with pytest.raises(ValueError, match="pairs: expected 3 x 2 entries"):
+        L.swap_de(ge, fake, [[1, 2], [3, 4]])
+    with pytest.raises(ValueError, match="codes: expected 3 entries"):
+        L.site_de(ge, fake, [1, 2, 3], [0, 1])
