@@ -434,7 +434,7 @@ int lmc_cmc_write_log(lmc_cmc *c, const char *const *filenames);
  * -> (E_kra, dE), SiLU, FP32 accumulation on the matrix cores) that stands in for
  * GetBarrierAndDiffFromLatticeIdPair (VacancyMigrationPredictorE0.cpp:153-159,
  * VacancyMigrationPredictorQuartic.cpp:247-276), with the cluster-expansion predictor as its
- * fallback and judge. Input rows are the 60 environment slots of a jump in the engine's fixed
+ * fallback and the reference of its errors. Input rows are the 60 environment slots of a jump in the engine's fixed
  * per-direction slot order (one species code per byte, rows of 64 bytes). Ea = max(0, E_kra +
  * dE/2); for the quartic model E_kra = Ea - dE/2. Supports 3 or 4 elements (plus vacancy). */
 typedef struct lmc_surrogate lmc_surrogate;

@@ -661,7 +661,8 @@ def jump_samples(model, batch, vac, stream=None):
 
 class Surrogate:
     """Ensemble of BF16 MLPs standing in for a barrier model's jump evaluation (see
-    lmc_surrogate_create); the barrier model stays the fallback and the judge."""
+    lmc_surrogate_create); the barrier model stays the fallback and the reference its errors
+    are measured against."""
 
     def __init__(self, barrier_model, n_members=4, seed=0):
         h = C.c_void_p()

@@ -4,7 +4,7 @@ numpy FP64 restatement of the network, loss and gradients of the engine's surrog
 (latticemontecarlo_amd/csrc/lmc_mlp.h, lmc_surrogate.hip). The surrogate has no counterpart in
 the reference; it stands in for GetBarrierAndDiffFromLatticeIdPair
 (lmc/pred/src/VacancyMigrationPredictorE0.cpp:153-159), and the reference's predictor (restated
-in lmc_oracle.cpp) remains its judge.
+in lmc_oracle.cpp) remains the reference its errors are measured against.
 
 With rounding=True the bf16 rounding points of the kernels are reproduced (weights, h1, h2 and
 the activation gradients fed to the matrix cores are rounded to bf16, round to nearest even);
