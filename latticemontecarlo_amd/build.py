@@ -22,6 +22,14 @@ FLAGS = [
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# Per-source flags. The KMC kernels are scheduled with LLVM's iterative ILP strategy: on 1 024
+# replicas of 20^3 cells it gave 1.336e8 against 1.298e8 steps/s on 20-step launches and 1.624e8
+# against 1.584e8 on 50 000-step runs (chain steps +1.7 %, the uneven batch unchanged), with the
+# same results. The surrogate kernel stays on the default scheduler, under which it runs 2.6x
+# faster than under this one.
+SOURCE_FLAGS = {
+    "lmc_jump_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+}
 
 
 def _run(cmd):
@@ -38,7 +46,8 @@ def build(verbose=False, jobs=None):
     for src in SOURCES:
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmds.append([hipcc, *FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([hipcc, *FLAGS, *SOURCE_FLAGS.get(src, []), *inc, "-c", os.path.join(CSRC, src),
+                     "-o", obj])
     jobs = jobs or min(len(cmds), max(1, min(16, os.cpu_count() or 1)))
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(_run, cmds))
