@@ -22,13 +22,18 @@ FLAGS = [
     # no packed FP32 instructions (v_pk_*_f32) anywhere in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-# Per-source flags. The KMC kernels are scheduled with LLVM's iterative ILP strategy: on 1 024
-# replicas of 20^3 cells it gave 1.336e8 against 1.298e8 steps/s on 20-step launches and 1.624e8
-# against 1.584e8 on 50 000-step runs (chain steps +1.7 %, the uneven batch unchanged), with the
-# same results. The surrogate kernel stays on the default scheduler, under which it runs 2.6x
-# faster than under this one.
+# Per-source instruction-scheduling strategies of LLVM's AMDGPU backend, each measured against
+# the default on its own workloads (results unchanged):
+# - KMC kernels, iterative ILP: 1 024 replicas of 20^3 cells, 1.336e8 against 1.298e8 steps/s
+#   on 20-step launches and 1.624e8 against 1.584e8 on 50 000-step runs; chain steps +1.7 %;
+#   the uneven batch unchanged.
+# - canonical-MC kernels, iterative ILP: annealing +2.6 %, CMC beside the uneven KMC batch
+#   +0.9 %, CanonicalMcOmp rounds -0.25 %.
+# - surrogate KMC kernel, iterative max-occupancy: +6.2 % (iterative ILP: 2.6x slower).
 SOURCE_FLAGS = {
     "lmc_jump_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "lmc_site_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+    "lmc_surrogate_kmc.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-maxocc"],
 }
 
 
