@@ -29,7 +29,8 @@ FLAGS = [
 #   the uneven batch unchanged.
 # - canonical-MC kernels, iterative ILP: annealing +2.6 %, CMC beside the uneven KMC batch
 #   +0.9 %, CanonicalMcOmp rounds -0.25 %.
-# - surrogate KMC kernel, iterative max-occupancy: +6.2 % (iterative ILP: 2.6x slower).
+# - surrogate KMC kernel, iterative max-occupancy: +6.2 % (iterative ILP 2.6x and iterative
+#   min-register 2.8x slower, max-ILP -34 %, max-memory-clause -4 %).
 SOURCE_FLAGS = {
     "lmc_jump_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
     "lmc_site_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
