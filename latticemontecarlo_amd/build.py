@@ -31,6 +31,8 @@ FLAGS = [
 #   +0.9 %, CanonicalMcOmp rounds -0.25 %.
 # - surrogate KMC kernel, iterative max-occupancy: +6.2 % (iterative ILP 2.6x and iterative
 #   min-register 2.8x slower, max-ILP -34 %, max-memory-clause -4 %).
+# The surrogate's training kernels keep the default: iterative ILP and max-memory-clause trained
+# within the run-to-run spread of 160 epochs (about 20 s), iterative max-occupancy 18 % slower.
 SOURCE_FLAGS = {
     "lmc_jump_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
     "lmc_site_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
